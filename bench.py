@@ -1,0 +1,121 @@
+#!/usr/bin/env python3
+"""Throughput of the LPS-ABC time loop on one MI355X, in node-steps/s.
+
+Workload (default): the large single domain - R = 500 mm lattice (dx = 0.5 mm, 3,141,549 nodes,
+25,116 ABC nodes, 213,195,828 bonds), delta = 2 mm, nu = 0.33, Gaussian pulse of radius 40 mm at
+(150, -100) mm, least-squares ABC with G_theta, 6000 velocity-Verlet steps at the swept dt. A step is
+one time step of the loop over the whole domain (dilatation pass, force pass with the fused Verlet/ABC
+epilogue); the timed region is the loop as the reference times it (src/main.cpp:716-1166): all steps
+plus the output block every 30 steps, with inputs resident on the GPU and no VTK output.
+
+Printed: one JSON line with the metric, the force-pass roofline (HIP-event timing of that kernel
+inside the timed region against its algorithmic bytes 68 N + 4 nnz) and the CPU oracle timed on a
+bounded sample of the same workload on the host's cores.
+
+  python bench.py [--steps K] [--warmup W] [--workload cfg2|cfg1] [--no-cpu-baseline]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "peridynamic-lps-absorbing-boundary_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=60)
+    ap.add_argument("--workload", default="cfg2")
+    ap.add_argument("--exact", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=30)
+    ap.add_argument("--workers", type=int, default=None)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): the path is measured on a "
+              f"single MI355X; multi-GPU sharding is a design note, not a measured configuration.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from lps_amd import configs, solver
+    w = configs.WORKLOADS[args.workload]
+    steps = args.steps if args.steps is not None else w.steps
+    workers = args.workers or min(16, os.cpu_count() or 1)
+    t = time.time()
+    log(f"[bench] generating {w.name} ({w.description}) with {workers} workers")
+    prob = configs.make(w.name, workers=workers, steps=steps)
+    log(f"[bench] generated N={prob.N} C={prob.C} dt={prob.dt:.3e} in {time.time() - t:.1f}s")
+
+    t = time.time()
+    s = solver.Solver(prob, exact=args.exact, use_graph=True)
+    rp, _ = s.horizons()
+    nnz = int(rp[-1]) - prob.N  # bonds, self excluded
+    log(f"[bench] GPU setup (horizons, m_i, theta history) {time.time() - t:.1f}s, bonds={nnz}")
+    if args.warmup > 0:
+        s.run(args.warmup)
+        s.reset()
+    log("[bench] warmup done; timed run")
+
+    s.set_timing(True)
+    t0 = time.perf_counter()
+    recs = s.run(steps)
+    t1 = time.perf_counter()
+    pt = s.pass_times()
+    elapsed = t1 - t0
+    value = prob.N * steps / elapsed
+    last = recs[-1] if recs else None
+    refl = (1.0 + last["de_rel"]) if last else None
+    log(f"[bench] {steps} steps in {elapsed:.3f}s -> {value:.4e} node-steps/s; passes {pt}")
+
+    force_bytes = 68 * prob.N + 4 * nnz  # u, x, m, theta, rowptr, a_out per node + one index per bond
+    achieved = force_bytes / (pt["force_ms"] * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_force (force pass + fused Verlet/ABC epilogue)",
+                "algorithmic_bytes": force_bytes, "avg_ms": round(pt["force_ms"], 5),
+                "theta_pass_ms": round(pt["theta_ms"], 5), "step_ms": round(pt["step_ms"], 5)}
+    s.close()
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        import pyoracle
+        threads = min(16, os.cpu_count() or 1)
+        log(f"[bench] CPU oracle: setup then {args.cpu_steps} steps on {threads} threads")
+        o = pyoracle.Oracle(prob, threads=threads)
+        tc = time.perf_counter()
+        o.run(args.cpu_steps)
+        tcpu = time.perf_counter() - tc
+        cpu = {"value": prob.N * args.cpu_steps / tcpu, "unit": "node-steps/s", "cores": threads,
+               "kind": "port",
+               "sample": f"first {args.cpu_steps} steps of the same workload (oracle restatement of "
+                         f"src/main.cpp, OpenMP, -O2 no FMA), {tcpu:.1f}s"}
+        log(f"[bench] CPU {cpu['value']:.4e} node-steps/s")
+
+    out = {
+        "metric": "node-steps/s at reflected dE/E0<=1e-4 (plain FP64 LSQ ABC), 1 MI355X",
+        "value": value, "unit": "node-steps/s", "n_gpus": 1, "steps": steps,
+        "warmup": args.warmup, "ms_per_step": elapsed / steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{w.name}: {w.description}", "nodes": prob.N, "abc_nodes": prob.C,
+                   "bonds": nnz, "dt": prob.dt, "exact": bool(args.exact),
+                   "reflected_energy_final": refl,
+                   "gate_reflected_le_1e-4": (refl is not None and refl <= 1e-4)},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,222 @@
+"""ctypes binding of liblps_amd.so (the C ABI declared in include/lps_amd.h).
+
+This is the Python face of the drop-in: a `Solver` is one lps_ctx, created from a
+`preprocess.Problem` (the contents of the job files). There is no fallback: if the HIP library is
+missing or no GPU is present, creating a solver raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Dict, List, Optional
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "liblps_amd.so")
+HEADER_PATH = os.path.join(os.path.dirname(PKG_DIR), "include", "lps_amd.h")
+
+LPS_OK = 0
+LPS_ERR_ARG = -1
+LPS_ERR_HIP = -2
+LPS_ERR_HORIZON = -3
+LPS_ERR_NONFINITE = -4
+LPS_ERR_CONSISTENCY = -5
+LPS_THETA_IDW = 0
+LPS_THETA_GTHETA = 1
+
+
+class LpsError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"lps error {code}: {msg}")
+        self.code = code
+
+
+class LpsProblem(C.Structure):
+    _fields_ = [("node_no", C.c_int32), ("constrains_no", C.c_int32),
+                ("max_cloud_size", C.c_int32),
+                ("E", C.c_double), ("nu", C.c_double), ("rho", C.c_double), ("dt", C.c_double),
+                ("delta", C.c_double), ("dx", C.c_double), ("dy", C.c_double),
+                ("nodes", C.c_void_p), ("bctypes", C.c_void_p), ("icvalues", C.c_void_p),
+                ("icdvalues", C.c_void_p), ("inf_nodes", C.c_void_p), ("iclouds", C.c_void_p),
+                ("ci_nodes_u", C.c_void_p), ("ci_nodes_v", C.c_void_p),
+                ("ci_nodes_theta", C.c_void_p)]
+
+
+class LpsOptions(C.Structure):
+    _fields_ = [("theta_method", C.c_int32), ("abc_enabled", C.c_int32), ("exact", C.c_int32),
+                ("use_graph", C.c_int32), ("device", C.c_int32), ("horizon_radius", C.c_double)]
+
+
+class LpsDiag(C.Structure):
+    _fields_ = [("step", C.c_int64)] + [(n, C.c_double) for n in (
+        "t", "energy", "energy_k", "energy_p", "px", "py", "max_u", "max_v", "max_a",
+        "theta_max", "theta_min", "ub_rms", "vb_rms", "de_rel")] + [
+        ("max_u_node", C.c_int32), ("max_v_node", C.c_int32)]
+
+
+DIAG_FIELDS = [f[0] for f in LpsDiag._fields_]
+
+# Every function include/lps_amd.h declares, with its ctypes signature.
+SIGNATURES = {
+    "lps_default_options": (None, [C.POINTER(LpsOptions)]),
+    "lps_create": (C.c_int32, [C.POINTER(LpsProblem), C.POINTER(LpsOptions), C.POINTER(C.c_void_p)]),
+    "lps_reset": (C.c_int32, [C.c_void_p]),
+    "lps_step": (C.c_int32, [C.c_void_p, C.c_int32]),
+    "lps_run": (C.c_int32, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32]),
+    "lps_diagnostics": (C.c_int32, [C.c_void_p, C.POINTER(LpsDiag)]),
+    "lps_fetch": (C.c_int32, [C.c_void_p] + [C.c_void_p] * 5),
+    "lps_horizons": (C.c_int64, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "lps_step_index": (C.c_int64, [C.c_void_p]),
+    "lps_m_ref": (C.c_double, [C.c_void_p]),
+    "lps_set_timing": (C.c_int32, [C.c_void_p, C.c_int32]),
+    "lps_pass_times": (C.c_int32, [C.c_void_p, C.c_void_p]),
+    "lps_last_error": (C.c_char_p, [C.c_void_p]),
+    "lps_global_error": (C.c_char_p, []),
+    "lps_destroy": (None, [C.c_void_p]),
+}
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load liblps_amd.so; raises OSError when it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise OSError(f"{path} is missing; build it with __graft_entry__.build() or `make -C "
+                      f"{PKG_DIR}`")
+    L = C.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _ptr(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data
+
+
+def diag_dict(d: LpsDiag) -> Dict[str, float]:
+    return {k: getattr(d, k) for k in DIAG_FIELDS}
+
+
+class Solver:
+    """One lps_ctx on the GPU for a problem (lps_amd.preprocess.Problem)."""
+
+    def __init__(self, prob, theta_method: int = LPS_THETA_GTHETA, abc_enabled: bool = True,
+                 exact: bool = False, use_graph: bool = True, device: int = 0,
+                 horizon_radius: float = 0.0):
+        L = load_library()
+        self._keep: List[np.ndarray] = []
+
+        def arr(a, dt):
+            a = np.ascontiguousarray(a, dtype=dt)
+            self._keep.append(a)
+            return a
+
+        N, Cn, M = prob.N, prob.C, prob.max_cloud
+        d2 = prob.data2
+        p = LpsProblem()
+        p.node_no, p.constrains_no, p.max_cloud_size = N, Cn, M
+        p.E, p.nu, p.rho, p.dt = d2[0], d2[1], d2[2], d2[6]
+        p.delta, p.dx, p.dy = d2[7], d2[8], d2[9]
+        p.nodes = _ptr(arr(prob.nodes, np.float64))
+        p.bctypes = _ptr(arr(prob.bctypes, np.int32))
+        p.icvalues = _ptr(arr(prob.ic, np.float64))
+        p.icdvalues = _ptr(arr(prob.icd, np.float64))
+        if Cn > 0:
+            cl = np.asarray(prob.clouds)
+            # the file reader's (int)(x + 0.5): a -1 pad becomes node 0
+            cl = np.trunc(cl.astype(np.float64) + 0.5).astype(np.int32)
+            p.inf_nodes = _ptr(arr(prob.abc, np.int32))
+            p.iclouds = _ptr(arr(cl, np.int32))
+            p.ci_nodes_u = _ptr(arr(prob.Gu, np.float64))
+            p.ci_nodes_v = _ptr(arr(prob.Gv, np.float64))
+            if prob.Gth is not None and prob.Gth.size:
+                p.ci_nodes_theta = _ptr(arr(prob.Gth, np.float64))
+        o = LpsOptions()
+        L.lps_default_options(C.byref(o))
+        o.theta_method = theta_method
+        o.abc_enabled = 1 if abc_enabled else 0
+        o.exact = 1 if exact else 0
+        o.use_graph = 1 if use_graph else 0
+        o.device = device
+        o.horizon_radius = horizon_radius
+        h = C.c_void_p()
+        rc = L.lps_create(C.byref(p), C.byref(o), C.byref(h))
+        if rc != LPS_OK:
+            raise LpsError(rc, L.lps_global_error().decode())
+        self._h = h
+        self._L = L
+        self._keep.clear()
+        self.N = N
+        self.dt = float(d2[6])
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.lps_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc):
+        if rc < 0:
+            raise LpsError(rc, self._L.lps_last_error(self._h).decode())
+        return rc
+
+    def reset(self):
+        self._check(self._L.lps_reset(self._h))
+
+    def step(self, nsteps: int):
+        self._check(self._L.lps_step(self._h, nsteps))
+
+    def run(self, nsteps: int) -> List[Dict[str, float]]:
+        recs = (LpsDiag * max(1, nsteps // 30 + 1))()
+        n = self._check(self._L.lps_run(self._h, nsteps, recs, len(recs)))
+        return [diag_dict(recs[i]) for i in range(n)]
+
+    def diagnostics(self) -> Dict[str, float]:
+        d = LpsDiag()
+        self._check(self._L.lps_diagnostics(self._h, C.byref(d)))
+        return diag_dict(d)
+
+    def fetch(self) -> Dict[str, np.ndarray]:
+        N = self.N
+        u, v, a = (np.zeros(2 * N) for _ in range(3))
+        th, m = np.zeros(N), np.zeros(N)
+        self._check(self._L.lps_fetch(self._h, _ptr(u), _ptr(v), _ptr(a), _ptr(th), _ptr(m)))
+        return {"u": u, "v": v, "a": a, "theta": th, "m": m}
+
+    def horizons(self):
+        nnz = self._check(self._L.lps_horizons(self._h, None, None))
+        rp = np.zeros(self.N + 1, dtype=np.int64)
+        cols = np.zeros(max(nnz, 1), dtype=np.int32)
+        self._check(self._L.lps_horizons(self._h, _ptr(rp), _ptr(cols)))
+        return rp, cols[:nnz]
+
+    @property
+    def step_index(self) -> int:
+        return self._L.lps_step_index(self._h)
+
+    @property
+    def m_ref(self) -> float:
+        return self._L.lps_m_ref(self._h)
+
+    def set_timing(self, on: bool):
+        self._check(self._L.lps_set_timing(self._h, 1 if on else 0))
+
+    def pass_times(self):
+        out = np.zeros(3)
+        self._check(self._L.lps_pass_times(self._h, _ptr(out)))
+        return {"theta_ms": out[0], "force_ms": out[1], "step_ms": out[2]}

@@ -1,0 +1,139 @@
+"""Writes the golden fixtures of tests/golden/ from the reference's own programs.
+
+Run in a container where the reference tree and its builds exist (`make -C oracle`):
+    python tests/golden/make_golden.py
+
+Every fixture is data: inputs and the outputs the reference produced for them.
+  abc_*.npz      a generated job (ABC operators rounded to float32 and written with 17 digits, so
+                 the text the reference reads and the arrays stored here are the same doubles) and
+                 the reference solver's .energy / .verify series plus the fields of its last VTK
+                 snapshot (oracle/_ref/PD2_LPS_ABC, built from src/main.cpp).
+  noabc_r30.npz  the input of tests/generate_test_data.cpp (run as built from the reference source)
+                 and the reference solver's outputs for it.
+  kdtree_sets.npz  neighbour lists of the reference's build_clouds
+                 (src/helpers/kd_tree_interface.h:41-126, through oracle/_ref/libref_kdtree.so) for
+                 a lattice, jittered points and a lattice with ties exactly at the radius.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, os.path.join(ROOT, "peridynamic-lps-absorbing-boundary_amd"))
+from lps_amd import files, preprocess as pp  # noqa: E402
+
+REF_BIN = os.path.join(ROOT, "oracle", "_ref", "PD2_LPS_ABC")
+REF_GEN = os.path.join(ROOT, "oracle", "_ref", "generate_test_data")
+REF_KD = os.path.join(ROOT, "oracle", "_ref", "libref_kdtree.so")
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+
+def run_reference(prob: pp.Problem, tmp: str):
+    job = os.path.join(tmp, "job")
+    files.write_job(prob, job, exact_text=True)
+    r = subprocess.run([REF_BIN, "_out", "job"], cwd=tmp, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(r.stdout[-2000:] + r.stderr[-2000:])
+    energy = files.read_series(job + "_out.energy")
+    verify = files.read_series(job + "_out.verify")
+    last = (prob.steps // 30) * 30
+    u = files.read_vtk_field(f"{job}_out-{last}.vtk")
+    v = files.read_vtk_field(f"{job}_out-vel-{last}.vtk")
+    th = files.read_vtk_field(f"{job}_out-theta-{last}.vtk")
+    q = files.read_job(job)  # exactly what the reference read
+    return q, energy, verify, last, u, v, th
+
+
+def save_case(name: str, q: pp.Problem, energy, verify, last, u, v, th, g32: bool):
+    arrays = dict(nodes=q.nodes, data2=q.data2, steps=np.int64(q.steps), bctypes=q.bctypes.astype(np.int32),
+                  ic=q.ic, icd=q.icd, abc=q.abc.astype(np.int32), clouds=q.clouds.astype(np.int32),
+                  energy=energy, verify=verify, vtk_step=np.int64(last), vtk_points=u["points"],
+                  vtk_u=u["vectors"], vtk_v=v["vectors"], vtk_theta=th["scalars"])
+    if q.C > 0:
+        if g32:  # the float32 values, exact in double
+            assert np.array_equal(q.Gu.astype(np.float32).astype(np.float64), q.Gu)
+            arrays.update(Gu32=q.Gu.astype(np.float32), Gv32=q.Gv.astype(np.float32),
+                          Gth32=q.Gth.astype(np.float32))
+        else:
+            arrays.update(Gu=q.Gu, Gv=q.Gv, Gth=q.Gth)
+    np.savez_compressed(os.path.join(OUT, name + ".npz"), **arrays)
+    print(name, q.N, q.C, q.steps, os.path.getsize(os.path.join(OUT, name + ".npz")))
+
+
+def abc_case(name, R, nu, steps, pulse):
+    p = pp.make_problem(R, nu, steps=steps, pulses=[pulse], workers=8)
+    p.Gu = p.Gu.astype(np.float32).astype(np.float64)
+    p.Gv = p.Gv.astype(np.float32).astype(np.float64)
+    p.Gth = p.Gth.astype(np.float32).astype(np.float64)
+    tmp = tempfile.mkdtemp()
+    try:
+        q, *rest = run_reference(p, tmp)
+        save_case(name, q, *rest, g32=True)
+    finally:
+        shutil.rmtree(tmp)
+
+
+def noabc_case():
+    tmp = tempfile.mkdtemp()
+    try:
+        subprocess.run([REF_GEN, "gen"], cwd=tmp, check=True, capture_output=True)
+        q = files.read_job(os.path.join(tmp, "gen"))
+        r = subprocess.run([REF_BIN, "_out", "gen"], cwd=tmp, capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr
+        job = os.path.join(tmp, "gen")
+        energy = files.read_series(job + "_out.energy")
+        verify = files.read_series(job + "_out.verify")
+        last = (q.steps // 30) * 30
+        u = files.read_vtk_field(f"{job}_out-{last}.vtk")
+        v = files.read_vtk_field(f"{job}_out-vel-{last}.vtk")
+        th = files.read_vtk_field(f"{job}_out-theta-{last}.vtk")
+        save_case("noabc_r30", q, energy, verify, last, u, v, th, g32=False)
+    finally:
+        shutil.rmtree(tmp)
+
+
+def kdtree_sets():
+    L = C.CDLL(REF_KD)
+    L.ref_build_clouds.restype = C.c_int
+    L.ref_build_clouds.argtypes = [C.c_int, C.c_void_p, C.c_double, C.c_int, C.c_void_p, C.c_void_p]
+    rng = np.random.default_rng(20261203)
+    sets = {
+        "lattice": (pp.lattice_domain(6.0), 2.0 + 0.25 + 1e-5),
+        "jitter": (pp.lattice_domain(5.0) + rng.uniform(-0.2, 0.2, size=pp.lattice_domain(5.0).shape),
+                   2.25001),
+        "ties": (pp.lattice_domain(6.0, 0.75, 0.75), 1.5),
+    }
+    out = {}
+    for name, (nodes, r) in sets.items():
+        nodes = np.ascontiguousarray(nodes, dtype=np.float64)
+        n = nodes.shape[0]
+        cl = np.zeros(n * 128, dtype=np.int32)
+        ln = np.zeros(n, dtype=np.int32)
+        mx = L.ref_build_clouds(n, nodes.ctypes.data, r, 128, cl.ctypes.data, ln.ctypes.data)
+        assert mx > 0
+        cl = cl.reshape(n, 128)
+        rp = np.concatenate([[0], np.cumsum(ln)]).astype(np.int64)
+        cols = np.concatenate([cl[i, :ln[i]] for i in range(n)]).astype(np.int32)
+        out[name + "_nodes"] = nodes
+        out[name + "_r"] = np.float64(r)
+        out[name + "_row_ptr"] = rp
+        out[name + "_cols"] = cols  # the reference's order (distance, kd-tree tie order)
+    np.savez_compressed(os.path.join(OUT, "kdtree_sets.npz"), **out)
+    print("kdtree_sets", {k: v.shape for k, v in out.items() if k.endswith("_cols")})
+
+
+if __name__ == "__main__":
+    for f in (REF_BIN, REF_GEN, REF_KD):
+        if not os.path.exists(f):
+            sys.exit(f"{f} missing: run `make -C oracle` where the reference tree is present")
+    kdtree_sets()
+    noabc_case()
+    abc_case("abc_r10_nu025", 10.0, 0.25, 600, pp.Pulse(3.0, -2.0, 4.0, 1.0, (1.0, 1.0)))
+    abc_case("abc_r12_nu033", 12.0, 0.33, 300, pp.Pulse(3.6, -2.4, 4.8, 1.0, (1.0, 1.0)))

@@ -1,0 +1,106 @@
+"""The HIP path against the oracle and the reference's golden outputs, through the C ABI."""
+import sys
+
+import numpy as np
+import pytest
+from conftest import (ENERGY_KEYS, VERIFY_KEYS, assert_series_close, load_golden, records_table,
+                      rel_l2, vtk_index)
+
+import pyoracle
+from lps_amd import preprocess as pp, solver
+
+pytestmark = pytest.mark.gpu
+
+CASES = ["noabc_r30", "abc_r10_nu025", "abc_r12_nu033"]
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("name", ["lattice", "jitter", "ties"])
+def test_gpu_horizons_equal_oracle_and_reference_sets(name):
+    say("horizons", name)
+    z = np.load(__import__("conftest").GOLDEN + "/kdtree_sets.npz")
+    nodes, r = z[name + "_nodes"], float(z[name + "_r"])
+    n = nodes.shape[0]
+    prob = pp.Problem(nodes=nodes, steps=0, data2=np.array([0.19, 0.25, 8e-15, 0, 0, 0, 1e-8, 2.0,
+                                                            0.5, 0.5, 0, 0, 0, 0, 0]),
+                      bctypes=-np.ones(2 * n, dtype=np.int64), bcvalues=np.zeros(2 * n),
+                      ic=np.zeros(2 * n), icd=np.zeros(2 * n), abc=np.zeros(0, np.int64),
+                      clouds=np.zeros((0, 0), np.int64), Gu=np.zeros((0, 0)), Gv=np.zeros((0, 0)),
+                      Gth=np.zeros((0, 0)))
+    with solver.Solver(prob, exact=True, horizon_radius=r) as s:
+        rp, cols = s.horizons()
+    orp, ocols = pyoracle.build_horizons(nodes, r, 128)
+    assert np.array_equal(rp, orp) and np.array_equal(cols, ocols)  # same sets, same order
+    rrp, rcols = z[name + "_row_ptr"], z[name + "_cols"]
+    assert np.array_equal(rp, rrp)
+    for i in range(n):
+        assert np.array_equal(np.sort(cols[rp[i]:rp[i + 1]]), np.sort(rcols[rrp[i]:rrp[i + 1]]))
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_exact_mode_matches_oracle_bitwise(name):
+    """Exact mode keeps the reference's operation order and neighbour order: the fields after the
+    run equal the oracle's to the last bit."""
+    say("exact vs oracle", name)
+    prob, ref = load_golden(name)
+    o = pyoracle.Oracle(prob, theta_method=1)
+    orecs = o.run(prob.steps)
+    of = o.fetch()
+    with solver.Solver(prob, exact=True) as s:
+        grecs = s.run(prob.steps)
+        gf = s.fetch()
+    for k in ("u", "v", "a", "theta", "m"):
+        assert np.array_equal(gf[k], of[k]), (k, np.max(np.abs(gf[k] - of[k])))
+    # output block: same per-node terms, reduction order differs (1e-14)
+    assert_series_close(records_table(grecs, ENERGY_KEYS), records_table(orecs, ENERGY_KEYS),
+                        ENERGY_KEYS, 1e-14, prob, name)
+
+
+@pytest.mark.parametrize("name", CASES)
+@pytest.mark.parametrize("graph", [True, False])
+def test_fast_mode_matches_reference_golden(name, graph):
+    """Fast path (tabulated geometry, FMA) against the reference's outputs: 1e-10 relative L2."""
+    say("fast vs reference golden", name, graph)
+    prob, ref = load_golden(name)
+    with solver.Solver(prob, exact=False, use_graph=graph) as s:
+        recs = s.run(prob.steps)
+        f = s.fetch()
+    assert_series_close(records_table(recs, ENERGY_KEYS), ref["energy"], ENERGY_KEYS, 1e-10, prob, name)
+    assert_series_close(records_table(recs, VERIFY_KEYS), ref["verify"], VERIFY_KEYS, 1e-10, prob, name)
+    idx = vtk_index(prob.nodes, ref["vtk_points"])
+    assert rel_l2(f["u"].reshape(-1, 2)[idx], ref["vtk_u"]) <= 1e-10
+    assert rel_l2(f["v"].reshape(-1, 2)[idx], ref["vtk_v"]) <= 1e-10
+    assert rel_l2(f["theta"][idx], ref["vtk_theta"]) <= 1e-10
+
+
+@pytest.mark.parametrize("exact", [True, False])
+def test_idw_variant_matches_oracle(exact):
+    say("idw", exact)
+    prob, _ = load_golden("abc_r12_nu033")
+    o = pyoracle.Oracle(prob, theta_method=0)
+    o.run(150)
+    of = o.fetch()
+    with solver.Solver(prob, theta_method=solver.LPS_THETA_IDW, exact=exact) as s:
+        s.run(150)
+        gf = s.fetch()
+    for k in ("u", "v", "theta"):
+        if exact:
+            assert np.array_equal(gf[k], of[k]), k
+        else:
+            assert rel_l2(gf[k], of[k]) <= 1e-10, k
+
+
+def test_nonfinite_state_is_reported_at_first_output_step():
+    say("nonfinite")
+    prob, _ = load_golden("abc_r10_nu025")
+    prob.ic = prob.ic.copy()
+    prob.ic[10] = np.nan
+    with solver.Solver(prob) as s:
+        with pytest.raises(solver.LpsError) as e:
+            s.run(90)
+        assert e.value.code == solver.LPS_ERR_NONFINITE
+        assert s.step_index == 30
