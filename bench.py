@@ -38,10 +38,25 @@ def main():
     ap.add_argument("--warmup", type=int, default=60)
     ap.add_argument("--workload", default="cfg2")
     ap.add_argument("--exact", action="store_true")
+    ap.add_argument("--variant", type=int, default=0,
+                    help="fast-path node kernels: 0 autotune, 1 generic tile, 2 unrolled stencil")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=30)
     ap.add_argument("--workers", type=int, default=None)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="run only a plain copy kernel over BYTES (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        n = args.copy_probe // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            torch.add(src, 1.0, out=dst)  # reads and writes args.copy_probe bytes each
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": n * 8, "copy_probe_bytes_written": n * 8,
+                          "launches": 5}))
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): the path is measured on a "
               f"single MI355X; multi-GPU sharding is a design note, not a measured configuration.",
@@ -58,10 +73,12 @@ def main():
     log(f"[bench] generated N={prob.N} C={prob.C} dt={prob.dt:.3e} in {time.time() - t:.1f}s")
 
     t = time.time()
-    s = solver.Solver(prob, exact=args.exact, use_graph=True)
+    s = solver.Solver(prob, exact=args.exact, use_graph=True, variant=args.variant)
     rp, _ = s.horizons()
+    path = s.path
     nnz = int(rp[-1]) - prob.N  # bonds, self excluded
-    log(f"[bench] GPU setup (horizons, m_i, theta history) {time.time() - t:.1f}s, bonds={nnz}")
+    log(f"[bench] GPU setup (horizons, m_i, theta history) {time.time() - t:.1f}s, bonds={nnz}, "
+        f"path: {s.path}")
     if args.warmup > 0:
         s.run(args.warmup)
         s.reset()
@@ -108,7 +125,7 @@ def main():
         "warmup": args.warmup, "ms_per_step": elapsed / steps * 1e3, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"{w.name}: {w.description}", "nodes": prob.N, "abc_nodes": prob.C,
-                   "bonds": nnz, "dt": prob.dt, "exact": bool(args.exact),
+                   "bonds": nnz, "dt": prob.dt, "exact": bool(args.exact), "kernels": path,
                    "reflected_energy_final": refl,
                    "gate_reflected_le_1e-4": (refl is not None and refl <= 1e-4)},
         "roofline": roofline,

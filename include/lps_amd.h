@@ -60,6 +60,9 @@ typedef struct lps_options {
   int32_t use_graph;     /* 1: replay the 30-step inner loop as a captured HIP graph */
   int32_t device;        /* HIP device ordinal */
   double horizon_radius; /* > 0: horizon search radius; 0: delta + dx/2 + 1e-5 (src/main.cpp:352) */
+  int32_t variant;       /* fast-path node kernels: 0 autotune at creation (time each candidate
+                            on the problem, keep the fastest), 1 generic tile kernels, 2 unrolled
+                            68-bond stencil (when the stencil applies) */
 } lps_options;
 
 /* One output record: the quantities src/main.cpp:986-1062 computes every 30 steps. */
@@ -113,11 +116,18 @@ int64_t lps_horizons(lps_ctx *ctx, int64_t *row_ptr, int32_t *cols);
 int64_t lps_step_index(const lps_ctx *ctx);
 double lps_m_ref(const lps_ctx *ctx);
 
-/* Average device time (ms) per step of the kernels launched by lps_step since the last reset,
- * measured with HIP events on the context's stream, split by pass: out[0] dilatation pass,
- * out[1] force pass, out[2] whole step. Timing is enabled with lps_set_timing(ctx, 1). */
+/* Average device time (ms) per step of the kernels launched by lps_step since timing was switched
+ * on, measured with HIP events on the context's stream: out[0] dilatation pass (with the G_theta /
+ * IDW ABC rows), out[1] force pass of the nodes (on the exact path with the ABC rows), out[2]
+ * on the fast path the time the ABC rows' force/G_u/G_v kernel, which runs beside the node force
+ * kernel on a second stream, outlasts it (0 on the exact path), out[3] whole step.
+ * Timing is enabled with lps_set_timing(ctx, 1); timed steps run without the HIP graph. */
 int32_t lps_set_timing(lps_ctx *ctx, int32_t on);
-int32_t lps_pass_times(lps_ctx *ctx, double *out3);
+int32_t lps_pass_times(lps_ctx *ctx, double *out4);
+
+/* Which kernels the context runs: the tile path (lattice inputs whose stencil reproduces every
+ * GPU-built horizon) or the exact CSR path, with the layout. */
+const char *lps_describe(const lps_ctx *ctx);
 
 const char *lps_last_error(const lps_ctx *ctx);
 const char *lps_global_error(void); /* message of a failed lps_create */

@@ -1,12 +1,12 @@
 // C ABI of the LPS-ABC time loop: context setup, step loop, output block, fetch.
 //
-// Setup follows src/main.cpp:178-684, the time loop :718-1164 (see lps_kernels.hip for the
-// passes). Buffer rotation (src/main.cpp:1145-1163) is index arithmetic on fixed buffers:
-// at step s (s completed steps) U^n lives in U[s % 3], U^{n-1} in U[(s + 2) % 3] and U^{n+1} is
-// written to U[(s + 1) % 3]; V likewise; A and the prediction alternate between two buffers; the
-// dilatation history uses three (theta of step s is written to TH[(s + 1) % 3], ThetaN is
-// TH[s % 3], ThetaNm1 is TH[(s + 2) % 3]). The pattern repeats every 6 steps, so a captured
-// 30-step graph replays from any step that is a multiple of 30.
+// Setup follows src/main.cpp:178-684, the time loop :718-1164 (lps_kernels.hip has the passes).
+// Buffer rotation (src/main.cpp:1145-1163) is index arithmetic on fixed buffers: at step s
+// (s completed steps) U^n lives in U[s % 3], U^{n-1} in U[(s + 2) % 3] and U^{n+1} is written to
+// U[(s + 1) % 3]; V likewise; A and the prediction alternate between two buffers; the dilatation
+// history uses three (theta of step s is written to TH[(s + 1) % 3], ThetaN is TH[s % 3],
+// ThetaNm1 is TH[(s + 2) % 3]). The pattern repeats every 6 steps, so a captured 30-step graph
+// replays from any step that is a multiple of 30.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -31,35 +31,36 @@ struct Dev2 {
 struct lps_ctx {
   lps_options opt{};
   hipStream_t stream = nullptr;
-  std::string err;
-  int32_t n = 0, n_int = 0, n_abc = 0, M = 0;
+  hipStream_t stream2 = nullptr;  // the ABC rows' force kernel beside the node force kernel
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  std::string err, path;
+  int32_t n = 0, n_slots = 0, n_abc = 0, M = 0;
   double E = 0, nu = 0, rho = 0, dt = 0, delta = 0, dx = 0, dy = 0;
   double lam = 0, mu = 0, kappa = 0, m_ref = 0, exclude_r = -1, E0 = -1;
   bool fast = false;
+  bool stencil68_ok = false;  // the unrolled 68-bond kernels apply to this problem
   int64_t step = 0;
-  std::vector<int32_t> perm_h;  // new -> original
+  std::vector<int32_t> perm_h;  // slot -> original index, -1 at holes
   HorizonBuild hb;
   std::vector<void *> allocs;
-  // device arrays
-  double *x = nullptr, *y = nullptr, *m = nullptr, *bcoef = nullptr, *c2m = nullptr;
-  int32_t *lat = nullptr;
-  Geo *geo = nullptr;
-  int32_t geo_k = 0;
-  int8_t *is_bc = nullptr;
+  // device arrays (per slot unless noted)
+  double *x = nullptr, *y = nullptr, *m = nullptr, *c2m = nullptr;
+  int8_t *kind = nullptr, *is_bc = nullptr;
+  int32_t *perm_d = nullptr, *abc_slot = nullptr;
   int32_t *cloud = nullptr;
   double *gu = nullptr, *gv = nullptr, *gth = nullptr;
+  TileArgs tile{};
   Dev2 U[3], V[3], A[2], P[2], IC, ICD;
   double *TH[3] = {nullptr, nullptr, nullptr};
-  double *acoef = nullptr;
   DiagPartial *partials = nullptr, *dout = nullptr, *hout = nullptr;
   int32_t n_partials = 0;
   // graph of 30 steps from a step that is a multiple of 30
   hipGraphExec_t graph30 = nullptr;
   // timing
   bool timing = false;
-  std::vector<hipEvent_t> ev;   // 3 per step
+  std::vector<hipEvent_t> ev;  // 4 per timed step
   int32_t ev_used = 0;
-  double t_theta = 0, t_force = 0, t_step = 0;
+  double t_pass[4] = {0, 0, 0, 0};  // dilatation, force (nodes), force (ABC rows), whole step
   int64_t t_count = 0;
 
   template <class T>
@@ -67,6 +68,13 @@ struct lps_ctx {
     hipError_t e = hipMalloc((void **)p, std::max<size_t>(count, 1) * sizeof(T));
     if (e == hipSuccess) allocs.push_back(*p);
     return e;
+  }
+  template <class T>
+  hipError_t upload(T **p, const std::vector<T> &h) {
+    hipError_t e = alloc(p, h.size());
+    if (e != hipSuccess) return e;
+    if (h.empty()) return hipSuccess;
+    return hipMemcpyAsync(*p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, stream);
   }
   int32_t fail(int32_t code, const std::string &msg) {
     err = msg;
@@ -88,16 +96,16 @@ struct lps_ctx {
 
 StepArgs lps_ctx::args() const {
   StepArgs a{};
-  a.n = n; a.n_int = n_int; a.n_abc = n_abc; a.M = M;
+  a.n = n_slots; a.n_abc = n_abc; a.M = M;
   a.dt = dt; a.rho = rho; a.lam = lam; a.mu = mu; a.kappa = kappa;
   a.delta = delta; a.dx = dx; a.dy = dy;
-  a.x = x; a.y = y; a.lat = lat; a.geo = geo; a.geo_k = geo_k;
+  a.x = x; a.y = y; a.kind = kind; a.abc_slot = abc_slot;
   a.row_ptr = hb.row_ptr; a.col = hb.col;
   a.sell_off = hb.sell_off; a.sell_w = hb.sell_w; a.sell_col = hb.sell_col;
-  a.m = m; a.bcoef = bcoef; a.c2m = c2m; a.is_bc = is_bc;
+  a.m = m; a.c2m = c2m; a.is_bc = is_bc;
   a.cloud = cloud; a.gu = gu; a.gv = gv; a.gth = gth;
   a.theta_method = opt.theta_method;
-  a.abc_on = opt.abc_enabled;
+  a.tile = tile;
   return a;
 }
 
@@ -112,7 +120,6 @@ StepState lps_ctx::state(int64_t s) const {
   st.vx = Vc.x; st.vy = Vc.y; st.vxm = Vm.x; st.vym = Vm.y;
   st.ax = Ac.x; st.ay = Ac.y;
   st.theta = TH[(s + 1) % 3];
-  st.acoef = acoef;
   st.thn = TH[s % 3];
   st.thnm1 = TH[(s + 2) % 3];
   st.ux1 = Un.x; st.uy1 = Un.y; st.vx1 = Vn.x; st.vy1 = Vn.y; st.ax1 = An.x; st.ay1 = An.y;
@@ -120,16 +127,40 @@ StepState lps_ctx::state(int64_t s) const {
   return st;
 }
 
-static double beta_host(double xi, double R, double dx) {
+static double beta_host(double xi, double R, double dx) {  // src/main.cpp:100-107
   if (xi <= R - dx / 2.0) return 1.0;
   else if (xi <= R + dx / 2.0) return (R + dx / 2.0 - xi) / dx;
   else return 0.0;
+}
+
+// Lattice coordinates of every node, when all nodes sit exactly on x0 + ix dx, y0 + iy dy.
+static bool lattice_coords(const lps_problem *p, std::vector<int32_t> &ix, std::vector<int32_t> &iy) {
+  const int32_t N = p->node_no;
+  double xmin = p->nodes[0], ymin = p->nodes[1];
+  for (int32_t i = 1; i < N; i++) {
+    xmin = std::min(xmin, p->nodes[2 * (int64_t)i]);
+    ymin = std::min(ymin, p->nodes[2 * (int64_t)i + 1]);
+  }
+  ix.resize(N);
+  iy.resize(N);
+  for (int32_t i = 0; i < N; i++) {
+    const double x = p->nodes[2 * (int64_t)i], y = p->nodes[2 * (int64_t)i + 1];
+    const long long a = std::llround((x - xmin) / p->dx), b = std::llround((y - ymin) / p->dy);
+    if (a < 0 || b < 0 || a > 32767 * kTile || b > 32767 * kTile) return false;
+    if (std::abs(xmin + (double)a * p->dx - x) > 1e-9 * p->dx ||
+        std::abs(ymin + (double)b * p->dy - y) > 1e-9 * p->dy)
+      return false;
+    ix[i] = (int32_t)a;
+    iy[i] = (int32_t)b;
+  }
+  return true;
 }
 
 extern "C" {
 
 static int32_t reset_state(lps_ctx *c);
 static int32_t flush_events(lps_ctx *c);
+static int32_t choose_variant(lps_ctx *c);
 
 void lps_default_options(lps_options *opt) {
   opt->theta_method = LPS_THETA_GTHETA;
@@ -138,10 +169,12 @@ void lps_default_options(lps_options *opt) {
   opt->use_graph = 1;
   opt->device = 0;
   opt->horizon_radius = 0.0;
+  opt->variant = 0;
 }
 
 const char *lps_global_error(void) { return g_create_error.c_str(); }
 const char *lps_last_error(const lps_ctx *ctx) { return ctx ? ctx->err.c_str() : g_create_error.c_str(); }
+const char *lps_describe(const lps_ctx *ctx) { return ctx->path.c_str(); }
 int64_t lps_step_index(const lps_ctx *ctx) { return ctx->step; }
 double lps_m_ref(const lps_ctx *ctx) { return ctx->m_ref; }
 
@@ -153,6 +186,10 @@ void lps_destroy(lps_ctx *ctx) {
   for (void *p : ctx->allocs) (void)hipFree(p);
   free_horizons(&ctx->hb);
   if (ctx->hout) (void)hipHostFree(ctx->hout);
+  if (ctx->stream2) (void)hipStreamSynchronize(ctx->stream2);
+  if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+  if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
+  if (ctx->stream2) (void)hipStreamDestroy(ctx->stream2);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -169,6 +206,7 @@ static int32_t create_impl(lps_ctx *c, const lps_problem *p) {
     return c->fail(LPS_ERR_ARG, "the G_theta method needs ci_nodes_theta");
   if (c->opt.theta_method != LPS_THETA_GTHETA && c->opt.theta_method != LPS_THETA_IDW)
     return c->fail(LPS_ERR_ARG, "theta_method must be LPS_THETA_GTHETA or LPS_THETA_IDW");
+  if (!(p->dx > 0.0) || !(p->dy > 0.0)) return c->fail(LPS_ERR_ARG, "dx and dy must be positive");
   c->n = N;
   c->M = abc_on ? M : 0;
   c->E = p->E; c->nu = p->nu; c->rho = p->rho; c->dt = p->dt;
@@ -205,72 +243,106 @@ static int32_t create_impl(lps_ctx *c, const lps_problem *p) {
     }
   }
   c->n_abc = (int32_t)active_rows.size();
-  c->n_int = N - c->n_abc;
   const hipStream_t s = c->stream;
+  const double search_r = c->opt.horizon_radius > 0.0 ? c->opt.horizon_radius
+                                                      : c->delta + c->dx / 2.0 + 1e-5;  // src/main.cpp:352
 
-  // coordinates (original order) -> device, horizon build
-  double *xo, *yo;
-  int8_t *dseg;
-  int32_t *drank;
-  CTX_TRY(c, c->alloc(&xo, N));
-  CTX_TRY(c, c->alloc(&yo, N));
-  CTX_TRY(c, c->alloc(&dseg, N));
-  CTX_TRY(c, c->alloc(&drank, N));
-  {
+  // ---- slot layout ----
+  std::vector<int32_t> lix, liy;
+  const int32_t reach = (int32_t)std::floor(search_r / std::min(c->dx, c->dy));
+  bool tile_layout = !c->opt.exact && reach <= kMaxHalo && lattice_coords(p, lix, liy);
+  std::vector<int32_t> slot_of(N);
+  std::vector<int32_t> tile_xy, tile_map;
+  if (tile_layout) {
+    int32_t nx = 0, ny = 0;
+    for (int32_t i = 0; i < N; i++) { nx = std::max(nx, lix[i] + 1); ny = std::max(ny, liy[i] + 1); }
+    const int32_t ntx = (nx + kTile - 1) / kTile, nty = (ny + kTile - 1) / kTile;
+    tile_map.assign((size_t)ntx * nty, -1);
+    for (int32_t i = 0; i < N; i++) tile_map[(size_t)(lix[i] / kTile) * nty + liy[i] / kTile] = 0;
+    for (int32_t tx = 0; tx < ntx; tx++)
+      for (int32_t ty = 0; ty < nty; ty++)
+        if (tile_map[(size_t)tx * nty + ty] == 0) {
+          tile_map[(size_t)tx * nty + ty] = (int32_t)tile_xy.size();
+          tile_xy.push_back((tx << 16) | ty);
+        }
+    for (int32_t i = 0; i < N; i++) {
+      const int32_t t = tile_map[(size_t)(lix[i] / kTile) * nty + liy[i] / kTile];
+      slot_of[i] = t * kTile * kTile + (lix[i] % kTile) * kTile + (liy[i] % kTile);
+    }
+    c->n_slots = (int32_t)tile_xy.size() * kTile * kTile;
+    c->tile.n_tiles = (int32_t)tile_xy.size();
+    c->tile.ntx = ntx;
+    c->tile.nty = nty;
+    c->tile.halo = reach;
+    c->tile.win = kTile + 2 * reach;
+  } else {
     std::vector<double> hx(N), hy(N);
     for (int32_t i = 0; i < N; i++) {
       hx[i] = p->nodes[2 * (int64_t)i];
       hy[i] = p->nodes[2 * (int64_t)i + 1];
     }
-    CTX_TRY(c, hipMemcpyAsync(xo, hx.data(), sizeof(double) * N, hipMemcpyHostToDevice, s));
-    CTX_TRY(c, hipMemcpyAsync(yo, hy.data(), sizeof(double) * N, hipMemcpyHostToDevice, s));
-    CTX_TRY(c, hipMemcpyAsync(dseg, seg.data(), N, hipMemcpyHostToDevice, s));
-    CTX_TRY(c, hipMemcpyAsync(drank, seg_rank.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice, s));
+    double *xo, *yo;
+    int8_t *dseg;
+    int32_t *drank, *dperm;
+    CTX_TRY(c, c->upload(&xo, hx));
+    CTX_TRY(c, c->upload(&yo, hy));
+    CTX_TRY(c, c->upload(&dseg, seg));
+    CTX_TRY(c, c->upload(&drank, seg_rank));
+    CTX_TRY(c, c->alloc(&dperm, N));
+    CTX_TRY(c, morton_order(N, xo, yo, dseg, drank, search_r, dperm, s));
+    std::vector<int32_t> perm(N);
+    CTX_TRY(c, hipMemcpyAsync(perm.data(), dperm, sizeof(int32_t) * N, hipMemcpyDeviceToHost, s));
     CTX_TRY(c, hipStreamSynchronize(s));
+    for (int32_t k = 0; k < N; k++) slot_of[perm[k]] = k;
+    c->n_slots = N;
   }
-  const double search_r = c->opt.horizon_radius > 0.0 ? c->opt.horizon_radius
-                                                      : c->delta + c->dx / 2.0 + 1e-5;  // src/main.cpp:352
-  bool overflow = false;
-  CTX_TRY(c, build_horizons(N, xo, yo, dseg, drank, search_r, kMaxHorizon, &c->hb, &overflow, s));
-  if (overflow) return c->fail(LPS_ERR_HORIZON, "found_node_no > max_node_no; increase max_node_no.");
-  c->perm_h.resize(N);
-  CTX_TRY(c, hipMemcpyAsync(c->perm_h.data(), c->hb.perm, sizeof(int32_t) * N, hipMemcpyDeviceToHost, s));
+  const int32_t NS = c->n_slots;
+  c->perm_h.assign(NS, -1);
+  std::vector<int8_t> kind(NS, 0), isbc(NS, 0);
+  std::vector<double> xs(NS, 0.0), ys(NS, 0.0);
+  for (int32_t i = 0; i < N; i++) {
+    const int32_t k = slot_of[i];
+    c->perm_h[k] = i;
+    kind[k] = seg[i] ? 2 : 1;
+    isbc[k] = p->bctypes[2 * (int64_t)i] == 1;
+    xs[k] = p->nodes[2 * (int64_t)i];
+    ys[k] = p->nodes[2 * (int64_t)i + 1];
+  }
+  std::vector<int32_t> abc_slot(c->n_abc);
+  for (int32_t r = 0; r < c->n_abc; r++) abc_slot[r] = slot_of[p->inf_nodes[active_rows[r]]];
+  CTX_TRY(c, c->upload(&c->x, xs));
+  CTX_TRY(c, c->upload(&c->y, ys));
+  CTX_TRY(c, c->upload(&c->kind, kind));
+  CTX_TRY(c, c->upload(&c->is_bc, isbc));
+  CTX_TRY(c, c->upload(&c->perm_d, c->perm_h));
+  CTX_TRY(c, c->upload(&c->abc_slot, abc_slot));
   CTX_TRY(c, hipStreamSynchronize(s));
-  std::vector<int32_t> iperm(N);
-  for (int32_t k = 0; k < N; k++) iperm[c->perm_h[k]] = k;
 
-  // permuted coordinates, bc flags
-  std::vector<double> hx(N), hy(N);
-  std::vector<int8_t> hbc(N);
-  for (int32_t k = 0; k < N; k++) {
-    const int32_t o = c->perm_h[k];
-    hx[k] = p->nodes[2 * (int64_t)o];
-    hy[k] = p->nodes[2 * (int64_t)o + 1];
-    hbc[k] = p->bctypes[2 * (int64_t)o] == 1;
-  }
-  CTX_TRY(c, c->alloc(&c->x, N));
-  CTX_TRY(c, c->alloc(&c->y, N));
-  CTX_TRY(c, c->alloc(&c->is_bc, N));
-  CTX_TRY(c, hipMemcpyAsync(c->x, hx.data(), sizeof(double) * N, hipMemcpyHostToDevice, s));
-  CTX_TRY(c, hipMemcpyAsync(c->y, hy.data(), sizeof(double) * N, hipMemcpyHostToDevice, s));
-  CTX_TRY(c, hipMemcpyAsync(c->is_bc, hbc.data(), N, hipMemcpyHostToDevice, s));
+  // ---- horizons ----
+  bool overflow = false;
+  CTX_TRY(c, build_horizons(NS, c->x, c->y, c->kind, c->perm_d, search_r, kMaxHorizon, &c->hb,
+                            &overflow, s));
+  if (overflow) return c->fail(LPS_ERR_HORIZON, "found_node_no > max_node_no; increase max_node_no.");
 
-  // m_i on the device, m_ref and the ABC override on the host (src/main.cpp:408-480)
-  CTX_TRY(c, c->alloc(&c->m, N));
+  // ---- m_i on the device, m_ref and the ABC override on the host (src/main.cpp:408-480) ----
+  CTX_TRY(c, c->alloc(&c->m, NS));
   {
     StepArgs a = c->args();
     CTX_TRY(c, launch_weighted_volume(a, c->m, s));
   }
-  std::vector<double> hm(N);
-  CTX_TRY(c, hipMemcpyAsync(hm.data(), c->m, sizeof(double) * N, hipMemcpyDeviceToHost, s));
+  std::vector<double> hm(NS);
+  CTX_TRY(c, hipMemcpyAsync(hm.data(), c->m, sizeof(double) * NS, hipMemcpyDeviceToHost, s));
   CTX_TRY(c, hipStreamSynchronize(s));
   double m_max = 0.0;
-  for (int32_t k = 0; k < N; k++)
-    if (hm[k] > 0.0) m_max = std::max(m_max, hm[k]);
+  for (int32_t k = 0; k < NS; k++)
+    if (kind[k] && hm[k] > 0.0) m_max = std::max(m_max, hm[k]);
   c->m_ref = m_max;
   if (abc_on)
-    for (int32_t a = 0; a < C; a++) hm[iperm[p->inf_nodes[a]]] = c->m_ref;
-  CTX_TRY(c, hipMemcpyAsync(c->m, hm.data(), sizeof(double) * N, hipMemcpyHostToDevice, s));
+    for (int32_t a = 0; a < C; a++) hm[slot_of[p->inf_nodes[a]]] = c->m_ref;
+  CTX_TRY(c, hipMemcpyAsync(c->m, hm.data(), sizeof(double) * NS, hipMemcpyHostToDevice, s));
+  std::vector<double> hc(NS);
+  for (int32_t k = 0; k < NS; k++) hc[k] = hm[k] > 0.0 ? 2.0 / hm[k] : 0.0;
+  CTX_TRY(c, c->upload(&c->c2m, hc));
   // LPS constants and the lambda_PD identity (src/main.cpp:485-504)
   {
     const double AlphaLPS = 8.0 * c->mu / c->m_ref;
@@ -290,59 +362,119 @@ static int32_t create_impl(lps_ctx *c, const lps_problem *p) {
     c->exclude_r = abc_on ? domainRadius - c->delta - c->dx / 100.0 : -1.0;
   }
 
-  // fast path: lattice coordinates and the offset table
-  c->fast = !c->opt.exact;
-  if (c->fast) {
-    double xmin = hx[0], ymin = hy[0];
-    for (int32_t k = 1; k < N; k++) { xmin = std::min(xmin, hx[k]); ymin = std::min(ymin, hy[k]); }
-    std::vector<int32_t> hl(N);
-    for (int32_t k = 0; k < N && c->fast; k++) {
-      const double fx = (hx[k] - xmin) / c->dx, fy = (hy[k] - ymin) / c->dy;
-      const long long ix = std::llround(fx), iy = std::llround(fy);
-      if (ix < 0 || iy < 0 || ix > 32767 || iy > 65535 ||
-          std::abs(xmin + (double)ix * c->dx - hx[k]) > 1e-9 * c->dx ||
-          std::abs(ymin + (double)iy * c->dy - hy[k]) > 1e-9 * c->dy)
-        c->fast = false;
-      else
-        hl[k] = (int32_t)((ix << 16) | iy);
-    }
-    const int32_t K = (int32_t)std::ceil(search_r / std::min(c->dx, c->dy));
-    if (K > kMaxGeoK) c->fast = false;
-    if (c->fast) {
-      c->geo_k = K;
-      const int32_t W = 2 * K + 1;
-      std::vector<Geo> g(W * W);
-      for (int32_t a = -K; a <= K; a++)
-        for (int32_t b = -K; b <= K; b++) {
-          Geo &e = g[(a + K) * W + (b + K)];
-          const double xi_x = a * c->dx, xi_y = b * c->dy;
-          const double xn = std::sqrt(xi_x * xi_x + xi_y * xi_y);
-          if (xn == 0.0) { e.hx = e.hy = e.g1 = e.g2 = 0.0; continue; }
-          const double bt = beta_host(xn, c->delta, c->dx);
-          e.hx = xi_x / xn;
-          e.hy = xi_y / xn;
-          e.g1 = xn * bt * c->dx * c->dy;
-          e.g2 = bt * c->dx * c->dy;
-        }
-      std::vector<double> hb(N), hc(N);
-      for (int32_t k = 0; k < N; k++) {
-        hb[k] = hm[k] > 0.0 ? 8.0 * c->mu / hm[k] : -1.0;
-        hc[k] = hm[k] > 0.0 ? 2.0 / hm[k] : 0.0;
+  // ---- tile path: the stencil of a full horizon and the check against the CSR ----
+  c->fast = false;
+  c->path = tile_layout ? "exact kernels on the tile layout" : "exact kernels, Morton layout";
+  if (tile_layout) {
+    struct Off { double d; int32_t ox, oy; };
+    std::vector<Off> offs;
+    const int32_t K = c->tile.halo;
+    for (int32_t ox = -K; ox <= K; ox++)
+      for (int32_t oy = -K; oy <= K; oy++) {
+        if (ox == 0 && oy == 0) continue;
+        const double xx = ox * c->dx, yy = oy * c->dy;
+        const double d = std::sqrt(xx * xx + yy * yy);
+        if (d <= search_r) offs.push_back({d, ox, oy});
       }
-      CTX_TRY(c, c->alloc(&c->lat, N));
-      CTX_TRY(c, c->alloc(&c->geo, g.size()));
-      CTX_TRY(c, c->alloc(&c->bcoef, N));
-      CTX_TRY(c, c->alloc(&c->c2m, N));
-      CTX_TRY(c, c->alloc(&c->acoef, N));
-      CTX_TRY(c, hipMemcpyAsync(c->lat, hl.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice, s));
-      CTX_TRY(c, hipMemcpyAsync(c->geo, g.data(), sizeof(Geo) * g.size(), hipMemcpyHostToDevice, s));
-      CTX_TRY(c, hipMemcpyAsync(c->bcoef, hb.data(), sizeof(double) * N, hipMemcpyHostToDevice, s));
-      CTX_TRY(c, hipMemcpyAsync(c->c2m, hc.data(), sizeof(double) * N, hipMemcpyHostToDevice, s));
-      CTX_TRY(c, hipStreamSynchronize(s));
+    std::sort(offs.begin(), offs.end(), [](const Off &a, const Off &b) {
+      if (a.d != b.d) return a.d < b.d;
+      if (a.ox != b.ox) return a.ox < b.ox;
+      return a.oy < b.oy;
+    });
+    const int32_t nt = (int32_t)offs.size();
+    std::vector<int32_t> dl(nt);
+    std::vector<double> thx(nt), thy(nt), tg1(nt), tg2(nt);
+    for (int32_t k = 0; k < nt; k++) {
+      const double xx = offs[k].ox * c->dx, yy = offs[k].oy * c->dy, d = offs[k].d;
+      const double bt = beta_host(d, c->delta, c->dx);
+      dl[k] = offs[k].ox * c->tile.win + offs[k].oy;
+      thx[k] = xx / d;
+      thy[k] = yy / d;
+      tg1[k] = d * bt * c->dx * c->dy;
+      tg2[k] = bt * c->dx * c->dy;
+    }
+    // 3 x 3 neighbour tiles and "full" windows (every cell a node with m > 0)
+    const int32_t ntl = c->tile.n_tiles;
+    std::vector<int32_t> nbr((size_t)ntl * 9, -1);
+    std::vector<int8_t> full(ntl, 0);
+    for (int32_t t = 0; t < ntl; t++) {
+      const int32_t tx = tile_xy[t] >> 16, ty = tile_xy[t] & 0xffff;
+      for (int32_t ax = -1; ax <= 1; ax++)
+        for (int32_t ay = -1; ay <= 1; ay++) {
+          const int32_t qx = tx + ax, qy = ty + ay;
+          if (qx < 0 || qy < 0 || qx >= c->tile.ntx || qy >= c->tile.nty) continue;
+          nbr[(size_t)t * 9 + (ax + 1) * 3 + (ay + 1)] = tile_map[(size_t)qx * c->tile.nty + qy];
+        }
+      bool f = true;
+      const int32_t W = c->tile.win;
+      for (int32_t cc = 0; cc < W * W && f; cc++) {
+        const int32_t rx = cc / W - K, ry = cc % W - K;
+        const int32_t ax = rx < 0 ? 0 : (rx < kTile ? 1 : 2), ay = ry < 0 ? 0 : (ry < kTile ? 1 : 2);
+        const int32_t tt = nbr[(size_t)t * 9 + ax * 3 + ay];
+        if (tt < 0) { f = false; break; }
+        const int32_t sl = tt * kTile * kTile + (rx - (ax - 1) * kTile) * kTile + (ry - (ay - 1) * kTile);
+        f = kind[sl] != 0 && hm[sl] > 0.0;
+      }
+      full[t] = f;
+    }
+    // the reference stencil (dx == dy, the 68 offsets 0 < ox^2 + oy^2 <= 20) runs unrolled
+    bool s68 = (K == 4) && (c->dx == c->dy) && (nt == 68);
+    for (int32_t k = 0; k < nt && s68; k++) {
+      const int32_t r2 = offs[k].ox * offs[k].ox + offs[k].oy * offs[k].oy;
+      s68 = r2 > 0 && r2 <= 20;
+    }
+    c->stencil68_ok = s68;
+    c->tile.stencil68 = (s68 && c->opt.variant != 1) ? 3 : 0;
+    if (s68) {
+      for (int32_t q = 0; q < 12; q++) {
+        const double xi = c->dx * std::sqrt((double)kStencilR2[q]);
+        const double bt = beta_host(xi, c->delta, c->dx);
+        const double cc = c->dx / xi;
+        c->tile.c1[q] = xi * bt * c->dx * c->dy * cc;
+        c->tile.c2[q] = bt * c->dx * c->dy * cc * cc;
+      }
+    }
+    int32_t *d_txy, *d_tmap, *d_delta, *d_nbr;
+    int8_t *d_full;
+    double *d_hx, *d_hy, *d_g1, *d_g2;
+    CTX_TRY(c, c->upload(&d_txy, tile_xy));
+    CTX_TRY(c, c->upload(&d_tmap, tile_map));
+    CTX_TRY(c, c->upload(&d_nbr, nbr));
+    CTX_TRY(c, c->upload(&d_full, full));
+    c->tile.tile_nbr = d_nbr;
+    c->tile.tile_full = d_full;
+    CTX_TRY(c, c->upload(&d_delta, dl));
+    CTX_TRY(c, c->upload(&d_hx, thx));
+    CTX_TRY(c, c->upload(&d_hy, thy));
+    CTX_TRY(c, c->upload(&d_g1, tg1));
+    CTX_TRY(c, c->upload(&d_g2, tg2));
+    c->tile.tile_xy = d_txy;
+    c->tile.tile_map = d_tmap;
+    c->tile.nt = nt;
+    c->tile.delta = d_delta;
+    c->tile.hx = d_hx;
+    c->tile.hy = d_hy;
+    c->tile.g1 = d_g1;
+    c->tile.g2 = d_g2;
+    int32_t *d_mis;
+    CTX_TRY(c, c->alloc(&d_mis, 1));
+    CTX_TRY(c, hipMemsetAsync(d_mis, 0, sizeof(int32_t), s));
+    CTX_TRY(c, launch_tile_check(c->args(), d_mis, s));
+    int32_t mis = 0;
+    CTX_TRY(c, hipMemcpyAsync(&mis, d_mis, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    CTX_TRY(c, hipStreamSynchronize(s));
+    if (mis == 0) {
+      c->fast = true;
+      int32_t nfull = 0;
+      for (int8_t f : full) nfull += f;
+      c->path = " (16x16 tiles, " + std::to_string(ntl) + " tiles, " + std::to_string(nfull) +
+                " full, " + std::to_string(nt) + " bonds, halo " + std::to_string(K) + ")";
+    } else {
+      c->path += " (" + std::to_string(mis) + " horizons differ from the stencil)";
     }
   }
 
-  // ABC rows on the device, active rows only, permuted cloud ids
+  // ---- ABC rows on the device, active rows only, cloud ids as slots ----
   if (c->n_abc > 0) {
     const int32_t R = c->n_abc;
     std::vector<int32_t> hcl((size_t)R * M);
@@ -352,53 +484,45 @@ static int32_t create_impl(lps_ctx *c, const lps_problem *p) {
       const int64_t a = active_rows[r];
       for (int32_t j = 0; j < M; j++) {
         const int32_t k = p->iclouds[a * M + j];
-        hcl[(size_t)r * M + j] = (k < 0) ? -1 : iperm[k];
+        hcl[(size_t)r * M + j] = (k < 0) ? -1 : slot_of[k];
       }
       std::memcpy(&hgu[(size_t)r * 8 * M], p->ci_nodes_u + a * 8 * M, sizeof(double) * 8 * M);
       std::memcpy(&hgv[(size_t)r * 8 * M], p->ci_nodes_v + a * 8 * M, sizeof(double) * 8 * M);
       if (!hgt.empty())
         std::memcpy(&hgt[(size_t)r * 2 * M], p->ci_nodes_theta + a * 2 * M, sizeof(double) * 2 * M);
     }
-    CTX_TRY(c, c->alloc(&c->cloud, hcl.size()));
-    CTX_TRY(c, c->alloc(&c->gu, hgu.size()));
-    CTX_TRY(c, c->alloc(&c->gv, hgv.size()));
-    CTX_TRY(c, hipMemcpyAsync(c->cloud, hcl.data(), sizeof(int32_t) * hcl.size(), hipMemcpyHostToDevice, s));
-    CTX_TRY(c, hipMemcpyAsync(c->gu, hgu.data(), sizeof(double) * hgu.size(), hipMemcpyHostToDevice, s));
-    CTX_TRY(c, hipMemcpyAsync(c->gv, hgv.data(), sizeof(double) * hgv.size(), hipMemcpyHostToDevice, s));
-    if (!hgt.empty()) {
-      CTX_TRY(c, c->alloc(&c->gth, hgt.size()));
-      CTX_TRY(c, hipMemcpyAsync(c->gth, hgt.data(), sizeof(double) * hgt.size(), hipMemcpyHostToDevice, s));
-    }
+    CTX_TRY(c, c->upload(&c->cloud, hcl));
+    CTX_TRY(c, c->upload(&c->gu, hgu));
+    CTX_TRY(c, c->upload(&c->gv, hgv));
+    if (!hgt.empty()) CTX_TRY(c, c->upload(&c->gth, hgt));
     CTX_TRY(c, hipStreamSynchronize(s));
   }
 
-  // state buffers; the initial conditions stay on the device for lps_reset
+  // ---- state buffers; the initial conditions stay on the device for lps_reset ----
   for (int b = 0; b < 3; b++) {
-    CTX_TRY(c, c->alloc(&c->U[b].x, N)); CTX_TRY(c, c->alloc(&c->U[b].y, N));
-    CTX_TRY(c, c->alloc(&c->V[b].x, N)); CTX_TRY(c, c->alloc(&c->V[b].y, N));
-    CTX_TRY(c, c->alloc(&c->TH[b], N));
+    CTX_TRY(c, c->alloc(&c->U[b].x, NS)); CTX_TRY(c, c->alloc(&c->U[b].y, NS));
+    CTX_TRY(c, c->alloc(&c->V[b].x, NS)); CTX_TRY(c, c->alloc(&c->V[b].y, NS));
+    CTX_TRY(c, c->alloc(&c->TH[b], NS));
   }
   for (int b = 0; b < 2; b++) {
-    CTX_TRY(c, c->alloc(&c->A[b].x, N)); CTX_TRY(c, c->alloc(&c->A[b].y, N));
-    CTX_TRY(c, c->alloc(&c->P[b].x, N)); CTX_TRY(c, c->alloc(&c->P[b].y, N));
+    CTX_TRY(c, c->alloc(&c->A[b].x, NS)); CTX_TRY(c, c->alloc(&c->A[b].y, NS));
+    CTX_TRY(c, c->alloc(&c->P[b].x, NS)); CTX_TRY(c, c->alloc(&c->P[b].y, NS));
   }
-  CTX_TRY(c, c->alloc(&c->IC.x, N)); CTX_TRY(c, c->alloc(&c->IC.y, N));
-  CTX_TRY(c, c->alloc(&c->ICD.x, N)); CTX_TRY(c, c->alloc(&c->ICD.y, N));
   {
-    std::vector<double> ux(N), uy(N), vx(N), vy(N);
-    for (int32_t k = 0; k < N; k++) {
-      const int64_t o = c->perm_h[k];
-      ux[k] = p->icvalues[2 * o]; uy[k] = p->icvalues[2 * o + 1];
-      vx[k] = p->icdvalues[2 * o]; vy[k] = p->icdvalues[2 * o + 1];
+    std::vector<double> ux(NS, 0.0), uy(NS, 0.0), vx(NS, 0.0), vy(NS, 0.0);
+    for (int32_t i = 0; i < N; i++) {
+      const int32_t k = slot_of[i];
+      ux[k] = p->icvalues[2 * (int64_t)i]; uy[k] = p->icvalues[2 * (int64_t)i + 1];
+      vx[k] = p->icdvalues[2 * (int64_t)i]; vy[k] = p->icdvalues[2 * (int64_t)i + 1];
     }
-    CTX_TRY(c, hipMemcpyAsync(c->IC.x, ux.data(), sizeof(double) * N, hipMemcpyHostToDevice, s));
-    CTX_TRY(c, hipMemcpyAsync(c->IC.y, uy.data(), sizeof(double) * N, hipMemcpyHostToDevice, s));
-    CTX_TRY(c, hipMemcpyAsync(c->ICD.x, vx.data(), sizeof(double) * N, hipMemcpyHostToDevice, s));
-    CTX_TRY(c, hipMemcpyAsync(c->ICD.y, vy.data(), sizeof(double) * N, hipMemcpyHostToDevice, s));
+    CTX_TRY(c, c->upload(&c->IC.x, ux));
+    CTX_TRY(c, c->upload(&c->IC.y, uy));
+    CTX_TRY(c, c->upload(&c->ICD.x, vx));
+    CTX_TRY(c, c->upload(&c->ICD.y, vy));
     CTX_TRY(c, hipStreamSynchronize(s));
   }
   // output block buffers
-  c->n_partials = diag_partials(N);
+  c->n_partials = diag_partials(NS);
   CTX_TRY(c, c->alloc(&c->partials, c->n_partials));
   CTX_TRY(c, c->alloc(&c->dout, 1));
   CTX_TRY(c, hipHostMalloc((void **)&c->hout, sizeof(DiagPartial)));
@@ -407,9 +531,9 @@ static int32_t create_impl(lps_ctx *c, const lps_problem *p) {
 
 // Initial state (src/main.cpp:624-634), theta history (:664-682) and the prediction of step 1.
 static int32_t reset_state(lps_ctx *c) {
-  const int32_t N = c->n;
+  const int32_t NS = c->n_slots;
   const hipStream_t s = c->stream;
-  const size_t bytes = sizeof(double) * N;
+  const size_t bytes = sizeof(double) * NS;
   for (int b = 0; b < 3; b++) {
     CTX_TRY(c, hipMemsetAsync(c->U[b].x, 0, bytes, s));
     CTX_TRY(c, hipMemsetAsync(c->U[b].y, 0, bytes, s));
@@ -466,13 +590,18 @@ int32_t lps_create(const lps_problem *problem, const lps_options *options, lps_c
     return LPS_ERR_ARG;
   }
   e = hipSetDevice(c->opt.device);
+  // both streams at the default priority
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming);
   if (e != hipSuccess) {
     g_create_error = std::string("HIP setup failed: ") + hipGetErrorString(e);
     delete c;
     return LPS_ERR_HIP;
   }
-  const int32_t rc = create_impl(c, problem);
+  int32_t rc = create_impl(c, problem);
+  if (rc == LPS_OK && c->fast) rc = choose_variant(c);
   if (rc != LPS_OK) {
     g_create_error = c->err;
     lps_destroy(c);
@@ -482,25 +611,67 @@ int32_t lps_create(const lps_problem *problem, const lps_options *options, lps_c
   return LPS_OK;
 }
 
+// Fast path: pick the node kernels of each pass. With variant 0 both candidates (generic tile
+// kernels, unrolled 68-bond kernels) run a few timed steps on the problem itself and each pass
+// keeps its faster kernel; the state is reset afterwards.
+static int32_t choose_variant(lps_ctx *c) {
+  int32_t mode;  // bit 0: unrolled dilatation kernel, bit 1: unrolled force kernel
+  const bool tune = c->stencil68_ok && c->opt.variant == 0;
+  if (!c->stencil68_ok || c->opt.variant == 1) mode = 0;
+  else if (c->opt.variant == 2) mode = 3;
+  else {
+    double t_gen[4] = {0, 0, 0, 0}, t_unr[4] = {0, 0, 0, 0};
+    for (int32_t m : {0, 3}) {
+      c->tile.stencil68 = m;
+      int32_t rc = lps_set_timing(c, 1);
+      if (rc == LPS_OK) rc = lps_step(c, 2);
+      if (rc == LPS_OK) rc = lps_set_timing(c, 1);
+      if (rc == LPS_OK) rc = lps_step(c, 6);
+      if (rc == LPS_OK) rc = lps_pass_times(c, m ? t_unr : t_gen);
+      if (rc == LPS_OK) rc = lps_set_timing(c, 0);
+      if (rc == LPS_OK) rc = reset_state(c);
+      if (rc != LPS_OK) return rc;
+    }
+    mode = (t_unr[0] < t_gen[0] ? 1 : 0) | (t_unr[1] < t_gen[1] ? 2 : 0);
+  }
+  c->tile.stencil68 = mode;
+  const char *names[4] = {"generic", "unrolled"};
+  c->path = std::string("tile kernels: dilatation ") + names[mode & 1] + ", force " +
+            names[(mode >> 1) & 1] + (tune ? " [autotuned]" : "") + c->path;
+  return LPS_OK;
+}
+
+constexpr int kEv = 4;  // events per timed step
+
 static hipError_t step_once(lps_ctx *c, bool record_events) {
   const StepArgs a = c->args();
   const StepState st = c->state(c->step);
-  const bool exact = !c->fast;
   hipEvent_t *ev = nullptr;
+  hipError_t e;
   if (record_events) {
-    ev = &c->ev[3 * (size_t)c->ev_used];
-    hipError_t e = hipEventRecord(ev[0], c->stream);
-    if (e != hipSuccess) return e;
+    ev = &c->ev[kEv * (size_t)c->ev_used];
+    if ((e = hipEventRecord(ev[0], c->stream)) != hipSuccess) return e;
   }
-  hipError_t e = launch_theta(a, st, exact, c->stream);
-  if (e == hipSuccess && a.abc_on && a.n_abc > 0 && a.theta_method == LPS_THETA_IDW)
-    e = launch_idw(a, st, exact, c->stream);
+  e = launch_theta(a, st, c->fast, c->stream);
+  if (e == hipSuccess && a.n_abc > 0 && a.theta_method == LPS_THETA_IDW)
+    e = launch_idw(a, st, c->stream);
   if (e != hipSuccess) return e;
   if (record_events && (e = hipEventRecord(ev[1], c->stream)) != hipSuccess) return e;
-  e = launch_force(a, st, exact, c->stream);
+  // fast path: the ABC rows' force/G_u/G_v kernel runs beside the node force kernel on the second
+  // stream (both only read the dilatation pass's output and write disjoint slots)
+  const bool fork = c->fast && a.n_abc > 0;
+  if (fork) {
+    if ((e = hipEventRecord(c->ev_fork, c->stream)) != hipSuccess) return e;
+    if ((e = hipStreamWaitEvent(c->stream2, c->ev_fork, 0)) != hipSuccess) return e;
+    if ((e = launch_force_abc(a, st, c->stream2)) != hipSuccess) return e;
+    if ((e = hipEventRecord(c->ev_join, c->stream2)) != hipSuccess) return e;
+  }
+  e = launch_force(a, st, c->fast, c->stream);
   if (e != hipSuccess) return e;
+  if (record_events && (e = hipEventRecord(ev[2], c->stream)) != hipSuccess) return e;
+  if (fork && (e = hipStreamWaitEvent(c->stream, c->ev_join, 0)) != hipSuccess) return e;
   if (record_events) {
-    if ((e = hipEventRecord(ev[2], c->stream)) != hipSuccess) return e;
+    if ((e = hipEventRecord(ev[3], c->stream)) != hipSuccess) return e;
     c->ev_used++;
   }
   c->step++;
@@ -509,16 +680,14 @@ static hipError_t step_once(lps_ctx *c, bool record_events) {
 
 static int32_t flush_events(lps_ctx *c) {
   if (c->ev_used == 0) return LPS_OK;
-  CTX_TRY(c, hipEventSynchronize(c->ev[3 * (size_t)(c->ev_used - 1) + 2]));
+  CTX_TRY(c, hipEventSynchronize(c->ev[kEv * (size_t)(c->ev_used - 1) + kEv - 1]));
   for (int32_t k = 0; k < c->ev_used; k++) {
-    float t01 = 0, t12 = 0, t02 = 0;
-    hipEvent_t *e = &c->ev[3 * (size_t)k];
-    CTX_TRY(c, hipEventElapsedTime(&t01, e[0], e[1]));
-    CTX_TRY(c, hipEventElapsedTime(&t12, e[1], e[2]));
-    CTX_TRY(c, hipEventElapsedTime(&t02, e[0], e[2]));
-    c->t_theta += t01;
-    c->t_force += t12;
-    c->t_step += t02;
+    float t[kEv - 1] = {0, 0, 0}, tall = 0;
+    hipEvent_t *e = &c->ev[kEv * (size_t)k];
+    for (int q = 0; q < kEv - 1; q++) CTX_TRY(c, hipEventElapsedTime(&t[q], e[q], e[q + 1]));
+    CTX_TRY(c, hipEventElapsedTime(&tall, e[0], e[kEv - 1]));
+    for (int q = 0; q < kEv - 1; q++) c->t_pass[q] += t[q];
+    c->t_pass[kEv - 1] += tall;
     c->t_count++;
   }
   c->ev_used = 0;
@@ -529,22 +698,20 @@ int32_t lps_set_timing(lps_ctx *c, int32_t on) {
   int32_t rc = flush_events(c);
   if (rc != LPS_OK) return rc;
   c->timing = on != 0;
-  c->t_theta = c->t_force = c->t_step = 0;
+  for (double &t : c->t_pass) t = 0.0;
   c->t_count = 0;
   if (c->timing && c->ev.empty()) {
-    c->ev.resize(3 * 30);
+    c->ev.resize(kEv * 30);
     for (auto &e : c->ev) CTX_TRY(c, hipEventCreate(&e));
   }
   return LPS_OK;
 }
 
-int32_t lps_pass_times(lps_ctx *c, double *out3) {
+int32_t lps_pass_times(lps_ctx *c, double *out4) {
   int32_t rc = flush_events(c);
   if (rc != LPS_OK) return rc;
   const double n = c->t_count > 0 ? (double)c->t_count : 1.0;
-  out3[0] = c->t_theta / n;
-  out3[1] = c->t_force / n;
-  out3[2] = c->t_step / n;
+  for (int q = 0; q < kEv; q++) out4[q] = c->t_pass[q] / n;
   return LPS_OK;
 }
 
@@ -597,7 +764,7 @@ int32_t lps_diagnostics(lps_ctx *c, lps_diag *out) {
   const StepArgs a = c->args();
   const Dev2 &U = c->U[s % 3], &V = c->V[s % 3], &A = c->A[s % 2];
   const double *th = c->TH[s % 3];
-  CTX_TRY(c, launch_diag(a, U.x, U.y, V.x, V.y, A.x, A.y, th, c->hb.perm, c->exclude_r, !c->fast,
+  CTX_TRY(c, launch_diag(a, U.x, U.y, V.x, V.y, A.x, A.y, th, c->perm_d, c->exclude_r,
                          c->partials, c->n_partials, c->dout, c->stream));
   CTX_TRY(c, hipMemcpyAsync(c->hout, c->dout, sizeof(DiagPartial), hipMemcpyDeviceToHost, c->stream));
   CTX_TRY(c, hipStreamSynchronize(c->stream));
@@ -654,15 +821,16 @@ int32_t lps_run(lps_ctx *c, int32_t nsteps, lps_diag *records, int32_t max_recor
 
 int32_t lps_fetch(lps_ctx *c, double *u, double *v, double *a, double *theta, double *m) {
   const int64_t s = c->step;
-  const int32_t N = c->n;
-  std::vector<double> bx(N), by(N);
+  const int32_t NS = c->n_slots;
+  std::vector<double> bx(NS), by(NS);
   auto pull2 = [&](const Dev2 &d, double *out) -> int32_t {
     if (!out) return LPS_OK;
-    CTX_TRY(c, hipMemcpyAsync(bx.data(), d.x, sizeof(double) * N, hipMemcpyDeviceToHost, c->stream));
-    CTX_TRY(c, hipMemcpyAsync(by.data(), d.y, sizeof(double) * N, hipMemcpyDeviceToHost, c->stream));
+    CTX_TRY(c, hipMemcpyAsync(bx.data(), d.x, sizeof(double) * NS, hipMemcpyDeviceToHost, c->stream));
+    CTX_TRY(c, hipMemcpyAsync(by.data(), d.y, sizeof(double) * NS, hipMemcpyDeviceToHost, c->stream));
     CTX_TRY(c, hipStreamSynchronize(c->stream));
-    for (int32_t k = 0; k < N; k++) {
+    for (int32_t k = 0; k < NS; k++) {
       const int64_t o = c->perm_h[k];
+      if (o < 0) continue;
       out[2 * o] = bx[k];
       out[2 * o + 1] = by[k];
     }
@@ -670,9 +838,10 @@ int32_t lps_fetch(lps_ctx *c, double *u, double *v, double *a, double *theta, do
   };
   auto pull1 = [&](const double *d, double *out) -> int32_t {
     if (!out) return LPS_OK;
-    CTX_TRY(c, hipMemcpyAsync(bx.data(), d, sizeof(double) * N, hipMemcpyDeviceToHost, c->stream));
+    CTX_TRY(c, hipMemcpyAsync(bx.data(), d, sizeof(double) * NS, hipMemcpyDeviceToHost, c->stream));
     CTX_TRY(c, hipStreamSynchronize(c->stream));
-    for (int32_t k = 0; k < N; k++) out[c->perm_h[k]] = bx[k];
+    for (int32_t k = 0; k < NS; k++)
+      if (c->perm_h[k] >= 0) out[c->perm_h[k]] = bx[k];
     return LPS_OK;
   };
   int32_t rc;
@@ -685,19 +854,20 @@ int32_t lps_fetch(lps_ctx *c, double *u, double *v, double *a, double *theta, do
 }
 
 int64_t lps_horizons(lps_ctx *c, int64_t *row_ptr, int32_t *cols) {
-  const int32_t N = c->n;
+  const int32_t N = c->n, NS = c->n_slots;
   if (!cols && !row_ptr) return c->hb.nnz;
-  std::vector<int64_t> rp(N + 1);
+  std::vector<int64_t> rp(NS + 1);
   std::vector<int32_t> cl(std::max<int64_t>(c->hb.nnz, 1));
-  CTX_TRY(c, hipMemcpyAsync(rp.data(), c->hb.row_ptr, sizeof(int64_t) * (N + 1), hipMemcpyDeviceToHost, c->stream));
+  CTX_TRY(c, hipMemcpyAsync(rp.data(), c->hb.row_ptr, sizeof(int64_t) * (NS + 1), hipMemcpyDeviceToHost, c->stream));
   CTX_TRY(c, hipMemcpyAsync(cl.data(), c->hb.col, sizeof(int32_t) * c->hb.nnz, hipMemcpyDeviceToHost, c->stream));
   CTX_TRY(c, hipStreamSynchronize(c->stream));
-  std::vector<int32_t> iperm(N);
-  for (int32_t k = 0; k < N; k++) iperm[c->perm_h[k]] = k;
+  std::vector<int32_t> slot_of(N);
+  for (int32_t k = 0; k < NS; k++)
+    if (c->perm_h[k] >= 0) slot_of[c->perm_h[k]] = k;
   int64_t pos = 0;
   if (row_ptr) row_ptr[0] = 0;
   for (int32_t o = 0; o < N; o++) {
-    const int32_t k = iperm[o];
+    const int32_t k = slot_of[o];
     const int64_t len = rp[k + 1] - rp[k];
     if (cols)
       for (int64_t t = 0; t < len; t++) cols[pos + t] = c->perm_h[cl[rp[k] + t]];

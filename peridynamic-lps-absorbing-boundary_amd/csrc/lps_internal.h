@@ -1,13 +1,16 @@
 // Internal declarations shared by the HIP kernels and the C-ABI layer.
 //
-// Device data layout (all arrays in the permuted node order):
-//   [0, n_int)       non-ABC nodes in Morton order of their lattice position
-//   [n_int, n)       active ABC nodes (bctypes == 1), one per ABC row, in row order
-// Fields are SoA (x and y components in separate arrays) so that a wavefront's 64 lanes load
-// 64 consecutive doubles. Horizons are kept twice: as CSR (self first, distance-sorted) for the
-// setup passes and the export, and as SELL-64 (self dropped, slice = one wavefront of rows,
-// column-major inside a slice) for the per-step passes, so that every index load is one
-// coalesced 256-byte line per wavefront.
+// Device data layout. Every per-node array is indexed by "slot":
+//   tile layout (fast path, lattice inputs): the lattice is cut into 16 x 16 tiles; each tile that
+//     holds a node owns 256 consecutive slots (x-major, y fastest inside the tile); lattice points
+//     without a node are holes (kind 0). ABC nodes stay in place.
+//   Morton layout (exact path, or nodes not on a lattice): non-ABC nodes in Morton order of their
+//     position, then the ABC nodes in ABC-row order; no holes.
+// kind[slot]: 0 hole, 1 node, 2 node of an active ABC row; abc_slot[row] gives the slot of a row.
+// Fields are SoA (x and y components in separate arrays), so a wavefront's 64 lanes load 64
+// consecutive doubles. Horizons exist as CSR (self first, distance-sorted; the setup passes, the
+// ABC rows and the export read it) and as SELL-64 (self dropped, one slice = one wavefront of
+// rows, column-major inside a slice) for the exact per-step passes.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -16,56 +19,72 @@
 
 namespace lps {
 
-constexpr int kSlice = 64;      // SELL slice height = wavefront size
-constexpr int kBlock = 256;     // threads per block of the node passes (4 wavefronts)
-constexpr int kMaxHorizon = 128;  // MAX_HORIZON_LENGTH, src/main.cpp:35
+constexpr int kSlice = 64;        // SELL slice height = wavefront size
+constexpr int kBlock = 256;       // threads per block of the node passes (4 wavefronts)
 constexpr int kWavesPerBlock = kBlock / 64;
-constexpr int kMaxGeoK = 6;       // largest lattice-offset radius of the fast path's LDS table
+constexpr int kMaxHorizon = 128;  // MAX_HORIZON_LENGTH, src/main.cpp:35
+constexpr int kTile = 16;         // tile edge (lattice points)
+constexpr int kMaxHalo = 4;       // largest stencil reach of the tile path (lattice cells)
+constexpr int kMaxWin = kTile + 2 * kMaxHalo;  // 24
 
-// Bond geometry of one lattice offset (fast path): unit vector, |xi| beta dV, beta dV.
-struct Geo {
-  double hx, hy, g1, g2;
+// Tile-path geometry: the stencil (bond offsets of a full horizon, in summation order) and the
+// tile map. Bond k of slot (lx, ly) reads window cell c + delta[k].
+struct TileArgs {
+  int32_t n_tiles, ntx, nty;       // active tiles; tile grid
+  int32_t halo, win;               // stencil reach K, window edge 16 + 2K
+  const int32_t *tile_xy;          // n_tiles: tx << 16 | ty
+  const int32_t *tile_map;         // ntx * nty -> tile index or -1
+  const int32_t *tile_nbr;         // n_tiles x 9: the 3 x 3 tiles around a tile (-1 absent)
+  const int8_t *tile_full;         // n_tiles: every window cell holds a node with m > 0
+  int32_t nt;                      // stencil bonds
+  const int32_t *delta;            // nt, window-cell offset ox * win + oy
+  const double *hx, *hy, *g1, *g2; // nt: unit vector, |xi| beta dV, beta dV
+  // The reference stencil (dx == dy, delta = 4 dx: the 68 offsets 0 < ox^2 + oy^2 <= 20) has 12
+  // distance classes; with c = dx / |xi|, class constants c1 = |xi| beta dV c and
+  // c2 = beta dV c^2 turn the bond terms into integer multiples (xi_hat = (ox, oy) c).
+  int32_t stencil68;               // bit 0: unrolled dilatation kernel, bit 1: unrolled force kernel
+  double c1[12], c2[12];
 };
+
+// Distance classes of the 68-bond stencil: ox^2 + oy^2.
+constexpr int kStencilR2[12] = {1, 2, 4, 5, 8, 9, 10, 13, 16, 17, 18, 20};
 
 // Everything a step kernel reads; device pointers.
 struct StepArgs {
-  int32_t n, n_int, n_abc, M;      // nodes, non-ABC nodes, ABC rows, cloud width
+  int32_t n;                       // slots
+  int32_t n_abc, M;                // ABC rows, cloud width
   double dt, rho, lam, mu, kappa, delta, dx, dy;
-  // geometry
-  const double *x, *y;           // coordinates
-  const int32_t *lat;            // packed lattice coords (ix << 16 | iy), fast path
-  const Geo *geo;                // (2K+1)^2 offset table, fast path
-  int32_t geo_k;                 // K: offsets in [-K, K]
+  const double *x, *y;             // coordinates (0 at holes)
+  const int8_t *kind;              // 0 hole, 1 node, 2 ABC-row node
+  const int32_t *abc_slot;         // n_abc
   // horizons
-  const int64_t *row_ptr;        // CSR, self first
+  const int64_t *row_ptr;          // CSR, self first (holes: empty)
   const int32_t *col;
-  const int64_t *sell_off;       // per slice, in elements
-  const int32_t *sell_w;         // per slice width (self excluded)
-  const int32_t *sell_col;       // -1 padded
+  const int64_t *sell_off;         // per slice, in elements
+  const int32_t *sell_w;           // per slice width (self excluded)
+  const int32_t *sell_col;         // -1 padded
   // per node constants
-  const double *m;               // weighted volume m_i
-  const double *bcoef;           // 8 mu / m_i, or -1 when m_i <= 0 (fast path)
-  const double *c2m;             // 2 / m_i, or 0 when m_i <= 0 (fast path)
-  const int8_t *is_bc;           // bctypes[2i] == 1 (IDW neighbour filter)
+  const double *m;                 // weighted volume m_i
+  const double *c2m;               // 2 / m_i, or 0 when m_i <= 0
+  const int8_t *is_bc;             // bctypes[2i] == 1 (IDW neighbour filter)
   // ABC rows
-  const int32_t *cloud;          // n_abc x M, permuted ids, -1 = skip
-  const double *gu, *gv;         // n_abc x 2 x 4M
-  const double *gth;             // n_abc x 2M
+  const int32_t *cloud;            // n_abc x M, slots, -1 = skip
+  const double *gu, *gv;           // n_abc x 2 x 4M
+  const double *gth;               // n_abc x 2M
   int32_t theta_method;
-  int32_t abc_on;
+  TileArgs tile;
 };
 
 // State buffers for one step (SoA).
 struct StepState {
-  const double *px, *py;         // prediction U^{n+1} before the ABC overwrite
+  const double *px, *py;           // prediction U^{n+1} before the ABC overwrite
   const double *ux, *uy, *uxm, *uym;   // U^n, U^{n-1}
   const double *vx, *vy, *vxm, *vym;   // V^n, V^{n-1}
-  const double *ax, *ay;         // A^n
-  double *theta;                 // theta^{n+1} (written by the dilatation pass)
-  double *acoef;                 // 2 (lambda - mu) theta_i / m_i (fast path, written with theta)
-  const double *thn, *thnm1;     // theta history
+  const double *ax, *ay;           // A^n
+  double *theta;                   // theta^{n+1} (written by the dilatation pass)
+  const double *thn, *thnm1;       // theta history
   double *ux1, *uy1, *vx1, *vy1, *ax1, *ay1;  // U^{n+1}, V^{n+1}, A^{n+1}
-  double *pnx, *pny;             // next prediction U^{n+2} (fused predictor)
+  double *pnx, *pny;               // next prediction U^{n+2} (fused predictor)
 };
 
 // Per-block partial sums of the output block.
@@ -74,40 +93,47 @@ struct DiagPartial {
   int64_t max_u_orig, max_v_orig;
 };
 
-// Kernel launchers (lps_kernels.hip).
+// Kernel launchers (lps_kernels.hip). fast selects the tile kernels (a.tile must be set).
 hipError_t launch_predict(const StepArgs &a, const double *ux, const double *uy, const double *vx,
                           const double *vy, const double *ax, const double *ay, double *px,
                           double *py, hipStream_t s);
-hipError_t launch_theta(const StepArgs &a, const StepState &st, bool exact, hipStream_t s);
-hipError_t launch_idw(const StepArgs &a, const StepState &st, bool exact, hipStream_t s);
-hipError_t launch_force(const StepArgs &a, const StepState &st, bool exact, hipStream_t s);
+hipError_t launch_theta(const StepArgs &a, const StepState &st, bool fast, hipStream_t s);
+hipError_t launch_idw(const StepArgs &a, const StepState &st, hipStream_t s);
+// Force pass of the nodes; on the exact path it also handles the ABC rows, on the fast path
+// launch_force_abc does (after it, same stream).
+hipError_t launch_force(const StepArgs &a, const StepState &st, bool fast, hipStream_t s);
+hipError_t launch_force_abc(const StepArgs &a, const StepState &st, hipStream_t s);
 hipError_t launch_theta_field(const StepArgs &a, const double *ux, const double *uy, double *theta,
                               hipStream_t s);
 hipError_t launch_weighted_volume(const StepArgs &a, double *m, hipStream_t s);
 hipError_t launch_diag(const StepArgs &a, const double *ux, const double *uy, const double *vx,
                        const double *vy, const double *ax, const double *ay, const double *theta,
-                       const int32_t *perm, double exclude_r, bool exact, DiagPartial *partials,
+                       const int32_t *perm, double exclude_r, DiagPartial *partials,
                        int32_t n_partials, DiagPartial *out, hipStream_t s);
 int32_t diag_partials(int32_t n);
+// Tile path check: for every node slot, the stencil cells that hold a node are exactly the set of
+// its CSR row (self excluded). Adds the number of rows that differ to *d_mismatch.
+hipError_t launch_tile_check(const StepArgs &a, int32_t *d_mismatch, hipStream_t s);
 
-// Horizon build (lps_neighbors.hip). Builds perm (new -> original), CSR and SELL on the device.
+// Horizon build (lps_neighbors.hip) over slots: the CSR and SELL of the nodes' horizons.
 struct HorizonBuild {
   int64_t nnz = 0;                 // CSR entries including self
-  int32_t *perm = nullptr;         // n
-  int32_t *iperm = nullptr;        // n
-  int64_t *row_ptr = nullptr;      // n + 1
-  int32_t *col = nullptr;          // nnz
+  int64_t *row_ptr = nullptr;      // n_slots + 1
+  int32_t *col = nullptr;          // nnz (slots)
   int64_t *sell_off = nullptr;     // slices + 1
   int32_t *sell_w = nullptr;       // slices
   int32_t *sell_col = nullptr;
   int64_t sell_len = 0;
   int32_t max_len = 0;
 };
-// xo, yo: original coordinates on the device; seg: 1 for nodes of the ABC segment (original order),
-// seg_rank: their rank inside the segment (ABC row). Returns hipSuccess or an error; *overflow is
-// set when a horizon exceeds max_len.
-hipError_t build_horizons(int32_t n, const double *xo, const double *yo, const int8_t *seg,
-                          const int32_t *seg_rank, double r, int32_t max_len, HorizonBuild *hb,
+// Morton order of the nodes (non-ABC by position, then seg == 1 nodes by seg_rank): perm[k] is
+// the original index of slot k.
+hipError_t morton_order(int32_t n, const double *xo, const double *yo, const int8_t *seg,
+                        const int32_t *seg_rank, double cell, int32_t *perm, hipStream_t s);
+// xs, ys: slot coordinates; kind: slot kind (0 = hole); perm: slot -> original index (ties of
+// equal distance are ranked by it). *overflow is set when a horizon exceeds max_len.
+hipError_t build_horizons(int32_t n_slots, const double *xs, const double *ys, const int8_t *kind,
+                          const int32_t *perm, double r, int32_t max_len, HorizonBuild *hb,
                           bool *overflow, hipStream_t s);
 void free_horizons(HorizonBuild *hb);
 

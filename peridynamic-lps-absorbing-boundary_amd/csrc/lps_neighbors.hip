@@ -5,9 +5,9 @@
 // (libraries/libkdtree++/kdtree++/region.hpp:44-53, 86-95), keeps the nodes with
 // sqrt(dx^2 + dy^2) <= r (kd_tree_interface.h:90-94) and sorts them by that distance (:107).
 // Here:
-//   1. nodes are reordered: non-ABC nodes by the Morton code of their position (so that the
-//      nodes of a wavefront and their neighbours share cache lines), ABC nodes after them in ABC-row
-//      order; a stable radix sort keeps equal keys in original order;
+//   1. the caller places the nodes in slots (lps_api.hip: lattice tiles, or the Morton order built
+//      here with a stable radix sort - non-ABC nodes by the Morton code of their position, ABC
+//      nodes after them in ABC-row order, equal keys in original order);
 //   2. nodes are binned into square cells of side slightly above r, so the 3x3 cells around a node
 //      hold every candidate;
 //   3. count pass, scan, fill pass: the same inclusive box test and the same FP64 distance
@@ -20,16 +20,6 @@
 #include "lps_internal.h"
 
 namespace lps {
-
-__device__ __forceinline__ uint64_t spread_bits(uint32_t v) {
-  uint64_t x = v & 0x1fffffull;  // 21 bits
-  x = (x | (x << 32)) & 0x1f00000000ffffull;
-  x = (x | (x << 16)) & 0x1f0000ff0000ffull;
-  x = (x | (x << 8)) & 0x100f00f00f00f00full;
-  x = (x | (x << 4)) & 0x10c30c30c30c30c3ull;
-  x = (x | (x << 2)) & 0x1249249249249249ull;
-  return x;
-}
 
 __device__ __forceinline__ uint64_t spread2(uint32_t v) {
   // 2D interleave: bit b of v -> bit 2b
@@ -60,16 +50,6 @@ __global__ void k_morton_keys(int32_t n, const double *xo, const double *yo, con
   vals[i] = i;
 }
 
-__global__ void k_apply_perm(int32_t n, const int32_t *perm, const double *xo, const double *yo,
-                             int32_t *iperm, double *xn, double *yn) {
-  const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
-  const int32_t o = perm[k];
-  iperm[o] = k;
-  xn[k] = xo[o];
-  yn[k] = yo[o];
-}
-
 struct CellGrid {
   double xmin, ymin, h;
   int32_t ncx, ncy;
@@ -81,19 +61,19 @@ __device__ __forceinline__ void cell_of(const CellGrid &g, double x, double y, i
   *cy = (int32_t)floor((y - g.ymin) / g.h);
 }
 
-__global__ void k_cell_count(int32_t n, const double *xn, const double *yn, CellGrid g,
-                             int32_t *cell_cnt) {
+__global__ void k_cell_count(int32_t n, const double *xn, const double *yn, const int8_t *kind,
+                             CellGrid g, int32_t *cell_cnt) {
   const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
+  if (k >= n || kind[k] == 0) return;
   int32_t cx, cy;
   cell_of(g, xn[k], yn[k], &cx, &cy);
   atomicAdd(&cell_cnt[(int64_t)cx * g.ncy + cy], 1);
 }
 
-__global__ void k_cell_fill(int32_t n, const double *xn, const double *yn, CellGrid g,
-                            int32_t *cell_fill, int32_t *cell_nodes) {
+__global__ void k_cell_fill(int32_t n, const double *xn, const double *yn, const int8_t *kind,
+                            CellGrid g, int32_t *cell_fill, int32_t *cell_nodes) {
   const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
+  if (k >= n || kind[k] == 0) return;
   int32_t cx, cy;
   cell_of(g, xn[k], yn[k], &cx, &cy);
   const int32_t pos = atomicAdd(&cell_fill[(int64_t)cx * g.ncy + cy], 1);
@@ -114,12 +94,12 @@ __device__ __forceinline__ bool in_horizon(double xi, double yi, double lox, dou
 }
 
 template <bool FILL>
-__global__ void k_search(int32_t n, const double *xn, const double *yn, CellGrid g,
-                         const int32_t *cell_start, const int32_t *cell_nodes, double r,
+__global__ void k_search(int32_t n, const double *xn, const double *yn, const int8_t *kind,
+                         CellGrid g, const int32_t *cell_start, const int32_t *cell_nodes, double r,
                          int32_t max_len, int64_t *len_or_ptr, int32_t *cand_col,
                          double *cand_dist, int32_t *overflow) {
   const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
+  if (k >= n || kind[k] == 0) return;
   double xi, yi, lox, hix, loy, hiy;
   {
 #pragma clang fp contract(off)
@@ -192,7 +172,7 @@ __global__ void k_slice_width(int32_t n, int32_t nslices, const int64_t *row_ptr
   if (s >= nslices) return;
   const int32_t row = s * kSlice + lane;
   int32_t len = 0;
-  if (row < n) len = (int32_t)(row_ptr[row + 1] - row_ptr[row]) - 1;  // self excluded
+  if (row < n) len = max((int32_t)(row_ptr[row + 1] - row_ptr[row]) - 1, 0);  // self excluded
   for (int o = 32; o > 0; o >>= 1) len = max(len, __shfl_xor(len, o, 64));
   if (lane == 0) {
     w[s] = len;
@@ -205,10 +185,11 @@ __global__ void k_sell_fill(int32_t n, const int64_t *row_ptr, const int32_t *co
   const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   const int32_t s = k / kSlice, lane = k % kSlice;
-  const int64_t b = row_ptr[k] + 1, len = row_ptr[k + 1] - b;
+  const int64_t b = row_ptr[k] + 1, len = row_ptr[k + 1] - b;  // holes: len = -1, all pads
   int32_t *out = sell_col + sell_off[s] + lane;
   const int32_t w = sell_w[s];
   for (int32_t t = 0; t < w; t++) out[(int64_t)t * kSlice] = (t < len) ? col[b + t] : -1;
+  (void)n;
 }
 
 __global__ void k_max_len(int32_t n, const int64_t *row_ptr, int32_t *out) {
@@ -226,7 +207,7 @@ __global__ void k_max_len(int32_t n, const int64_t *row_ptr, int32_t *out) {
 static inline unsigned grid_of(int64_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
 
 void free_horizons(HorizonBuild *hb) {
-  (void)hipFree(hb->perm); (void)hipFree(hb->iperm); (void)hipFree(hb->row_ptr);
+  (void)hipFree(hb->row_ptr);
   (void)hipFree(hb->col); (void)hipFree(hb->sell_off); (void)hipFree(hb->sell_w);
   (void)hipFree(hb->sell_col);
   *hb = HorizonBuild();
@@ -249,18 +230,58 @@ struct Scratch {
 };
 }  // namespace
 
-hipError_t build_horizons(int32_t n, const double *xo, const double *yo, const int8_t *seg,
-                          const int32_t *seg_rank, double r, int32_t max_len, HorizonBuild *hb,
-                          bool *overflow, hipStream_t s) {
-  *overflow = false;
+hipError_t morton_order(int32_t n, const double *xo, const double *yo, const int8_t *seg,
+                        const int32_t *seg_rank, double cell, int32_t *perm, hipStream_t s) {
   Scratch sc;
-  // bounding box (host reads back 2 x n doubles once; n is at most a few million)
   std::vector<double> hx(n), hy(n);
   LPS_TRY(hipMemcpyAsync(hx.data(), xo, sizeof(double) * n, hipMemcpyDeviceToHost, s));
   LPS_TRY(hipMemcpyAsync(hy.data(), yo, sizeof(double) * n, hipMemcpyDeviceToHost, s));
   LPS_TRY(hipStreamSynchronize(s));
-  double xmin = hx[0], xmax = hx[0], ymin = hy[0], ymax = hy[0];
+  double xmin = hx[0], ymin = hy[0];
   for (int32_t i = 1; i < n; i++) {
+    xmin = std::min(xmin, hx[i]);
+    ymin = std::min(ymin, hy[i]);
+  }
+  uint64_t *keys, *keys_out;
+  int32_t *vals;
+  LPS_TRY(sc.get(&keys, n));
+  LPS_TRY(sc.get(&keys_out, n));
+  LPS_TRY(sc.get(&vals, n));
+  hipLaunchKernelGGL(k_morton_keys, dim3(grid_of(n)), dim3(256), 0, s, n, xo, yo, seg, seg_rank,
+                     xmin, ymin, cell / 8.0, keys, vals);
+  LPS_TRY(hipGetLastError());
+  size_t tmp_bytes = 0;
+  LPS_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, keys, keys_out, vals, perm, n, 0,
+                                             64, s));
+  char *tmp;
+  LPS_TRY(sc.get(&tmp, tmp_bytes));
+  LPS_TRY(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, keys, keys_out, vals, perm, n, 0, 64,
+                                             s));
+  LPS_TRY(hipStreamSynchronize(s));
+  return hipSuccess;
+}
+
+hipError_t build_horizons(int32_t n, const double *xn, const double *yn, const int8_t *kind,
+                          const int32_t *perm, double r, int32_t max_len, HorizonBuild *hb,
+                          bool *overflow, hipStream_t s) {
+  *overflow = false;
+  Scratch sc;
+  // bounding box of the nodes (holes excluded)
+  std::vector<double> hx(n), hy(n);
+  std::vector<int8_t> hk(n);
+  LPS_TRY(hipMemcpyAsync(hx.data(), xn, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+  LPS_TRY(hipMemcpyAsync(hy.data(), yn, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+  LPS_TRY(hipMemcpyAsync(hk.data(), kind, n, hipMemcpyDeviceToHost, s));
+  LPS_TRY(hipStreamSynchronize(s));
+  double xmin = 0, xmax = 0, ymin = 0, ymax = 0;
+  bool first = true;
+  for (int32_t i = 0; i < n; i++) {
+    if (!hk[i]) continue;
+    if (first) {
+      xmin = xmax = hx[i];
+      ymin = ymax = hy[i];
+      first = false;
+    }
     xmin = std::min(xmin, hx[i]); xmax = std::max(xmax, hx[i]);
     ymin = std::min(ymin, hy[i]); ymax = std::max(ymax, hy[i]);
   }
@@ -273,60 +294,35 @@ hipError_t build_horizons(int32_t n, const double *xo, const double *yo, const i
   const int64_t ncell = (int64_t)g.ncx * g.ncy;
   if (ncell > (int64_t)1 << 30) return hipErrorInvalidValue;
 
-  // 1. Morton order (ABC segment last)
-  uint64_t *keys, *keys_out;
-  int32_t *vals;
-  LPS_TRY(sc.get(&keys, n));
-  LPS_TRY(sc.get(&keys_out, n));
-  LPS_TRY(sc.get(&vals, n));
-  LPS_TRY(hipMalloc((void **)&hb->perm, sizeof(int32_t) * std::max(n, 1)));
-  LPS_TRY(hipMalloc((void **)&hb->iperm, sizeof(int32_t) * std::max(n, 1)));
-  hipLaunchKernelGGL(k_morton_keys, dim3(grid_of(n)), dim3(256), 0, s, n, xo, yo, seg, seg_rank,
-                     xmin, ymin, r / 8.0, keys, vals);
-  LPS_TRY(hipGetLastError());
-  size_t tmp_bytes = 0;
-  LPS_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, keys, keys_out, vals, hb->perm, n,
-                                             0, 64, s));
-  void *tmp;
-  LPS_TRY(sc.get((char **)&tmp, tmp_bytes));
-  LPS_TRY(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, keys, keys_out, vals, hb->perm, n, 0,
-                                             64, s));
-  double *xn, *yn;
-  LPS_TRY(sc.get(&xn, n));
-  LPS_TRY(sc.get(&yn, n));
-  hipLaunchKernelGGL(k_apply_perm, dim3(grid_of(n)), dim3(256), 0, s, n, hb->perm, xo, yo,
-                     hb->iperm, xn, yn);
-  LPS_TRY(hipGetLastError());
-
-  // 2. cell binning
+  // cell binning
   int32_t *cell_cnt, *cell_start, *cell_fill, *cell_nodes;
   LPS_TRY(sc.get(&cell_cnt, ncell + 1));
   LPS_TRY(sc.get(&cell_start, ncell + 1));
   LPS_TRY(sc.get(&cell_fill, ncell + 1));
   LPS_TRY(sc.get(&cell_nodes, n));
   LPS_TRY(hipMemsetAsync(cell_cnt, 0, sizeof(int32_t) * (ncell + 1), s));
-  hipLaunchKernelGGL(k_cell_count, dim3(grid_of(n)), dim3(256), 0, s, n, xn, yn, g, cell_cnt);
+  hipLaunchKernelGGL(k_cell_count, dim3(grid_of(n)), dim3(256), 0, s, n, xn, yn, kind, g, cell_cnt);
   LPS_TRY(hipGetLastError());
-  tmp_bytes = 0;
+  size_t tmp_bytes = 0;
   LPS_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, cell_cnt, cell_start, ncell + 1, s));
-  void *tmp2;
-  LPS_TRY(sc.get((char **)&tmp2, tmp_bytes));
+  char *tmp2;
+  LPS_TRY(sc.get(&tmp2, tmp_bytes));
   LPS_TRY(hipcub::DeviceScan::ExclusiveSum(tmp2, tmp_bytes, cell_cnt, cell_start, ncell + 1, s));
   LPS_TRY(hipMemcpyAsync(cell_fill, cell_start, sizeof(int32_t) * (ncell + 1),
                          hipMemcpyDeviceToDevice, s));
-  hipLaunchKernelGGL(k_cell_fill, dim3(grid_of(n)), dim3(256), 0, s, n, xn, yn, g, cell_fill,
+  hipLaunchKernelGGL(k_cell_fill, dim3(grid_of(n)), dim3(256), 0, s, n, xn, yn, kind, g, cell_fill,
                      cell_nodes);
   LPS_TRY(hipGetLastError());
 
-  // 3. count, scan, fill
+  // count, scan, fill
   int64_t *lens;
   int32_t *d_over;
   LPS_TRY(sc.get(&lens, n + 1));
   LPS_TRY(sc.get(&d_over, 1));
   LPS_TRY(hipMemsetAsync(d_over, 0, sizeof(int32_t), s));
   LPS_TRY(hipMemsetAsync(lens, 0, sizeof(int64_t) * (n + 1), s));
-  hipLaunchKernelGGL(k_search<false>, dim3(grid_of(n)), dim3(256), 0, s, n, xn, yn, g, cell_start,
-                     cell_nodes, r, max_len, lens, nullptr, nullptr, d_over);
+  hipLaunchKernelGGL(k_search<false>, dim3(grid_of(n)), dim3(256), 0, s, n, xn, yn, kind, g,
+                     cell_start, cell_nodes, r, max_len, lens, nullptr, nullptr, d_over);
   LPS_TRY(hipGetLastError());
   int32_t h_over = 0;
   LPS_TRY(hipMemcpyAsync(&h_over, d_over, sizeof(int32_t), hipMemcpyDeviceToHost, s));
@@ -338,8 +334,8 @@ hipError_t build_horizons(int32_t n, const double *xo, const double *yo, const i
   LPS_TRY(hipMalloc((void **)&hb->row_ptr, sizeof(int64_t) * (n + 1)));
   tmp_bytes = 0;
   LPS_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, lens, hb->row_ptr, n + 1, s));
-  void *tmp3;
-  LPS_TRY(sc.get((char **)&tmp3, tmp_bytes));
+  char *tmp3;
+  LPS_TRY(sc.get(&tmp3, tmp_bytes));
   LPS_TRY(hipcub::DeviceScan::ExclusiveSum(tmp3, tmp_bytes, lens, hb->row_ptr, n + 1, s));
   LPS_TRY(hipMemcpyAsync(&hb->nnz, hb->row_ptr + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
   LPS_TRY(hipStreamSynchronize(s));
@@ -347,17 +343,17 @@ hipError_t build_horizons(int32_t n, const double *xo, const double *yo, const i
   double *cand_dist;
   LPS_TRY(sc.get(&cand_col, hb->nnz));
   LPS_TRY(sc.get(&cand_dist, hb->nnz));
-  hipLaunchKernelGGL(k_search<true>, dim3(grid_of(n)), dim3(256), 0, s, n, xn, yn, g, cell_start,
-                     cell_nodes, r, max_len, hb->row_ptr, cand_col, cand_dist, d_over);
+  hipLaunchKernelGGL(k_search<true>, dim3(grid_of(n)), dim3(256), 0, s, n, xn, yn, kind, g,
+                     cell_start, cell_nodes, r, max_len, hb->row_ptr, cand_col, cand_dist, d_over);
   LPS_TRY(hipGetLastError());
 
-  // 4. sort rows
+  // rank rows by (distance, original index)
   LPS_TRY(hipMalloc((void **)&hb->col, sizeof(int32_t) * std::max<int64_t>(hb->nnz, 1)));
   hipLaunchKernelGGL(k_sort_rows, dim3(grid_of(n, 4)), dim3(256), 0, s, n, hb->row_ptr, cand_col,
-                     cand_dist, hb->perm, hb->col);
+                     cand_dist, perm, hb->col);
   LPS_TRY(hipGetLastError());
 
-  // 5. SELL-64 without self
+  // SELL-64 without self
   const int32_t nslices = (n + kSlice - 1) / kSlice;
   int64_t *wsz;
   LPS_TRY(sc.get(&wsz, nslices + 1));
@@ -369,8 +365,8 @@ hipError_t build_horizons(int32_t n, const double *xo, const double *yo, const i
   LPS_TRY(hipGetLastError());
   tmp_bytes = 0;
   LPS_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, wsz, hb->sell_off, nslices + 1, s));
-  void *tmp4;
-  LPS_TRY(sc.get((char **)&tmp4, tmp_bytes));
+  char *tmp4;
+  LPS_TRY(sc.get(&tmp4, tmp_bytes));
   LPS_TRY(hipcub::DeviceScan::ExclusiveSum(tmp4, tmp_bytes, wsz, hb->sell_off, nslices + 1, s));
   LPS_TRY(hipMemcpyAsync(&hb->sell_len, hb->sell_off + nslices, sizeof(int64_t),
                          hipMemcpyDeviceToHost, s));

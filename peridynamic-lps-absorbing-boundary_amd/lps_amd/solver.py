@@ -45,7 +45,8 @@ class LpsProblem(C.Structure):
 
 class LpsOptions(C.Structure):
     _fields_ = [("theta_method", C.c_int32), ("abc_enabled", C.c_int32), ("exact", C.c_int32),
-                ("use_graph", C.c_int32), ("device", C.c_int32), ("horizon_radius", C.c_double)]
+                ("use_graph", C.c_int32), ("device", C.c_int32), ("horizon_radius", C.c_double),
+                ("variant", C.c_int32)]
 
 
 class LpsDiag(C.Structure):
@@ -71,6 +72,7 @@ SIGNATURES = {
     "lps_m_ref": (C.c_double, [C.c_void_p]),
     "lps_set_timing": (C.c_int32, [C.c_void_p, C.c_int32]),
     "lps_pass_times": (C.c_int32, [C.c_void_p, C.c_void_p]),
+    "lps_describe": (C.c_char_p, [C.c_void_p]),
     "lps_last_error": (C.c_char_p, [C.c_void_p]),
     "lps_global_error": (C.c_char_p, []),
     "lps_destroy": (None, [C.c_void_p]),
@@ -109,7 +111,7 @@ class Solver:
 
     def __init__(self, prob, theta_method: int = LPS_THETA_GTHETA, abc_enabled: bool = True,
                  exact: bool = False, use_graph: bool = True, device: int = 0,
-                 horizon_radius: float = 0.0):
+                 horizon_radius: float = 0.0, variant: int = 0):
         L = load_library()
         self._keep: List[np.ndarray] = []
 
@@ -146,6 +148,7 @@ class Solver:
         o.use_graph = 1 if use_graph else 0
         o.device = device
         o.horizon_radius = horizon_radius
+        o.variant = variant
         h = C.c_void_p()
         rc = L.lps_create(C.byref(p), C.byref(o), C.byref(h))
         if rc != LPS_OK:
@@ -206,6 +209,10 @@ class Solver:
         return rp, cols[:nnz]
 
     @property
+    def path(self) -> str:
+        return self._L.lps_describe(self._h).decode()
+
+    @property
     def step_index(self) -> int:
         return self._L.lps_step_index(self._h)
 
@@ -217,6 +224,7 @@ class Solver:
         self._check(self._L.lps_set_timing(self._h, 1 if on else 0))
 
     def pass_times(self):
-        out = np.zeros(3)
+        out = np.zeros(4)
         self._check(self._L.lps_pass_times(self._h, _ptr(out)))
-        return {"theta_ms": out[0], "force_ms": out[1], "step_ms": out[2]}
+        return {"theta_ms": float(out[0]), "force_ms": float(out[1]), "abc_ms": float(out[2]),
+                "step_ms": float(out[3])}

@@ -104,3 +104,39 @@ def test_nonfinite_state_is_reported_at_first_output_step():
             s.run(90)
         assert e.value.code == solver.LPS_ERR_NONFINITE
         assert s.step_index == 30
+
+
+def test_lattice_inputs_take_the_tile_path_and_others_fall_back():
+    say("path selection")
+    prob, _ = load_golden("abc_r10_nu025")
+    with solver.Solver(prob) as s:
+        assert s.path.startswith("tile"), s.path
+    with solver.Solver(prob, exact=True) as s:
+        assert s.path.startswith("exact"), s.path
+    jit = pp.Problem(**{**prob.__dict__})
+    rng = np.random.default_rng(5)
+    jit.nodes = prob.nodes + rng.uniform(-0.05, 0.05, size=prob.nodes.shape)
+    with solver.Solver(jit) as s:
+        assert s.path.startswith("exact"), s.path
+        recs = s.run(60)
+    o = pyoracle.Oracle(jit)
+    orecs = o.run(60)
+    assert abs(recs[-1]["energy"] - orecs[-1]["energy"]) <= 1e-13 * abs(orecs[-1]["energy"])
+
+
+def test_tile_path_matches_oracle_on_a_larger_domain():
+    """R = 40 mm (20k nodes, 89 tiles incl. partial ones), nu = 0.3, off-centre pulse, 600 steps."""
+    say("tile R=40")
+    prob = pp.make_problem(40.0, 0.3, steps=600, dt=1.7e-8, dk_factor=0.7,
+                           pulses=[pp.Pulse(8.0, -5.0, 12.0, 1.0, (1.0, 1.0))], workers=8)
+    with solver.Solver(prob) as s:
+        assert s.path.startswith("tile")
+        recs = s.run(600)
+        f = s.fetch()
+    o = pyoracle.Oracle(prob)
+    orecs = o.run(600)
+    of = o.fetch()
+    for k in ("u", "v", "theta"):
+        assert rel_l2(f[k], of[k]) <= 1e-10, k
+    assert_series_close(records_table(recs, ENERGY_KEYS), records_table(orecs, ENERGY_KEYS),
+                        ENERGY_KEYS, 1e-10, prob, "R40")
