@@ -43,6 +43,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=30)
     ap.add_argument("--workers", type=int, default=None)
+    ap.add_argument("--cache", default=None, metavar="NPZ",
+                    help="load the generated workload from NPZ, or generate and save it there")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="run only a plain copy kernel over BYTES (HBM counter calibration)")
     args = ap.parse_args()
@@ -68,9 +70,16 @@ def main():
     steps = args.steps if args.steps is not None else w.steps
     workers = args.workers or min(16, os.cpu_count() or 1)
     t = time.time()
-    log(f"[bench] generating {w.name} ({w.description}) with {workers} workers")
-    prob = configs.make(w.name, workers=workers, steps=steps)
-    log(f"[bench] generated N={prob.N} C={prob.C} dt={prob.dt:.3e} in {time.time() - t:.1f}s")
+    if args.cache and os.path.exists(args.cache):
+        prob = configs.load(args.cache)
+        prob.steps = steps
+        log(f"[bench] loaded {w.name} from {args.cache}")
+    else:
+        log(f"[bench] generating {w.name} ({w.description}) with {workers} workers")
+        prob = configs.make(w.name, workers=workers, steps=steps)
+        if args.cache:
+            configs.save(prob, args.cache)
+    log(f"[bench] N={prob.N} C={prob.C} dt={prob.dt:.3e} ready in {time.time() - t:.1f}s")
 
     t = time.time()
     s = solver.Solver(prob, exact=args.exact, use_graph=True, variant=args.variant)

@@ -383,7 +383,7 @@ static int32_t create_impl(lps_ctx *c, const lps_problem *p) {
     });
     const int32_t nt = (int32_t)offs.size();
     std::vector<int32_t> dl(nt);
-    std::vector<double> thx(nt), thy(nt), tg1(nt), tg2(nt);
+    std::vector<double> thx(nt), thy(nt), tg1(nt), tg2(nt), txi(nt);
     for (int32_t k = 0; k < nt; k++) {
       const double xx = offs[k].ox * c->dx, yy = offs[k].oy * c->dy, d = offs[k].d;
       const double bt = beta_host(d, c->delta, c->dx);
@@ -392,6 +392,7 @@ static int32_t create_impl(lps_ctx *c, const lps_problem *p) {
       thy[k] = yy / d;
       tg1[k] = d * bt * c->dx * c->dy;
       tg2[k] = bt * c->dx * c->dy;
+      txi[k] = d;
     }
     // 3 x 3 neighbour tiles and "full" windows (every cell a node with m > 0)
     const int32_t ntl = c->tile.n_tiles;
@@ -436,18 +437,29 @@ static int32_t create_impl(lps_ctx *c, const lps_problem *p) {
     }
     int32_t *d_txy, *d_tmap, *d_delta, *d_nbr;
     int8_t *d_full;
-    double *d_hx, *d_hy, *d_g1, *d_g2;
+    double *d_hx, *d_hy, *d_g1, *d_g2, *d_xi;
     CTX_TRY(c, c->upload(&d_txy, tile_xy));
     CTX_TRY(c, c->upload(&d_tmap, tile_map));
     CTX_TRY(c, c->upload(&d_nbr, nbr));
     CTX_TRY(c, c->upload(&d_full, full));
     c->tile.tile_nbr = d_nbr;
     c->tile.tile_full = d_full;
+    std::vector<int32_t> full_list, part_list;
+    for (int32_t t = 0; t < ntl; t++) (full[t] ? full_list : part_list).push_back(t);
+    int32_t *d_fl, *d_pl;
+    CTX_TRY(c, c->upload(&d_fl, full_list));
+    CTX_TRY(c, c->upload(&d_pl, part_list));
+    c->tile.full_list = d_fl;
+    c->tile.part_list = d_pl;
+    c->tile.n_full = (int32_t)full_list.size();
+    c->tile.n_part = (int32_t)part_list.size();
     CTX_TRY(c, c->upload(&d_delta, dl));
     CTX_TRY(c, c->upload(&d_hx, thx));
     CTX_TRY(c, c->upload(&d_hy, thy));
     CTX_TRY(c, c->upload(&d_g1, tg1));
     CTX_TRY(c, c->upload(&d_g2, tg2));
+    CTX_TRY(c, c->upload(&d_xi, txi));
+    c->tile.xi = d_xi;
     c->tile.tile_xy = d_txy;
     c->tile.tile_map = d_tmap;
     c->tile.nt = nt;
@@ -523,7 +535,7 @@ static int32_t create_impl(lps_ctx *c, const lps_problem *p) {
   }
   // output block buffers
   c->n_partials = diag_partials(NS);
-  CTX_TRY(c, c->alloc(&c->partials, c->n_partials));
+  CTX_TRY(c, c->alloc(&c->partials, std::max(c->n_partials, c->tile.n_tiles)));
   CTX_TRY(c, c->alloc(&c->dout, 1));
   CTX_TRY(c, hipHostMalloc((void **)&c->hout, sizeof(DiagPartial)));
   return reset_state(c);
@@ -659,7 +671,7 @@ static hipError_t step_once(lps_ctx *c, bool record_events) {
   if (record_events && (e = hipEventRecord(ev[1], c->stream)) != hipSuccess) return e;
   // fast path: the ABC rows' force/G_u/G_v kernel runs beside the node force kernel on the second
   // stream (both only read the dilatation pass's output and write disjoint slots)
-  const bool fork = c->fast && a.n_abc > 0;
+  const bool fork = c->fast && (a.n_abc > 0 || ((a.tile.stencil68 & 2) && a.tile.n_part > 0));
   if (fork) {
     if ((e = hipEventRecord(c->ev_fork, c->stream)) != hipSuccess) return e;
     if ((e = hipStreamWaitEvent(c->stream2, c->ev_fork, 0)) != hipSuccess) return e;
@@ -764,8 +776,12 @@ int32_t lps_diagnostics(lps_ctx *c, lps_diag *out) {
   const StepArgs a = c->args();
   const Dev2 &U = c->U[s % 3], &V = c->V[s % 3], &A = c->A[s % 2];
   const double *th = c->TH[s % 3];
-  CTX_TRY(c, launch_diag(a, U.x, U.y, V.x, V.y, A.x, A.y, th, c->perm_d, c->exclude_r,
-                         c->partials, c->n_partials, c->dout, c->stream));
+  if (c->fast)
+    CTX_TRY(c, launch_diag_tile(a, U.x, U.y, V.x, V.y, A.x, A.y, th, c->perm_d, c->exclude_r,
+                                c->partials, c->dout, c->stream));
+  else
+    CTX_TRY(c, launch_diag(a, U.x, U.y, V.x, V.y, A.x, A.y, th, c->perm_d, c->exclude_r,
+                           c->partials, c->n_partials, c->dout, c->stream));
   CTX_TRY(c, hipMemcpyAsync(c->hout, c->dout, sizeof(DiagPartial), hipMemcpyDeviceToHost, c->stream));
   CTX_TRY(c, hipStreamSynchronize(c->stream));
   const DiagPartial &d = *c->hout;

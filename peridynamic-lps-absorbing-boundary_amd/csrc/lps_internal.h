@@ -36,9 +36,13 @@ struct TileArgs {
   const int32_t *tile_map;         // ntx * nty -> tile index or -1
   const int32_t *tile_nbr;         // n_tiles x 9: the 3 x 3 tiles around a tile (-1 absent)
   const int8_t *tile_full;         // n_tiles: every window cell holds a node with m > 0
+  const int32_t *full_list;        // the full tiles, in tile order
+  const int32_t *part_list;        // the other tiles
+  int32_t n_full, n_part;
   int32_t nt;                      // stencil bonds
   const int32_t *delta;            // nt, window-cell offset ox * win + oy
   const double *hx, *hy, *g1, *g2; // nt: unit vector, |xi| beta dV, beta dV
+  const double *xi;                // nt: |xi|
   // The reference stencil (dx == dy, delta = 4 dx: the 68 offsets 0 < ox^2 + oy^2 <= 20) has 12
   // distance classes; with c = dx / |xi|, class constants c1 = |xi| beta dV c and
   // c2 = beta dV c^2 turn the bond terms into integer multiples (xi_hat = (ox, oy) c).
@@ -102,6 +106,8 @@ hipError_t launch_idw(const StepArgs &a, const StepState &st, hipStream_t s);
 // Force pass of the nodes; on the exact path it also handles the ABC rows, on the fast path
 // launch_force_abc does (after it, same stream).
 hipError_t launch_force(const StepArgs &a, const StepState &st, bool fast, hipStream_t s);
+// Fast path, second stream: the ABC rows and, when the unrolled force kernel takes the full tiles,
+// the partial tiles.
 hipError_t launch_force_abc(const StepArgs &a, const StepState &st, hipStream_t s);
 hipError_t launch_theta_field(const StepArgs &a, const double *ux, const double *uy, double *theta,
                               hipStream_t s);
@@ -111,6 +117,12 @@ hipError_t launch_diag(const StepArgs &a, const double *ux, const double *uy, co
                        const int32_t *perm, double exclude_r, DiagPartial *partials,
                        int32_t n_partials, DiagPartial *out, hipStream_t s);
 int32_t diag_partials(int32_t n);
+// Fast-path output block: one block per tile (n_partials = a.tile.n_tiles), the strain energy
+// from the LDS-staged window and the stencil table.
+hipError_t launch_diag_tile(const StepArgs &a, const double *ux, const double *uy,
+                            const double *vx, const double *vy, const double *ax, const double *ay,
+                            const double *theta, const int32_t *perm, double exclude_r,
+                            DiagPartial *partials, DiagPartial *out, hipStream_t s);
 // Tile path check: for every node slot, the stencil cells that hold a node are exactly the set of
 // its CSR row (self excluded). Adds the number of rows that differ to *d_mismatch.
 hipError_t launch_tile_check(const StepArgs &a, int32_t *d_mismatch, hipStream_t s);

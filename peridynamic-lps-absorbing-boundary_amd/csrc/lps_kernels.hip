@@ -652,27 +652,24 @@ __device__ __forceinline__ void force_sum68(const double2 *su, const double2 *sa
   *F1 = (fy[0] + fy[1]) + (fy[2] + fy[3]);
 }
 
-__global__ void __launch_bounds__(kBlock) k_force_t68(StepArgs a, StepState st) {
+// Unrolled force kernel for the full tiles (no masks: every window cell holds a node with m > 0).
+// Five waves per SIMD fit without spills.
+__global__ void __launch_bounds__(kBlock, 5) k_force_t68(StepArgs a, StepState st) {
   __shared__ Tile68Shared sh;
   const TileArgs &T = a.tile;
-  const int32_t t = tile_of_block(blockIdx.x, T.n_tiles);
-  if (t >= T.n_tiles) return;
+  const int32_t q = tile_of_block(blockIdx.x, T.n_full);
+  if (q >= T.n_full) return;
+  const int32_t t = T.full_list[q];
   const int32_t *nbr = T.tile_nbr + 9 * t;
-  const bool full = T.tile_full[t] != 0;
   const double lm = a.lam - a.mu, mu4 = 4.0 * a.mu;
 #pragma unroll
-  for (int q = 0; q < 3; q++) {
-    const int32_t c = threadIdx.x + q * kBlock;
+  for (int r = 0; r < 3; r++) {
+    const int32_t c = threadIdx.x + r * kBlock;
     if (c < kCells68) {
       const int32_t s = window_slot68(nbr, c);
-      double2 u = make_double2(0.0, 0.0), ab = make_double2(0.0, 0.0);
-      if (s >= 0 && (full || a.kind[s] != 0)) {
-        const double c2 = a.c2m[s];  // 0 when m <= 0: the bond is skipped (src/main.cpp:852)
-        u = make_double2(st.px[s], st.py[s]);
-        ab = make_double2(lm * c2 * st.theta[s], mu4 * c2);
-      }
-      sh.fo.u[c] = u;
-      sh.fo.ab[c] = ab;
+      const double c2 = a.c2m[s];
+      sh.fo.u[c] = make_double2(st.px[s], st.py[s]);
+      sh.fo.ab[c] = make_double2(lm * c2 * st.theta[s], mu4 * c2);
     }
   }
   __syncthreads();
@@ -682,8 +679,7 @@ __global__ void __launch_bounds__(kBlock) k_force_t68(StepArgs a, StepState st) 
   const double2 ui = sh.fo.u[c];
   const double2 abi = sh.fo.ab[c];
   double F0 = 0.0, F1 = 0.0;
-  if (full) force_sum68<true>(sh.fo.u, sh.fo.ab, c, ui, abi, T, &F0, &F1);
-  else if (abi.y != 0.0) force_sum68<false>(sh.fo.u, sh.fo.ab, c, ui, abi, T, &F0, &F1);
+  force_sum68<true>(sh.fo.u, sh.fo.ab, c, ui, abi, T, &F0, &F1);
   const double inv_rho = 1.0 / a.rho;
   const double a1x = F0 * inv_rho, a1y = F1 * inv_rho;
   const double h = 0.5 * a.dt;
@@ -705,11 +701,14 @@ __global__ void __launch_bounds__(kBlock) k_force_abc(StepArgs a, StepState st) 
   if (row < a.n_abc) abc_force_row<EXACT>(a, st, row, lane, &sabc[wid]);
 }
 
-__global__ void __launch_bounds__(kBlock) k_force_tile(StepArgs a, StepState st) {
+// Generic force kernel over a list of tiles (all tiles when list is null).
+__global__ void __launch_bounds__(kBlock) k_force_tile(StepArgs a, StepState st,
+                                                       const int32_t *list, int32_t count) {
   __shared__ TileShared sh;
   const TileArgs &T = a.tile;
-  const int32_t t = tile_of_block(blockIdx.x, T.n_tiles);
-  if (t >= T.n_tiles) return;
+  const int32_t q = tile_of_block(blockIdx.x, count);
+  if (q >= count) return;
+  const int32_t t = list ? list[q] : q;
   const int32_t txy = T.tile_xy[t];
   const int32_t gx0 = (txy >> 16) * kTile - T.halo, gy0 = (txy & 0xffff) * kTile - T.halo;
   const int32_t W = T.win;
@@ -892,6 +891,75 @@ __global__ void __launch_bounds__(kBlock) k_diag(StepArgs a, const double *ux, c
   if (threadIdx.x == 0) partials[blockIdx.x] = d;
 }
 
+// Output block on the tile layout: the per-node terms of src/main.cpp:992-1043 with the strain
+// energy sum taken over the stencil on the LDS-staged window of U^{n+1}.
+__global__ void __launch_bounds__(kBlock) k_diag_tile(StepArgs a, const double *ux,
+                                                      const double *uy, const double *vx,
+                                                      const double *vy, const double *ax,
+                                                      const double *ay, const double *theta,
+                                                      const int32_t *perm, double exclude_r,
+                                                      DiagPartial *partials) {
+  __shared__ DiagPartial sh[kBlock];
+  __shared__ double2 su[kMaxWin * kMaxWin];
+  __shared__ int8_t sok[kMaxWin * kMaxWin];
+  const TileArgs &T = a.tile;
+  const int32_t t = blockIdx.x;
+  const int32_t txy = T.tile_xy[t];
+  const int32_t gx0 = (txy >> 16) * kTile - T.halo, gy0 = (txy & 0xffff) * kTile - T.halo;
+  const int32_t W = T.win;
+  for (int32_t c = threadIdx.x; c < W * W; c += kBlock) {
+    const int32_t s = lattice_slot(T, gx0 + c / W, gy0 + c % W);
+    const bool ok = s >= 0 && a.kind[s] != 0;
+    su[c] = ok ? make_double2(ux[s], uy[s]) : make_double2(0.0, 0.0);
+    sok[c] = ok;
+  }
+  __syncthreads();
+  DiagPartial d;
+  diag_identity(d);
+  const int32_t i = t * (kTile * kTile) + threadIdx.x;
+  const int8_t kd = a.kind[i];
+  if (kd != 0) {
+    const double x = a.x[i], y = a.y[i];
+    const double r = sqrt(x * x + y * y);
+    const double uxi = ux[i], uyi = uy[i], vxi = vx[i], vyi = vy[i];
+    const double u_mag = sqrt(uxi * uxi + uyi * uyi);
+    const double v_mag = sqrt(vxi * vxi + vyi * vyi);
+    const double a_mag = sqrt(ax[i] * ax[i] + ay[i] * ay[i]);
+    const int64_t orig = perm[i];
+    if (u_mag > 0.0) { d.max_u = u_mag; d.max_u_orig = orig; }
+    if (v_mag > 0.0) { d.max_v = v_mag; d.max_v_orig = orig; }
+    d.max_a = a_mag;
+    const double th = theta[i];
+    d.th_min = th;
+    d.th_max = th;
+    if (kd == 2) {
+      d.ub = uxi * uxi + uyi * uyi;
+      d.vb = vxi * vxi + vyi * vyi;
+    }
+    if (!(exclude_r > 0.0 && r > exclude_r)) {
+      d.px = a.rho * vxi * a.dx * a.dy;
+      d.py = a.rho * vyi * a.dx * a.dy;
+      const double ek = 0.5 * a.rho * (vxi * vxi + vyi * vyi) * a.dx * a.dy;
+      d.ek = ek;
+      const int32_t c = (threadIdx.x / kTile + T.halo) * W + (threadIdx.x % kTile + T.halo);
+      double acc = 0.0;
+      for (int32_t k = 0; k < T.nt; k++) {
+        const int32_t cj = c + T.delta[k];
+        const double hx = T.hx[k], hy = T.hy[k], g2 = T.g2[k], xin = T.xi[k];
+        const double2 uj = su[cj];
+        const double ed = (uj.x - uxi) * hx + (uj.y - uyi) * hy - th * xin * 0.5;
+        acc += sok[cj] ? ed * ed * g2 : 0.0;
+      }
+      const double W_i = 0.5 * a.kappa * th * th + (4.0 * a.mu / a.m[i]) * acc;
+      const double ep = W_i * a.dx * a.dy;
+      d.ep = ep;
+      d.energy = ek + ep;
+    }
+  }
+  block_reduce_diag(d, sh);
+  if (threadIdx.x == 0) partials[t] = d;
+}
+
 __global__ void __launch_bounds__(kBlock) k_diag_final(const DiagPartial *partials, int32_t np,
                                                        DiagPartial *out) {
   __shared__ DiagPartial sh[kBlock];
@@ -940,12 +1008,18 @@ hipError_t launch_idw(const StepArgs &a, const StepState &st, hipStream_t s) {
   return hipGetLastError();
 }
 
+static inline int32_t list_blocks(int32_t count) { return (count + 7) / 8 * 8; }
+
 hipError_t launch_force(const StepArgs &a, const StepState &st, bool fast, hipStream_t s) {
   if (fast) {
-    if (a.tile.stencil68 & 2)
-      hipLaunchKernelGGL(k_force_t68, dim3(tile_blocks(a)), dim3(kBlock), 0, s, a, st);
-    else
-      hipLaunchKernelGGL(k_force_tile, dim3(tile_blocks(a)), dim3(kBlock), 0, s, a, st);
+    if (a.tile.stencil68 & 2) {
+      if (a.tile.n_full > 0)
+        hipLaunchKernelGGL(k_force_t68, dim3(list_blocks(a.tile.n_full)), dim3(kBlock), 0, s, a,
+                           st);
+    } else {
+      hipLaunchKernelGGL(k_force_tile, dim3(tile_blocks(a)), dim3(kBlock), 0, s, a, st,
+                         (const int32_t *)nullptr, a.tile.n_tiles);
+    }
   } else {
     const int32_t nb = blocks_for(a.n);
     hipLaunchKernelGGL(k_force_exact, dim3(nb + abc_blocks(a)), dim3(kBlock), 0, s, a, st, nb);
@@ -954,6 +1028,12 @@ hipError_t launch_force(const StepArgs &a, const StepState &st, bool fast, hipSt
 }
 
 hipError_t launch_force_abc(const StepArgs &a, const StepState &st, hipStream_t s) {
+  if ((a.tile.stencil68 & 2) && a.tile.n_part > 0) {
+    hipLaunchKernelGGL(k_force_tile, dim3(list_blocks(a.tile.n_part)), dim3(kBlock), 0, s, a, st,
+                       a.tile.part_list, a.tile.n_part);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
   if (abc_blocks(a) == 0) return hipSuccess;
   hipLaunchKernelGGL(k_force_abc<false>, dim3(abc_blocks(a)), dim3(kBlock), 0, s, a, st);
   return hipGetLastError();
@@ -994,6 +1074,18 @@ hipError_t launch_diag(const StepArgs &a, const double *ux, const double *uy, co
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(kBlock), 0, s, partials, n_partials, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_diag_tile(const StepArgs &a, const double *ux, const double *uy,
+                            const double *vx, const double *vy, const double *ax, const double *ay,
+                            const double *theta, const int32_t *perm, double exclude_r,
+                            DiagPartial *partials, DiagPartial *out, hipStream_t s) {
+  hipLaunchKernelGGL(k_diag_tile, dim3(a.tile.n_tiles), dim3(kBlock), 0, s, a, ux, uy, vx, vy, ax,
+                     ay, theta, perm, exclude_r, partials);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(kBlock), 0, s, partials, a.tile.n_tiles, out);
   return hipGetLastError();
 }
 

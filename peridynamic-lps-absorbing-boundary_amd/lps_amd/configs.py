@@ -39,6 +39,21 @@ WORKLOADS = {
 }
 
 
+_ARRAYS = ("nodes", "data2", "bctypes", "bcvalues", "ic", "icd", "abc", "clouds", "Gu", "Gv", "Gth")
+
+
+def save(p: pp.Problem, path: str) -> None:
+    """Binary copy of a generated problem (numpy .npz, no pickled objects)."""
+    import numpy as np
+    np.savez(path, steps=np.int64(p.steps), **{k: getattr(p, k) for k in _ARRAYS})
+
+
+def load(path: str) -> pp.Problem:
+    import numpy as np
+    z = np.load(path, allow_pickle=False)
+    return pp.Problem(steps=int(z["steps"]), **{k: z[k] for k in _ARRAYS})
+
+
 def make(name: str, workers: Optional[int] = None, steps: Optional[int] = None) -> pp.Problem:
     w = WORKLOADS[name]
     p = pp.make_problem(w.R, w.nu, pulses=[w.pulse], steps=steps or w.steps, dt=w.dt,

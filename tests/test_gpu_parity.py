@@ -124,13 +124,22 @@ def test_lattice_inputs_take_the_tile_path_and_others_fall_back():
     assert abs(recs[-1]["energy"] - orecs[-1]["energy"]) <= 1e-13 * abs(orecs[-1]["energy"])
 
 
-def test_tile_path_matches_oracle_on_a_larger_domain():
-    """R = 40 mm (20k nodes, 89 tiles incl. partial ones), nu = 0.3, off-centre pulse, 600 steps."""
-    say("tile R=40")
-    prob = pp.make_problem(40.0, 0.3, steps=600, dt=1.7e-8, dk_factor=0.7,
+@pytest.fixture(scope="module")
+def r40_problem():
+    return pp.make_problem(40.0, 0.3, steps=600, dt=1.7e-8, dk_factor=0.7,
                            pulses=[pp.Pulse(8.0, -5.0, 12.0, 1.0, (1.0, 1.0))], workers=8)
-    with solver.Solver(prob) as s:
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2])
+def test_tile_path_matches_oracle_on_a_larger_domain(variant, r40_problem):
+    """R = 40 mm (20k nodes, 121 tiles, full and partial ones), nu = 0.3, off-centre pulse,
+    600 steps; autotuned, generic and unrolled node kernels."""
+    say("tile R=40 variant", variant)
+    prob = r40_problem
+    with solver.Solver(prob, variant=variant) as s:
         assert s.path.startswith("tile")
+        if variant == 2:
+            assert "dilatation unrolled, force unrolled" in s.path, s.path
         recs = s.run(600)
         f = s.fetch()
     o = pyoracle.Oracle(prob)
