@@ -40,6 +40,9 @@ def main():
     ap.add_argument("--exact", action="store_true")
     ap.add_argument("--variant", type=int, default=0,
                     help="fast-path node kernels: 0 autotune, 1 generic tile, 2 unrolled stencil")
+    ap.add_argument("--serial", action="store_true",
+                    help="run the ABC rows' kernels on the main stream instead of beside the "
+                         "tile kernels on a second stream")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=30)
     ap.add_argument("--workers", type=int, default=None)
@@ -82,7 +85,8 @@ def main():
     log(f"[bench] N={prob.N} C={prob.C} dt={prob.dt:.3e} ready in {time.time() - t:.1f}s")
 
     t = time.time()
-    s = solver.Solver(prob, exact=args.exact, use_graph=True, variant=args.variant)
+    s = solver.Solver(prob, exact=args.exact, use_graph=True, variant=args.variant,
+                      concurrent=not args.serial)
     rp, _ = s.horizons()
     path = s.path
     nnz = int(rp[-1]) - prob.N  # bonds, self excluded

@@ -61,8 +61,11 @@ typedef struct lps_options {
   int32_t device;        /* HIP device ordinal */
   double horizon_radius; /* > 0: horizon search radius; 0: delta + dx/2 + 1e-5 (src/main.cpp:352) */
   int32_t variant;       /* fast-path node kernels: 0 autotune at creation (time each candidate
-                            on the problem, keep the fastest), 1 generic tile kernels, 2 unrolled
-                            68-bond stencil (when the stencil applies) */
+                            on the problem, keep the fastest per pass), 1 generic tile kernels,
+                            2 unrolled 68-bond stencil, 3 register-blocked 68-bond stencil (2 and
+                            3 when the 68-bond stencil applies, else generic) */
+  int32_t concurrent;    /* fast path: 1 runs the ABC rows' kernels (G_theta, G_u, G_v) on a
+                            second stream beside the tile kernels; 0 on the one stream */
 } lps_options;
 
 /* One output record: the quantities src/main.cpp:986-1062 computes every 30 steps. */
@@ -117,10 +120,11 @@ int64_t lps_step_index(const lps_ctx *ctx);
 double lps_m_ref(const lps_ctx *ctx);
 
 /* Average device time (ms) per step of the kernels launched by lps_step since timing was switched
- * on, measured with HIP events on the context's stream: out[0] dilatation pass (with the G_theta /
- * IDW ABC rows), out[1] force pass of the nodes (on the exact path with the ABC rows), out[2]
- * on the fast path the time the ABC rows' force/G_u/G_v kernel, which runs beside the node force
- * kernel on a second stream, outlasts it (0 on the exact path), out[3] whole step.
+ * on, measured with HIP events on the context's stream: out[0] dilatation pass (exact path with
+ * the ABC rows; fast path the tile kernels, and the IDW override), out[1] the force pass's main
+ * kernel (exact path: nodes and ABC rows; fast path: the selected tile kernel), out[2] the rest of
+ * the step (fast path: the partial tiles' force kernel, the wait for the ABC rows' G_u / G_v, which
+ * run on a second stream beside the tile kernels, and the ABC prediction), out[3] whole step.
  * Timing is enabled with lps_set_timing(ctx, 1); timed steps run without the HIP graph. */
 int32_t lps_set_timing(lps_ctx *ctx, int32_t on);
 int32_t lps_pass_times(lps_ctx *ctx, double *out4);

@@ -32,7 +32,7 @@ struct lps_ctx {
   lps_options opt{};
   hipStream_t stream = nullptr;
   hipStream_t stream2 = nullptr;  // the ABC rows' force kernel beside the node force kernel
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  hipEvent_t ev_fork[2] = {nullptr, nullptr}, ev_join[2] = {nullptr, nullptr};
   std::string err, path;
   int32_t n = 0, n_slots = 0, n_abc = 0, M = 0;
   double E = 0, nu = 0, rho = 0, dt = 0, delta = 0, dx = 0, dy = 0;
@@ -170,6 +170,7 @@ void lps_default_options(lps_options *opt) {
   opt->device = 0;
   opt->horizon_radius = 0.0;
   opt->variant = 0;
+  opt->concurrent = 1;
 }
 
 const char *lps_global_error(void) { return g_create_error.c_str(); }
@@ -187,8 +188,10 @@ void lps_destroy(lps_ctx *ctx) {
   free_horizons(&ctx->hb);
   if (ctx->hout) (void)hipHostFree(ctx->hout);
   if (ctx->stream2) (void)hipStreamSynchronize(ctx->stream2);
-  if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
-  if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
+  for (int q = 0; q < 2; q++) {
+    if (ctx->ev_fork[q]) (void)hipEventDestroy(ctx->ev_fork[q]);
+    if (ctx->ev_join[q]) (void)hipEventDestroy(ctx->ev_join[q]);
+  }
   if (ctx->stream2) (void)hipStreamDestroy(ctx->stream2);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -394,7 +397,7 @@ static int32_t create_impl(lps_ctx *c, const lps_problem *p) {
       tg2[k] = bt * c->dx * c->dy;
       txi[k] = d;
     }
-    // 3 x 3 neighbour tiles and "full" windows (every cell a node with m > 0)
+    // 3 x 3 neighbour tiles and "full" windows (every cell a node with m == m_ref)
     const int32_t ntl = c->tile.n_tiles;
     std::vector<int32_t> nbr((size_t)ntl * 9, -1);
     std::vector<int8_t> full(ntl, 0);
@@ -414,7 +417,7 @@ static int32_t create_impl(lps_ctx *c, const lps_problem *p) {
         const int32_t tt = nbr[(size_t)t * 9 + ax * 3 + ay];
         if (tt < 0) { f = false; break; }
         const int32_t sl = tt * kTile * kTile + (rx - (ax - 1) * kTile) * kTile + (ry - (ay - 1) * kTile);
-        f = kind[sl] != 0 && hm[sl] > 0.0;
+        f = kind[sl] != 0 && hm[sl] == c->m_ref;
       }
       full[t] = f;
     }
@@ -425,14 +428,26 @@ static int32_t create_impl(lps_ctx *c, const lps_problem *p) {
       s68 = r2 > 0 && r2 <= 20;
     }
     c->stencil68_ok = s68;
-    c->tile.stencil68 = (s68 && c->opt.variant != 1) ? 3 : 0;
+    c->tile.theta_kernel = c->tile.force_kernel = kGeneric;
     if (s68) {
+      const double C = 2.0 / c->m_ref;
       for (int32_t q = 0; q < 12; q++) {
         const double xi = c->dx * std::sqrt((double)kStencilR2[q]);
         const double bt = beta_host(xi, c->delta, c->dx);
         const double cc = c->dx / xi;
         c->tile.c1[q] = xi * bt * c->dx * c->dy * cc;
         c->tile.c2[q] = bt * c->dx * c->dy * cc * cc;
+        int32_t ra = 0, rb = 0;  // class representative a >= b >= 0
+        for (int32_t aa = 0; aa <= 4; aa++)
+          for (int32_t bb = 0; bb <= aa; bb++)
+            if (aa * aa + bb * bb == kStencilR2[q]) { ra = aa; rb = bb; }
+        const double ms[2] = {(double)ra, (double)rb};
+        const double ps[3] = {(double)(ra * ra), (double)(ra * rb), (double)(rb * rb)};
+        for (int32_t z = 0; z < 2; z++) {
+          c->tile.wt[q][z] = C * c->tile.c1[q] * ms[z];
+          c->tile.wa[q][z] = (c->lam - c->mu) * C * c->tile.c1[q] * ms[z];
+        }
+        for (int32_t z = 0; z < 3; z++) c->tile.wb[q][z] = 8.0 * c->mu * C * c->tile.c2[q] * ps[z];
       }
     }
     int32_t *d_txy, *d_tmap, *d_delta, *d_nbr;
@@ -471,7 +486,12 @@ static int32_t create_impl(lps_ctx *c, const lps_problem *p) {
     int32_t *d_mis;
     CTX_TRY(c, c->alloc(&d_mis, 1));
     CTX_TRY(c, hipMemsetAsync(d_mis, 0, sizeof(int32_t), s));
-    CTX_TRY(c, launch_tile_check(c->args(), d_mis, s));
+    const int32_t E = 2 * K + 1;
+    std::vector<int16_t> bond_of((size_t)E * E, -1);
+    for (int32_t k = 0; k < nt; k++) bond_of[(size_t)(offs[k].ox + K) * E + offs[k].oy + K] = (int16_t)k;
+    int16_t *d_bond_of;
+    CTX_TRY(c, c->upload(&d_bond_of, bond_of));
+    CTX_TRY(c, launch_tile_check(c->args(), d_bond_of, d_mis, s));
     int32_t mis = 0;
     CTX_TRY(c, hipMemcpyAsync(&mis, d_mis, sizeof(int32_t), hipMemcpyDeviceToHost, s));
     CTX_TRY(c, hipStreamSynchronize(s));
@@ -605,8 +625,10 @@ int32_t lps_create(const lps_problem *problem, const lps_options *options, lps_c
   // both streams at the default priority
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking);
-  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming);
-  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming);
+  for (int q = 0; q < 2; q++) {
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_fork[q], hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_join[q], hipEventDisableTiming);
+  }
   if (e != hipSuccess) {
     g_create_error = std::string("HIP setup failed: ") + hipGetErrorString(e);
     delete c;
@@ -623,33 +645,40 @@ int32_t lps_create(const lps_problem *problem, const lps_options *options, lps_c
   return LPS_OK;
 }
 
-// Fast path: pick the node kernels of each pass. With variant 0 both candidates (generic tile
-// kernels, unrolled 68-bond kernels) run a few timed steps on the problem itself and each pass
-// keeps its faster kernel; the state is reset afterwards.
+// Fast path: pick the node kernels of each pass. With variant 0 every candidate (generic tile
+// kernels, unrolled 68-bond kernels, register-blocked 68-bond kernels) runs a few timed steps on
+// the problem itself and each pass keeps its fastest; the state is reset afterwards.
 static int32_t choose_variant(lps_ctx *c) {
-  int32_t mode;  // bit 0: unrolled dilatation kernel, bit 1: unrolled force kernel
   const bool tune = c->stencil68_ok && c->opt.variant == 0;
-  if (!c->stencil68_ok || c->opt.variant == 1) mode = 0;
-  else if (c->opt.variant == 2) mode = 3;
-  else {
-    double t_gen[4] = {0, 0, 0, 0}, t_unr[4] = {0, 0, 0, 0};
-    for (int32_t m : {0, 3}) {
-      c->tile.stencil68 = m;
+  int32_t th = kGeneric, fo = kGeneric;
+  if (!c->stencil68_ok || c->opt.variant == 1) {
+    th = fo = kGeneric;
+  } else if (c->opt.variant == 2) {
+    th = fo = kUnrolled;
+  } else if (c->opt.variant == 3) {
+    th = fo = kBlocked;
+  } else {
+    double best_th = 1e300, best_fo = 1e300;
+    for (int32_t m : {kGeneric, kUnrolled, kBlocked}) {
+      double t[4] = {0, 0, 0, 0};
+      c->tile.theta_kernel = c->tile.force_kernel = m;
       int32_t rc = lps_set_timing(c, 1);
       if (rc == LPS_OK) rc = lps_step(c, 2);
       if (rc == LPS_OK) rc = lps_set_timing(c, 1);
       if (rc == LPS_OK) rc = lps_step(c, 6);
-      if (rc == LPS_OK) rc = lps_pass_times(c, m ? t_unr : t_gen);
+      if (rc == LPS_OK) rc = lps_pass_times(c, t);
       if (rc == LPS_OK) rc = lps_set_timing(c, 0);
       if (rc == LPS_OK) rc = reset_state(c);
       if (rc != LPS_OK) return rc;
+      if (t[0] < best_th) { best_th = t[0]; th = m; }
+      if (t[1] + t[2] < best_fo) { best_fo = t[1] + t[2]; fo = m; }
     }
-    mode = (t_unr[0] < t_gen[0] ? 1 : 0) | (t_unr[1] < t_gen[1] ? 2 : 0);
   }
-  c->tile.stencil68 = mode;
-  const char *names[4] = {"generic", "unrolled"};
-  c->path = std::string("tile kernels: dilatation ") + names[mode & 1] + ", force " +
-            names[(mode >> 1) & 1] + (tune ? " [autotuned]" : "") + c->path;
+  c->tile.theta_kernel = th;
+  c->tile.force_kernel = fo;
+  const char *names[3] = {"generic", "unrolled", "blocked"};
+  c->path = std::string("tile kernels: dilatation ") + names[th] + ", force " + names[fo] +
+            (tune ? " [autotuned]" : "") + c->path;
   return LPS_OK;
 }
 
@@ -664,24 +693,34 @@ static hipError_t step_once(lps_ctx *c, bool record_events) {
     ev = &c->ev[kEv * (size_t)c->ev_used];
     if ((e = hipEventRecord(ev[0], c->stream)) != hipSuccess) return e;
   }
-  e = launch_theta(a, st, c->fast, c->stream);
-  if (e == hipSuccess && a.n_abc > 0 && a.theta_method == LPS_THETA_IDW)
-    e = launch_idw(a, st, c->stream);
-  if (e != hipSuccess) return e;
-  if (record_events && (e = hipEventRecord(ev[1], c->stream)) != hipSuccess) return e;
-  // fast path: the ABC rows' force/G_u/G_v kernel runs beside the node force kernel on the second
-  // stream (both only read the dilatation pass's output and write disjoint slots)
-  const bool fork = c->fast && (a.n_abc > 0 || ((a.tile.stencil68 & 2) && a.tile.n_part > 0));
+  // fast path: the ABC rows' G_theta and G_u / G_v read only the previous steps' state and run on
+  // the second stream beside the tile passes; the force pass waits for G_theta (ev_join[0]), the
+  // ABC prediction for G_u / G_v (ev_join[1])
+  const bool abc = c->fast && a.n_abc > 0;
+  const bool fork = abc && c->opt.concurrent;
   if (fork) {
-    if ((e = hipEventRecord(c->ev_fork, c->stream)) != hipSuccess) return e;
-    if ((e = hipStreamWaitEvent(c->stream2, c->ev_fork, 0)) != hipSuccess) return e;
-    if ((e = launch_force_abc(a, st, c->stream2)) != hipSuccess) return e;
-    if ((e = hipEventRecord(c->ev_join, c->stream2)) != hipSuccess) return e;
+    if ((e = hipEventRecord(c->ev_fork[0], c->stream)) != hipSuccess) return e;
+    if ((e = hipStreamWaitEvent(c->stream2, c->ev_fork[0], 0)) != hipSuccess) return e;
+    if ((e = launch_abc_theta(a, st, c->stream2)) != hipSuccess) return e;
+    if ((e = hipEventRecord(c->ev_join[0], c->stream2)) != hipSuccess) return e;
+    if ((e = launch_abc_uv(a, st, c->stream2)) != hipSuccess) return e;
+    if ((e = hipEventRecord(c->ev_join[1], c->stream2)) != hipSuccess) return e;
+  } else if (abc && (e = launch_abc_theta(a, st, c->stream)) != hipSuccess) {
+    return e;
   }
-  e = launch_force(a, st, c->fast, c->stream);
-  if (e != hipSuccess) return e;
+  if ((e = launch_theta(a, st, c->fast, c->stream)) != hipSuccess) return e;
+  if (fork && (e = hipStreamWaitEvent(c->stream, c->ev_join[0], 0)) != hipSuccess) return e;
+  if (a.n_abc > 0 && a.theta_method == LPS_THETA_IDW && (e = launch_idw(a, st, c->stream)) != hipSuccess)
+    return e;
+  if (record_events && (e = hipEventRecord(ev[1], c->stream)) != hipSuccess) return e;
+  if ((e = launch_force(a, st, c->fast, c->stream)) != hipSuccess) return e;
   if (record_events && (e = hipEventRecord(ev[2], c->stream)) != hipSuccess) return e;
-  if (fork && (e = hipStreamWaitEvent(c->stream, c->ev_join, 0)) != hipSuccess) return e;
+  if (c->fast) {
+    if ((e = launch_force_part(a, st, c->stream)) != hipSuccess) return e;
+    if (fork && (e = hipStreamWaitEvent(c->stream, c->ev_join[1], 0)) != hipSuccess) return e;
+    if (abc && !fork && (e = launch_abc_uv(a, st, c->stream)) != hipSuccess) return e;
+    if (abc && (e = launch_abc_predict(a, st, c->stream)) != hipSuccess) return e;
+  }
   if (record_events) {
     if ((e = hipEventRecord(ev[3], c->stream)) != hipSuccess) return e;
     c->ev_used++;

@@ -35,7 +35,7 @@ struct TileArgs {
   const int32_t *tile_xy;          // n_tiles: tx << 16 | ty
   const int32_t *tile_map;         // ntx * nty -> tile index or -1
   const int32_t *tile_nbr;         // n_tiles x 9: the 3 x 3 tiles around a tile (-1 absent)
-  const int8_t *tile_full;         // n_tiles: every window cell holds a node with m > 0
+  const int8_t *tile_full;         // n_tiles: every window cell holds a node with m == m_ref
   const int32_t *full_list;        // the full tiles, in tile order
   const int32_t *part_list;        // the other tiles
   int32_t n_full, n_part;
@@ -46,9 +46,20 @@ struct TileArgs {
   // The reference stencil (dx == dy, delta = 4 dx: the 68 offsets 0 < ox^2 + oy^2 <= 20) has 12
   // distance classes; with c = dx / |xi|, class constants c1 = |xi| beta dV c and
   // c2 = beta dV c^2 turn the bond terms into integer multiples (xi_hat = (ox, oy) c).
-  int32_t stencil68;               // bit 0: unrolled dilatation kernel, bit 1: unrolled force kernel
+  // Node kernel of each pass: kGeneric (any stencil, table-driven), kUnrolled (68 bonds, one node
+  // per thread), kBlocked (68 bonds, four nodes per thread on the full tiles).
+  int32_t theta_kernel, force_kernel;
   double c1[12], c2[12];
+  // Full tiles, where every window cell holds a node with m == m_ref (C = 2 / m_ref): class
+  // constants folded with the material, for the class representative (a, b), a >= b >= 0,
+  // a^2 + b^2 = r^2 of the class:
+  //   wt[cls][0 / 1]     = C c1 {a, b}                    (dilatation)
+  //   wa[cls][0 / 1]     = (lambda - mu) C c1 {a, b}      (force, theta_j terms)
+  //   wb[cls][0 / 1 / 2] = 8 mu C c2 {a^2, a b, b^2}      (force, bond-extension terms)
+  double wt[12][2], wa[12][2], wb[12][3];
 };
+
+enum NodeKernel : int32_t { kGeneric = 0, kUnrolled = 1, kBlocked = 2 };
 
 // Distance classes of the 68-bond stencil: ox^2 + oy^2.
 constexpr int kStencilR2[12] = {1, 2, 4, 5, 8, 9, 10, 13, 16, 17, 18, 20};
@@ -101,14 +112,22 @@ struct DiagPartial {
 hipError_t launch_predict(const StepArgs &a, const double *ux, const double *uy, const double *vx,
                           const double *vy, const double *ax, const double *ay, double *px,
                           double *py, hipStream_t s);
+// Dilatation pass of the nodes: exact path all nodes and the ABC rows; fast path the tiles (the
+// selected kernel on the full tiles, then the unrolled masked kernel on the others).
 hipError_t launch_theta(const StepArgs &a, const StepState &st, bool fast, hipStream_t s);
 hipError_t launch_idw(const StepArgs &a, const StepState &st, hipStream_t s);
-// Force pass of the nodes; on the exact path it also handles the ABC rows, on the fast path
-// launch_force_abc does (after it, same stream).
+// Force pass: exact path all nodes and ABC rows. Fast path: launch_force = the selected kernel
+// (force at every node, ABC nodes included, with the Verlet epilogue), launch_force_part = the
+// partial tiles when the selected kernel takes only the full ones.
 hipError_t launch_force(const StepArgs &a, const StepState &st, bool fast, hipStream_t s);
-// Fast path, second stream: the ABC rows and, when the unrolled force kernel takes the full tiles,
-// the partial tiles.
-hipError_t launch_force_abc(const StepArgs &a, const StepState &st, hipStream_t s);
+hipError_t launch_force_part(const StepArgs &a, const StepState &st, hipStream_t s);
+// Fast path, ABC rows. launch_abc_theta (G_theta / direct IDW value) reads only the theta history
+// and the prediction, launch_abc_uv (G_u, G_v) only the previous steps' state: both may run on a
+// second stream beside the tile passes; the force pass needs abc_theta, launch_abc_predict (the
+// next prediction at the ABC nodes) needs abc_uv and the force pass.
+hipError_t launch_abc_theta(const StepArgs &a, const StepState &st, hipStream_t s);
+hipError_t launch_abc_uv(const StepArgs &a, const StepState &st, hipStream_t s);
+hipError_t launch_abc_predict(const StepArgs &a, const StepState &st, hipStream_t s);
 hipError_t launch_theta_field(const StepArgs &a, const double *ux, const double *uy, double *theta,
                               hipStream_t s);
 hipError_t launch_weighted_volume(const StepArgs &a, double *m, hipStream_t s);
@@ -124,8 +143,11 @@ hipError_t launch_diag_tile(const StepArgs &a, const double *ux, const double *u
                             const double *theta, const int32_t *perm, double exclude_r,
                             DiagPartial *partials, DiagPartial *out, hipStream_t s);
 // Tile path check: for every node slot, the stencil cells that hold a node are exactly the set of
-// its CSR row (self excluded). Adds the number of rows that differ to *d_mismatch.
-hipError_t launch_tile_check(const StepArgs &a, int32_t *d_mismatch, hipStream_t s);
+// its CSR row (self excluded). bond_of: (2 halo + 1)^2 entries, the bond of offset (ox, oy) at
+// (ox + halo) (2 halo + 1) + oy + halo, -1 off the stencil. Adds the number of rows that differ to
+// *d_mismatch.
+hipError_t launch_tile_check(const StepArgs &a, const int16_t *bond_of, int32_t *d_mismatch,
+                             hipStream_t s);
 
 // Horizon build (lps_neighbors.hip) over slots: the CSR and SELL of the nodes' horizons.
 struct HorizonBuild {

@@ -19,6 +19,8 @@
 //          at a fixed window offset; the bond geometry (unit vector, |xi| beta dV, beta dV) is a
 //          wave-uniform table entry (scalar loads). No index stream, no square root or division
 //          per bond, FMA on.
+#include <utility>
+
 #include "lps_internal.h"
 
 namespace lps {
@@ -478,16 +480,9 @@ union TileShared {
   AbcScratch abc[kWavesPerBlock];
 };
 
-__global__ void __launch_bounds__(kBlock) k_theta_tile(StepArgs a, StepState st,
-                                                       int32_t n_tile_blocks) {
+__global__ void __launch_bounds__(kBlock) k_theta_tile(StepArgs a, StepState st) {
   __shared__ TileShared sh;
   const TileArgs &T = a.tile;
-  if ((int32_t)blockIdx.x >= n_tile_blocks) {  // ABC rows
-    const int32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    const int32_t row = (blockIdx.x - n_tile_blocks) * kWavesPerBlock + wid;
-    if (row < a.n_abc) abc_theta_row<false>(a, st, row, lane, &sh.abc[wid]);
-    return;
-  }
   const int32_t t = tile_of_block(blockIdx.x, T.n_tiles);
   if (t >= T.n_tiles) return;
   const int32_t txy = T.tile_xy[t];
@@ -577,18 +572,14 @@ union Tile68Shared {
   AbcScratch abc[kWavesPerBlock];
 };
 
+// Dilatation over the tiles of list (all tiles when list is null), one node per thread.
 __global__ void __launch_bounds__(kBlock) k_theta_t68(StepArgs a, StepState st,
-                                                      int32_t n_tile_blocks) {
+                                                      const int32_t *list, int32_t count) {
   __shared__ Tile68Shared sh;
   const TileArgs &T = a.tile;
-  if ((int32_t)blockIdx.x >= n_tile_blocks) {  // ABC rows
-    const int32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    const int32_t row = (blockIdx.x - n_tile_blocks) * kWavesPerBlock + wid;
-    if (row < a.n_abc) abc_theta_row<false>(a, st, row, lane, &sh.abc[wid]);
-    return;
-  }
-  const int32_t t = tile_of_block(blockIdx.x, T.n_tiles);
-  if (t >= T.n_tiles) return;
+  const int32_t q = tile_of_block(blockIdx.x, count);
+  if (q >= count) return;
+  const int32_t t = list ? list[q] : q;
   const int32_t *nbr = T.tile_nbr + 9 * t;
   const bool full = T.tile_full[t] != 0;
 #pragma unroll
@@ -652,14 +643,37 @@ __device__ __forceinline__ void force_sum68(const double2 *su, const double2 *sa
   *F1 = (fy[0] + fy[1]) + (fy[2] + fy[3]);
 }
 
-// Unrolled force kernel for the full tiles (no masks: every window cell holds a node with m > 0).
-// Five waves per SIMD fit without spills.
-__global__ void __launch_bounds__(kBlock, 5) k_force_t68(StepArgs a, StepState st) {
+// Force-pass epilogue of node slot i: acceleration, velocity-Verlet corrector and the next step's
+// prediction (src/main.cpp:874, 922-925, 725-728). At an ABC node (kind 2) only the acceleration:
+// G_u / G_v replace its displacement and velocity (k_abc_uv) and k_abc_predict then predicts it.
+// vi, ai: V^n and A^n of the node (read by the caller, possibly early).
+__device__ __forceinline__ void node_update(const StepArgs &a, const StepState &st, int32_t i,
+                                            int8_t kd, double2 ui, double2 vi, double2 ai,
+                                            double F0, double F1) {
+  const double inv_rho = 1.0 / a.rho;
+  const double a1x = F0 * inv_rho, a1y = F1 * inv_rho;
+  st.ax1[i] = a1x; st.ay1[i] = a1y;
+  if (kd != 1) return;
+  const double h = 0.5 * a.dt;
+  const double v1x = vi.x + h * (ai.x + a1x);
+  const double v1y = vi.y + h * (ai.y + a1y);
+  st.ux1[i] = ui.x; st.uy1[i] = ui.y;
+  st.vx1[i] = v1x; st.vy1[i] = v1y;
+  st.pnx[i] = ui.x + a.dt * (v1x + h * a1x);
+  st.pny[i] = ui.y + a.dt * (v1y + h * a1y);
+}
+
+// Unrolled force kernel over the tiles of list. FULL: the full tiles, no masks (every window cell
+// holds a node with m == m_ref); otherwise a bond to a cell without a node or with m <= 0 is
+// skipped, and a node with m <= 0 gets no force (src/main.cpp:852).
+template <bool FULL>
+__global__ void __launch_bounds__(kBlock, 5) k_force_t68(StepArgs a, StepState st,
+                                                         const int32_t *list, int32_t count) {
   __shared__ Tile68Shared sh;
   const TileArgs &T = a.tile;
-  const int32_t q = tile_of_block(blockIdx.x, T.n_full);
-  if (q >= T.n_full) return;
-  const int32_t t = T.full_list[q];
+  const int32_t q = tile_of_block(blockIdx.x, count);
+  if (q >= count) return;
+  const int32_t t = list[q];
   const int32_t *nbr = T.tile_nbr + 9 * t;
   const double lm = a.lam - a.mu, mu4 = 4.0 * a.mu;
 #pragma unroll
@@ -667,38 +681,508 @@ __global__ void __launch_bounds__(kBlock, 5) k_force_t68(StepArgs a, StepState s
     const int32_t c = threadIdx.x + r * kBlock;
     if (c < kCells68) {
       const int32_t s = window_slot68(nbr, c);
-      const double c2 = a.c2m[s];
-      sh.fo.u[c] = make_double2(st.px[s], st.py[s]);
-      sh.fo.ab[c] = make_double2(lm * c2 * st.theta[s], mu4 * c2);
+      double2 u = make_double2(0.0, 0.0), ab = make_double2(0.0, 0.0);
+      if (FULL || (s >= 0 && a.kind[s] != 0)) {
+        const double c2 = a.c2m[s];  // 0 when m <= 0
+        u = make_double2(st.px[s], st.py[s]);
+        ab = make_double2(lm * c2 * st.theta[s], mu4 * c2);
+      }
+      sh.fo.u[c] = u;
+      sh.fo.ab[c] = ab;
     }
   }
   __syncthreads();
   const int32_t i = t * (kTile * kTile) + threadIdx.x;
-  if (a.kind[i] != 1) return;
+  const int8_t kd = a.kind[i];
+  if (kd == 0) return;
   const int32_t c = (threadIdx.x / kTile + 4) * kW68 + (threadIdx.x % kTile + 4);
   const double2 ui = sh.fo.u[c];
   const double2 abi = sh.fo.ab[c];
   double F0 = 0.0, F1 = 0.0;
-  force_sum68<true>(sh.fo.u, sh.fo.ab, c, ui, abi, T, &F0, &F1);
-  const double inv_rho = 1.0 / a.rho;
-  const double a1x = F0 * inv_rho, a1y = F1 * inv_rho;
-  const double h = 0.5 * a.dt;
-  const double v1x = st.vx[i] + h * (st.ax[i] + a1x);
-  const double v1y = st.vy[i] + h * (st.ay[i] + a1y);
-  st.ax1[i] = a1x; st.ay1[i] = a1y;
-  st.ux1[i] = ui.x; st.uy1[i] = ui.y;
-  st.vx1[i] = v1x; st.vy1[i] = v1y;
-  st.pnx[i] = ui.x + a.dt * (v1x + h * a1x);
-  st.pny[i] = ui.y + a.dt * (v1y + h * a1y);
+  if (FULL || abi.y != 0.0) force_sum68<FULL>(sh.fo.u, sh.fo.ab, c, ui, abi, T, &F0, &F1);
+  node_update(a, st, i, kd, ui, make_double2(st.vx[i], st.vy[i]), make_double2(st.ax[i], st.ay[i]),
+              F0, F1);
 }
 
-// Force pass of the ABC rows alone (the tile kernels leave kind-2 slots to it).
-template <bool EXACT>
-__global__ void __launch_bounds__(kBlock) k_force_abc(StepArgs a, StepState st) {
+// ---------------------------------------------------------------------------------------------
+// Register-blocked passes over the full tiles (every window cell a node with m == m_ref).
+// One wavefront per tile; lane l owns the kRB nodes (4 (l / 16) + r, l % 16), r < kRB, of the
+// tile. The wavefront sweeps the window rows that these nodes reach: each row's cells are read
+// from LDS once and feed the bonds of every node of the lane that reaches the row, so a node
+// costs (9 kRB + 60) / kRB cell reads instead of 68.
+// On a full tile m_j = m_i = m_ref for every bond, so the bond term collapses to class weights
+// folded with the material (TileArgs::wt, wa, wb) and the theta_i part of the force,
+// theta_i sum_k c1_k xi_hat_k, vanishes by the stencil's symmetry:
+//   theta_i = sum_k C c1_k (u_j . o_k)
+//   F_i     = sum_k (lambda - mu) C c1_k theta_j o_k + 8 mu C c2_k ((u_j - u_i) . o_k) o_k
+// (o_k the integer offset, xi_hat = o_k dx / |xi|). The u_i term of the dilatation is dropped the
+// same way (sum_k c1_k o_k = 0); the force keeps u_j - u_i, whose difference of nearby values is
+// exact, since there the sum is a second difference.
+// ---------------------------------------------------------------------------------------------
+constexpr int kRB = 4;
+
+// The 24 x 24 window (halo 4) of a full tile in 9 wavefront-wide pieces: piece k of lane l is the
+// pair (slot, window cell). Pieces 0-3 are the tile itself (64 consecutive slots each), 4 / 5 the
+// rows above / below (the last / first 4 rows of the x-neighbours: 64 consecutive slots), 6 / 7 the
+// columns left / right (4 slots of each of 16 rows), 8 the four 4 x 4 corners. nb: the 3 x 3
+// neighbour tiles (wave-uniform; all present on a full tile, -1 where absent), nb[(dx + 1) * 3 +
+// dy + 1].
+struct WinCell {
+  int32_t slot, cell, tile;  // tile < 0: the cell lies in an absent tile (slot is not valid)
+};
+template <int K>
+__device__ __forceinline__ WinCell window_piece68(const int32_t *nb, int32_t l) {
+  constexpr int kT = kTile * kTile;
+  if constexpr (K < 4) {
+    return WinCell{nb[4] * kT + l + 64 * K, ((l >> 4) + 4 * K + 4) * kW68 + (l & 15) + 4, nb[4]};
+  } else if constexpr (K == 4) {
+    return WinCell{nb[1] * kT + 192 + l, (l >> 4) * kW68 + (l & 15) + 4, nb[1]};
+  } else if constexpr (K == 5) {
+    return WinCell{nb[7] * kT + l, (20 + (l >> 4)) * kW68 + (l & 15) + 4, nb[7]};
+  } else if constexpr (K == 6) {
+    return WinCell{nb[3] * kT + (l >> 2) * kTile + 12 + (l & 3), ((l >> 2) + 4) * kW68 + (l & 3),
+                   nb[3]};
+  } else if constexpr (K == 7) {
+    return WinCell{nb[5] * kT + (l >> 2) * kTile + (l & 3), ((l >> 2) + 4) * kW68 + 20 + (l & 3),
+                   nb[5]};
+  } else {
+    const int32_t hx = (l >> 5) & 1, hy = (l >> 4) & 1, crx = (l >> 2) & 3, cry = l & 3;
+    // corner tile nb[6 hx + 2 hy] as arithmetic on the four values (a select between them would be
+    // turned into an indexed load from a stack copy of nb)
+    const int32_t tq = nb[0] + hx * (nb[6] - nb[0]) + hy * (nb[2] - nb[0]) +
+                       hx * hy * (nb[8] - nb[6] - nb[2] + nb[0]);
+    return WinCell{tq * kT + ((hx ? 0 : 12) + crx) * kTile + (hy ? 0 : 12) + cry,
+                   ((hx ? 20 : 0) + crx) * kW68 + (hy ? 20 : 0) + cry, tq};
+  }
+}
+
+template <class F, int... K>
+__device__ __forceinline__ void fill_window68(const int32_t *nb, int32_t l, F &&f,
+                                              std::integer_sequence<int, K...>) {
+  (f(window_piece68<K>(nb, l)), ...);
+}
+
+struct ClassRep {
+  int a, b;
+};
+constexpr ClassRep class_rep(int cls) {
+  for (int a = 0; a <= 4; a++)
+    for (int b = 0; b <= a; b++)
+      if (a * a + b * b == kStencilR2[cls]) return ClassRep{a, b};
+  return ClassRep{-1, -1};
+}
+constexpr int iabs(int v) { return v < 0 ? -v : v; }
+// Weight slot of the multiplier |component| of a bond of class cls: 0 for a, 1 for b.
+constexpr int rep_slot(int cls, int m) { return class_rep(cls).a == m ? 0 : 1; }
+// Weight slot of the product p in {a^2, a b, b^2}.
+constexpr int prod_slot(int cls, int p) {
+  return p == class_rep(cls).a * class_rep(cls).a ? 0
+         : (p == class_rep(cls).a * class_rep(cls).b ? 1 : 2);
+}
+
+constexpr bool in_stencil68(int ox, int oy) {
+  return ox * ox + oy * oy > 0 && ox * ox + oy * oy <= 20;
+}
+
+// The sweep: rows D = -4 .. kRB + 3 (window row lx0 + 4 + D), columns OY = -4 .. 4 (window column
+// ly + 4 + OY). A cell is read once if a node r of the lane has the bond (D - r, OY); the pass
+// object S reads it (S::cell<D, OY>) and applies its bonds (S::bond<D - r, OY, r>). At the end of
+// each row S::end_row pins the accumulators.
+template <class S, int D, int OY, int... R>
+__device__ __forceinline__ void rb_cell(S &s, std::integer_sequence<int, R...>) {
+  if constexpr ((in_stencil68(D - R, OY) || ...)) s.template cell<D, OY>();
+}
+
+template <class S, int D, int... J>
+__device__ __forceinline__ void rb_row(S &s, std::integer_sequence<int, J...>) {
+  (rb_cell<S, D, J - 4>(s, std::make_integer_sequence<int, kRB>{}), ...);
+  s.end_row();
+}
+
+template <class S, int... E>
+__device__ __forceinline__ void rb_sweep(S &s, std::integer_sequence<int, E...>) {
+  (rb_row<S, E - 4>(s, std::make_integer_sequence<int, 9>{}), ...);
+}
+
+template <int OX, int OY, int R, class S>
+__device__ __forceinline__ void rb_bond_if(S &s) {
+  if constexpr (in_stencil68(OX, OY)) s.template bond<OX, OY, R>();
+}
+
+// Applies bond<D - r, OY, r> for every node r of the lane that has that bond.
+template <int D, int OY, class S, int... R>
+__device__ __forceinline__ void rb_bonds(S &s, std::integer_sequence<int, R...>) {
+  (rb_bond_if<D - R, OY, R>(s), ...);
+}
+
+// Keep each row's reads next to its bonds (the accumulators are complete at the end of the row):
+// hoisting every read of the sweep ahead of the arithmetic needs more registers than a wavefront
+// has.
+__device__ __forceinline__ void pin(double &v) { asm volatile("" : "+v"(v)); }
+
+// Full-tile dilatation: theta_i = sum_k C c1_k (u_j . o_k).
+struct ThetaFull {
+  const double2 *su;
+  int32_t c0;
+  const TileArgs &T;
+  double acc[kRB];
+  double2 u;
+  template <int D, int OY>
+  __device__ __forceinline__ void cell() {
+    u = su[c0 + D * kW68 + OY];
+    rb_bonds<D, OY>(*this, std::make_integer_sequence<int, kRB>{});
+  }
+  template <int OX, int OY, int R>
+  __device__ __forceinline__ void bond() {
+    constexpr int cls = stencil_class(OX * OX + OY * OY);
+    if constexpr (OX != 0) {
+      constexpr int z = rep_slot(cls, iabs(OX));
+      acc[R] = fma(OX > 0 ? T.wt[cls][z] : -T.wt[cls][z], u.x, acc[R]);
+    }
+    if constexpr (OY != 0) {
+      constexpr int z = rep_slot(cls, iabs(OY));
+      acc[R] = fma(OY > 0 ? T.wt[cls][z] : -T.wt[cls][z], u.y, acc[R]);
+    }
+  }
+  __device__ __forceinline__ void end_row() {
+#pragma unroll
+    for (int r = 0; r < kRB; r++) pin(acc[r]);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+};
+
+// Partial-tile dilatation, the reference's bond term: theta_i = c2m_i sum_k c1_k (u_j - u_i) . o_k
+// over the cells that hold a node (mask 1).
+struct ThetaMasked {
+  const double2 *su;
+  const double *sm;
+  int32_t c0;
+  const TileArgs &T;
+  double2 ui[kRB];
+  double acc[kRB];
+  double2 u;
+  double msk;
+  template <int D, int OY>
+  __device__ __forceinline__ void cell() {
+    u = su[c0 + D * kW68 + OY];
+    msk = sm[c0 + D * kW68 + OY];
+    rb_bonds<D, OY>(*this, std::make_integer_sequence<int, kRB>{});
+  }
+  template <int OX, int OY, int R>
+  __device__ __forceinline__ void bond() {
+    constexpr int cls = stencil_class(OX * OX + OY * OY);
+    acc[R] = fma(T.c1[cls] * msk, dot_int(u.x - ui[R].x, u.y - ui[R].y, OX, OY), acc[R]);
+  }
+  __device__ __forceinline__ void end_row() {
+#pragma unroll
+    for (int r = 0; r < kRB; r++) pin(acc[r]);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+};
+
+// Full-tile force: F_i = sum_k (lambda - mu) C c1_k theta_j o_k + 8 mu C c2_k ((u_j - u_i) . o_k) o_k.
+struct ForceFull {
+  const double2 *su;
+  const double *sth;
+  int32_t c0;
+  const TileArgs &T;
+  double2 ui[kRB];
+  double fx[kRB], fy[kRB];
+  double2 u;
+  double th;
+  template <int D, int OY>
+  __device__ __forceinline__ void cell() {
+    u = su[c0 + D * kW68 + OY];
+    th = sth[c0 + D * kW68 + OY];
+    rb_bonds<D, OY>(*this, std::make_integer_sequence<int, kRB>{});
+  }
+  template <int OX, int OY, int R>
+  __device__ __forceinline__ void bond() {
+    constexpr int cls = stencil_class(OX * OX + OY * OY);
+    const double dux = u.x - ui[R].x, duy = u.y - ui[R].y;
+    if constexpr (OX != 0) {
+      constexpr int za = rep_slot(cls, iabs(OX)), zb = prod_slot(cls, OX * OX);
+      fx[R] = fma(OX > 0 ? T.wa[cls][za] : -T.wa[cls][za], th, fx[R]);
+      fx[R] = fma(T.wb[cls][zb], dux, fx[R]);
+    }
+    if constexpr (OY != 0) {
+      constexpr int za = rep_slot(cls, iabs(OY)), zb = prod_slot(cls, OY * OY);
+      fy[R] = fma(OY > 0 ? T.wa[cls][za] : -T.wa[cls][za], th, fy[R]);
+      fy[R] = fma(T.wb[cls][zb], duy, fy[R]);
+    }
+    if constexpr (OX != 0 && OY != 0) {
+      constexpr int z = prod_slot(cls, iabs(OX * OY));
+      const double w = ((OX > 0) == (OY > 0)) ? T.wb[cls][z] : -T.wb[cls][z];
+      fx[R] = fma(w, duy, fx[R]);
+      fy[R] = fma(w, dux, fy[R]);
+    }
+  }
+  __device__ __forceinline__ void end_row() {
+#pragma unroll
+    for (int r = 0; r < kRB; r++) {
+      pin(fx[r]);
+      pin(fy[r]);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  }
+};
+
+// Partial-tile force, the reference's bond term with ab = ((lambda - mu) c2m theta, 4 mu c2m):
+// s = (ab_i.x + ab_j.x) c1 + (ab_i.y + ab_j.y) c2 ((u_j - u_i) . o), skipped where ab_j.y == 0
+// (no node, or m_j <= 0; src/main.cpp:852).
+struct ForceMasked {
+  const double2 *su;
+  const double2 *sab;
+  int32_t c0;
+  const TileArgs &T;
+  double2 ui[kRB], abi[kRB];
+  double fx[kRB], fy[kRB];
+  double2 u, ab;
+  template <int D, int OY>
+  __device__ __forceinline__ void cell() {
+    u = su[c0 + D * kW68 + OY];
+    ab = sab[c0 + D * kW68 + OY];
+    rb_bonds<D, OY>(*this, std::make_integer_sequence<int, kRB>{});
+  }
+  template <int OX, int OY, int R>
+  __device__ __forceinline__ void bond() {
+    constexpr int cls = stencil_class(OX * OX + OY * OY);
+    const double tt = dot_int(u.x - ui[R].x, u.y - ui[R].y, OX, OY);
+    double sp = (abi[R].x + ab.x) * T.c1[cls] + (abi[R].y + ab.y) * tt * T.c2[cls];
+    sp = (ab.y != 0.0) ? sp : 0.0;
+    if constexpr (OX != 0) fx[R] = fma(sp, (double)OX, fx[R]);
+    if constexpr (OY != 0) fy[R] = fma(sp, (double)OY, fy[R]);
+  }
+  __device__ __forceinline__ void end_row() {
+#pragma unroll
+    for (int r = 0; r < kRB; r++) {
+      pin(fx[r]);
+      pin(fy[r]);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  }
+};
+
+// Dilatation of the full tiles; one wavefront per tile (full_list order).
+__global__ void __launch_bounds__(64, 4) k_theta_rb(StepArgs a, StepState st) {
+  __shared__ double2 su[kCells68];
+  const TileArgs &T = a.tile;
+  const int32_t q = tile_of_block(blockIdx.x, T.n_full);
+  if (q >= T.n_full) return;
+  const int32_t t = T.full_list[q];
+  int32_t nb[9];
+#pragma unroll
+  for (int k = 0; k < 9; k++) nb[k] = T.tile_nbr[9 * t + k];
+  const int32_t lane = threadIdx.x;
+  const int32_t ly = lane & 15, lx0 = (lane >> 4) * kRB;
+  int8_t kd[kRB];
+#pragma unroll
+  for (int r = 0; r < kRB; r++) kd[r] = a.kind[t * (kTile * kTile) + (lx0 + r) * kTile + ly];
+  fill_window68(nb, lane, [&](WinCell w) { su[w.cell] = make_double2(st.px[w.slot], st.py[w.slot]); },
+                std::make_integer_sequence<int, 9>{});
+  __syncthreads();
+  ThetaFull P{su, (lx0 + 4) * kW68 + ly + 4, T, {}, {}};
+#pragma unroll
+  for (int r = 0; r < kRB; r++) P.acc[r] = 0.0;
+  rb_sweep(P, std::make_integer_sequence<int, kRB + 8>{});
+#pragma unroll
+  for (int r = 0; r < kRB; r++)
+    if (kd[r] == 1) st.theta[t * (kTile * kTile) + (lx0 + r) * kTile + ly] = P.acc[r];
+}
+
+// Dilatation of the partial tiles; one wavefront per tile (part_list order).
+__global__ void __launch_bounds__(64, 4) k_theta_rbm(StepArgs a, StepState st) {
+  __shared__ double2 su[kCells68];
+  __shared__ double sm[kCells68];
+  const TileArgs &T = a.tile;
+  const int32_t q = tile_of_block(blockIdx.x, T.n_part);
+  if (q >= T.n_part) return;
+  const int32_t t = T.part_list[q];
+  int32_t nb[9];
+#pragma unroll
+  for (int k = 0; k < 9; k++) nb[k] = T.tile_nbr[9 * t + k];
+  const int32_t lane = threadIdx.x;
+  const int32_t ly = lane & 15, lx0 = (lane >> 4) * kRB;
+  int8_t kd[kRB];
+  double c2[kRB];
+#pragma unroll
+  for (int r = 0; r < kRB; r++) {
+    const int32_t i = t * (kTile * kTile) + (lx0 + r) * kTile + ly;
+    kd[r] = a.kind[i];
+    c2[r] = a.c2m[i];
+  }
+  fill_window68(nb, lane, [&](WinCell w) {
+    const bool ok = w.tile >= 0 && a.kind[w.slot] != 0;
+    su[w.cell] = ok ? make_double2(st.px[w.slot], st.py[w.slot]) : make_double2(0.0, 0.0);
+    sm[w.cell] = ok ? 1.0 : 0.0;
+  }, std::make_integer_sequence<int, 9>{});
+  __syncthreads();
+  const int32_t c0 = (lx0 + 4) * kW68 + ly + 4;
+  ThetaMasked P{su, sm, c0, T, {}, {}, {}, 0.0};
+#pragma unroll
+  for (int r = 0; r < kRB; r++) {
+    P.ui[r] = su[c0 + r * kW68];
+    P.acc[r] = 0.0;
+  }
+  rb_sweep(P, std::make_integer_sequence<int, kRB + 8>{});
+#pragma unroll
+  for (int r = 0; r < kRB; r++)
+    if (kd[r] == 1) st.theta[t * (kTile * kTile) + (lx0 + r) * kTile + ly] = c2[r] * P.acc[r];
+}
+
+// Force pass of the full tiles with the fused Verlet corrector and predictor; one wavefront per
+// tile (full_list order).
+__global__ void __launch_bounds__(64, 3) k_force_rb(StepArgs a, StepState st) {
+  __shared__ double2 su[kCells68];
+  __shared__ double sth[kCells68];
+  const TileArgs &T = a.tile;
+  const int32_t q = tile_of_block(blockIdx.x, T.n_full);
+  if (q >= T.n_full) return;
+  const int32_t t = T.full_list[q];
+  int32_t nb[9];
+#pragma unroll
+  for (int k = 0; k < 9; k++) nb[k] = T.tile_nbr[9 * t + k];
+  const int32_t lane = threadIdx.x;
+  const int32_t ly = lane & 15, lx0 = (lane >> 4) * kRB;
+  // the epilogue's inputs first: their loads overlap the window's
+  int8_t kd[kRB];
+  double2 vi[kRB], ai[kRB];
+#pragma unroll
+  for (int r = 0; r < kRB; r++) {
+    const int32_t i = t * (kTile * kTile) + (lx0 + r) * kTile + ly;
+    kd[r] = a.kind[i];
+    vi[r] = make_double2(st.vx[i], st.vy[i]);
+    ai[r] = make_double2(st.ax[i], st.ay[i]);
+  }
+  fill_window68(nb, lane, [&](WinCell w) {
+    su[w.cell] = make_double2(st.px[w.slot], st.py[w.slot]);
+    sth[w.cell] = st.theta[w.slot];
+  }, std::make_integer_sequence<int, 9>{});
+  __syncthreads();
+  const int32_t c0 = (lx0 + 4) * kW68 + ly + 4;
+  ForceFull P{su, sth, c0, T, {}, {}, {}, {}, 0.0};
+#pragma unroll
+  for (int r = 0; r < kRB; r++) {
+    P.ui[r] = su[c0 + r * kW68];
+    P.fx[r] = 0.0;
+    P.fy[r] = 0.0;
+  }
+  rb_sweep(P, std::make_integer_sequence<int, kRB + 8>{});
+#pragma unroll
+  for (int r = 0; r < kRB; r++) {
+    const int32_t i = t * (kTile * kTile) + (lx0 + r) * kTile + ly;
+    if (kd[r] != 0) node_update(a, st, i, kd[r], P.ui[r], vi[r], ai[r], P.fx[r], P.fy[r]);
+  }
+}
+
+// Force pass of the partial tiles; one wavefront per tile (part_list order).
+__global__ void __launch_bounds__(64, 3) k_force_rbm(StepArgs a, StepState st) {
+  __shared__ double2 su[kCells68];
+  __shared__ double2 sab[kCells68];
+  const TileArgs &T = a.tile;
+  const int32_t q = tile_of_block(blockIdx.x, T.n_part);
+  if (q >= T.n_part) return;
+  const int32_t t = T.part_list[q];
+  int32_t nb[9];
+#pragma unroll
+  for (int k = 0; k < 9; k++) nb[k] = T.tile_nbr[9 * t + k];
+  const int32_t lane = threadIdx.x;
+  const int32_t ly = lane & 15, lx0 = (lane >> 4) * kRB;
+  const double lm = a.lam - a.mu, mu4 = 4.0 * a.mu;
+  int8_t kd[kRB];
+  double2 vi[kRB], ai[kRB];
+#pragma unroll
+  for (int r = 0; r < kRB; r++) {
+    const int32_t i = t * (kTile * kTile) + (lx0 + r) * kTile + ly;
+    kd[r] = a.kind[i];
+    vi[r] = make_double2(st.vx[i], st.vy[i]);
+    ai[r] = make_double2(st.ax[i], st.ay[i]);
+  }
+  fill_window68(nb, lane, [&](WinCell w) {
+    double2 u = make_double2(0.0, 0.0), ab = make_double2(0.0, 0.0);
+    if (w.tile >= 0 && a.kind[w.slot] != 0) {
+      const double c2 = a.c2m[w.slot];  // 0 when m <= 0
+      u = make_double2(st.px[w.slot], st.py[w.slot]);
+      ab = make_double2(lm * c2 * st.theta[w.slot], mu4 * c2);
+    }
+    su[w.cell] = u;
+    sab[w.cell] = ab;
+  }, std::make_integer_sequence<int, 9>{});
+  __syncthreads();
+  const int32_t c0 = (lx0 + 4) * kW68 + ly + 4;
+  ForceMasked P{su, sab, c0, T, {}, {}, {}, {}, {}, {}};
+#pragma unroll
+  for (int r = 0; r < kRB; r++) {
+    P.ui[r] = su[c0 + r * kW68];
+    P.abi[r] = sab[c0 + r * kW68];
+    P.fx[r] = 0.0;
+    P.fy[r] = 0.0;
+  }
+  rb_sweep(P, std::make_integer_sequence<int, kRB + 8>{});
+#pragma unroll
+  for (int r = 0; r < kRB; r++) {
+    const int32_t i = t * (kTile * kTile) + (lx0 + r) * kTile + ly;
+    const bool on = P.abi[r].y != 0.0;  // a node with m <= 0 gets no force
+    if (kd[r] != 0)
+      node_update(a, st, i, kd[r], P.ui[r], vi[r], ai[r], on ? P.fx[r] : 0.0, on ? P.fy[r] : 0.0);
+  }
+}
+
+// Fast path, ABC rows' dilatation (G_theta, or the direct value for IDW), one wavefront per row.
+// Reads only the theta history (or the prediction), so it may run beside the tile kernels.
+__global__ void __launch_bounds__(kBlock) k_abc_theta(StepArgs a, StepState st) {
   __shared__ AbcScratch sabc[kWavesPerBlock];
   const int32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int32_t row = blockIdx.x * kWavesPerBlock + wid;
-  if (row < a.n_abc) abc_force_row<EXACT>(a, st, row, lane, &sabc[wid]);
+  if (row < a.n_abc) abc_theta_row<false>(a, st, row, lane, &sabc[wid]);
+}
+
+// Fast path, ABC rows: U^{n+1} = G_u [U^n, U^{n-1}] and V^{n+1} = G_v [V^n, V^{n-1}] over the
+// row's cloud (src/main.cpp:883-916, 933-974), one wavefront per row, lane k on cloud member k.
+// The tile kernels compute the force at the ABC nodes like at any node.
+__global__ void __launch_bounds__(kBlock) k_abc_uv(StepArgs a, StepState st) {
+  const int32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int32_t row = blockIdx.x * kWavesPerBlock + wid;
+  if (row >= a.n_abc) return;
+  const double *gu0 = a.gu + (int64_t)row * 8 * a.M, *gu1 = gu0 + 4 * a.M;
+  const double *gv0 = a.gv + (int64_t)row * 8 * a.M, *gv1 = gv0 + 4 * a.M;
+  const int32_t *cl = a.cloud + (int64_t)row * a.M;
+  double Ux = 0.0, Uy = 0.0, Vx = 0.0, Vy = 0.0;
+  for (int base = 0; base < a.M; base += 64) {
+    const int jj = base + lane;
+    const int32_t k = jj < a.M ? cl[jj] : -1;
+    double t0 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0;
+    if (k >= 0) {
+      const double4 g0 = *(const double4 *)(gu0 + 4 * jj), g1 = *(const double4 *)(gu1 + 4 * jj);
+      const double4 h0 = *(const double4 *)(gv0 + 4 * jj), h1 = *(const double4 *)(gv1 + 4 * jj);
+      const double ux = st.ux[k], uy = st.uy[k], uxm = st.uxm[k], uym = st.uym[k];
+      const double vx = st.vx[k], vy = st.vy[k], vxm = st.vxm[k], vym = st.vym[k];
+      t0 = g0.x * ux + g0.y * uy + g0.z * uxm + g0.w * uym;
+      t1 = g1.x * ux + g1.y * uy + g1.z * uxm + g1.w * uym;
+      t2 = h0.x * vx + h0.y * vy + h0.z * vxm + h0.w * vym;
+      t3 = h1.x * vx + h1.y * vy + h1.z * vxm + h1.w * vym;
+    }
+    Ux += wave_sum(t0);
+    Uy += wave_sum(t1);
+    Vx += wave_sum(t2);
+    Vy += wave_sum(t3);
+  }
+  if (lane == 0) {
+    const int32_t node = a.abc_slot[row];
+    st.ux1[node] = Ux; st.uy1[node] = Uy;
+    st.vx1[node] = Vx; st.vy1[node] = Vy;
+  }
+}
+
+// Fast path, after the force pass: the next prediction at the ABC nodes from G_u, G_v and the
+// force kernels' acceleration (src/main.cpp:725-728).
+__global__ void __launch_bounds__(kBlock) k_abc_predict(StepArgs a, StepState st) {
+  const int32_t row = blockIdx.x * kBlock + threadIdx.x;
+  if (row >= a.n_abc) return;
+  const int32_t i = a.abc_slot[row];
+  const double h = 0.5 * a.dt;
+  st.pnx[i] = st.ux1[i] + a.dt * (st.vx1[i] + h * st.ax1[i]);
+  st.pny[i] = st.uy1[i] + a.dt * (st.vy1[i] + h * st.ay1[i]);
 }
 
 // Generic force kernel over a list of tiles (all tiles when list is null).
@@ -726,7 +1210,8 @@ __global__ void __launch_bounds__(kBlock) k_force_tile(StepArgs a, StepState st,
   }
   __syncthreads();
   const int32_t i = t * (kTile * kTile) + threadIdx.x;
-  if (a.kind[i] != 1) return;
+  const int8_t kd = a.kind[i];
+  if (kd == 0) return;
   const int32_t c = (threadIdx.x / kTile + T.halo) * W + (threadIdx.x % kTile + T.halo);
   const double2 ui = sh.fo.u[c];
   const double2 abi = sh.fo.ab[c];
@@ -745,22 +1230,16 @@ __global__ void __launch_bounds__(kBlock) k_force_tile(StepArgs a, StepState st,
       F1 += s * hy;
     }
   }
-  const double inv_rho = 1.0 / a.rho;
-  const double a1x = F0 * inv_rho, a1y = F1 * inv_rho;
-  const double h = 0.5 * a.dt;
-  const double u1x = ui.x, u1y = ui.y;
-  const double v1x = st.vx[i] + h * (st.ax[i] + a1x);
-  const double v1y = st.vy[i] + h * (st.ay[i] + a1y);
-  st.ax1[i] = a1x; st.ay1[i] = a1y;
-  st.ux1[i] = u1x; st.uy1[i] = u1y;
-  st.vx1[i] = v1x; st.vy1[i] = v1y;
-  st.pnx[i] = u1x + a.dt * (v1x + h * a1x);
-  st.pny[i] = u1y + a.dt * (v1y + h * a1y);
+  node_update(a, st, i, kd, ui, make_double2(st.vx[i], st.vy[i]), make_double2(st.ax[i], st.ay[i]),
+              F0, F1);
 }
 
 // For every node slot, the stencil cells holding a node must be exactly the set of its CSR row
-// (self excluded) for the tile path to reproduce the reference's horizons.
-__global__ void __launch_bounds__(kBlock) k_tile_check(StepArgs a, int32_t *mismatch) {
+// (self excluded) for the tile path to reproduce the reference's horizons. Both sets as bit masks
+// over the stencil's bonds (nt <= 128): bond_of maps an offset (ox, oy), |ox|, |oy| <= halo, to
+// its bond, -1 off the stencil.
+__global__ void __launch_bounds__(kBlock) k_tile_check(StepArgs a, const int16_t *bond_of,
+                                                       int32_t *mismatch) {
   const TileArgs &T = a.tile;
   const int32_t t = blockIdx.x;
   const int32_t i = t * (kTile * kTile) + threadIdx.x;
@@ -768,22 +1247,29 @@ __global__ void __launch_bounds__(kBlock) k_tile_check(StepArgs a, int32_t *mism
   const int32_t txy = T.tile_xy[t];
   const int32_t gx = (txy >> 16) * kTile + threadIdx.x / kTile;
   const int32_t gy = (txy & 0xffff) * kTile + threadIdx.x % kTile;
-  const int64_t q0 = a.row_ptr[i] + 1, q1 = a.row_ptr[i + 1];
-  int64_t found = 0;
-  bool bad = false;
-  for (int32_t k = 0; k < T.nt && !bad; k++) {
+  uint64_t want[2] = {0, 0}, got[2] = {0, 0};
+  for (int32_t k = 0; k < T.nt; k++) {
     // delta = ox * win + oy with |ox|, |oy| <= halo < win / 2
     const int32_t d = T.delta[k];
     const int32_t ox = (d + T.halo + T.halo * T.win) / T.win - T.halo;
     const int32_t oy = d - ox * T.win;
     const int32_t s = lattice_slot(T, gx + ox, gy + oy);
-    if (s < 0 || a.kind[s] == 0) continue;
-    bool in_row = false;
-    for (int64_t q = q0; q < q1 && !in_row; q++) in_row = (a.col[q] == s);
-    bad = !in_row;
-    found++;
+    if (s >= 0 && a.kind[s] != 0) want[k >> 6] |= 1ull << (k & 63);
   }
-  if (bad || found != q1 - q0) atomicAdd(mismatch, 1);
+  const int32_t E = 2 * T.halo + 1;
+  bool bad = false;
+  for (int64_t q = a.row_ptr[i] + 1; q < a.row_ptr[i + 1] && !bad; q++) {
+    const int32_t j = a.col[q];
+    const int32_t tj = j / (kTile * kTile), lj = j % (kTile * kTile);
+    const int32_t ox = (T.tile_xy[tj] >> 16) * kTile + lj / kTile - gx;
+    const int32_t oy = (T.tile_xy[tj] & 0xffff) * kTile + lj % kTile - gy;
+    const int32_t k = (ox < -T.halo || ox > T.halo || oy < -T.halo || oy > T.halo)
+                          ? -1 : bond_of[(ox + T.halo) * E + oy + T.halo];
+    const uint64_t bit = k >= 0 ? 1ull << (k & 63) : 0;
+    bad = k < 0 || (got[k >> 6] & bit) != 0;
+    if (!bad) got[k >> 6] |= bit;
+  }
+  if (bad || want[0] != got[0] || want[1] != got[1]) atomicAdd(mismatch, 1);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -988,17 +1474,33 @@ hipError_t launch_predict(const StepArgs &a, const double *ux, const double *uy,
   return hipGetLastError();
 }
 
+static inline int32_t list_blocks(int32_t count) { return (count + 7) / 8 * 8; }
+
 hipError_t launch_theta(const StepArgs &a, const StepState &st, bool fast, hipStream_t s) {
   if (fast) {
-    const int32_t nt = tile_blocks(a);
-    if (a.tile.stencil68 & 1)
-      hipLaunchKernelGGL(k_theta_t68, dim3(nt + abc_blocks(a)), dim3(kBlock), 0, s, a, st, nt);
-    else
-      hipLaunchKernelGGL(k_theta_tile, dim3(nt + abc_blocks(a)), dim3(kBlock), 0, s, a, st, nt);
+    const TileArgs &T = a.tile;
+    if (T.theta_kernel == kBlocked) {
+      if (T.n_full > 0)
+        hipLaunchKernelGGL(k_theta_rb, dim3(list_blocks(T.n_full)), dim3(64), 0, s, a, st);
+      hipError_t e = hipGetLastError();
+      if (e != hipSuccess || T.n_part == 0) return e;
+      hipLaunchKernelGGL(k_theta_rbm, dim3(list_blocks(T.n_part)), dim3(64), 0, s, a, st);
+    } else if (T.theta_kernel == kUnrolled) {
+      hipLaunchKernelGGL(k_theta_t68, dim3(tile_blocks(a)), dim3(kBlock), 0, s, a, st,
+                         (const int32_t *)nullptr, T.n_tiles);
+    } else {
+      hipLaunchKernelGGL(k_theta_tile, dim3(tile_blocks(a)), dim3(kBlock), 0, s, a, st);
+    }
   } else {
     const int32_t nb = blocks_for(a.n);
     hipLaunchKernelGGL(k_theta_exact, dim3(nb + abc_blocks(a)), dim3(kBlock), 0, s, a, st, nb);
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_abc_theta(const StepArgs &a, const StepState &st, hipStream_t s) {
+  if (a.n_abc == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_abc_theta, dim3(abc_blocks(a)), dim3(kBlock), 0, s, a, st);
   return hipGetLastError();
 }
 
@@ -1008,14 +1510,15 @@ hipError_t launch_idw(const StepArgs &a, const StepState &st, hipStream_t s) {
   return hipGetLastError();
 }
 
-static inline int32_t list_blocks(int32_t count) { return (count + 7) / 8 * 8; }
-
 hipError_t launch_force(const StepArgs &a, const StepState &st, bool fast, hipStream_t s) {
   if (fast) {
-    if (a.tile.stencil68 & 2) {
-      if (a.tile.n_full > 0)
-        hipLaunchKernelGGL(k_force_t68, dim3(list_blocks(a.tile.n_full)), dim3(kBlock), 0, s, a,
-                           st);
+    const int32_t nf = a.tile.n_full;
+    if (a.tile.force_kernel == kBlocked) {
+      if (nf > 0) hipLaunchKernelGGL(k_force_rb, dim3(list_blocks(nf)), dim3(64), 0, s, a, st);
+    } else if (a.tile.force_kernel == kUnrolled) {
+      if (nf > 0)
+        hipLaunchKernelGGL(k_force_t68<true>, dim3(list_blocks(nf)), dim3(kBlock), 0, s, a, st,
+                           a.tile.full_list, nf);
     } else {
       hipLaunchKernelGGL(k_force_tile, dim3(tile_blocks(a)), dim3(kBlock), 0, s, a, st,
                          (const int32_t *)nullptr, a.tile.n_tiles);
@@ -1027,21 +1530,33 @@ hipError_t launch_force(const StepArgs &a, const StepState &st, bool fast, hipSt
   return hipGetLastError();
 }
 
-hipError_t launch_force_abc(const StepArgs &a, const StepState &st, hipStream_t s) {
-  if ((a.tile.stencil68 & 2) && a.tile.n_part > 0) {
-    hipLaunchKernelGGL(k_force_tile, dim3(list_blocks(a.tile.n_part)), dim3(kBlock), 0, s, a, st,
-                       a.tile.part_list, a.tile.n_part);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-  }
-  if (abc_blocks(a) == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_force_abc<false>, dim3(abc_blocks(a)), dim3(kBlock), 0, s, a, st);
+hipError_t launch_force_part(const StepArgs &a, const StepState &st, hipStream_t s) {
+  if (a.tile.force_kernel == kGeneric || a.tile.n_part == 0) return hipSuccess;
+  if (a.tile.force_kernel == kBlocked)
+    hipLaunchKernelGGL(k_force_rbm, dim3(list_blocks(a.tile.n_part)), dim3(64), 0, s, a, st);
+  else
+    hipLaunchKernelGGL(k_force_t68<false>, dim3(list_blocks(a.tile.n_part)), dim3(kBlock), 0, s,
+                       a, st, a.tile.part_list, a.tile.n_part);
   return hipGetLastError();
 }
 
-hipError_t launch_tile_check(const StepArgs &a, int32_t *d_mismatch, hipStream_t s) {
+hipError_t launch_abc_uv(const StepArgs &a, const StepState &st, hipStream_t s) {
+  if (a.n_abc == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_abc_uv, dim3(abc_blocks(a)), dim3(kBlock), 0, s, a, st);
+  return hipGetLastError();
+}
+
+hipError_t launch_abc_predict(const StepArgs &a, const StepState &st, hipStream_t s) {
+  if (a.n_abc == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_abc_predict, dim3(blocks_for(a.n_abc)), dim3(kBlock), 0, s, a, st);
+  return hipGetLastError();
+}
+
+hipError_t launch_tile_check(const StepArgs &a, const int16_t *bond_of, int32_t *d_mismatch,
+                             hipStream_t s) {
   if (a.tile.n_tiles == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_tile_check, dim3(a.tile.n_tiles), dim3(kBlock), 0, s, a, d_mismatch);
+  hipLaunchKernelGGL(k_tile_check, dim3(a.tile.n_tiles), dim3(kBlock), 0, s, a, bond_of,
+                     d_mismatch);
   return hipGetLastError();
 }
 

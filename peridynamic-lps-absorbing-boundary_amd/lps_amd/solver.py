@@ -46,7 +46,7 @@ class LpsProblem(C.Structure):
 class LpsOptions(C.Structure):
     _fields_ = [("theta_method", C.c_int32), ("abc_enabled", C.c_int32), ("exact", C.c_int32),
                 ("use_graph", C.c_int32), ("device", C.c_int32), ("horizon_radius", C.c_double),
-                ("variant", C.c_int32)]
+                ("variant", C.c_int32), ("concurrent", C.c_int32)]
 
 
 class LpsDiag(C.Structure):
@@ -111,7 +111,7 @@ class Solver:
 
     def __init__(self, prob, theta_method: int = LPS_THETA_GTHETA, abc_enabled: bool = True,
                  exact: bool = False, use_graph: bool = True, device: int = 0,
-                 horizon_radius: float = 0.0, variant: int = 0):
+                 horizon_radius: float = 0.0, variant: int = 0, concurrent: bool = True):
         L = load_library()
         self._keep: List[np.ndarray] = []
 
@@ -149,6 +149,7 @@ class Solver:
         o.device = device
         o.horizon_radius = horizon_radius
         o.variant = variant
+        o.concurrent = 1 if concurrent else 0
         h = C.c_void_p()
         rc = L.lps_create(C.byref(p), C.byref(o), C.byref(h))
         if rc != LPS_OK:

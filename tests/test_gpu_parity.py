@@ -61,12 +61,15 @@ def test_exact_mode_matches_oracle_bitwise(name):
 
 
 @pytest.mark.parametrize("name", CASES)
-@pytest.mark.parametrize("graph", [True, False])
-def test_fast_mode_matches_reference_golden(name, graph):
-    """Fast path (tabulated geometry, FMA) against the reference's outputs: 1e-10 relative L2."""
-    say("fast vs reference golden", name, graph)
+@pytest.mark.parametrize("graph,variant,concurrent", [(True, 0, True), (False, 0, True),
+                                                      (True, 3, True), (True, 2, False)])
+def test_fast_mode_matches_reference_golden(name, graph, variant, concurrent):
+    """Fast path (tile kernels, FMA) against the reference's outputs: 1e-10 relative L2; autotuned,
+    register-blocked and unrolled node kernels, ABC rows beside or after the tile kernels."""
+    say("fast vs reference golden", name, graph, variant, concurrent)
     prob, ref = load_golden(name)
-    with solver.Solver(prob, exact=False, use_graph=graph) as s:
+    with solver.Solver(prob, exact=False, use_graph=graph, variant=variant,
+                       concurrent=concurrent) as s:
         recs = s.run(prob.steps)
         f = s.fetch()
     assert_series_close(records_table(recs, ENERGY_KEYS), ref["energy"], ENERGY_KEYS, 1e-10, prob, name)
@@ -130,16 +133,18 @@ def r40_problem():
                            pulses=[pp.Pulse(8.0, -5.0, 12.0, 1.0, (1.0, 1.0))], workers=8)
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
 def test_tile_path_matches_oracle_on_a_larger_domain(variant, r40_problem):
     """R = 40 mm (20k nodes, 121 tiles, full and partial ones), nu = 0.3, off-centre pulse,
-    600 steps; autotuned, generic and unrolled node kernels."""
+    600 steps; autotuned, generic, unrolled and register-blocked node kernels."""
     say("tile R=40 variant", variant)
     prob = r40_problem
+    names = {1: "generic", 2: "unrolled", 3: "blocked"}
     with solver.Solver(prob, variant=variant) as s:
         assert s.path.startswith("tile")
-        if variant == 2:
-            assert "dilatation unrolled, force unrolled" in s.path, s.path
+        if variant:
+            want = names[variant]
+            assert f"dilatation {want}, force {want}" in s.path, s.path
         recs = s.run(600)
         f = s.fetch()
     o = pyoracle.Oracle(prob)
