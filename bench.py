@@ -8,9 +8,9 @@ one time step of the loop over the whole domain (dilatation pass, force pass wit
 epilogue); the timed region is the loop as the reference times it (src/main.cpp:716-1166): all steps
 plus the output block every 30 steps, with inputs resident on the GPU and no VTK output.
 
-Printed: one JSON line with the metric, the force-pass roofline (HIP-event timing of that kernel
-inside the timed region against its algorithmic bytes 68 N + 4 nnz) and the CPU oracle timed on a
-bounded sample of the same workload on the host's cores.
+Printed: one JSON line with the metric, the roofline of the force pass's main kernel (its average
+duration from HIP events on its stream inside the timed region, against its compulsory HBM bytes,
+121 per node) and the CPU oracle timed on a bounded sample of the same workload on the host's cores.
 
   python bench.py [--steps K] [--warmup W] [--workload cfg2|cfg1] [--no-cpu-baseline]
 """
@@ -29,6 +29,20 @@ HBM_PEAK_GBS = 8000.0
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of kernel on workload from the committed counter summary (FETCH_SIZE
+    and WRITE_SIZE passes of rocprofv3, calibrated on a copy kernel), or None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        table = json.load(f)
+    e = table.get(f"{kernel}@{workload}")
+    if not e:
+        return None
+    return {"bytes_per_launch": e["bytes_per_launch"], "source": f"profiles/hbm_traffic.json ({e['command']})"}
 
 
 def main():
@@ -108,13 +122,29 @@ def main():
     refl = (1.0 + last["de_rel"]) if last else None
     log(f"[bench] {steps} steps in {elapsed:.3f}s -> {value:.4e} node-steps/s; passes {pt}")
 
-    force_bytes = 68 * prob.N + 4 * nnz  # u, x, m, theta, rowptr, a_out per node + one index per bond
+    # Roofline of the force pass's main kernel (the longest launch of a step). Its compulsory HBM
+    # traffic, each element once: reads u, theta, v, a (8 B each component) and kind (1 B), writes
+    # a, v, u and the next prediction: 121 B per node. The halo reads of the LDS-staged windows are
+    # L2 hits and not counted. A CSR-gather formulation of the same pass would move 68 B per node
+    # plus 4 B per bond (coordinates, m, row pointer, column index; csr_gather_bytes).
+    ti = s.tile_info()
+    fk = path.split("force ")[1].split(" ")[0].rstrip(",") if "force " in path else "exact"
+    kernel = {"blocked": "k_force_rb", "unrolled": "k_force_t68<true>",
+              "generic": "k_force_tile", "exact": "k_force_exact"}.get(fk, fk)
+    units = ti["full_tile_nodes"] if fk in ("blocked", "unrolled") else prob.N
+    per_unit = 121
+    force_bytes = per_unit * units
     achieved = force_bytes / (pt["force_ms"] * 1e-3) / 1e9
+    traffic = measured_traffic(kernel, w.name)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_force (force pass + fused Verlet/ABC epilogue)",
-                "algorithmic_bytes": force_bytes, "avg_ms": round(pt["force_ms"], 5),
-                "theta_pass_ms": round(pt["theta_ms"], 5), "step_ms": round(pt["step_ms"], 5)}
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": traffic["bytes_per_launch"] if traffic else None,
+                "kernel": kernel, "units": units, "unit_name": "nodes of the full tiles",
+                "bytes_per_unit": per_unit, "algorithmic_bytes": force_bytes,
+                "avg_ms": round(pt["force_ms"], 5),
+                "csr_gather_bytes": 68 * units + 4 * 68 * units,
+                "traffic_source": traffic["source"] if traffic else None,
+                "passes_ms": {k: round(v, 5) for k, v in pt.items()}}
     s.close()
 
     cpu = None
@@ -133,12 +163,14 @@ def main():
         log(f"[bench] CPU {cpu['value']:.4e} node-steps/s")
 
     out = {
-        "metric": "node-steps/s at reflected dE/E0<=1e-4 (plain FP64 LSQ ABC), 1 MI355X",
+        "metric": "node-steps/s at reflected dE/E0\u22641e-4: learned vs LSQ ABC, 1 MI355X",
         "value": value, "unit": "node-steps/s", "n_gpus": 1, "steps": steps,
         "warmup": args.warmup, "ms_per_step": elapsed / steps * 1e3, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"{w.name}: {w.description}", "nodes": prob.N, "abc_nodes": prob.C,
                    "bonds": nnz, "dt": prob.dt, "exact": bool(args.exact), "kernels": path,
+                   "abc": "least-squares G_u/G_v/G_theta (the learned closure is not built)",
+                   "streams": "serial" if args.serial else "concurrent",
                    "reflected_energy_final": refl,
                    "gate_reflected_le_1e-4": (refl is not None and refl <= 1e-4)},
         "roofline": roofline,

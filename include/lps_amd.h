@@ -133,6 +133,11 @@ int32_t lps_pass_times(lps_ctx *ctx, double *out4);
  * GPU-built horizon) or the exact CSR path, with the layout. */
 const char *lps_describe(const lps_ctx *ctx);
 
+/* Tile-path layout of the context: out[0] tiles, out[1] full tiles (every window cell a node with
+ * m == m_ref; the register-blocked kernels take them), out[2] partial tiles, out[3] nodes in the
+ * full tiles, out[4] nodes (all zeros on the exact path). */
+int32_t lps_tile_info(const lps_ctx *ctx, int64_t *out5);
+
 const char *lps_last_error(const lps_ctx *ctx);
 const char *lps_global_error(void); /* message of a failed lps_create */
 void lps_destroy(lps_ctx *ctx);

@@ -39,6 +39,7 @@ struct lps_ctx {
   double lam = 0, mu = 0, kappa = 0, m_ref = 0, exclude_r = -1, E0 = -1;
   bool fast = false;
   bool stencil68_ok = false;  // the unrolled 68-bond kernels apply to this problem
+  int64_t n_full_nodes = 0;   // nodes in the full tiles
   int64_t step = 0;
   std::vector<int32_t> perm_h;  // slot -> original index, -1 at holes
   HorizonBuild hb;
@@ -176,6 +177,17 @@ void lps_default_options(lps_options *opt) {
 const char *lps_global_error(void) { return g_create_error.c_str(); }
 const char *lps_last_error(const lps_ctx *ctx) { return ctx ? ctx->err.c_str() : g_create_error.c_str(); }
 const char *lps_describe(const lps_ctx *ctx) { return ctx->path.c_str(); }
+
+int32_t lps_tile_info(const lps_ctx *ctx, int64_t *out5) {
+  for (int q = 0; q < 5; q++) out5[q] = 0;
+  if (!ctx->fast) return LPS_OK;
+  out5[0] = ctx->tile.n_tiles;
+  out5[1] = ctx->tile.n_full;
+  out5[2] = ctx->tile.n_part;
+  out5[3] = ctx->n_full_nodes;
+  out5[4] = ctx->n;
+  return LPS_OK;
+}
 int64_t lps_step_index(const lps_ctx *ctx) { return ctx->step; }
 double lps_m_ref(const lps_ctx *ctx) { return ctx->m_ref; }
 
@@ -499,6 +511,9 @@ static int32_t create_impl(lps_ctx *c, const lps_problem *p) {
       c->fast = true;
       int32_t nfull = 0;
       for (int8_t f : full) nfull += f;
+      for (int32_t t = 0; t < ntl; t++)
+        if (full[t])
+          for (int32_t k = 0; k < kTile * kTile; k++) c->n_full_nodes += kind[(size_t)t * kTile * kTile + k] != 0;
       c->path = " (16x16 tiles, " + std::to_string(ntl) + " tiles, " + std::to_string(nfull) +
                 " full, " + std::to_string(nt) + " bonds, halo " + std::to_string(K) + ")";
     } else {

@@ -73,6 +73,7 @@ SIGNATURES = {
     "lps_set_timing": (C.c_int32, [C.c_void_p, C.c_int32]),
     "lps_pass_times": (C.c_int32, [C.c_void_p, C.c_void_p]),
     "lps_describe": (C.c_char_p, [C.c_void_p]),
+    "lps_tile_info": (C.c_int32, [C.c_void_p, C.c_void_p]),
     "lps_last_error": (C.c_char_p, [C.c_void_p]),
     "lps_global_error": (C.c_char_p, []),
     "lps_destroy": (None, [C.c_void_p]),
@@ -229,3 +230,11 @@ class Solver:
         self._check(self._L.lps_pass_times(self._h, _ptr(out)))
         return {"theta_ms": float(out[0]), "force_ms": float(out[1]), "abc_ms": float(out[2]),
                 "step_ms": float(out[3])}
+
+    def tile_info(self):
+        """Tile-path layout: tiles, full tiles, partial tiles, nodes in full tiles, nodes (zeros on
+        the exact path)."""
+        out = np.zeros(5, dtype=np.int64)
+        self._check(self._L.lps_tile_info(self._h, _ptr(out)))
+        return {"tiles": int(out[0]), "full_tiles": int(out[1]), "partial_tiles": int(out[2]),
+                "full_tile_nodes": int(out[3]), "nodes": int(out[4])}
