@@ -53,7 +53,8 @@ def main():
     ap.add_argument("--workload", default="cfg2")
     ap.add_argument("--exact", action="store_true")
     ap.add_argument("--variant", type=int, default=0,
-                    help="fast-path node kernels: 0 autotune, 1 generic tile, 2 unrolled stencil")
+                    help="fast-path node kernels: 0 autotune, 1 generic tile, 2 unrolled stencil, "
+                         "3 register-blocked stencil")
     ap.add_argument("--serial", action="store_true",
                     help="run the ABC rows' kernels on the main stream instead of beside the "
                          "tile kernels on a second stream")
@@ -121,6 +122,14 @@ def main():
     last = recs[-1] if recs else None
     refl = (1.0 + last["de_rel"]) if last else None
     log(f"[bench] {steps} steps in {elapsed:.3f}s -> {value:.4e} node-steps/s; passes {pt}")
+    # the same steps without the per-kernel timing events
+    s.reset()
+    s.set_timing(False)
+    tg = time.perf_counter()
+    s.run(steps)
+    untimed_s = time.perf_counter() - tg
+    log(f"[bench] untimed: {steps} steps in {untimed_s:.3f}s -> {prob.N * steps / untimed_s:.4e} "
+        f"node-steps/s")
 
     # Roofline of the force pass's main kernel (the longest launch of a step). Its compulsory HBM
     # traffic, each element once: reads u, theta, v, a (8 B each component) and kind (1 B), writes
@@ -171,6 +180,7 @@ def main():
                    "bonds": nnz, "dt": prob.dt, "exact": bool(args.exact), "kernels": path,
                    "abc": "least-squares G_u/G_v/G_theta (the learned closure is not built)",
                    "streams": "serial" if args.serial else "concurrent",
+                   "untimed_node_steps_per_s": prob.N * steps / untimed_s,
                    "reflected_energy_final": refl,
                    "gate_reflected_le_1e-4": (refl is not None and refl <= 1e-4)},
         "roofline": roofline,

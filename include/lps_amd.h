@@ -125,7 +125,8 @@ double lps_m_ref(const lps_ctx *ctx);
  * kernel (exact path: nodes and ABC rows; fast path: the selected tile kernel), out[2] the rest of
  * the step (fast path: the partial tiles' force kernel, the wait for the ABC rows' G_u / G_v, which
  * run on a second stream beside the tile kernels, and the ABC prediction), out[3] whole step.
- * Timing is enabled with lps_set_timing(ctx, 1); timed steps run without the HIP graph. */
+ * Timing is enabled with lps_set_timing(ctx, 1); timed steps run as separate launches, without
+ * the HIP graph (HIP cannot time events recorded inside a graph). */
 int32_t lps_set_timing(lps_ctx *ctx, int32_t on);
 int32_t lps_pass_times(lps_ctx *ctx, double *out4);
 

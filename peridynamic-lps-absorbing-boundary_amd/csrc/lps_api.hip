@@ -162,6 +162,7 @@ extern "C" {
 static int32_t reset_state(lps_ctx *c);
 static int32_t flush_events(lps_ctx *c);
 static int32_t choose_variant(lps_ctx *c);
+static void drop_graphs(lps_ctx *c);
 
 void lps_default_options(lps_options *opt) {
   opt->theta_method = LPS_THETA_GTHETA;
@@ -691,6 +692,7 @@ static int32_t choose_variant(lps_ctx *c) {
   }
   c->tile.theta_kernel = th;
   c->tile.force_kernel = fo;
+  drop_graphs(c);  // graphs hold the kernels they were captured with
   const char *names[3] = {"generic", "unrolled", "blocked"};
   c->path = std::string("tile kernels: dilatation ") + names[th] + ", force " + names[fo] +
             (tune ? " [autotuned]" : "") + c->path;
@@ -781,7 +783,10 @@ int32_t lps_pass_times(lps_ctx *c, double *out4) {
   return LPS_OK;
 }
 
+// 30 steps from a step that is a multiple of 30 (the buffer rotation repeats every 6 steps),
+// captured once and replayed.
 static int32_t build_graph30(lps_ctx *c) {
+  hipGraphExec_t *out = &c->graph30;
   hipGraph_t g = nullptr;
   CTX_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   const int64_t s0 = c->step;
@@ -791,13 +796,18 @@ static int32_t build_graph30(lps_ctx *c) {
   hipError_t e2 = hipStreamEndCapture(c->stream, &g);
   if (err != hipSuccess) return c->hip_fail(err, "capture of the step loop");
   if (e2 != hipSuccess) return c->hip_fail(e2, "hipStreamEndCapture");
-  hipError_t e3 = hipGraphInstantiate(&c->graph30, g, nullptr, nullptr, 0);
+  hipError_t e3 = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
   if (e3 != hipSuccess) {
-    c->graph30 = nullptr;
+    *out = nullptr;
     return c->hip_fail(e3, "hipGraphInstantiate");
   }
   return LPS_OK;
+}
+
+static void drop_graphs(lps_ctx *c) {
+  if (c->graph30) (void)hipGraphExecDestroy(c->graph30);
+  c->graph30 = nullptr;
 }
 
 int32_t lps_step(lps_ctx *c, int32_t nsteps) {
@@ -805,6 +815,7 @@ int32_t lps_step(lps_ctx *c, int32_t nsteps) {
   int32_t done = 0;
   while (done < nsteps) {
     const int32_t left = nsteps - done;
+    // timed steps run as separate launches: HIP cannot time events recorded inside a graph
     if (c->opt.use_graph && !c->timing && c->step % 30 == 0 && left >= 30) {
       if (!c->graph30) {
         int32_t rc = build_graph30(c);

@@ -30,3 +30,16 @@ def test_create_without_gpu_fails_loudly():
     with pytest.raises(solver.LpsError) as e:
         solver.Solver(pp.reference_test_problem(R=3.0, steps=30))
     assert e.value.code == solver.LPS_ERR_HIP
+
+
+def test_cli_usage_and_missing_input_exit_codes(tmp_path):
+    """PD2_LPS_ABC keeps the reference's exit codes (src/main.cpp:154-160, 178-185): 0 with the
+    usage message, 1 with a FATAL message when a job file is missing. Neither touches the GPU."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(solver.LIB_PATH), "PD2_LPS_ABC")
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0 and "Usage" in r.stdout
+    r = subprocess.run([exe, "_out", "nojob"], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode == 1 and "FATAL: Cannot read nojob.data1" in r.stderr

@@ -1,0 +1,57 @@
+"""The drop-in executable PD2_LPS_ABC on golden jobs written in the reference's file format: the
+.energy/.verify files and the last VTK snapshot match the reference solver's within 1e-10, and a
+non-finite state ends the run with exit code 1 (src/main.cpp:1126-1133)."""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+from conftest import (ENERGY_KEYS, PKG, VERIFY_KEYS, assert_series_close, load_golden, rel_l2,
+                      vtk_index)
+
+from lps_amd import files
+
+pytestmark = pytest.mark.gpu
+
+EXE = os.path.join(PKG, "PD2_LPS_ABC")
+
+
+def run_cli(tmp_path, prob, *opts):
+    job = str(tmp_path / "job")
+    files.write_job(prob, job, exact_text=True)
+    r = subprocess.run([EXE, "_out", "job", *opts], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=240)
+    return job, r
+
+
+@pytest.mark.parametrize("name", ["abc_r10_nu025", "noabc_r30"])
+def test_cli_reproduces_the_reference_outputs(name, tmp_path):
+    print("cli", name, flush=True)
+    sys.stdout.flush()
+    prob, ref = load_golden(name)
+    job, r = run_cli(tmp_path, prob)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert_series_close(files.read_series(job + "_out.energy"), ref["energy"], ENERGY_KEYS, 1e-10,
+                        prob, name)
+    assert_series_close(files.read_series(job + "_out.verify"), ref["verify"], VERIFY_KEYS, 1e-10,
+                        prob, name)
+    s = ref["vtk_step"]
+    u = files.read_vtk_field(f"{job}_out-{s}.vtk")
+    v = files.read_vtk_field(f"{job}_out-vel-{s}.vtk")
+    th = files.read_vtk_field(f"{job}_out-theta-{s}.vtk")
+    assert np.array_equal(u["points"], ref["vtk_points"])
+    assert rel_l2(u["vectors"], ref["vtk_u"]) <= 1e-10
+    assert rel_l2(v["vectors"], ref["vtk_v"]) <= 1e-10
+    assert rel_l2(th["scalars"], ref["vtk_theta"]) <= 1e-10
+    vtk_index(prob.nodes, u["points"])  # every point is a node
+
+
+def test_cli_exits_1_on_a_nonfinite_state(tmp_path):
+    print("cli nonfinite", flush=True)
+    prob, _ = load_golden("abc_r10_nu025")
+    prob.ic = prob.ic.copy()
+    prob.ic[10] = np.nan
+    job, r = run_cli(tmp_path, prob, "--vtk=off")
+    assert r.returncode == 1, r.stdout[-2000:]
+    assert "FATAL" in r.stdout + r.stderr

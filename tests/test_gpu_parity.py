@@ -154,3 +154,24 @@ def test_tile_path_matches_oracle_on_a_larger_domain(variant, r40_problem):
         assert rel_l2(f[k], of[k]) <= 1e-10, k
     assert_series_close(records_table(recs, ENERGY_KEYS), records_table(orecs, ENERGY_KEYS),
                         ENERGY_KEYS, 1e-10, prob, "R40")
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_timing_mode_reports_pass_times_and_changes_no_result(graph, r40_problem):
+    """lps_set_timing: timed steps run as separate launches with per-pass events (also when graphs
+    are on); the state afterwards is the same bits as an untimed run."""
+    say("timing", graph)
+    prob = r40_problem
+    with solver.Solver(prob, use_graph=graph) as s:
+        s.run(60)
+        plain = s.fetch()
+        s.reset()
+        s.set_timing(True)
+        s.run(60)
+        pt = s.pass_times()
+        s.set_timing(False)
+        timed = s.fetch()
+    for k in ("u", "v", "theta"):
+        assert np.array_equal(plain[k], timed[k]), k
+    assert all(pt[k] > 0 for k in ("theta_ms", "force_ms", "step_ms")), pt
+    assert pt["step_ms"] >= pt["theta_ms"] + pt["force_ms"], pt

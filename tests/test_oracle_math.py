@@ -178,3 +178,33 @@ def test_energy_is_conserved_without_abc():
     assert np.max(np.abs(e / e[0] - 1.0)) < 1e-3
     p = np.array([[r["px"], r["py"]] for r in recs])
     assert np.max(np.abs(p)) <= 1e-12 * math.sqrt(2 * e[0] * 8e-15 * 0.25 * len(nodes))
+
+
+def test_idw_abc_dilatation_of_an_isotropic_strain():
+    """IDW at the ABC nodes (src/main.cpp:765-790) for u = eps x: every non-ABC node has
+    theta = 2 eps (its m_i is the sum over its own horizon), so the weighted mean over an ABC
+    node's non-ABC neighbours is exactly 2 eps, while the direct value, biased by the m_i := m_ref
+    override on the truncated horizons of the outer ring, is not."""
+    R, eps, delta = 10.0, 1e-3, 2.0
+    mc = pp.material(0.25)
+    nodes = pp.lattice_domain(R)
+    N = nodes.shape[0]
+    abc = np.nonzero(np.hypot(nodes[:, 0], nodes[:, 1]) > R - delta)[0]
+    C, M = len(abc), 49
+    bctypes = -np.ones(2 * N, dtype=np.int64)
+    bctypes[2 * abc] = 1
+    bctypes[2 * abc + 1] = 1
+    x, y = nodes[:, 0], nodes[:, 1]
+    data2 = np.array([mc.E, 0.25, mc.rho, 0, 0, 0, 1.7e-8, delta, 0.5, 0.5, mc.s0, 0, 0, 0, 0])
+    prob = pp.Problem(nodes=nodes, steps=1, data2=data2, bctypes=bctypes, bcvalues=np.zeros(2 * N),
+                      ic=np.stack([eps * x, eps * y], 1).reshape(-1), icd=np.zeros(2 * N),
+                      abc=abc.astype(np.int64), clouds=np.zeros((C, M), np.int64),
+                      Gu=np.zeros((2 * C, 4 * M)), Gv=np.zeros((2 * C, 4 * M)), Gth=np.zeros((C, 2 * M)))
+    o = pyoracle.Oracle(prob, theta_method=0)
+    direct = o.dilatation(prob.ic)
+    o.run(1)
+    th = o.fetch()["theta"]
+    is_abc = np.zeros(N, bool)
+    is_abc[abc] = True
+    assert np.allclose(th, 2 * eps, rtol=1e-12, atol=0)
+    assert np.max(np.abs(direct[is_abc] / (2 * eps) - 1)) > 1e-2
