@@ -91,6 +91,28 @@ int32_t lps_create(const lps_problem *problem, const lps_options *options, lps_c
  * rebuilding the horizons. */
 int32_t lps_reset(lps_ctx *ctx);
 
+/* Batches. No counterpart in the reference, which runs one job per process: many independent
+ * problems (own radius, nu, dt, ABC operators, initial conditions; the same dx, dy and delta)
+ * advanced together, every launch of a step covering all of them. Each problem is set up as
+ * lps_create would (GPU horizons, m_i, tile check, theta history, prediction), then the instances
+ * are packed into one slot space. Batches run the tile path only: every problem must be a lattice
+ * with the 68-bond stencil (LPS_ERR_ARG otherwise); options->variant 2 selects the unrolled node
+ * kernels, anything else the register-blocked ones. lps_step, lps_reset, lps_set_timing,
+ * lps_pass_times apply as for one problem; lps_run, lps_diagnostics, lps_fetch and lps_horizons
+ * return LPS_ERR_ARG on a batch. */
+int32_t lps_create_batch(const lps_problem *problems, int32_t count, const lps_options *options,
+                         lps_ctx **out);
+int32_t lps_instance_count(const lps_ctx *ctx);
+/* lps_run for every instance: records[k * count + i] is output record k of instance i (E0, dE/E0,
+ * time and ABC-row RMS per instance). Returns the number of output steps, or LPS_ERR_NONFINITE
+ * (the message names the instance). Works on a single-problem context too (count 1). */
+int32_t lps_run_batch(lps_ctx *ctx, int32_t nsteps, lps_diag *records, int32_t max_records);
+/* The output block of every instance at the current step (count records). Tile path only. */
+int32_t lps_diagnostics_batch(lps_ctx *ctx, lps_diag *out);
+/* lps_fetch of one instance, in that problem's node order. */
+int32_t lps_fetch_instance(lps_ctx *ctx, int32_t instance, double *u, double *v, double *a,
+                           double *theta, double *m);
+
 /* Advance nsteps steps of the time loop (src/main.cpp:718-1164, steps 1-4 and 7). Stream-ordered;
  * returns without waiting for the device. */
 int32_t lps_step(lps_ctx *ctx, int32_t nsteps);

@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <unordered_set>
 #include <vector>
 
@@ -40,6 +41,18 @@ struct lps_ctx {
   bool fast = false;
   bool stencil68_ok = false;  // the unrolled 68-bond kernels apply to this problem
   int64_t n_full_nodes = 0;   // nodes in the full tiles
+  // instances: one for lps_create, several for lps_create_batch
+  std::vector<InstParams> inst_h;
+  InstParams *inst_d = nullptr;
+  bool batch = false;
+  int32_t n_inst = 1;
+  std::vector<int32_t> tile_off, abc_n, node_n;  // per instance (tile_off: n_inst + 1 entries)
+  std::vector<int64_t> slot_off, node_off;       // n_inst + 1 entries
+  std::vector<double> E0_inst;
+  int32_t *tile_off_d = nullptr;
+  DiagPartial *dout_inst = nullptr, *hout_inst = nullptr;
+  Dev2 P0;                        // batch: prediction of step 1 and theta history of the setup
+  double *TH0 = nullptr;
   int64_t step = 0;
   std::vector<int32_t> perm_h;  // slot -> original index, -1 at holes
   HorizonBuild hb;
@@ -106,6 +119,7 @@ StepArgs lps_ctx::args() const {
   a.m = m; a.c2m = c2m; a.is_bc = is_bc;
   a.cloud = cloud; a.gu = gu; a.gv = gv; a.gth = gth;
   a.theta_method = opt.theta_method;
+  a.inst = inst_d;
   a.tile = tile;
   return a;
 }
@@ -200,6 +214,7 @@ void lps_destroy(lps_ctx *ctx) {
   for (void *p : ctx->allocs) (void)hipFree(p);
   free_horizons(&ctx->hb);
   if (ctx->hout) (void)hipHostFree(ctx->hout);
+  if (ctx->hout_inst) (void)hipHostFree(ctx->hout_inst);
   if (ctx->stream2) (void)hipStreamSynchronize(ctx->stream2);
   for (int q = 0; q < 2; q++) {
     if (ctx->ev_fork[q]) (void)hipEventDestroy(ctx->ev_fork[q]);
@@ -377,6 +392,10 @@ static int32_t create_impl(lps_ctx *c, const lps_problem *p) {
     }
     c->exclude_r = abc_on ? domainRadius - c->delta - c->dx / 100.0 : -1.0;
   }
+  // the problem's instance entry (the fast path's kernels read the material through it)
+  InstParams ip{};
+  ip.dt = c->dt; ip.rho = c->rho; ip.lam = c->lam; ip.mu = c->mu; ip.kappa = c->kappa;
+  ip.exclude_r = c->exclude_r;
 
   // ---- tile path: the stencil of a full horizon and the check against the CSR ----
   c->fast = false;
@@ -458,11 +477,15 @@ static int32_t create_impl(lps_ctx *c, const lps_problem *p) {
         const double ps[3] = {(double)(ra * ra), (double)(ra * rb), (double)(rb * rb)};
         for (int32_t z = 0; z < 2; z++) {
           c->tile.wt[q][z] = C * c->tile.c1[q] * ms[z];
-          c->tile.wa[q][z] = (c->lam - c->mu) * C * c->tile.c1[q] * ms[z];
+          ip.wa[q][z] = (c->lam - c->mu) * C * c->tile.c1[q] * ms[z];
         }
-        for (int32_t z = 0; z < 3; z++) c->tile.wb[q][z] = 8.0 * c->mu * C * c->tile.c2[q] * ps[z];
+        for (int32_t z = 0; z < 3; z++) ip.wb[q][z] = 8.0 * c->mu * C * c->tile.c2[q] * ps[z];
       }
     }
+    std::vector<int32_t> tinst(c->tile.n_tiles, 0);
+    int32_t *d_tinst;
+    CTX_TRY(c, c->upload(&d_tinst, tinst));
+    c->tile.tile_inst = d_tinst;
     int32_t *d_txy, *d_tmap, *d_delta, *d_nbr;
     int8_t *d_full;
     double *d_hx, *d_hy, *d_g1, *d_g2, *d_xi;
@@ -569,6 +592,8 @@ static int32_t create_impl(lps_ctx *c, const lps_problem *p) {
     CTX_TRY(c, c->upload(&c->ICD.y, vy));
     CTX_TRY(c, hipStreamSynchronize(s));
   }
+  c->inst_h.assign(1, ip);
+  CTX_TRY(c, c->upload(&c->inst_d, c->inst_h));
   // output block buffers
   c->n_partials = diag_partials(NS);
   CTX_TRY(c, c->alloc(&c->partials, std::max(c->n_partials, c->tile.n_tiles)));
@@ -582,6 +607,31 @@ static int32_t reset_state(lps_ctx *c) {
   const int32_t NS = c->n_slots;
   const hipStream_t s = c->stream;
   const size_t bytes = sizeof(double) * NS;
+  if (c->batch) {  // the instances' setup state, kept from lps_create_batch
+    for (int b = 0; b < 3; b++) {
+      CTX_TRY(c, hipMemsetAsync(c->U[b].x, 0, bytes, s));
+      CTX_TRY(c, hipMemsetAsync(c->U[b].y, 0, bytes, s));
+      CTX_TRY(c, hipMemsetAsync(c->V[b].x, 0, bytes, s));
+      CTX_TRY(c, hipMemsetAsync(c->V[b].y, 0, bytes, s));
+      CTX_TRY(c, hipMemsetAsync(c->TH[b], 0, bytes, s));
+    }
+    for (int b = 0; b < 2; b++) {
+      CTX_TRY(c, hipMemsetAsync(c->A[b].x, 0, bytes, s));
+      CTX_TRY(c, hipMemsetAsync(c->A[b].y, 0, bytes, s));
+    }
+    CTX_TRY(c, hipMemcpyAsync(c->U[0].x, c->IC.x, bytes, hipMemcpyDeviceToDevice, s));
+    CTX_TRY(c, hipMemcpyAsync(c->U[0].y, c->IC.y, bytes, hipMemcpyDeviceToDevice, s));
+    CTX_TRY(c, hipMemcpyAsync(c->V[0].x, c->ICD.x, bytes, hipMemcpyDeviceToDevice, s));
+    CTX_TRY(c, hipMemcpyAsync(c->V[0].y, c->ICD.y, bytes, hipMemcpyDeviceToDevice, s));
+    CTX_TRY(c, hipMemcpyAsync(c->TH[0], c->TH0, bytes, hipMemcpyDeviceToDevice, s));
+    CTX_TRY(c, hipMemcpyAsync(c->P[0].x, c->P0.x, bytes, hipMemcpyDeviceToDevice, s));
+    CTX_TRY(c, hipMemcpyAsync(c->P[0].y, c->P0.y, bytes, hipMemcpyDeviceToDevice, s));
+    CTX_TRY(c, hipStreamSynchronize(s));
+    c->step = 0;
+    c->E0 = -1.0;
+    c->E0_inst.assign(c->n_inst, -1.0);
+    return LPS_OK;
+  }
   for (int b = 0; b < 3; b++) {
     CTX_TRY(c, hipMemsetAsync(c->U[b].x, 0, bytes, s));
     CTX_TRY(c, hipMemsetAsync(c->U[b].y, 0, bytes, s));
@@ -616,12 +666,8 @@ int32_t lps_reset(lps_ctx *c) {
   return reset_state(c);
 }
 
-int32_t lps_create(const lps_problem *problem, const lps_options *options, lps_ctx **out) {
-  *out = nullptr;
-  if (!problem) {
-    g_create_error = "problem is NULL";
-    return LPS_ERR_ARG;
-  }
+// A context with its device, streams and events, nothing else.
+static int32_t new_context(const lps_options *options, lps_ctx **out) {
   lps_ctx *c = new lps_ctx();
   if (options) c->opt = *options;
   else lps_default_options(&c->opt);
@@ -647,10 +693,23 @@ int32_t lps_create(const lps_problem *problem, const lps_options *options, lps_c
   }
   if (e != hipSuccess) {
     g_create_error = std::string("HIP setup failed: ") + hipGetErrorString(e);
-    delete c;
+    lps_destroy(c);
     return LPS_ERR_HIP;
   }
-  int32_t rc = create_impl(c, problem);
+  *out = c;
+  return LPS_OK;
+}
+
+int32_t lps_create(const lps_problem *problem, const lps_options *options, lps_ctx **out) {
+  *out = nullptr;
+  if (!problem) {
+    g_create_error = "problem is NULL";
+    return LPS_ERR_ARG;
+  }
+  lps_ctx *c = nullptr;
+  int32_t rc = new_context(options, &c);
+  if (rc != LPS_OK) return rc;
+  rc = create_impl(c, problem);
   if (rc == LPS_OK && c->fast) rc = choose_variant(c);
   if (rc != LPS_OK) {
     g_create_error = c->err;
@@ -660,6 +719,274 @@ int32_t lps_create(const lps_problem *problem, const lps_options *options, lps_c
   *out = c;
   return LPS_OK;
 }
+
+// ---- batches: instances set up one by one (lps_create), then packed into one context ----
+
+extern "C++" {
+// dst[total] = the parts one after another (device to device).
+template <class T>
+static int32_t cat_dev(lps_ctx *c, T **dst, size_t total,
+                       const std::vector<std::pair<const T *, size_t>> &parts) {
+  CTX_TRY(c, c->alloc(dst, total));
+  size_t off = 0;
+  for (const auto &pt : parts) {
+    if (pt.second)
+      CTX_TRY(c, hipMemcpyAsync(*dst + off, pt.first, sizeof(T) * pt.second,
+                                hipMemcpyDeviceToDevice, c->stream));
+    off += pt.second;
+  }
+  return LPS_OK;
+}
+
+template <class T>
+static int32_t to_host(lps_ctx *c, std::vector<T> &h, const T *d, size_t n) {
+  h.resize(n);
+  if (n) CTX_TRY(c, hipMemcpy(h.data(), d, sizeof(T) * n, hipMemcpyDeviceToHost));
+  return LPS_OK;
+}
+}  // extern "C++"
+
+static int32_t merge_batch(lps_ctx *c, const std::vector<lps_ctx *> &sub) {
+  const int32_t B = (int32_t)sub.size();
+  const lps_ctx &s0 = *sub[0];
+  for (int32_t i = 0; i < B; i++) {
+    const lps_ctx &q = *sub[i];
+    if (!q.fast || !q.stencil68_ok)
+      return c->fail(LPS_ERR_ARG, "batch instance " + std::to_string(i) +
+                                      ": a batch needs lattice inputs on the 68-bond tile path");
+    if (q.dx != s0.dx || q.dy != s0.dy || q.delta != s0.delta || q.m_ref != s0.m_ref ||
+        q.tile.halo != s0.tile.halo)
+      return c->fail(LPS_ERR_ARG, "batch instance " + std::to_string(i) +
+                                      ": every instance needs the same dx, dy and delta");
+  }
+  c->batch = true;
+  c->fast = true;
+  c->stencil68_ok = true;
+  c->n_inst = B;
+  // scalars shared by the instances (the fast kernels take the material from InstParams)
+  c->E = s0.E; c->nu = s0.nu; c->rho = s0.rho; c->dt = s0.dt;
+  c->delta = s0.delta; c->dx = s0.dx; c->dy = s0.dy;
+  c->lam = s0.lam; c->mu = s0.mu; c->kappa = s0.kappa; c->m_ref = s0.m_ref;
+  c->exclude_r = s0.exclude_r;
+  // offsets
+  c->slot_off.assign(B + 1, 0);
+  c->node_off.assign(B + 1, 0);
+  c->tile_off.assign(B + 1, 0);
+  c->abc_n.resize(B);
+  c->node_n.resize(B);
+  int32_t M = 0, n_full = 0, n_part = 0;
+  for (int32_t i = 0; i < B; i++) {
+    c->slot_off[i + 1] = c->slot_off[i] + sub[i]->n_slots;
+    c->node_off[i + 1] = c->node_off[i] + sub[i]->n;
+    c->tile_off[i + 1] = c->tile_off[i] + sub[i]->tile.n_tiles;
+    c->abc_n[i] = sub[i]->n_abc;
+    c->node_n[i] = sub[i]->n;
+    M = std::max(M, sub[i]->M);
+    n_full += sub[i]->tile.n_full;
+    n_part += sub[i]->tile.n_part;
+    c->n_full_nodes += sub[i]->n_full_nodes;
+  }
+  if (c->slot_off[B] > INT32_MAX || c->node_off[B] > INT32_MAX)
+    return c->fail(LPS_ERR_ARG, "a batch holds at most 2^31 - 1 slots");
+  const int32_t NS = (int32_t)c->slot_off[B], NT = c->tile_off[B];
+  c->n_slots = NS;
+  c->n = (int32_t)c->node_off[B];
+  c->M = M;
+  // per-slot arrays
+  auto parts_d = [&](auto get) {
+    using T = std::remove_const_t<std::remove_pointer_t<decltype(get(*sub[0]))>>;
+    std::vector<std::pair<const T *, size_t>> v;
+    for (int32_t i = 0; i < B; i++) v.push_back({get(*sub[i]), (size_t)sub[i]->n_slots});
+    return v;
+  };
+  int32_t rc;
+  if ((rc = cat_dev(c, &c->x, NS, parts_d([](const lps_ctx &q) { return (const double *)q.x; }))) ||
+      (rc = cat_dev(c, &c->y, NS, parts_d([](const lps_ctx &q) { return (const double *)q.y; }))) ||
+      (rc = cat_dev(c, &c->kind, NS, parts_d([](const lps_ctx &q) { return (const int8_t *)q.kind; }))) ||
+      (rc = cat_dev(c, &c->is_bc, NS, parts_d([](const lps_ctx &q) { return (const int8_t *)q.is_bc; }))) ||
+      (rc = cat_dev(c, &c->m, NS, parts_d([](const lps_ctx &q) { return (const double *)q.m; }))) ||
+      (rc = cat_dev(c, &c->c2m, NS, parts_d([](const lps_ctx &q) { return (const double *)q.c2m; }))) ||
+      (rc = cat_dev(c, &c->perm_d, NS, parts_d([](const lps_ctx &q) { return (const int32_t *)q.perm_d; }))) ||
+      (rc = cat_dev(c, &c->IC.x, NS, parts_d([](const lps_ctx &q) { return (const double *)q.IC.x; }))) ||
+      (rc = cat_dev(c, &c->IC.y, NS, parts_d([](const lps_ctx &q) { return (const double *)q.IC.y; }))) ||
+      (rc = cat_dev(c, &c->ICD.x, NS, parts_d([](const lps_ctx &q) { return (const double *)q.ICD.x; }))) ||
+      (rc = cat_dev(c, &c->ICD.y, NS, parts_d([](const lps_ctx &q) { return (const double *)q.ICD.y; }))) ||
+      (rc = cat_dev(c, &c->TH0, NS, parts_d([](const lps_ctx &q) { return (const double *)q.TH[0]; }))) ||
+      (rc = cat_dev(c, &c->P0.x, NS, parts_d([](const lps_ctx &q) { return (const double *)q.P[0].x; }))) ||
+      (rc = cat_dev(c, &c->P0.y, NS, parts_d([](const lps_ctx &q) { return (const double *)q.P[0].y; }))))
+    return rc;
+  // slot -> node of the batch (instances' nodes one after another); perm_d stays per instance
+  c->perm_h.assign(NS, -1);
+  for (int32_t i = 0; i < B; i++)
+    for (int32_t k = 0; k < sub[i]->n_slots; k++) {
+      const int32_t o = sub[i]->perm_h[k];
+      c->perm_h[c->slot_off[i] + k] = o < 0 ? -1 : (int32_t)(o + c->node_off[i]);
+    }
+  // tiles: neighbour tables and lists shifted by the instances' tile offsets
+  std::vector<int32_t> nbr, full_list, part_list, tinst(NT), txy;
+  std::vector<int8_t> tfull;
+  for (int32_t i = 0; i < B; i++) {
+    const TileArgs &T = sub[i]->tile;
+    std::vector<int32_t> h;
+    if ((rc = to_host(c, h, T.tile_nbr, (size_t)9 * T.n_tiles))) return rc;
+    for (int32_t v : h) nbr.push_back(v < 0 ? -1 : v + c->tile_off[i]);
+    if ((rc = to_host(c, h, T.full_list, (size_t)T.n_full))) return rc;
+    for (int32_t v : h) full_list.push_back(v + c->tile_off[i]);
+    if ((rc = to_host(c, h, T.part_list, (size_t)T.n_part))) return rc;
+    for (int32_t v : h) part_list.push_back(v + c->tile_off[i]);
+    if ((rc = to_host(c, h, T.tile_xy, (size_t)T.n_tiles))) return rc;
+    txy.insert(txy.end(), h.begin(), h.end());
+    std::vector<int8_t> f;
+    if ((rc = to_host(c, f, T.tile_full, (size_t)T.n_tiles))) return rc;
+    tfull.insert(tfull.end(), f.begin(), f.end());
+    for (int32_t t = 0; t < T.n_tiles; t++) tinst[c->tile_off[i] + t] = i;
+  }
+  TileArgs &T = c->tile;
+  T = s0.tile;  // stencil constants, halo, window, class weights
+  T.n_tiles = NT;
+  T.ntx = T.nty = 0;
+  T.tile_map = nullptr;  // one lattice per instance: the batch kernels use the neighbour tables
+  T.n_full = n_full;
+  T.n_part = n_part;
+  T.theta_kernel = s0.tile.theta_kernel;
+  T.force_kernel = s0.tile.force_kernel;
+  int32_t *d_nbr, *d_fl, *d_pl, *d_ti, *d_txy;
+  int8_t *d_full;
+  CTX_TRY(c, c->upload(&d_nbr, nbr));
+  CTX_TRY(c, c->upload(&d_fl, full_list));
+  CTX_TRY(c, c->upload(&d_pl, part_list));
+  CTX_TRY(c, c->upload(&d_ti, tinst));
+  CTX_TRY(c, c->upload(&d_txy, txy));
+  CTX_TRY(c, c->upload(&d_full, tfull));
+  T.tile_nbr = d_nbr; T.full_list = d_fl; T.part_list = d_pl; T.tile_inst = d_ti;
+  T.tile_xy = d_txy; T.tile_full = d_full;
+  {  // the stencil tables (the same for every instance)
+    const size_t nt = (size_t)s0.tile.nt;
+    int32_t *d_delta;
+    double *d_hx, *d_hy, *d_g1, *d_g2, *d_xi;
+    if ((rc = cat_dev(c, &d_delta, nt, {{s0.tile.delta, nt}})) ||
+        (rc = cat_dev(c, &d_hx, nt, {{s0.tile.hx, nt}})) ||
+        (rc = cat_dev(c, &d_hy, nt, {{s0.tile.hy, nt}})) ||
+        (rc = cat_dev(c, &d_g1, nt, {{s0.tile.g1, nt}})) ||
+        (rc = cat_dev(c, &d_g2, nt, {{s0.tile.g2, nt}})) ||
+        (rc = cat_dev(c, &d_xi, nt, {{s0.tile.xi, nt}})))
+      return rc;
+    T.delta = d_delta; T.hx = d_hx; T.hy = d_hy; T.g1 = d_g1; T.g2 = d_g2; T.xi = d_xi;
+  }
+  // ABC rows: slots shifted, clouds and operators padded to the widest cloud
+  int32_t R = 0;
+  for (int32_t i = 0; i < B; i++) R += sub[i]->n_abc;
+  c->n_abc = R;
+  if (R > 0) {
+    std::vector<int32_t> aslot, cl((size_t)R * M, -1);
+    CTX_TRY(c, c->alloc(&c->gu, (size_t)R * 8 * M));
+    CTX_TRY(c, c->alloc(&c->gv, (size_t)R * 8 * M));
+    CTX_TRY(c, hipMemsetAsync(c->gu, 0, sizeof(double) * R * 8 * M, c->stream));
+    CTX_TRY(c, hipMemsetAsync(c->gv, 0, sizeof(double) * R * 8 * M, c->stream));
+    const bool gth = c->opt.theta_method == LPS_THETA_GTHETA;
+    if (gth) {
+      CTX_TRY(c, c->alloc(&c->gth, (size_t)R * 2 * M));
+      CTX_TRY(c, hipMemsetAsync(c->gth, 0, sizeof(double) * R * 2 * M, c->stream));
+    }
+    int32_t r0 = 0;
+    for (int32_t i = 0; i < B; i++) {
+      const lps_ctx &q = *sub[i];
+      const int32_t Ri = q.n_abc, Mi = q.M;
+      if (Ri == 0) continue;
+      std::vector<int32_t> h;
+      if ((rc = to_host(c, h, q.abc_slot, (size_t)Ri))) return rc;
+      for (int32_t v : h) aslot.push_back((int32_t)(v + c->slot_off[i]));
+      if ((rc = to_host(c, h, q.cloud, (size_t)Ri * Mi))) return rc;
+      for (int32_t r = 0; r < Ri; r++)
+        for (int32_t j = 0; j < Mi; j++) {
+          const int32_t v = h[(size_t)r * Mi + j];
+          cl[(size_t)(r0 + r) * M + j] = v < 0 ? -1 : (int32_t)(v + c->slot_off[i]);
+        }
+      // G_u / G_v rows: two component blocks of 4 Mi doubles each, placed at 0 and 4 M
+      CTX_TRY(c, hipMemcpy2DAsync(c->gu + (size_t)r0 * 8 * M, sizeof(double) * 4 * M, q.gu,
+                                  sizeof(double) * 4 * Mi, sizeof(double) * 4 * Mi, 2 * (size_t)Ri,
+                                  hipMemcpyDeviceToDevice, c->stream));
+      CTX_TRY(c, hipMemcpy2DAsync(c->gv + (size_t)r0 * 8 * M, sizeof(double) * 4 * M, q.gv,
+                                  sizeof(double) * 4 * Mi, sizeof(double) * 4 * Mi, 2 * (size_t)Ri,
+                                  hipMemcpyDeviceToDevice, c->stream));
+      if (gth)
+        CTX_TRY(c, hipMemcpy2DAsync(c->gth + (size_t)r0 * 2 * M, sizeof(double) * 2 * M, q.gth,
+                                    sizeof(double) * 2 * Mi, sizeof(double) * 2 * Mi, (size_t)Ri,
+                                    hipMemcpyDeviceToDevice, c->stream));
+      r0 += Ri;
+    }
+    CTX_TRY(c, c->upload(&c->abc_slot, aslot));
+    CTX_TRY(c, c->upload(&c->cloud, cl));
+  }
+  // instances
+  c->inst_h.clear();
+  for (int32_t i = 0; i < B; i++) c->inst_h.push_back(sub[i]->inst_h[0]);
+  CTX_TRY(c, c->upload(&c->inst_d, c->inst_h));
+  CTX_TRY(c, c->upload(&c->tile_off_d, c->tile_off));
+  // state and output buffers
+  for (int b = 0; b < 3; b++) {
+    CTX_TRY(c, c->alloc(&c->U[b].x, NS)); CTX_TRY(c, c->alloc(&c->U[b].y, NS));
+    CTX_TRY(c, c->alloc(&c->V[b].x, NS)); CTX_TRY(c, c->alloc(&c->V[b].y, NS));
+    CTX_TRY(c, c->alloc(&c->TH[b], NS));
+  }
+  for (int b = 0; b < 2; b++) {
+    CTX_TRY(c, c->alloc(&c->A[b].x, NS)); CTX_TRY(c, c->alloc(&c->A[b].y, NS));
+    CTX_TRY(c, c->alloc(&c->P[b].x, NS)); CTX_TRY(c, c->alloc(&c->P[b].y, NS));
+  }
+  CTX_TRY(c, c->alloc(&c->partials, NT));
+  CTX_TRY(c, c->alloc(&c->dout, 1));
+  CTX_TRY(c, c->alloc(&c->dout_inst, B));
+  CTX_TRY(c, hipHostMalloc((void **)&c->hout, sizeof(DiagPartial)));
+  CTX_TRY(c, hipHostMalloc((void **)&c->hout_inst, sizeof(DiagPartial) * B));
+  CTX_TRY(c, hipStreamSynchronize(c->stream));
+  const char *names[3] = {"generic", "unrolled", "blocked"};
+  c->path = "batch of " + std::to_string(B) + " instances: tile kernels: dilatation " +
+            names[T.theta_kernel] + ", force " + names[T.force_kernel] + " (16x16 tiles, " +
+            std::to_string(NT) + " tiles, " + std::to_string(n_full) + " full, " +
+            std::to_string(c->n) + " nodes, " + std::to_string(R) + " ABC rows)";
+  return reset_state(c);
+}
+
+int32_t lps_create_batch(const lps_problem *problems, int32_t count, const lps_options *options,
+                         lps_ctx **out) {
+  *out = nullptr;
+  if (!problems || count < 1) {
+    g_create_error = "a batch needs at least one problem";
+    return LPS_ERR_ARG;
+  }
+  lps_options opt;
+  if (options) opt = *options;
+  else lps_default_options(&opt);
+  if (opt.exact) {
+    g_create_error = "batches run the tile path only (exact = 0)";
+    return LPS_ERR_ARG;
+  }
+  // the instances run the register-blocked kernels unless the unrolled ones are asked for
+  lps_options sub_opt = opt;
+  sub_opt.variant = opt.variant == 2 ? 2 : 3;
+  sub_opt.use_graph = 0;
+  lps_ctx *c = nullptr;
+  int32_t rc = new_context(&opt, &c);
+  if (rc != LPS_OK) return rc;
+  std::vector<lps_ctx *> sub;
+  for (int32_t i = 0; i < count && rc == LPS_OK; i++) {
+    lps_ctx *q = nullptr;
+    rc = lps_create(&problems[i], &sub_opt, &q);
+    if (rc != LPS_OK) c->err = "batch instance " + std::to_string(i) + ": " + g_create_error;
+    else sub.push_back(q);
+  }
+  if (rc == LPS_OK) rc = merge_batch(c, sub);
+  for (lps_ctx *q : sub) lps_destroy(q);
+  if (rc != LPS_OK) {
+    g_create_error = c->err;
+    lps_destroy(c);
+    return rc;
+  }
+  *out = c;
+  return LPS_OK;
+}
+
+int32_t lps_instance_count(const lps_ctx *ctx) { return ctx->n_inst; }
 
 // Fast path: pick the node kernels of each pass. With variant 0 every candidate (generic tile
 // kernels, unrolled 68-bond kernels, register-blocked 68-bond kernels) runs a few timed steps on
@@ -836,22 +1163,11 @@ int32_t lps_step(lps_ctx *c, int32_t nsteps) {
   return LPS_OK;
 }
 
-int32_t lps_diagnostics(lps_ctx *c, lps_diag *out) {
+static int32_t diag_fill(lps_ctx *c, const DiagPartial &d, double dt, double E0, int32_t n_abc,
+                         lps_diag *out) {
   const int64_t s = c->step;
-  const StepArgs a = c->args();
-  const Dev2 &U = c->U[s % 3], &V = c->V[s % 3], &A = c->A[s % 2];
-  const double *th = c->TH[s % 3];
-  if (c->fast)
-    CTX_TRY(c, launch_diag_tile(a, U.x, U.y, V.x, V.y, A.x, A.y, th, c->perm_d, c->exclude_r,
-                                c->partials, c->dout, c->stream));
-  else
-    CTX_TRY(c, launch_diag(a, U.x, U.y, V.x, V.y, A.x, A.y, th, c->perm_d, c->exclude_r,
-                           c->partials, c->n_partials, c->dout, c->stream));
-  CTX_TRY(c, hipMemcpyAsync(c->hout, c->dout, sizeof(DiagPartial), hipMemcpyDeviceToHost, c->stream));
-  CTX_TRY(c, hipStreamSynchronize(c->stream));
-  const DiagPartial &d = *c->hout;
   out->step = s;
-  out->t = (double)s * c->dt;
+  out->t = (double)s * dt;
   out->energy = d.energy;
   out->energy_k = d.ek;
   out->energy_p = d.ep;
@@ -862,15 +1178,93 @@ int32_t lps_diagnostics(lps_ctx *c, lps_diag *out) {
   out->max_a = d.max_a;
   out->theta_max = d.th_max;
   out->theta_min = d.th_min;
-  out->ub_rms = c->n_abc > 0 ? std::sqrt(d.ub / c->n_abc) : 0.0;
-  out->vb_rms = c->n_abc > 0 ? std::sqrt(d.vb / c->n_abc) : 0.0;
+  out->ub_rms = n_abc > 0 ? std::sqrt(d.ub / n_abc) : 0.0;
+  out->vb_rms = n_abc > 0 ? std::sqrt(d.vb / n_abc) : 0.0;
   out->max_u_node = d.max_u_orig == INT64_MAX ? -1 : (int32_t)d.max_u_orig;
   out->max_v_node = d.max_v_orig == INT64_MAX ? -1 : (int32_t)d.max_v_orig;
-  out->de_rel = (c->E0 > 0.0) ? (d.energy - c->E0) / c->E0 : 0.0;
+  out->de_rel = (E0 > 0.0) ? (d.energy - E0) / E0 : 0.0;
   return LPS_OK;
 }
 
+int32_t lps_diagnostics_batch(lps_ctx *c, lps_diag *out) {
+  const int64_t s = c->step;
+  const StepArgs a = c->args();
+  if (!c->fast) return c->fail(LPS_ERR_ARG, "per-instance diagnostics need the tile path");
+  const Dev2 &U = c->U[s % 3], &V = c->V[s % 3], &A = c->A[s % 2];
+  const double *th = c->TH[s % 3];
+  if (c->batch) {
+    CTX_TRY(c, launch_diag_tile(a, U.x, U.y, V.x, V.y, A.x, A.y, th, c->perm_d, c->partials,
+                                nullptr, c->stream));
+    CTX_TRY(c, launch_diag_final_inst(c->partials, c->tile_off_d, c->n_inst, c->dout_inst,
+                                      c->stream));
+    CTX_TRY(c, hipMemcpyAsync(c->hout_inst, c->dout_inst, sizeof(DiagPartial) * c->n_inst,
+                              hipMemcpyDeviceToHost, c->stream));
+    CTX_TRY(c, hipStreamSynchronize(c->stream));
+    if ((int32_t)c->E0_inst.size() != c->n_inst) c->E0_inst.assign(c->n_inst, -1.0);
+    for (int32_t i = 0; i < c->n_inst; i++)
+      diag_fill(c, c->hout_inst[i], c->inst_h[i].dt, c->E0_inst[i], c->abc_n[i], &out[i]);
+    return LPS_OK;
+  }
+  return lps_diagnostics(c, out);
+}
+
+int32_t lps_run_batch(lps_ctx *c, int32_t nsteps, lps_diag *records, int32_t max_records) {
+  if (nsteps < 0) return c->fail(LPS_ERR_ARG, "nsteps must be >= 0");
+  const int32_t B = c->n_inst;
+  if ((int32_t)c->E0_inst.size() != B) c->E0_inst.assign(B, -1.0);
+  std::vector<lps_diag> d(B);
+  int32_t done = 0, nrec = 0;
+  while (done < nsteps) {
+    const int32_t to_out = 30 - (int32_t)(c->step % 30);
+    const int32_t chunk = std::min(to_out, nsteps - done);
+    int32_t rc = lps_step(c, chunk);
+    if (rc != LPS_OK) return rc;
+    done += chunk;
+    if (c->step % 30 == 0) {
+      rc = c->batch ? lps_diagnostics_batch(c, d.data()) : lps_diagnostics(c, d.data());
+      if (rc != LPS_OK) return rc;
+      for (int32_t i = 0; i < B; i++) {
+        double &E0 = c->batch ? c->E0_inst[i] : c->E0;
+        if (E0 < 0.0) E0 = d[i].energy;  // src/main.cpp:1046, per instance
+        d[i].de_rel = (E0 > 0.0) ? (d[i].energy - E0) / E0 : 0.0;
+        if (records && nrec < max_records) records[(size_t)nrec * B + i] = d[i];
+      }
+      nrec++;
+      for (int32_t i = 0; i < B; i++)
+        if (std::isnan(d[i].energy) || std::isinf(d[i].energy) || std::isnan(d[i].max_u) ||
+            std::isinf(d[i].max_u))
+          return c->fail(LPS_ERR_NONFINITE, "instance " + std::to_string(i) +
+                                                ": energy or displacement is NaN/Inf at step " +
+                                                std::to_string(c->step));
+    }
+  }
+  if (c->timing) {
+    int32_t rc = flush_events(c);
+    if (rc != LPS_OK) return rc;
+  }
+  CTX_TRY(c, hipStreamSynchronize(c->stream));
+  return nrec;
+}
+
+int32_t lps_diagnostics(lps_ctx *c, lps_diag *out) {
+  if (c->batch) return c->fail(LPS_ERR_ARG, "a batched context reports per instance: use lps_diagnostics_batch");
+  const int64_t s = c->step;
+  const StepArgs a = c->args();
+  const Dev2 &U = c->U[s % 3], &V = c->V[s % 3], &A = c->A[s % 2];
+  const double *th = c->TH[s % 3];
+  if (c->fast)
+    CTX_TRY(c, launch_diag_tile(a, U.x, U.y, V.x, V.y, A.x, A.y, th, c->perm_d, c->partials,
+                                c->dout, c->stream));
+  else
+    CTX_TRY(c, launch_diag(a, U.x, U.y, V.x, V.y, A.x, A.y, th, c->perm_d, c->exclude_r,
+                           c->partials, c->n_partials, c->dout, c->stream));
+  CTX_TRY(c, hipMemcpyAsync(c->hout, c->dout, sizeof(DiagPartial), hipMemcpyDeviceToHost, c->stream));
+  CTX_TRY(c, hipStreamSynchronize(c->stream));
+  return diag_fill(c, *c->hout, c->dt, c->E0, c->n_abc, out);
+}
+
 int32_t lps_run(lps_ctx *c, int32_t nsteps, lps_diag *records, int32_t max_records) {
+  if (c->batch) return c->fail(LPS_ERR_ARG, "a batched context runs with lps_run_batch");
   if (nsteps < 0) return c->fail(LPS_ERR_ARG, "nsteps must be >= 0");
   int32_t done = 0, nrec = 0;
   while (done < nsteps) {
@@ -900,29 +1294,31 @@ int32_t lps_run(lps_ctx *c, int32_t nsteps, lps_diag *records, int32_t max_recor
   return nrec;
 }
 
-int32_t lps_fetch(lps_ctx *c, double *u, double *v, double *a, double *theta, double *m) {
+// Fields of the slots [s0, s1) scattered to node order; node ids are perm_h minus node0.
+static int32_t fetch_range(lps_ctx *c, int64_t s0, int64_t s1, int64_t node0, double *u, double *v,
+                           double *a, double *theta, double *m) {
   const int64_t s = c->step;
-  const int32_t NS = c->n_slots;
+  const size_t NS = (size_t)(s1 - s0);
   std::vector<double> bx(NS), by(NS);
   auto pull2 = [&](const Dev2 &d, double *out) -> int32_t {
     if (!out) return LPS_OK;
-    CTX_TRY(c, hipMemcpyAsync(bx.data(), d.x, sizeof(double) * NS, hipMemcpyDeviceToHost, c->stream));
-    CTX_TRY(c, hipMemcpyAsync(by.data(), d.y, sizeof(double) * NS, hipMemcpyDeviceToHost, c->stream));
+    CTX_TRY(c, hipMemcpyAsync(bx.data(), d.x + s0, sizeof(double) * NS, hipMemcpyDeviceToHost, c->stream));
+    CTX_TRY(c, hipMemcpyAsync(by.data(), d.y + s0, sizeof(double) * NS, hipMemcpyDeviceToHost, c->stream));
     CTX_TRY(c, hipStreamSynchronize(c->stream));
-    for (int32_t k = 0; k < NS; k++) {
-      const int64_t o = c->perm_h[k];
+    for (size_t k = 0; k < NS; k++) {
+      const int64_t o = c->perm_h[s0 + k];
       if (o < 0) continue;
-      out[2 * o] = bx[k];
-      out[2 * o + 1] = by[k];
+      out[2 * (o - node0)] = bx[k];
+      out[2 * (o - node0) + 1] = by[k];
     }
     return LPS_OK;
   };
   auto pull1 = [&](const double *d, double *out) -> int32_t {
     if (!out) return LPS_OK;
-    CTX_TRY(c, hipMemcpyAsync(bx.data(), d, sizeof(double) * NS, hipMemcpyDeviceToHost, c->stream));
+    CTX_TRY(c, hipMemcpyAsync(bx.data(), d + s0, sizeof(double) * NS, hipMemcpyDeviceToHost, c->stream));
     CTX_TRY(c, hipStreamSynchronize(c->stream));
-    for (int32_t k = 0; k < NS; k++)
-      if (c->perm_h[k] >= 0) out[c->perm_h[k]] = bx[k];
+    for (size_t k = 0; k < NS; k++)
+      if (c->perm_h[s0 + k] >= 0) out[c->perm_h[s0 + k] - node0] = bx[k];
     return LPS_OK;
   };
   int32_t rc;
@@ -934,7 +1330,21 @@ int32_t lps_fetch(lps_ctx *c, double *u, double *v, double *a, double *theta, do
   return LPS_OK;
 }
 
+int32_t lps_fetch(lps_ctx *c, double *u, double *v, double *a, double *theta, double *m) {
+  if (c->batch) return c->fail(LPS_ERR_ARG, "a batched context fetches per instance: use lps_fetch_instance");
+  return fetch_range(c, 0, c->n_slots, 0, u, v, a, theta, m);
+}
+
+int32_t lps_fetch_instance(lps_ctx *c, int32_t inst, double *u, double *v, double *a,
+                           double *theta, double *m) {
+  if (inst < 0 || inst >= c->n_inst) return c->fail(LPS_ERR_ARG, "instance out of range");
+  if (!c->batch) return fetch_range(c, 0, c->n_slots, 0, u, v, a, theta, m);
+  return fetch_range(c, c->slot_off[inst], c->slot_off[inst + 1], c->node_off[inst], u, v, a,
+                     theta, m);
+}
+
 int64_t lps_horizons(lps_ctx *c, int64_t *row_ptr, int32_t *cols) {
+  if (c->batch) return c->fail(LPS_ERR_ARG, "a batched context keeps no horizons");
   const int32_t N = c->n, NS = c->n_slots;
   if (!cols && !row_ptr) return c->hb.nnz;
   std::vector<int64_t> rp(NS + 1);

@@ -51,12 +51,20 @@ struct TileArgs {
   int32_t theta_kernel, force_kernel;
   double c1[12], c2[12];
   // Full tiles, where every window cell holds a node with m == m_ref (C = 2 / m_ref): class
-  // constants folded with the material, for the class representative (a, b), a >= b >= 0,
-  // a^2 + b^2 = r^2 of the class:
-  //   wt[cls][0 / 1]     = C c1 {a, b}                    (dilatation)
-  //   wa[cls][0 / 1]     = (lambda - mu) C c1 {a, b}      (force, theta_j terms)
-  //   wb[cls][0 / 1 / 2] = 8 mu C c2 {a^2, a b, b^2}      (force, bond-extension terms)
-  double wt[12][2], wa[12][2], wb[12][3];
+  // constants folded with C, for the class representative (a, b), a >= b >= 0, a^2 + b^2 = r^2:
+  //   wt[cls][0 / 1] = C c1 {a, b}  (dilatation); the force's weights depend on the material and
+  //   sit in InstParams.
+  double wt[12][2];
+  const int32_t *tile_inst;        // n_tiles: the instance (domain) a tile belongs to
+};
+
+// Per instance (one domain of a batch; a single problem is a batch of one): time step, material and
+// the full tiles' force weights folded with it (C = 2 / m_ref, class representative (a, b)):
+//   wa[cls][0 / 1]     = (lambda - mu) C c1 {a, b}      (theta_j terms)
+//   wb[cls][0 / 1 / 2] = 8 mu C c2 {a^2, a b, b^2}      (bond-extension terms)
+struct InstParams {
+  double dt, rho, lam, mu, kappa, exclude_r;
+  double wa[12][2], wb[12][3];
 };
 
 enum NodeKernel : int32_t { kGeneric = 0, kUnrolled = 1, kBlocked = 2 };
@@ -87,6 +95,8 @@ struct StepArgs {
   const double *gu, *gv;           // n_abc x 2 x 4M
   const double *gth;               // n_abc x 2M
   int32_t theta_method;
+  const InstParams *inst;          // per instance; the fast-path kernels read it through
+                                   // tile.tile_inst, the exact path uses the scalars above
   TileArgs tile;
 };
 
@@ -140,8 +150,12 @@ int32_t diag_partials(int32_t n);
 // from the LDS-staged window and the stencil table.
 hipError_t launch_diag_tile(const StepArgs &a, const double *ux, const double *uy,
                             const double *vx, const double *vy, const double *ax, const double *ay,
-                            const double *theta, const int32_t *perm, double exclude_r,
-                            DiagPartial *partials, DiagPartial *out, hipStream_t s);
+                            const double *theta, const int32_t *perm, DiagPartial *partials,
+                            DiagPartial *out, hipStream_t s);
+// Per instance: out[i] reduces the tile partials [tile_off[i], tile_off[i + 1]) in the order of
+// launch_diag_tile's final reduction (the same bits for an instance alone or in a batch).
+hipError_t launch_diag_final_inst(const DiagPartial *partials, const int32_t *tile_off,
+                                  int32_t n_inst, DiagPartial *out, hipStream_t s);
 // Tile path check: for every node slot, the stencil cells that hold a node are exactly the set of
 // its CSR row (self excluded). bond_of: (2 halo + 1)^2 entries, the bond of offset (ox, oy) at
 // (ox + halo) (2 halo + 1) + oy + halo, -1 off the stencil. Adds the number of rows that differ to

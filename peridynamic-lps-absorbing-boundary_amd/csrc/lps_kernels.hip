@@ -646,21 +646,21 @@ __device__ __forceinline__ void force_sum68(const double2 *su, const double2 *sa
 // Force-pass epilogue of node slot i: acceleration, velocity-Verlet corrector and the next step's
 // prediction (src/main.cpp:874, 922-925, 725-728). At an ABC node (kind 2) only the acceleration:
 // G_u / G_v replace its displacement and velocity (k_abc_uv) and k_abc_predict then predicts it.
-// vi, ai: V^n and A^n of the node (read by the caller, possibly early).
-__device__ __forceinline__ void node_update(const StepArgs &a, const StepState &st, int32_t i,
+// vi, ai: V^n and A^n of the node (read by the caller, possibly early); P: its instance.
+__device__ __forceinline__ void node_update(const InstParams &P, const StepState &st, int32_t i,
                                             int8_t kd, double2 ui, double2 vi, double2 ai,
                                             double F0, double F1) {
-  const double inv_rho = 1.0 / a.rho;
+  const double inv_rho = 1.0 / P.rho;
   const double a1x = F0 * inv_rho, a1y = F1 * inv_rho;
   st.ax1[i] = a1x; st.ay1[i] = a1y;
   if (kd != 1) return;
-  const double h = 0.5 * a.dt;
+  const double dt = P.dt, h = 0.5 * dt;
   const double v1x = vi.x + h * (ai.x + a1x);
   const double v1y = vi.y + h * (ai.y + a1y);
   st.ux1[i] = ui.x; st.uy1[i] = ui.y;
   st.vx1[i] = v1x; st.vy1[i] = v1y;
-  st.pnx[i] = ui.x + a.dt * (v1x + h * a1x);
-  st.pny[i] = ui.y + a.dt * (v1y + h * a1y);
+  st.pnx[i] = ui.x + dt * (v1x + h * a1x);
+  st.pny[i] = ui.y + dt * (v1y + h * a1y);
 }
 
 // Unrolled force kernel over the tiles of list. FULL: the full tiles, no masks (every window cell
@@ -675,7 +675,8 @@ __global__ void __launch_bounds__(kBlock, 5) k_force_t68(StepArgs a, StepState s
   if (q >= count) return;
   const int32_t t = list[q];
   const int32_t *nbr = T.tile_nbr + 9 * t;
-  const double lm = a.lam - a.mu, mu4 = 4.0 * a.mu;
+  const InstParams &P = a.inst[T.tile_inst[t]];
+  const double lm = P.lam - P.mu, mu4 = 4.0 * P.mu;
 #pragma unroll
   for (int r = 0; r < 3; r++) {
     const int32_t c = threadIdx.x + r * kBlock;
@@ -700,7 +701,7 @@ __global__ void __launch_bounds__(kBlock, 5) k_force_t68(StepArgs a, StepState s
   const double2 abi = sh.fo.ab[c];
   double F0 = 0.0, F1 = 0.0;
   if (FULL || abi.y != 0.0) force_sum68<FULL>(sh.fo.u, sh.fo.ab, c, ui, abi, T, &F0, &F1);
-  node_update(a, st, i, kd, ui, make_double2(st.vx[i], st.vy[i]), make_double2(st.ax[i], st.ay[i]),
+  node_update(P, st, i, kd, ui, make_double2(st.vx[i], st.vy[i]), make_double2(st.ax[i], st.ay[i]),
               F0, F1);
 }
 
@@ -885,7 +886,7 @@ struct ForceFull {
   const double2 *su;
   const double *sth;
   int32_t c0;
-  const TileArgs &T;
+  const InstParams &T;  // the instance's folded weights wa, wb
   double2 ui[kRB];
   double fx[kRB], fy[kRB];
   double2 u;
@@ -1060,7 +1061,8 @@ __global__ void __launch_bounds__(64, 3) k_force_rb(StepArgs a, StepState st) {
   }, std::make_integer_sequence<int, 9>{});
   __syncthreads();
   const int32_t c0 = (lx0 + 4) * kW68 + ly + 4;
-  ForceFull P{su, sth, c0, T, {}, {}, {}, {}, 0.0};
+  const InstParams &I = a.inst[T.tile_inst[t]];
+  ForceFull P{su, sth, c0, I, {}, {}, {}, {}, 0.0};
 #pragma unroll
   for (int r = 0; r < kRB; r++) {
     P.ui[r] = su[c0 + r * kW68];
@@ -1071,7 +1073,7 @@ __global__ void __launch_bounds__(64, 3) k_force_rb(StepArgs a, StepState st) {
 #pragma unroll
   for (int r = 0; r < kRB; r++) {
     const int32_t i = t * (kTile * kTile) + (lx0 + r) * kTile + ly;
-    if (kd[r] != 0) node_update(a, st, i, kd[r], P.ui[r], vi[r], ai[r], P.fx[r], P.fy[r]);
+    if (kd[r] != 0) node_update(I, st, i, kd[r], P.ui[r], vi[r], ai[r], P.fx[r], P.fy[r]);
   }
 }
 
@@ -1088,7 +1090,8 @@ __global__ void __launch_bounds__(64, 3) k_force_rbm(StepArgs a, StepState st) {
   for (int k = 0; k < 9; k++) nb[k] = T.tile_nbr[9 * t + k];
   const int32_t lane = threadIdx.x;
   const int32_t ly = lane & 15, lx0 = (lane >> 4) * kRB;
-  const double lm = a.lam - a.mu, mu4 = 4.0 * a.mu;
+  const InstParams &P = a.inst[T.tile_inst[t]];
+  const double lm = P.lam - P.mu, mu4 = 4.0 * P.mu;
   int8_t kd[kRB];
   double2 vi[kRB], ai[kRB];
 #pragma unroll
@@ -1110,21 +1113,21 @@ __global__ void __launch_bounds__(64, 3) k_force_rbm(StepArgs a, StepState st) {
   }, std::make_integer_sequence<int, 9>{});
   __syncthreads();
   const int32_t c0 = (lx0 + 4) * kW68 + ly + 4;
-  ForceMasked P{su, sab, c0, T, {}, {}, {}, {}, {}, {}};
+  ForceMasked Q{su, sab, c0, T, {}, {}, {}, {}, {}, {}};
 #pragma unroll
   for (int r = 0; r < kRB; r++) {
-    P.ui[r] = su[c0 + r * kW68];
-    P.abi[r] = sab[c0 + r * kW68];
-    P.fx[r] = 0.0;
-    P.fy[r] = 0.0;
+    Q.ui[r] = su[c0 + r * kW68];
+    Q.abi[r] = sab[c0 + r * kW68];
+    Q.fx[r] = 0.0;
+    Q.fy[r] = 0.0;
   }
-  rb_sweep(P, std::make_integer_sequence<int, kRB + 8>{});
+  rb_sweep(Q, std::make_integer_sequence<int, kRB + 8>{});
 #pragma unroll
   for (int r = 0; r < kRB; r++) {
     const int32_t i = t * (kTile * kTile) + (lx0 + r) * kTile + ly;
-    const bool on = P.abi[r].y != 0.0;  // a node with m <= 0 gets no force
+    const bool on = Q.abi[r].y != 0.0;  // a node with m <= 0 gets no force
     if (kd[r] != 0)
-      node_update(a, st, i, kd[r], P.ui[r], vi[r], ai[r], on ? P.fx[r] : 0.0, on ? P.fy[r] : 0.0);
+      node_update(P, st, i, kd[r], Q.ui[r], vi[r], ai[r], on ? Q.fx[r] : 0.0, on ? Q.fy[r] : 0.0);
   }
 }
 
@@ -1180,9 +1183,9 @@ __global__ void __launch_bounds__(kBlock) k_abc_predict(StepArgs a, StepState st
   const int32_t row = blockIdx.x * kBlock + threadIdx.x;
   if (row >= a.n_abc) return;
   const int32_t i = a.abc_slot[row];
-  const double h = 0.5 * a.dt;
-  st.pnx[i] = st.ux1[i] + a.dt * (st.vx1[i] + h * st.ax1[i]);
-  st.pny[i] = st.uy1[i] + a.dt * (st.vy1[i] + h * st.ay1[i]);
+  const double dt = a.inst[a.tile.tile_inst[i / (kTile * kTile)]].dt, h = 0.5 * dt;
+  st.pnx[i] = st.ux1[i] + dt * (st.vx1[i] + h * st.ax1[i]);
+  st.pny[i] = st.uy1[i] + dt * (st.vy1[i] + h * st.ay1[i]);
 }
 
 // Generic force kernel over a list of tiles (all tiles when list is null).
@@ -1196,7 +1199,8 @@ __global__ void __launch_bounds__(kBlock) k_force_tile(StepArgs a, StepState st,
   const int32_t txy = T.tile_xy[t];
   const int32_t gx0 = (txy >> 16) * kTile - T.halo, gy0 = (txy & 0xffff) * kTile - T.halo;
   const int32_t W = T.win;
-  const double lm = a.lam - a.mu, mu4 = 4.0 * a.mu;
+  const InstParams &P = a.inst[T.tile_inst[t]];
+  const double lm = P.lam - P.mu, mu4 = 4.0 * P.mu;
   for (int32_t c = threadIdx.x; c < W * W; c += kBlock) {
     const int32_t s = lattice_slot(T, gx0 + c / W, gy0 + c % W);
     double2 u = make_double2(0.0, 0.0), ab = make_double2(0.0, 0.0);
@@ -1230,7 +1234,7 @@ __global__ void __launch_bounds__(kBlock) k_force_tile(StepArgs a, StepState st,
       F1 += s * hy;
     }
   }
-  node_update(a, st, i, kd, ui, make_double2(st.vx[i], st.vy[i]), make_double2(st.ax[i], st.ay[i]),
+  node_update(P, st, i, kd, ui, make_double2(st.vx[i], st.vy[i]), make_double2(st.ax[i], st.ay[i]),
               F0, F1);
 }
 
@@ -1383,18 +1387,26 @@ __global__ void __launch_bounds__(kBlock) k_diag_tile(StepArgs a, const double *
                                                       const double *uy, const double *vx,
                                                       const double *vy, const double *ax,
                                                       const double *ay, const double *theta,
-                                                      const int32_t *perm, double exclude_r,
+                                                      const int32_t *perm,
                                                       DiagPartial *partials) {
   __shared__ DiagPartial sh[kBlock];
   __shared__ double2 su[kMaxWin * kMaxWin];
   __shared__ int8_t sok[kMaxWin * kMaxWin];
   const TileArgs &T = a.tile;
   const int32_t t = blockIdx.x;
-  const int32_t txy = T.tile_xy[t];
-  const int32_t gx0 = (txy >> 16) * kTile - T.halo, gy0 = (txy & 0xffff) * kTile - T.halo;
-  const int32_t W = T.win;
+  const int32_t *nbr = T.tile_nbr + 9 * t;
+  const InstParams &P = a.inst[T.tile_inst[t]];
+  const double exclude_r = P.exclude_r;
+  const int32_t W = T.win, K = T.halo;
   for (int32_t c = threadIdx.x; c < W * W; c += kBlock) {
-    const int32_t s = lattice_slot(T, gx0 + c / W, gy0 + c % W);
+    // window cell (rx, ry) relative to the tile's origin, |offset| <= halo < 16: one of the 3 x 3
+    // neighbour tiles
+    const int32_t rx = c / W - K, ry = c % W - K;
+    const int32_t tx = rx < 0 ? 0 : (rx < kTile ? 1 : 2), ty = ry < 0 ? 0 : (ry < kTile ? 1 : 2);
+    const int32_t tt = nbr[tx * 3 + ty];
+    const int32_t s = tt < 0 ? -1
+                             : tt * (kTile * kTile) + (rx - (tx - 1) * kTile) * kTile +
+                                   (ry - (ty - 1) * kTile);
     const bool ok = s >= 0 && a.kind[s] != 0;
     su[c] = ok ? make_double2(ux[s], uy[s]) : make_double2(0.0, 0.0);
     sok[c] = ok;
@@ -1423,9 +1435,9 @@ __global__ void __launch_bounds__(kBlock) k_diag_tile(StepArgs a, const double *
       d.vb = vxi * vxi + vyi * vyi;
     }
     if (!(exclude_r > 0.0 && r > exclude_r)) {
-      d.px = a.rho * vxi * a.dx * a.dy;
-      d.py = a.rho * vyi * a.dx * a.dy;
-      const double ek = 0.5 * a.rho * (vxi * vxi + vyi * vyi) * a.dx * a.dy;
+      d.px = P.rho * vxi * a.dx * a.dy;
+      d.py = P.rho * vyi * a.dx * a.dy;
+      const double ek = 0.5 * P.rho * (vxi * vxi + vyi * vyi) * a.dx * a.dy;
       d.ek = ek;
       const int32_t c = (threadIdx.x / kTile + T.halo) * W + (threadIdx.x % kTile + T.halo);
       double acc = 0.0;
@@ -1436,7 +1448,7 @@ __global__ void __launch_bounds__(kBlock) k_diag_tile(StepArgs a, const double *
         const double ed = (uj.x - uxi) * hx + (uj.y - uyi) * hy - th * xin * 0.5;
         acc += sok[cj] ? ed * ed * g2 : 0.0;
       }
-      const double W_i = 0.5 * a.kappa * th * th + (4.0 * a.mu / a.m[i]) * acc;
+      const double W_i = 0.5 * P.kappa * th * th + (4.0 * P.mu / a.m[i]) * acc;
       const double ep = W_i * a.dx * a.dy;
       d.ep = ep;
       d.energy = ek + ep;
@@ -1454,6 +1466,19 @@ __global__ void __launch_bounds__(kBlock) k_diag_final(const DiagPartial *partia
   for (int32_t t = threadIdx.x; t < np; t += kBlock) diag_merge(d, partials[t]);
   block_reduce_diag(d, sh);
   if (threadIdx.x == 0) *out = d;
+}
+
+// One block per instance over its tiles' partials, in k_diag_final's order.
+__global__ void __launch_bounds__(kBlock) k_diag_final_inst(const DiagPartial *partials,
+                                                            const int32_t *tile_off,
+                                                            DiagPartial *out) {
+  __shared__ DiagPartial sh[kBlock];
+  const int32_t i = blockIdx.x, t0 = tile_off[i], np = tile_off[i + 1] - t0;
+  DiagPartial d;
+  diag_identity(d);
+  for (int32_t t = threadIdx.x; t < np; t += kBlock) diag_merge(d, partials[t0 + t]);
+  block_reduce_diag(d, sh);
+  if (threadIdx.x == 0) out[i] = d;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1594,13 +1619,19 @@ hipError_t launch_diag(const StepArgs &a, const double *ux, const double *uy, co
 
 hipError_t launch_diag_tile(const StepArgs &a, const double *ux, const double *uy,
                             const double *vx, const double *vy, const double *ax, const double *ay,
-                            const double *theta, const int32_t *perm, double exclude_r,
-                            DiagPartial *partials, DiagPartial *out, hipStream_t s) {
+                            const double *theta, const int32_t *perm, DiagPartial *partials,
+                            DiagPartial *out, hipStream_t s) {
   hipLaunchKernelGGL(k_diag_tile, dim3(a.tile.n_tiles), dim3(kBlock), 0, s, a, ux, uy, vx, vy, ax,
-                     ay, theta, perm, exclude_r, partials);
+                     ay, theta, perm, partials);
   hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
+  if (e != hipSuccess || !out) return e;
   hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(kBlock), 0, s, partials, a.tile.n_tiles, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_diag_final_inst(const DiagPartial *partials, const int32_t *tile_off,
+                                  int32_t n_inst, DiagPartial *out, hipStream_t s) {
+  hipLaunchKernelGGL(k_diag_final_inst, dim3(n_inst), dim3(kBlock), 0, s, partials, tile_off, out);
   return hipGetLastError();
 }
 

@@ -63,6 +63,12 @@ SIGNATURES = {
     "lps_default_options": (None, [C.POINTER(LpsOptions)]),
     "lps_create": (C.c_int32, [C.POINTER(LpsProblem), C.POINTER(LpsOptions), C.POINTER(C.c_void_p)]),
     "lps_reset": (C.c_int32, [C.c_void_p]),
+    "lps_create_batch": (C.c_int32, [C.c_void_p, C.c_int32, C.POINTER(LpsOptions),
+                                     C.POINTER(C.c_void_p)]),
+    "lps_instance_count": (C.c_int32, [C.c_void_p]),
+    "lps_run_batch": (C.c_int32, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32]),
+    "lps_diagnostics_batch": (C.c_int32, [C.c_void_p, C.c_void_p]),
+    "lps_fetch_instance": (C.c_int32, [C.c_void_p, C.c_int32] + [C.c_void_p] * 5),
     "lps_step": (C.c_int32, [C.c_void_p, C.c_int32]),
     "lps_run": (C.c_int32, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32]),
     "lps_diagnostics": (C.c_int32, [C.c_void_p, C.POINTER(LpsDiag)]),
@@ -107,6 +113,51 @@ def diag_dict(d: LpsDiag) -> Dict[str, float]:
     return {k: getattr(d, k) for k in DIAG_FIELDS}
 
 
+def _lps_problem(prob, keep: List[np.ndarray]) -> "LpsProblem":
+    """The C struct of a problem; the arrays it points to are appended to keep."""
+    def arr(a, dt):
+        a = np.ascontiguousarray(a, dtype=dt)
+        keep.append(a)
+        return a
+
+    N, Cn, M = prob.N, prob.C, prob.max_cloud
+    d2 = prob.data2
+    p = LpsProblem()
+    p.node_no, p.constrains_no, p.max_cloud_size = N, Cn, M
+    p.E, p.nu, p.rho, p.dt = d2[0], d2[1], d2[2], d2[6]
+    p.delta, p.dx, p.dy = d2[7], d2[8], d2[9]
+    p.nodes = _ptr(arr(prob.nodes, np.float64))
+    p.bctypes = _ptr(arr(prob.bctypes, np.int32))
+    p.icvalues = _ptr(arr(prob.ic, np.float64))
+    p.icdvalues = _ptr(arr(prob.icd, np.float64))
+    if Cn > 0:
+        cl = np.asarray(prob.clouds)
+        # the file reader's (int)(x + 0.5): a -1 pad becomes node 0
+        cl = np.trunc(cl.astype(np.float64) + 0.5).astype(np.int32)
+        p.inf_nodes = _ptr(arr(prob.abc, np.int32))
+        p.iclouds = _ptr(arr(cl, np.int32))
+        p.ci_nodes_u = _ptr(arr(prob.Gu, np.float64))
+        p.ci_nodes_v = _ptr(arr(prob.Gv, np.float64))
+        if prob.Gth is not None and prob.Gth.size:
+            p.ci_nodes_theta = _ptr(arr(prob.Gth, np.float64))
+    return p
+
+
+def _lps_options(L, theta_method, abc_enabled, exact, use_graph, device, horizon_radius, variant,
+                 concurrent) -> "LpsOptions":
+    o = LpsOptions()
+    L.lps_default_options(C.byref(o))
+    o.theta_method = theta_method
+    o.abc_enabled = 1 if abc_enabled else 0
+    o.exact = 1 if exact else 0
+    o.use_graph = 1 if use_graph else 0
+    o.device = device
+    o.horizon_radius = horizon_radius
+    o.variant = variant
+    o.concurrent = 1 if concurrent else 0
+    return o
+
+
 class Solver:
     """One lps_ctx on the GPU for a problem (lps_amd.preprocess.Problem)."""
 
@@ -115,42 +166,11 @@ class Solver:
                  horizon_radius: float = 0.0, variant: int = 0, concurrent: bool = True):
         L = load_library()
         self._keep: List[np.ndarray] = []
-
-        def arr(a, dt):
-            a = np.ascontiguousarray(a, dtype=dt)
-            self._keep.append(a)
-            return a
-
-        N, Cn, M = prob.N, prob.C, prob.max_cloud
+        N = prob.N
         d2 = prob.data2
-        p = LpsProblem()
-        p.node_no, p.constrains_no, p.max_cloud_size = N, Cn, M
-        p.E, p.nu, p.rho, p.dt = d2[0], d2[1], d2[2], d2[6]
-        p.delta, p.dx, p.dy = d2[7], d2[8], d2[9]
-        p.nodes = _ptr(arr(prob.nodes, np.float64))
-        p.bctypes = _ptr(arr(prob.bctypes, np.int32))
-        p.icvalues = _ptr(arr(prob.ic, np.float64))
-        p.icdvalues = _ptr(arr(prob.icd, np.float64))
-        if Cn > 0:
-            cl = np.asarray(prob.clouds)
-            # the file reader's (int)(x + 0.5): a -1 pad becomes node 0
-            cl = np.trunc(cl.astype(np.float64) + 0.5).astype(np.int32)
-            p.inf_nodes = _ptr(arr(prob.abc, np.int32))
-            p.iclouds = _ptr(arr(cl, np.int32))
-            p.ci_nodes_u = _ptr(arr(prob.Gu, np.float64))
-            p.ci_nodes_v = _ptr(arr(prob.Gv, np.float64))
-            if prob.Gth is not None and prob.Gth.size:
-                p.ci_nodes_theta = _ptr(arr(prob.Gth, np.float64))
-        o = LpsOptions()
-        L.lps_default_options(C.byref(o))
-        o.theta_method = theta_method
-        o.abc_enabled = 1 if abc_enabled else 0
-        o.exact = 1 if exact else 0
-        o.use_graph = 1 if use_graph else 0
-        o.device = device
-        o.horizon_radius = horizon_radius
-        o.variant = variant
-        o.concurrent = 1 if concurrent else 0
+        p = _lps_problem(prob, self._keep)
+        o = _lps_options(L, theta_method, abc_enabled, exact, use_graph, device, horizon_radius,
+                         variant, concurrent)
         h = C.c_void_p()
         rc = L.lps_create(C.byref(p), C.byref(o), C.byref(h))
         if rc != LPS_OK:
@@ -238,3 +258,47 @@ class Solver:
         self._check(self._L.lps_tile_info(self._h, _ptr(out)))
         return {"tiles": int(out[0]), "full_tiles": int(out[1]), "partial_tiles": int(out[2]),
                 "full_tile_nodes": int(out[3]), "nodes": int(out[4])}
+
+
+class Batch(Solver):
+    """Many problems advanced together in one lps_ctx (lps_create_batch): the same dx, dy and
+    delta, each with its own radius, nu, dt, ABC operators and initial conditions."""
+
+    def __init__(self, probs, theta_method: int = LPS_THETA_GTHETA, use_graph: bool = True,
+                 device: int = 0, variant: int = 0, concurrent: bool = True):
+        L = load_library()
+        self._keep: List[np.ndarray] = []
+        structs = (LpsProblem * len(probs))()
+        for i, p in enumerate(probs):
+            structs[i] = _lps_problem(p, self._keep)
+        o = _lps_options(L, theta_method, True, False, use_graph, device, 0.0, variant,
+                         concurrent)
+        h = C.c_void_p()
+        rc = L.lps_create_batch(structs, len(probs), C.byref(o), C.byref(h))
+        if rc != LPS_OK:
+            raise LpsError(rc, L.lps_global_error().decode())
+        self._h = h
+        self._L = L
+        self._keep.clear()
+        self.sizes = [p.N for p in probs]
+        self.count = L.lps_instance_count(h)
+
+    def run(self, nsteps: int) -> List[List[Dict[str, float]]]:
+        """Output records of every instance: result[i] is instance i's list of records."""
+        nrec = max(1, nsteps // 30 + 1)
+        recs = (LpsDiag * (nrec * self.count))()
+        n = self._check(self._L.lps_run_batch(self._h, nsteps, recs, nrec))
+        return [[diag_dict(recs[k * self.count + i]) for k in range(n)] for i in range(self.count)]
+
+    def diagnostics(self) -> List[Dict[str, float]]:
+        d = (LpsDiag * self.count)()
+        self._check(self._L.lps_diagnostics_batch(self._h, d))
+        return [diag_dict(d[i]) for i in range(self.count)]
+
+    def fetch(self, instance: int = 0) -> Dict[str, np.ndarray]:
+        N = self.sizes[instance]
+        u, v, a = (np.zeros(2 * N) for _ in range(3))
+        th, m = np.zeros(N), np.zeros(N)
+        self._check(self._L.lps_fetch_instance(self._h, instance, _ptr(u), _ptr(v), _ptr(a),
+                                               _ptr(th), _ptr(m)))
+        return {"u": u, "v": v, "a": a, "theta": th, "m": m}
