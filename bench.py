@@ -45,6 +45,110 @@ def measured_traffic(kernel, workload):
     return {"bytes_per_launch": e["bytes_per_launch"], "source": f"profiles/hbm_traffic.json ({e['command']})"}
 
 
+def force_roofline(path, ti, n_nodes, pt, workload):
+    """Roofline of the force pass's main kernel (the longest launch of a step). Its compulsory HBM
+    traffic, each element once: reads u, theta, v, a (8 B each component) and kind (1 B), writes
+    a, v, u and the next prediction: 121 B per node. The halo reads of the LDS-staged windows are
+    L2 hits and not counted. A CSR-gather formulation of the same pass would move 68 B per node plus
+    4 B per bond (coordinates, m, row pointer, column index; csr_gather_bytes)."""
+    fk = path.split("force ")[1].split(" ")[0].rstrip(",") if "force " in path else "exact"
+    kernel = {"blocked": "k_force_rb", "unrolled": "k_force_t68<true>",
+              "generic": "k_force_tile", "exact": "k_force_exact"}.get(fk, fk)
+    units = ti["full_tile_nodes"] if fk in ("blocked", "unrolled") else n_nodes
+    per_unit = 121
+    force_bytes = per_unit * units
+    achieved = force_bytes / (pt["force_ms"] * 1e-3) / 1e9
+    traffic = measured_traffic(kernel, workload)
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic["bytes_per_launch"] if traffic else None,
+            "kernel": kernel, "units": units, "unit_name": "nodes of the full tiles",
+            "bytes_per_unit": per_unit, "algorithmic_bytes": force_bytes,
+            "avg_ms": round(pt["force_ms"], 5),
+            "csr_gather_bytes": 68 * units + 4 * 68 * units,
+            "traffic_source": traffic["source"] if traffic else None,
+            "passes_ms": {k: round(v, 5) for k, v in pt.items()}}
+
+
+def bench_batch(args, configs, solver):
+    """A batched workload: every instance advances the same number of steps in the same launches;
+    value = sum of the instances' nodes x steps / time."""
+    w = configs.BATCH_WORKLOADS[args.workload]
+    steps = args.steps if args.steps is not None else w.steps
+    workers = args.workers or min(16, os.cpu_count() or 1)
+    t = time.time()
+    if args.cache and os.path.exists(args.cache):
+        probs = configs.load_batch(args.cache)
+        for p in probs:
+            p.steps = steps
+        log(f"[bench] loaded {w.name} from {args.cache}")
+    else:
+        log(f"[bench] generating {w.name} ({w.description}) with {workers} workers")
+        probs = configs.make_batch(
+            w.name, workers=workers, steps=steps,
+            progress=lambda i, p: log(f"[bench]   instance {i}: R={p.info['R']} nu={p.info['nu']} "
+                                      f"N={p.N} C={p.C} dt={p.dt:.3e}"))
+        if args.cache:
+            configs.save_batch(probs, args.cache)
+    n_nodes = sum(p.N for p in probs)
+    n_abc = sum(p.C for p in probs)
+    log(f"[bench] {len(probs)} instances, {n_nodes} nodes, {n_abc} ABC rows, ready in "
+        f"{time.time() - t:.1f}s")
+    t = time.time()
+    b = solver.Batch(probs, variant=args.variant, concurrent=not args.serial)
+    path = b.path
+    log(f"[bench] GPU setup {time.time() - t:.1f}s, path: {path}")
+    if args.warmup > 0:
+        b.run(args.warmup)
+        b.reset()
+    b.set_timing(True)
+    t0 = time.perf_counter()
+    recs = b.run(steps)
+    elapsed = time.perf_counter() - t0
+    pt = b.pass_times()
+    value = n_nodes * steps / elapsed
+    log(f"[bench] {steps} steps in {elapsed:.3f}s -> {value:.4e} node-steps/s; passes {pt}")
+    b.reset()
+    b.set_timing(False)
+    tg = time.perf_counter()
+    b.run(steps)
+    untimed_s = time.perf_counter() - tg
+    log(f"[bench] untimed: {steps} steps in {untimed_s:.3f}s -> {n_nodes * steps / untimed_s:.4e} "
+        f"node-steps/s")
+    roofline = force_roofline(path, b.tile_info(), n_nodes, pt, w.name)
+    b.close()
+    refl = [1.0 + r[-1]["de_rel"] for r in recs if r]
+    cpu = None
+    if not args.no_cpu_baseline:
+        import pyoracle
+        threads = min(16, os.cpu_count() or 1)
+        p0 = probs[0]
+        o = pyoracle.Oracle(p0, threads=threads)
+        tc = time.perf_counter()
+        o.run(args.cpu_steps)
+        tcpu = time.perf_counter() - tc
+        cpu = {"value": p0.N * args.cpu_steps / tcpu, "unit": "node-steps/s", "cores": threads,
+               "kind": "port",
+               "sample": f"instance 0 ({p0.N} nodes), first {args.cpu_steps} steps (oracle "
+                         f"restatement of src/main.cpp, OpenMP, -O2 no FMA), {tcpu:.1f}s"}
+    out = {
+        "metric": "node-steps/s at reflected dE/E0\u22641e-4: learned vs LSQ ABC, 1 MI355X",
+        "value": value, "unit": "node-steps/s", "n_gpus": 1, "steps": steps,
+        "warmup": args.warmup, "ms_per_step": elapsed / steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{w.name}: {w.description}", "instances": len(probs),
+                   "nodes": n_nodes, "abc_nodes": n_abc, "kernels": path,
+                   "abc": "least-squares G_u/G_v/G_theta (the learned closure is not built)",
+                   "streams": "serial" if args.serial else "concurrent",
+                   "untimed_node_steps_per_s": n_nodes * steps / untimed_s,
+                   "reflected_energy_final_max": max(refl) if refl else None,
+                   "instances_gate_reflected_le_1e-4": sum(r <= 1e-4 for r in refl)},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -84,6 +188,9 @@ def main():
         sys.exit(2)
 
     from lps_amd import configs, solver
+    if args.workload in configs.BATCH_WORKLOADS:
+        bench_batch(args, configs, solver)
+        return
     w = configs.WORKLOADS[args.workload]
     steps = args.steps if args.steps is not None else w.steps
     workers = args.workers or min(16, os.cpu_count() or 1)
@@ -131,29 +238,7 @@ def main():
     log(f"[bench] untimed: {steps} steps in {untimed_s:.3f}s -> {prob.N * steps / untimed_s:.4e} "
         f"node-steps/s")
 
-    # Roofline of the force pass's main kernel (the longest launch of a step). Its compulsory HBM
-    # traffic, each element once: reads u, theta, v, a (8 B each component) and kind (1 B), writes
-    # a, v, u and the next prediction: 121 B per node. The halo reads of the LDS-staged windows are
-    # L2 hits and not counted. A CSR-gather formulation of the same pass would move 68 B per node
-    # plus 4 B per bond (coordinates, m, row pointer, column index; csr_gather_bytes).
-    ti = s.tile_info()
-    fk = path.split("force ")[1].split(" ")[0].rstrip(",") if "force " in path else "exact"
-    kernel = {"blocked": "k_force_rb", "unrolled": "k_force_t68<true>",
-              "generic": "k_force_tile", "exact": "k_force_exact"}.get(fk, fk)
-    units = ti["full_tile_nodes"] if fk in ("blocked", "unrolled") else prob.N
-    per_unit = 121
-    force_bytes = per_unit * units
-    achieved = force_bytes / (pt["force_ms"] * 1e-3) / 1e9
-    traffic = measured_traffic(kernel, w.name)
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": traffic["bytes_per_launch"] if traffic else None,
-                "kernel": kernel, "units": units, "unit_name": "nodes of the full tiles",
-                "bytes_per_unit": per_unit, "algorithmic_bytes": force_bytes,
-                "avg_ms": round(pt["force_ms"], 5),
-                "csr_gather_bytes": 68 * units + 4 * 68 * units,
-                "traffic_source": traffic["source"] if traffic else None,
-                "passes_ms": {k: round(v, 5) for k, v in pt.items()}}
+    roofline = force_roofline(path, s.tile_info(), prob.N, pt, w.name)
     s.close()
 
     cpu = None

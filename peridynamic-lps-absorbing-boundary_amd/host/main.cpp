@@ -2,12 +2,17 @@
 // time loop on an MI355X through the lps_amd C ABI.
 //
 //   PD2_LPS_ABC <prefix> <jobname> [--theta-method=gtheta|idw] [--vtk=on|off] [--exact]
-//                                  [--no-graph] [--device=N]
+//                                  [--no-graph] [--device=N] [--abc=lsq]
+//   PD2_LPS_ABC --batch=<manifest> [options]
 //
 // Same positional arguments, input files, .energy/.verify outputs (byte format of
 // src/main.cpp:694-707, 1106-1123) and exit codes (0 on success or on the usage message, 1 on a FATAL
 // condition) as the reference. VTK snapshots (src/main.cpp:689-691, 1091-1103) are written unless
 // --vtk=off. The options select the reference's compile-time variants at run time.
+//
+// --batch runs many jobs in one process (no counterpart in the reference): the manifest holds one
+// "<prefix> <jobname>" per line ('#' starts a comment); the jobs, which must share dx, dy, delta and
+// the step count, are advanced together (lps_create_batch) and each gets its own outputs.
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -15,6 +20,7 @@
 #include <fstream>
 #include <iomanip>
 #include <iostream>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -36,6 +42,274 @@ int fatal_read(const std::string &job, const char *ext) {
   return 1;
 }
 
+// One job: its files read with the reference's semantics (src/main.cpp:176-295) and its outputs.
+struct Job {
+  std::string prefix, name;
+  int32_t node_no = 0, step_no = 0, constrains_no = 0, max_cloud = 0;
+  std::vector<int32_t> data1;
+  std::vector<double> data2, nodes, bcvalues, icd, ic, Gu, Gv, Gth;
+  std::vector<int32_t> bctypes, iclouds, infnodes;
+  double exclude_r = -1.0;
+  std::ofstream energy, verify;
+  std::vector<double> u, v, th;
+
+  int read(const lps_options &opt) {
+    const std::string &J = name;
+    if (!lpsio::read_ints(J + ".data1", 6, data1)) return fatal_read(J, ".data1");
+    if (!lpsio::read_doubles(J + ".data2", 15, data2)) return fatal_read(J, ".data2");
+    node_no = data1[0];
+    step_no = data1[1];
+    const int32_t prenotch_no = data1[2];
+    constrains_no = data1[4];
+    max_cloud = data1[5];
+    if (prenotch_no > 0) {
+      std::cerr << "FATAL: prenotch bond cutting is not supported by this build" << std::endl;
+      return 1;
+    }
+    const int64_t N = node_no;
+    if (!lpsio::read_doubles(J + ".nodes", 2 * N, nodes)) return fatal_read(J, ".nodes");
+    if (!lpsio::read_ints(J + ".bctypes", 2 * N, bctypes)) return fatal_read(J, ".bctypes");
+    if (!lpsio::read_doubles(J + ".bcvalues", 2 * N, bcvalues)) return fatal_read(J, ".bcvalues");
+    if (!lpsio::read_doubles(J + ".icdvalues", 2 * N, icd)) return fatal_read(J, ".icdvalues");
+    if (!lpsio::read_doubles(J + ".icvalues", 2 * N, ic)) return fatal_read(J, ".icvalues");
+    if (opt.abc_enabled && constrains_no > 0) {
+      const int64_t C = constrains_no, M = max_cloud;
+      if (!lpsio::read_doubles(J + ".ciNodesU", 2 * C * 4 * M, Gu)) return fatal_read(J, ".ciNodesU");
+      if (!lpsio::read_doubles(J + ".ciNodesV", 2 * C * 4 * M, Gv)) return fatal_read(J, ".ciNodesV");
+      if (!lpsio::read_rounded_ints(J + ".iClouds", C * M, iclouds)) return fatal_read(J, ".iClouds");
+      if (!lpsio::read_rounded_ints(J + ".infNodesIndex", C, infnodes))
+        return fatal_read(J, ".infNodesIndex");
+      if (opt.theta_method == LPS_THETA_GTHETA) {
+        if (!lpsio::read_doubles(J + ".ciNodesTheta", C * 2 * M, Gth))
+          return fatal_read(J, ".ciNodesTheta");
+        std::cout << "(G_theta: " << constrains_no << " x " << 2 * max_cloud << ") ";
+      }
+    }
+    // exclusion radius of the VTK output (src/main.cpp:518-530)
+    double domain_r = 0.0;
+    for (int32_t i = 0; i < node_no; i++)
+      domain_r = std::max(domain_r, std::sqrt(nodes[2 * (size_t)i] * nodes[2 * (size_t)i] +
+                                              nodes[2 * (size_t)i + 1] * nodes[2 * (size_t)i + 1]));
+    exclude_r = (opt.abc_enabled && constrains_no > 0) ? domain_r - data2[7] - data2[8] / 100.0
+                                                       : -1.0;
+    u.resize(2 * (size_t)node_no);
+    v.resize(2 * (size_t)node_no);
+    th.resize((size_t)node_no);
+    return 0;
+  }
+
+  lps_problem problem() const {
+    lps_problem p{};
+    p.node_no = node_no;
+    p.constrains_no = constrains_no;
+    p.max_cloud_size = max_cloud;
+    p.E = data2[0]; p.nu = data2[1]; p.rho = data2[2]; p.dt = data2[6];
+    p.delta = data2[7]; p.dx = data2[8]; p.dy = data2[9];
+    p.nodes = nodes.data();
+    p.bctypes = bctypes.data();
+    p.icvalues = ic.data();
+    p.icdvalues = icd.data();
+    p.inf_nodes = infnodes.empty() ? nullptr : infnodes.data();
+    p.iclouds = iclouds.empty() ? nullptr : iclouds.data();
+    p.ci_nodes_u = Gu.empty() ? nullptr : Gu.data();
+    p.ci_nodes_v = Gv.empty() ? nullptr : Gv.data();
+    p.ci_nodes_theta = Gth.empty() ? nullptr : Gth.data();
+    return p;
+  }
+
+  void open_outputs() {
+    energy.open(name + prefix + ".energy", std::ios::out | std::ios::trunc);
+    energy << "# t  E_total  E_kinetic  E_potential  P_x  P_y  max_u  max_theta" << std::endl;
+    verify.open(name + prefix + ".verify", std::ios::out | std::ios::trunc);
+    verify << "# t  E_total  dE/E0  P_x  P_y  max_theta  max_u  max_v  max_a" << std::endl;
+  }
+
+  void write_vtk(int64_t s) {
+    const std::string ss = std::to_string(s);
+    lpsio::write_vtk_vector(name + prefix + "-" + ss + ".vtk", nodes, u, "displacement", node_no,
+                            exclude_r);
+    lpsio::write_vtk_vector(name + prefix + "-vel-" + ss + ".vtk", nodes, v, "velocity", node_no,
+                            exclude_r);
+    lpsio::write_vtk_scalar(name + prefix + "-theta-" + ss + ".vtk", nodes, th, "dilatation",
+                            node_no, exclude_r);
+  }
+
+  // One output record (src/main.cpp:1065-1123); returns 1 on the NaN/Inf FATAL (:1126-1133).
+  int write_record(const lps_diag &d, const char *tag) {
+    const int64_t s = d.step;
+    std::cout << tag << s / 30 << "\tt=" << std::setprecision(6) << d.t << "\tE="
+              << std::setprecision(8) << d.energy << "\tK=" << d.energy_k << "\tP=" << d.energy_p
+              << "\tdE/E0=" << std::setprecision(4) << d.de_rel
+              << "\t|p|=" << std::sqrt(d.px * d.px + d.py * d.py) << "\tuB=" << d.ub_rms
+              << "\tvB=" << d.vb_rms << std::setprecision(10) << std::endl;
+    energy << std::setprecision(16) << d.t << " " << d.energy << " " << d.energy_k << " "
+           << d.energy_p << " " << d.px << " " << d.py << " " << d.max_u << " " << d.theta_max
+           << " " << d.ub_rms << " " << d.vb_rms << std::endl;
+    verify << std::setprecision(16) << d.t << " " << d.energy << " " << d.de_rel << " " << d.px
+           << " " << d.py << " " << d.theta_max << " " << d.max_u << " " << d.max_v << " "
+           << d.max_a << std::endl;
+    if (std::isnan(d.energy) || std::isinf(d.energy)) {
+      std::cerr << "FATAL: " << tag << "Energy is NaN/Inf at step " << s << "!" << std::endl;
+      return 1;
+    }
+    if (std::isnan(d.max_u) || std::isinf(d.max_u)) {
+      std::cerr << "FATAL: " << tag << "Displacement is NaN/Inf at step " << s << "!" << std::endl;
+      return 1;
+    }
+    return 0;
+  }
+};
+
+int run_single(Job &job, const lps_options &opt, bool vtk) {
+  std::cout << "Reading data... " << std::flush;
+  if (job.read(opt)) return 1;
+  std::cout << "finished." << std::endl;
+  const lps_problem p = job.problem();
+  std::cout << std::endl << "=== Material Parameters ===" << std::endl << std::setprecision(10);
+  std::cout << "  E     = " << p.E << std::endl << "  Nu    = " << p.nu << std::endl
+            << "  Rho   = " << p.rho << std::endl;
+  std::cout << std::endl << "=== Discretization ===" << std::endl;
+  std::cout << "  NodeNo = " << job.node_no << std::endl << "  StepNo = " << job.step_no
+            << std::endl << "  delta (R) = " << p.delta << std::endl << "  dx = " << p.dx
+            << ", dy = " << p.dy << std::endl << "  dt = " << p.dt << std::endl
+            << "  ABC nodes = " << job.constrains_no << std::endl
+            << "  maxCloudSize = " << job.max_cloud << std::endl;
+
+  Timer timer;
+  timer.tic();
+  lps_ctx *ctx = nullptr;
+  int32_t rc = lps_create(&p, &opt, &ctx);
+  if (rc != LPS_OK) {
+    std::cerr << "FATAL: " << lps_global_error() << std::endl;
+    return 1;
+  }
+  std::cout << std::endl << "Setup on the GPU (horizons, m_i, theta history) took " << timer.toc()
+            << " s." << std::endl;
+  std::cout << "  m_ref      = " << lps_m_ref(ctx) << std::endl;
+  if (vtk) {
+    lps_fetch(ctx, job.u.data(), nullptr, nullptr, nullptr, nullptr);
+    lpsio::write_vtk_vector(job.name + job.prefix + "-0.vtk", job.nodes, job.u, "displacement",
+                            job.node_no, job.exclude_r);
+  }
+  job.open_outputs();
+  std::cout << std::endl << "Time marching started (" << job.step_no << " steps):" << std::endl;
+  timer.tic();
+  int32_t done = 0;
+  while (done < job.step_no) {
+    const int32_t chunk =
+        std::min<int32_t>(30 - (int32_t)(lps_step_index(ctx) % 30), job.step_no - done);
+    lps_diag d;
+    rc = lps_run(ctx, chunk, &d, 1);
+    done += chunk;
+    if (rc < 0 && rc != LPS_ERR_NONFINITE) {
+      std::cerr << "FATAL: " << lps_last_error(ctx) << std::endl;
+      lps_destroy(ctx);
+      return 1;
+    }
+    if (rc == 0) continue;
+    if (vtk) {
+      lps_fetch(ctx, job.u.data(), job.v.data(), nullptr, job.th.data(), nullptr);
+      job.write_vtk(d.step);
+    }
+    if (job.write_record(d, "")) {
+      lps_destroy(ctx);
+      return 1;
+    }
+  }
+  std::cout << std::endl << "Time marching took " << timer.toc() << " seconds." << std::endl;
+  lps_destroy(ctx);
+  return 0;
+}
+
+int run_batch(const std::string &manifest, const lps_options &opt, bool vtk) {
+  std::ifstream mf(manifest);
+  if (!mf) {
+    std::cerr << "FATAL: Cannot read " << manifest << std::endl;
+    return 1;
+  }
+  std::vector<Job> jobs;
+  std::string line;
+  while (std::getline(mf, line)) {
+    const size_t hash = line.find('#');
+    if (hash != std::string::npos) line.resize(hash);
+    std::istringstream ls(line);
+    std::string prefix, name;
+    if (!(ls >> prefix >> name)) continue;
+    jobs.emplace_back();
+    jobs.back().prefix = prefix;
+    jobs.back().name = name;
+  }
+  if (jobs.empty()) {
+    std::cerr << "FATAL: " << manifest << " lists no job" << std::endl;
+    return 1;
+  }
+  std::cout << "Reading " << jobs.size() << " jobs... " << std::flush;
+  std::vector<lps_problem> probs;
+  for (Job &j : jobs) {
+    if (j.read(opt)) return 1;
+    probs.push_back(j.problem());
+    if (j.step_no != jobs[0].step_no) {
+      std::cerr << "FATAL: the jobs of a batch need the same StepNo (" << j.name << ")"
+                << std::endl;
+      return 1;
+    }
+  }
+  std::cout << "finished." << std::endl;
+  Timer timer;
+  timer.tic();
+  lps_ctx *ctx = nullptr;
+  int32_t rc = lps_create_batch(probs.data(), (int32_t)probs.size(), &opt, &ctx);
+  if (rc != LPS_OK) {
+    std::cerr << "FATAL: " << lps_global_error() << std::endl;
+    return 1;
+  }
+  std::cout << "Setup on the GPU took " << timer.toc() << " s: " << lps_describe(ctx) << std::endl;
+  const int32_t B = (int32_t)jobs.size();
+  for (int32_t i = 0; i < B; i++) {
+    Job &j = jobs[i];
+    if (vtk) {
+      lps_fetch_instance(ctx, i, j.u.data(), nullptr, nullptr, nullptr, nullptr);
+      lpsio::write_vtk_vector(j.name + j.prefix + "-0.vtk", j.nodes, j.u, "displacement",
+                              j.node_no, j.exclude_r);
+    }
+    j.open_outputs();
+  }
+  const int32_t steps = jobs[0].step_no;
+  std::cout << std::endl << "Time marching started (" << steps << " steps, " << B << " jobs):"
+            << std::endl;
+  timer.tic();
+  std::vector<lps_diag> d(B);
+  int32_t done = 0;
+  while (done < steps) {
+    const int32_t chunk = std::min<int32_t>(30 - (int32_t)(lps_step_index(ctx) % 30), steps - done);
+    rc = lps_run_batch(ctx, chunk, d.data(), 1);
+    done += chunk;
+    if (rc < 0 && rc != LPS_ERR_NONFINITE) {
+      std::cerr << "FATAL: " << lps_last_error(ctx) << std::endl;
+      lps_destroy(ctx);
+      return 1;
+    }
+    if (rc == 0) continue;
+    int fatal = 0;
+    for (int32_t i = 0; i < B; i++) {
+      Job &j = jobs[i];
+      if (vtk) {
+        lps_fetch_instance(ctx, i, j.u.data(), j.v.data(), nullptr, j.th.data(), nullptr);
+        j.write_vtk(d[i].step);
+      }
+      const std::string tag = "[" + j.name + "] ";
+      fatal |= j.write_record(d[i], tag.c_str());
+    }
+    if (fatal) {
+      lps_destroy(ctx);
+      return 1;
+    }
+  }
+  std::cout << std::endl << "Time marching took " << timer.toc() << " seconds." << std::endl;
+  lps_destroy(ctx);
+  return 0;
+}
+
 }  // namespace
 
 int main(int argc, char *argv[]) {
@@ -48,6 +322,7 @@ int main(int argc, char *argv[]) {
             << std::endl;
 
   std::vector<std::string> pos;
+  std::string manifest;
   lps_options opt;
   lps_default_options(&opt);
   bool vtk = true;
@@ -61,155 +336,23 @@ int main(int argc, char *argv[]) {
     else if (a == "--exact") opt.exact = 1;
     else if (a == "--no-graph") opt.use_graph = 0;
     else if (a.rfind("--device=", 0) == 0) opt.device = std::atoi(a.c_str() + 9);
-    else {
+    else if (a.rfind("--batch=", 0) == 0) manifest = a.substr(8);
+    else if (a == "--abc=lsq") {}
+    else if (a == "--abc=learned") {
+      std::cerr << "FATAL: the learned ABC closure is not part of this build" << std::endl;
+      return 1;
+    } else {
       std::cerr << "unknown option " << a << std::endl;
       return 1;
     }
   }
+  if (!manifest.empty()) return run_batch(manifest, opt, vtk);
   if (pos.size() != 2) {
     std::cout << "Usage: " << argv[0] << " <prefix> <jobname>" << std::endl;
     return 0;
   }
-  const std::string Prefix = pos[0], JobName = pos[1];
-
-  // ---- read input data (src/main.cpp:176-295) ----
-  std::cout << "Reading data... " << std::flush;
-  std::vector<int32_t> data1;
-  std::vector<double> data2;
-  if (!lpsio::read_ints(JobName + ".data1", 6, data1)) return fatal_read(JobName, ".data1");
-  if (!lpsio::read_doubles(JobName + ".data2", 15, data2)) return fatal_read(JobName, ".data2");
-  const int32_t NodeNo = data1[0], StepNo = data1[1], PrenotchNo = data1[2];
-  const int32_t constrainsNo = data1[4], maxCloudSize = data1[5];
-  if (PrenotchNo > 0) {
-    std::cerr << "FATAL: prenotch bond cutting is not supported by this build" << std::endl;
-    return 1;
-  }
-  std::vector<double> nodes, bcvalues, icd, ic, Gu, Gv, Gth;
-  std::vector<int32_t> bctypes, iclouds, infnodes;
-  if (!lpsio::read_doubles(JobName + ".nodes", 2 * (int64_t)NodeNo, nodes)) return fatal_read(JobName, ".nodes");
-  if (!lpsio::read_ints(JobName + ".bctypes", 2 * (int64_t)NodeNo, bctypes)) return fatal_read(JobName, ".bctypes");
-  if (!lpsio::read_doubles(JobName + ".bcvalues", 2 * (int64_t)NodeNo, bcvalues)) return fatal_read(JobName, ".bcvalues");
-  if (!lpsio::read_doubles(JobName + ".icdvalues", 2 * (int64_t)NodeNo, icd)) return fatal_read(JobName, ".icdvalues");
-  if (!lpsio::read_doubles(JobName + ".icvalues", 2 * (int64_t)NodeNo, ic)) return fatal_read(JobName, ".icvalues");
-  if (opt.abc_enabled && constrainsNo > 0) {
-    const int64_t C = constrainsNo, M = maxCloudSize;
-    if (!lpsio::read_doubles(JobName + ".ciNodesU", 2 * C * 4 * M, Gu)) return fatal_read(JobName, ".ciNodesU");
-    if (!lpsio::read_doubles(JobName + ".ciNodesV", 2 * C * 4 * M, Gv)) return fatal_read(JobName, ".ciNodesV");
-    if (!lpsio::read_rounded_ints(JobName + ".iClouds", C * M, iclouds)) return fatal_read(JobName, ".iClouds");
-    if (!lpsio::read_rounded_ints(JobName + ".infNodesIndex", C, infnodes)) return fatal_read(JobName, ".infNodesIndex");
-    if (opt.theta_method == LPS_THETA_GTHETA) {
-      if (!lpsio::read_doubles(JobName + ".ciNodesTheta", C * 2 * M, Gth)) return fatal_read(JobName, ".ciNodesTheta");
-      std::cout << "(G_theta: " << constrainsNo << " x " << 2 * maxCloudSize << ") ";
-    }
-  }
-  std::cout << "finished." << std::endl;
-
-  lps_problem p{};
-  p.node_no = NodeNo;
-  p.constrains_no = constrainsNo;
-  p.max_cloud_size = maxCloudSize;
-  p.E = data2[0]; p.nu = data2[1]; p.rho = data2[2]; p.dt = data2[6];
-  p.delta = data2[7]; p.dx = data2[8]; p.dy = data2[9];
-  p.nodes = nodes.data();
-  p.bctypes = bctypes.data();
-  p.icvalues = ic.data();
-  p.icdvalues = icd.data();
-  p.inf_nodes = infnodes.empty() ? nullptr : infnodes.data();
-  p.iclouds = iclouds.empty() ? nullptr : iclouds.data();
-  p.ci_nodes_u = Gu.empty() ? nullptr : Gu.data();
-  p.ci_nodes_v = Gv.empty() ? nullptr : Gv.data();
-  p.ci_nodes_theta = Gth.empty() ? nullptr : Gth.data();
-
-  std::cout << std::endl << "=== Material Parameters ===" << std::endl << std::setprecision(10);
-  std::cout << "  E     = " << p.E << std::endl << "  Nu    = " << p.nu << std::endl
-            << "  Rho   = " << p.rho << std::endl;
-  std::cout << std::endl << "=== Discretization ===" << std::endl;
-  std::cout << "  NodeNo = " << NodeNo << std::endl << "  StepNo = " << StepNo << std::endl
-            << "  delta (R) = " << p.delta << std::endl << "  dx = " << p.dx << ", dy = " << p.dy
-            << std::endl << "  dt = " << p.dt << std::endl << "  ABC nodes = " << constrainsNo
-            << std::endl << "  maxCloudSize = " << maxCloudSize << std::endl;
-
-  Timer timer;
-  timer.tic();
-  lps_ctx *ctx = nullptr;
-  int32_t rc = lps_create(&p, &opt, &ctx);
-  if (rc != LPS_OK) {
-    std::cerr << "FATAL: " << lps_global_error() << std::endl;
-    return 1;
-  }
-  std::cout << std::endl << "Setup on the GPU (horizons, m_i, theta history) took " << timer.toc()
-            << " s." << std::endl;
-  std::cout << "  m_ref      = " << lps_m_ref(ctx) << std::endl;
-
-  // exclusion radius for the VTK output (src/main.cpp:518-530)
-  double domainRadius = 0.0;
-  for (int32_t i = 0; i < NodeNo; i++) {
-    const double r = std::sqrt(nodes[2 * (size_t)i] * nodes[2 * (size_t)i] +
-                               nodes[2 * (size_t)i + 1] * nodes[2 * (size_t)i + 1]);
-    if (r > domainRadius) domainRadius = r;
-  }
-  const double output_exclude_r = (opt.abc_enabled && constrainsNo > 0)
-                                      ? domainRadius - p.delta - p.dx / 100.0 : -1.0;
-
-  std::vector<double> u(2 * (size_t)NodeNo), v(2 * (size_t)NodeNo), th((size_t)NodeNo);
-  if (vtk) {
-    lps_fetch(ctx, u.data(), nullptr, nullptr, nullptr, nullptr);
-    lpsio::write_vtk_vector(JobName + Prefix + "-0.vtk", nodes, u, "displacement", NodeNo,
-                            output_exclude_r);
-  }
-  std::ofstream energyFile(JobName + Prefix + ".energy", std::ios::out | std::ios::trunc);
-  energyFile << "# t  E_total  E_kinetic  E_potential  P_x  P_y  max_u  max_theta" << std::endl;
-  std::ofstream verifyFile(JobName + Prefix + ".verify", std::ios::out | std::ios::trunc);
-  verifyFile << "# t  E_total  dE/E0  P_x  P_y  max_theta  max_u  max_v  max_a" << std::endl;
-
-  std::cout << std::endl << "Time marching started (" << StepNo << " steps):" << std::endl;
-  timer.tic();
-  int32_t done = 0;
-  while (done < StepNo) {
-    const int32_t chunk = std::min<int32_t>(30 - (int32_t)(lps_step_index(ctx) % 30), StepNo - done);
-    lps_diag d;
-    rc = lps_run(ctx, chunk, &d, 1);
-    done += chunk;
-    if (rc < 0 && rc != LPS_ERR_NONFINITE) {
-      std::cerr << "FATAL: " << lps_last_error(ctx) << std::endl;
-      lps_destroy(ctx);
-      return 1;
-    }
-    if (rc == 0) continue;
-    const int64_t s = d.step;
-    std::cout << s / 30 << "\tt=" << std::setprecision(6) << d.t << "\tE=" << std::setprecision(8)
-              << d.energy << "\tK=" << d.energy_k << "\tP=" << d.energy_p
-              << "\tdE/E0=" << std::setprecision(4) << d.de_rel
-              << "\t|p|=" << std::sqrt(d.px * d.px + d.py * d.py) << "\tuB=" << d.ub_rms
-              << "\tvB=" << d.vb_rms << std::setprecision(10) << std::endl;
-    if (vtk) {
-      lps_fetch(ctx, u.data(), v.data(), nullptr, th.data(), nullptr);
-      const std::string ss = std::to_string(s);
-      lpsio::write_vtk_vector(JobName + Prefix + "-" + ss + ".vtk", nodes, u, "displacement",
-                              NodeNo, output_exclude_r);
-      lpsio::write_vtk_vector(JobName + Prefix + "-vel-" + ss + ".vtk", nodes, v, "velocity",
-                              NodeNo, output_exclude_r);
-      lpsio::write_vtk_scalar(JobName + Prefix + "-theta-" + ss + ".vtk", nodes, th, "dilatation",
-                              NodeNo, output_exclude_r);
-    }
-    energyFile << std::setprecision(16) << d.t << " " << d.energy << " " << d.energy_k << " "
-               << d.energy_p << " " << d.px << " " << d.py << " " << d.max_u << " " << d.theta_max
-               << " " << d.ub_rms << " " << d.vb_rms << std::endl;
-    verifyFile << std::setprecision(16) << d.t << " " << d.energy << " " << d.de_rel << " " << d.px
-               << " " << d.py << " " << d.theta_max << " " << d.max_u << " " << d.max_v << " "
-               << d.max_a << std::endl;
-    if (std::isnan(d.energy) || std::isinf(d.energy)) {
-      std::cerr << "FATAL: Energy is NaN/Inf at step " << s << "!" << std::endl;
-      lps_destroy(ctx);
-      return 1;
-    }
-    if (std::isnan(d.max_u) || std::isinf(d.max_u)) {
-      std::cerr << "FATAL: Displacement is NaN/Inf at step " << s << "!" << std::endl;
-      lps_destroy(ctx);
-      return 1;
-    }
-  }
-  std::cout << std::endl << "Time marching took " << timer.toc() << " seconds." << std::endl;
-  lps_destroy(ctx);
-  return 0;
+  Job job;
+  job.prefix = pos[0];
+  job.name = pos[1];
+  return run_single(job, opt, vtk);
 }

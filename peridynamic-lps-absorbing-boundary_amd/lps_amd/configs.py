@@ -6,6 +6,7 @@ computed once and pinned here so that a run does not repeat the sweep's ~10^5 le
 """
 from __future__ import annotations
 
+import math
 from dataclasses import dataclass
 from typing import Optional
 
@@ -60,3 +61,101 @@ def make(name: str, workers: Optional[int] = None, steps: Optional[int] = None) 
                         dk_factor=w.dk_factor, workers=workers)
     p.info["workload"] = w.name
     return p
+
+
+# ---------------------------------------------------------------------------------------------
+# Batched workloads: many independent domains advanced together (lps_create_batch).
+# ---------------------------------------------------------------------------------------------
+NU_SET = (0.15, 0.20, 0.25, 0.30, 0.35, 0.40, 0.45)
+
+# (dt, delta_kappa factor) of the preprocessor's stability sweep (preprocess.make_problem with
+# dt=None) on an R = 100 mm domain, per nu (every pair stable: max ||G|| between 0.9961 and
+# 0.99996); each batch instance takes the pair of its nu and builds its own ABC operators for its
+# own radius.
+NU_SWEEP = {
+    0.15: (17.0e-9, 0.7),
+    0.20: (17.0e-9, 0.7),
+    0.25: (17.5e-9, 0.9),
+    0.30: (15.0e-9, 0.5),
+    0.35: (17.5e-9, 0.9),
+    0.40: (17.0e-9, 0.7),
+    0.45: (16.0e-9, 0.7),
+}
+
+
+@dataclass(frozen=True)
+class BatchWorkload:
+    name: str
+    count: int
+    r_min: float
+    r_max: float
+    steps: int
+    seed: int
+    description: str
+
+
+BATCH_WORKLOADS = {
+    # 64 instances, R ~ U(50, 250) mm snapped to dx, nu from NU_SET, 1-3 Gaussian pulses with
+    # r ~ U(10, 40) mm, centres uniform in the disk of radius R/2, amplitude N(0, 1e-3), direction at
+    # a uniform angle; 3000 steps.
+    "cfg3": BatchWorkload("cfg3", 64, 50.0, 250.0, 3000, 0,
+                          "64 domains, R ~ U(50,250) mm, nu in {0.15..0.45}, 1-3 pulses, LSQ ABC "
+                          "(G_theta), 3000 steps"),
+}
+
+
+def batch_specs(name: str, count: Optional[int] = None):
+    """The instances of a batched workload (radius, nu, pulses, dt, delta_kappa factor), drawn from
+    the workload's seed."""
+    import numpy as np
+    w = BATCH_WORKLOADS[name]
+    rng = np.random.default_rng(w.seed)
+    dx = pp.DX_DEFAULT
+    specs = []
+    for _ in range(count or w.count):
+        R = round(rng.uniform(w.r_min, w.r_max) / dx) * dx
+        nu = NU_SET[int(rng.integers(len(NU_SET)))]
+        pulses = []
+        for _ in range(int(rng.integers(1, 4))):
+            r = rng.uniform(10.0, 40.0)
+            rho = 0.5 * R * math.sqrt(rng.uniform())
+            phi = rng.uniform(0.0, 2.0 * math.pi)
+            ang = rng.uniform(0.0, 2.0 * math.pi)
+            amp = rng.normal(0.0, 1e-3)
+            pulses.append(pp.Pulse(rho * math.cos(phi), rho * math.sin(phi), r, amp,
+                                   (math.cos(ang), math.sin(ang))))
+        dt, dk = NU_SWEEP[nu]
+        specs.append(dict(R=R, nu=nu, pulses=pulses, dt=dt, dk_factor=dk))
+    return specs
+
+
+def make_batch(name: str, workers: Optional[int] = None, steps: Optional[int] = None,
+               count: Optional[int] = None, progress=None):
+    """The problems of a batched workload; progress(i, problem) is called after each one."""
+    w = BATCH_WORKLOADS[name]
+    probs = []
+    for i, sp in enumerate(batch_specs(name, count)):
+        p = pp.make_problem(sp["R"], sp["nu"], pulses=sp["pulses"], steps=steps or w.steps,
+                            dt=sp["dt"], dk_factor=sp["dk_factor"], workers=workers)
+        p.info["workload"] = name
+        probs.append(p)
+        if progress:
+            progress(i, p)
+    return probs
+
+
+def save_batch(probs, path: str) -> None:
+    import numpy as np
+    arrays = {"count": np.int64(len(probs))}
+    for i, p in enumerate(probs):
+        arrays[f"i{i}_steps"] = np.int64(p.steps)
+        for k in _ARRAYS:
+            arrays[f"i{i}_{k}"] = getattr(p, k)
+    np.savez(path, **arrays)
+
+
+def load_batch(path: str):
+    import numpy as np
+    z = np.load(path, allow_pickle=False)
+    return [pp.Problem(steps=int(z[f"i{i}_steps"]), **{k: z[f"i{i}_{k}"] for k in _ARRAYS})
+            for i in range(int(z["count"]))]

@@ -252,7 +252,10 @@ def _dgelsd(AT: np.ndarray, B: np.ndarray, rcond: float = SVD_RCOND) -> np.ndarr
     Bp[:m] = B
     x, _s, _rank, info = lapack.dgelsd(np.asfortranarray(AT), Bp, lwork, liwork, cond=rcond)
     if info != 0:
-        raise RuntimeError(f"dgelsd failed (info={info})")
+        # the reference warns and keeps what dgelsd left in B (preprocessor/main.cpp:605-611,
+        # 769-772); the stability sweep then judges the resulting operator by its norm
+        import warnings
+        warnings.warn(f"dgelsd did not converge (info={info})")
     return x[:n]
 
 

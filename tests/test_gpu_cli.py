@@ -55,3 +55,28 @@ def test_cli_exits_1_on_a_nonfinite_state(tmp_path):
     job, r = run_cli(tmp_path, prob, "--vtk=off")
     assert r.returncode == 1, r.stdout[-2000:]
     assert "FATAL" in r.stdout + r.stderr
+
+
+def test_cli_batch_manifest_runs_several_jobs_in_one_process(tmp_path):
+    """--batch=<manifest>: the jobs advance together and each gets the reference's outputs."""
+    print("cli batch", flush=True)
+    names = ["abc_r10_nu025", "abc_r12_nu033"]
+    steps = 300
+    lines = []
+    for k, name in enumerate(names):
+        prob, _ = load_golden(name)
+        prob.steps = steps
+        files.write_job(prob, str(tmp_path / f"job{k}"), exact_text=True)
+        lines.append(f"_out job{k}")
+    (tmp_path / "jobs.txt").write_text("# two jobs\n" + "\n".join(lines) + "\n")
+    r = subprocess.run([EXE, "--batch=jobs.txt", "--vtk=off"], cwd=tmp_path, capture_output=True,
+                       text=True, timeout=240)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    for k, name in enumerate(names):
+        prob, ref = load_golden(name)
+        n = steps // 30
+        job = str(tmp_path / f"job{k}")
+        assert_series_close(files.read_series(job + "_out.energy"), ref["energy"][:n], ENERGY_KEYS,
+                            1e-10, prob, name)
+        assert_series_close(files.read_series(job + "_out.verify"), ref["verify"][:n], VERIFY_KEYS,
+                            1e-10, prob, name)
