@@ -17,96 +17,14 @@
 #include <vector>
 
 #include "../../include/lps_amd.h"
+#include "lps_ctx.h"
 #include "lps_internal.h"
 
 using namespace lps;
 
 namespace {
 thread_local std::string g_create_error;
-
-struct Dev2 {
-  double *x = nullptr, *y = nullptr;
-};
 }  // namespace
-
-struct lps_ctx {
-  lps_options opt{};
-  hipStream_t stream = nullptr;
-  hipStream_t stream2 = nullptr;  // the ABC rows' force kernel beside the node force kernel
-  hipEvent_t ev_fork[2] = {nullptr, nullptr}, ev_join[2] = {nullptr, nullptr};
-  std::string err, path;
-  int32_t n = 0, n_slots = 0, n_abc = 0, M = 0;
-  double E = 0, nu = 0, rho = 0, dt = 0, delta = 0, dx = 0, dy = 0;
-  double lam = 0, mu = 0, kappa = 0, m_ref = 0, exclude_r = -1, E0 = -1;
-  bool fast = false;
-  bool stencil68_ok = false;  // the unrolled 68-bond kernels apply to this problem
-  int64_t n_full_nodes = 0;   // nodes in the full tiles
-  // instances: one for lps_create, several for lps_create_batch
-  std::vector<InstParams> inst_h;
-  InstParams *inst_d = nullptr;
-  bool batch = false;
-  int32_t n_inst = 1;
-  std::vector<int32_t> tile_off, abc_n, node_n;  // per instance (tile_off: n_inst + 1 entries)
-  std::vector<int64_t> slot_off, node_off;       // n_inst + 1 entries
-  std::vector<double> E0_inst;
-  int32_t *tile_off_d = nullptr;
-  DiagPartial *dout_inst = nullptr, *hout_inst = nullptr;
-  Dev2 P0;                        // batch: prediction of step 1 and theta history of the setup
-  double *TH0 = nullptr;
-  int64_t step = 0;
-  std::vector<int32_t> perm_h;  // slot -> original index, -1 at holes
-  HorizonBuild hb;
-  std::vector<void *> allocs;
-  // device arrays (per slot unless noted)
-  double *x = nullptr, *y = nullptr, *m = nullptr, *c2m = nullptr;
-  int8_t *kind = nullptr, *is_bc = nullptr;
-  int32_t *perm_d = nullptr, *abc_slot = nullptr;
-  int32_t *cloud = nullptr;
-  double *gu = nullptr, *gv = nullptr, *gth = nullptr;
-  TileArgs tile{};
-  Dev2 U[3], V[3], A[2], P[2], IC, ICD;
-  double *TH[3] = {nullptr, nullptr, nullptr};
-  DiagPartial *partials = nullptr, *dout = nullptr, *hout = nullptr;
-  int32_t n_partials = 0;
-  // graph of 30 steps from a step that is a multiple of 30
-  hipGraphExec_t graph30 = nullptr;
-  // timing
-  bool timing = false;
-  std::vector<hipEvent_t> ev;  // 4 per timed step
-  int32_t ev_used = 0;
-  double t_pass[4] = {0, 0, 0, 0};  // dilatation, force (nodes), force (ABC rows), whole step
-  int64_t t_count = 0;
-
-  template <class T>
-  hipError_t alloc(T **p, size_t count) {
-    hipError_t e = hipMalloc((void **)p, std::max<size_t>(count, 1) * sizeof(T));
-    if (e == hipSuccess) allocs.push_back(*p);
-    return e;
-  }
-  template <class T>
-  hipError_t upload(T **p, const std::vector<T> &h) {
-    hipError_t e = alloc(p, h.size());
-    if (e != hipSuccess) return e;
-    if (h.empty()) return hipSuccess;
-    return hipMemcpyAsync(*p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, stream);
-  }
-  int32_t fail(int32_t code, const std::string &msg) {
-    err = msg;
-    return code;
-  }
-  int32_t hip_fail(hipError_t e, const char *what) {
-    err = std::string(what) + ": " + hipGetErrorString(e);
-    return LPS_ERR_HIP;
-  }
-  StepArgs args() const;
-  StepState state(int64_t s) const;
-};
-
-#define CTX_TRY(ctx, call)                                   \
-  do {                                                       \
-    hipError_t e_ = (call);                                  \
-    if (e_ != hipSuccess) return (ctx)->hip_fail(e_, #call); \
-  } while (0)
 
 StepArgs lps_ctx::args() const {
   StepArgs a{};
