@@ -165,6 +165,93 @@ const char *lps_last_error(const lps_ctx *ctx);
 const char *lps_global_error(void); /* message of a failed lps_create */
 void lps_destroy(lps_ctx *ctx);
 
+/* ------------------------------------------------------------------------------------------------
+ * Learned ABC closure. No counterpart in the reference, whose ABC is the least-squares update
+ * G_u / G_v (src/main.cpp:883-974); the closure adds a learned correction to that update at every
+ * ABC row, and the least-squares update stays what it corrects and what a correction of zero
+ * gives back bit for bit.
+ *
+ * Network: 1600 inputs -> 256 -> 256 -> 256 (ReLU) -> 4. Layers 1-3 run on the matrix cores with
+ * BF16 operands and FP32 accumulation; the output layer is FP32. Inputs of an ABC row, in the
+ * lattice frame that maps the row's outward normal into the 0-45 degree sector: the last 8 steps
+ * of u and v at the 49 lattice offsets |o|^2 <= 16 around the node (zero where no node is), u
+ * divided by s_u = max |u| and v by s_v = max |v| over those inputs, then nu' = (nu - 0.3) / 0.15,
+ * the canonical normal (n'_x, n'_y) and the depth (R - |x|) / delta. Outputs, in the same frame:
+ * the corrections of U^{n+1} and V^{n+1} in units of s_u sigma_u and s_v sigma_v.
+ *
+ * Samples: inputs (1600 BF16 values, as uint16 bit patterns) and aux (8 floats: the 4 targets in
+ * the units above without sigma, the weight, s_u, s_v, and the row it came from).
+ * ---------------------------------------------------------------------------------------------- */
+#define LPS_CLOSURE_IN 1600
+#define LPS_CLOSURE_HIDDEN 256
+#define LPS_CLOSURE_OUT 4
+typedef struct lps_closure lps_closure;
+
+/* Number of FP32 parameters: W1 [256][1600], b1 [256], W2 [256][256], b2, W3 [256][256], b3,
+ * W4 [4][256], b4 [4], in that order, weights [out][in]. */
+int64_t lps_closure_param_count(void);
+/* A network on a device, initialised from seed (He-normal hidden layers, output layer near 0). */
+int32_t lps_closure_create(int32_t device, uint64_t seed, lps_closure **out);
+void lps_closure_destroy(lps_closure *clo);
+const char *lps_closure_error(const lps_closure *clo);
+int32_t lps_closure_get_params(lps_closure *clo, float *params);
+/* Replaces the parameters and clears the optimizer state. */
+int32_t lps_closure_set_params(lps_closure *clo, const float *params);
+/* Output scales sigma_u, sigma_v (the targets' spread; the network learns targets / sigma). */
+int32_t lps_closure_set_scales(lps_closure *clo, float sigma_u, float sigma_v);
+int32_t lps_closure_get_scales(lps_closure *clo, float *sigma2);
+
+/* Forward pass over n rows of inputs: out is n x 4 (units of sigma). */
+int32_t lps_closure_forward(lps_closure *clo, const uint16_t *x, int64_t n, float *out);
+/* One Adam step (lr; beta 0.9 / 0.999, eps 1e-8) on a batch of n rows (n a multiple of 64) given
+ * by the host; loss = sum_r w_r |f(x_r) - t_r / sigma|^2 / n. grads (param_count floats, may be
+ * NULL) receives the gradient of that loss before the update. lr = 0 leaves the weights. */
+int32_t lps_closure_train_batch(lps_closure *clo, const uint16_t *x, const float *target,
+                                const float *weight, int32_t n, float lr, float *loss,
+                                float *grads);
+
+/* The sample store on the closure's device. */
+int32_t lps_closure_dataset_reserve(lps_closure *clo, int64_t capacity);
+int64_t lps_closure_dataset_count(lps_closure *clo);
+int32_t lps_closure_dataset_clear(lps_closure *clo);
+int32_t lps_closure_dataset_append(lps_closure *clo, const uint16_t *x, const float *aux,
+                                   int64_t n);
+int32_t lps_closure_dataset_read(lps_closure *clo, int64_t first, int64_t n, uint16_t *x,
+                                 float *aux);
+/* nsteps Adam steps on batches of `batch` rows (a multiple of 64) drawn with replacement from
+ * samples [first, first + count); the step size follows a cosine from lr0 to lr1. loss_curve
+ * (nsteps floats, may be NULL) receives each step's batch loss. */
+int32_t lps_closure_train(lps_closure *clo, int32_t nsteps, int32_t batch, float lr0, float lr1,
+                          int64_t first, int64_t count, uint64_t seed, float *loss_curve);
+/* Over samples [first, first + count): out[0] = sum w |f - t/sigma|^2 (sigma units),
+ * out[1] = sum w |t/sigma|^2 (the least-squares ABC alone), out[2] = sum w. */
+int32_t lps_closure_evaluate(lps_closure *clo, int64_t first, int64_t count, double *out3);
+
+/* Teacher-forced samples from a reflection-free run. truth: a context on a larger domain of the
+ * same origin-centred lattice (the fast path); trunc: the truncated problem whose ABC rows are
+ * emulated (its nodes lie on truth's lattice). Runs truth for nsteps steps from its current state;
+ * after each step n + 1 with n + 1 >= first_step and (n + 1) % sample_every == 0, every row of
+ * trunc whose hash (seed, step, row) falls under keep and whose s_u >= amp_min appends one
+ * sample: its inputs at step n from truth's history, and as target truth's U^{n+1}, V^{n+1} at
+ * the row's node minus trunc's G_u / G_v applied to truth's U^n, U^{n-1}, V^n, V^{n-1}; weight
+ * (s_u / amp_ref)^2. Samples beyond the capacity are dropped (the count still grows). */
+int32_t lps_closure_collect(lps_closure *clo, lps_ctx *truth, const lps_problem *trunc,
+                            int32_t nsteps, int32_t first_step, int32_t sample_every, double keep,
+                            double amp_ref, double amp_min, uint64_t seed);
+
+/* Applies clo at every ABC row of ctx (fast path only; a batch uses each instance's nu and
+ * radius) from the next step on; the history starts empty (zeros) and is cleared by lps_reset.
+ * clo must outlive the attachment. Replaces an earlier attachment. */
+int32_t lps_closure_attach(lps_ctx *ctx, lps_closure *clo);
+int32_t lps_closure_detach(lps_ctx *ctx);
+/* Factor on the correction per instance (count = lps_instance_count; 1 after attach; 0 gives the
+ * least-squares ABC). */
+int32_t lps_closure_set_gain(lps_ctx *ctx, const float *gain);
+/* Inputs and scales of ctx's ABC rows from the history of the last completed step (n_abc x 1600
+ * BF16 bits, n_abc x 2), and the correction that step applied (n_abc x 8: 4 outputs, s_u, s_v,
+ * 0, 0); any pointer may be NULL. */
+int32_t lps_closure_inspect(lps_ctx *ctx, uint16_t *x, float *scale2, float *applied8);
+
 #ifdef __cplusplus
 }
 #endif

@@ -127,6 +127,8 @@ double lps_m_ref(const lps_ctx *ctx) { return ctx->m_ref; }
 void lps_destroy(lps_ctx *ctx) {
   if (!ctx) return;
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->stream2) (void)hipStreamSynchronize(ctx->stream2);
+  closure_free(ctx);
   if (ctx->graph30) (void)hipGraphExecDestroy(ctx->graph30);
   for (hipEvent_t e : ctx->ev) (void)hipEventDestroy(e);
   for (void *p : ctx->allocs) (void)hipFree(p);
@@ -581,7 +583,12 @@ static int32_t reset_state(lps_ctx *c) {
 int32_t lps_reset(lps_ctx *c) {
   int32_t rc = flush_events(c);
   if (rc != LPS_OK) return rc;
-  return reset_state(c);
+  rc = reset_state(c);
+  if (rc == LPS_OK && c->clo) {
+    CTX_TRY(c, closure_reset_hook(c));
+    CTX_TRY(c, hipStreamSynchronize(c->stream));
+  }
+  return rc;
 }
 
 // A context with its device, streams and events, nothing else.
@@ -966,6 +973,7 @@ static hipError_t step_once(lps_ctx *c, bool record_events) {
     if ((e = launch_abc_theta(a, st, c->stream2)) != hipSuccess) return e;
     if ((e = hipEventRecord(c->ev_join[0], c->stream2)) != hipSuccess) return e;
     if ((e = launch_abc_uv(a, st, c->stream2)) != hipSuccess) return e;
+    if (c->clo && (e = closure_step_hook(c, st, c->stream2)) != hipSuccess) return e;
     if ((e = hipEventRecord(c->ev_join[1], c->stream2)) != hipSuccess) return e;
   } else if (abc && (e = launch_abc_theta(a, st, c->stream)) != hipSuccess) {
     return e;
@@ -981,8 +989,10 @@ static hipError_t step_once(lps_ctx *c, bool record_events) {
     if ((e = launch_force_part(a, st, c->stream)) != hipSuccess) return e;
     if (fork && (e = hipStreamWaitEvent(c->stream, c->ev_join[1], 0)) != hipSuccess) return e;
     if (abc && !fork && (e = launch_abc_uv(a, st, c->stream)) != hipSuccess) return e;
+    if (abc && !fork && c->clo && (e = closure_step_hook(c, st, c->stream)) != hipSuccess) return e;
     if (abc && (e = launch_abc_predict(a, st, c->stream)) != hipSuccess) return e;
   }
+  if (c->clo && !abc && (e = closure_step_hook(c, st, c->stream)) != hipSuccess) return e;
   if (record_events) {
     if ((e = hipEventRecord(ev[3], c->stream)) != hipSuccess) return e;
     c->ev_used++;
@@ -1287,3 +1297,5 @@ int64_t lps_horizons(lps_ctx *c, int64_t *row_ptr, int32_t *cols) {
 }
 
 }  // extern "C"
+
+void lps::ctx_drop_graphs(lps_ctx *c) { drop_graphs(c); }

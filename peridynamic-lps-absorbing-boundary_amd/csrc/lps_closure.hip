@@ -1,0 +1,610 @@
+// Learned ABC closure, coupling to the time loop: the per-node history of u and v around the ABC
+// rows, the correction applied inside the step after the least-squares G_u / G_v update
+// (src/main.cpp:883-974 is the update it corrects), and teacher-forced sample collection from a
+// larger, reflection-free domain. No counterpart in the reference.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/lps_amd.h"
+#include "lps_closure.h"
+#include "lps_ctx.h"
+
+namespace lps {
+
+// The closure attached to a context (or, with clo == nullptr, the history alone).
+struct ClosureAttach {
+  lps_closure *clo = nullptr;
+  int32_t n_rows = 0, n_hist = 0, n_inst = 1;
+  int32_t *hist_slot = nullptr;  // n_hist: slot of each history node
+  int32_t *hidx = nullptr;       // n_rows x 49
+  int8_t *xf = nullptr;          // n_rows
+  float *feat = nullptr;         // n_rows x 4
+  int32_t *row_slot = nullptr, *row_inst = nullptr;
+  float *gain = nullptr;                            // n_inst
+  float *hist[2] = {nullptr, nullptr}, *mx[2] = {nullptr, nullptr};  // by step parity
+  float *dbg = nullptr;                             // n_rows x 8, last applied correction
+  std::vector<void *> allocs;
+  template <class T>
+  hipError_t alloc(T **q, size_t count) {
+    hipError_t e = hipMalloc((void **)q, std::max<size_t>(count, 1) * sizeof(T));
+    if (e == hipSuccess) allocs.push_back(*q);
+    return e;
+  }
+  template <class T>
+  hipError_t upload(T **q, const std::vector<T> &h, hipStream_t s) {
+    hipError_t e = alloc(q, h.size());
+    if (e == hipSuccess && !h.empty())
+      e = hipMemcpyAsync(*q, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, s);
+    return e;
+  }
+  ~ClosureAttach() {
+    for (void *q : allocs) (void)hipFree(q);
+  }
+  GatherSrc src(int par) const { return GatherSrc{hist[par], mx[par], hidx, xf, feat}; }
+};
+
+// History shift of step n: record = (U^n, V^n, the 7 steps before), and its max |u|, max |v|.
+__global__ void __launch_bounds__(256) k_hist_shift(int32_t n_hist, const int32_t *slot,
+                                                    const double *ux, const double *uy,
+                                                    const double *vx, const double *vy,
+                                                    const float *prev, float *cur, float *mx) {
+  const int32_t b = blockIdx.x * 256 + threadIdx.x;
+  if (b >= n_hist) return;
+  const int32_t i = slot[b];
+  const float4 *p = (const float4 *)(prev + (int64_t)b * kClRec);
+  float4 *q = (float4 *)(cur + (int64_t)b * kClRec);
+  float mu = 0.0f, mv = 0.0f;
+#pragma unroll
+  for (int half = 0; half < 2; half++) {
+    const float nx = (float)(half == 0 ? ux[i] : vx[i]), ny = (float)(half == 0 ? uy[i] : vy[i]);
+    float4 in[4], out[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) in[k] = p[4 * half + k];
+    // shift by one (x, y) pair: out pair h = in pair h - 1, out pair 0 = current
+    out[0] = make_float4(nx, ny, in[0].x, in[0].y);
+#pragma unroll
+    for (int k = 1; k < 4; k++) out[k] = make_float4(in[k - 1].z, in[k - 1].w, in[k].x, in[k].y);
+    float m = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      q[4 * half + k] = out[k];
+      m = fmaxf(m, fmaxf(fmaxf(fabsf(out[k].x), fabsf(out[k].y)), fmaxf(fabsf(out[k].z), fabsf(out[k].w))));
+    }
+    if (half == 0) mu = m;
+    else mv = m;
+  }
+  mx[2 * (int64_t)b] = mu;
+  mx[2 * (int64_t)b + 1] = mv;
+}
+
+hipError_t closure_step_hook(lps_ctx *c, const StepState &st, hipStream_t s) {
+  ClosureAttach *A = c->clo;
+  const int par = (int)(c->step & 1);
+  if (A->n_hist > 0) {
+    hipLaunchKernelGGL(k_hist_shift, dim3((A->n_hist + 255) / 256), dim3(256), 0, s, A->n_hist,
+                       A->hist_slot, st.ux, st.uy, st.vx, st.vy, A->hist[par ^ 1], A->hist[par],
+                       A->mx[par]);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  if (!A->clo || A->n_rows == 0) return hipSuccess;
+  const ApplyDst d{A->row_slot, A->row_inst, A->gain, st.ux1, st.uy1, st.vx1, st.vy1, A->dbg};
+  return mlp_forward_apply(A->clo->dev(), A->src(par), A->n_rows, d, s);
+}
+
+hipError_t closure_reset_hook(lps_ctx *c) {
+  ClosureAttach *A = c->clo;
+  const size_t hb = sizeof(float) * std::max<int64_t>(A->n_hist, 1);
+  for (int q = 0; q < 2; q++) {
+    hipError_t e = hipMemsetAsync(A->hist[q], 0, hb * kClRec, c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(A->mx[q], 0, hb * 2, c->stream);
+    if (e != hipSuccess) return e;
+  }
+  if (A->dbg) return hipMemsetAsync(A->dbg, 0, sizeof(float) * 8 * std::max(A->n_rows, 1), c->stream);
+  return hipSuccess;
+}
+
+void closure_free(lps_ctx *c) {
+  delete c->clo;
+  c->clo = nullptr;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Row tables
+// ---------------------------------------------------------------------------------------------
+
+// The 49 lattice offsets |o|^2 <= 16, x-major.
+static std::vector<std::pair<int, int>> canon_offsets() {
+  std::vector<std::pair<int, int>> o;
+  for (int x = -4; x <= 4; x++)
+    for (int y = -4; y <= 4; y++)
+      if (x * x + y * y <= 16) o.push_back({x, y});
+  return o;
+}
+
+// Frame code of a row whose outward normal is (nx, ny): the canonical normal is
+// (max(|nx|, |ny|), min(|nx|, |ny|)).
+static int frame_code(double nx, double ny) {
+  const double ax = std::fabs(nx), ay = std::fabs(ny);
+  if (ax >= ay) return (nx < 0 ? 2 : 0) | (ny < 0 ? 4 : 0);
+  return 1 | (ny < 0 ? 2 : 0) | (nx < 0 ? 4 : 0);
+}
+
+struct Grid {  // lattice position -> slot over a bounding box
+  int32_t x0 = 0, y0 = 0, nx = 0, ny = 0;
+  std::vector<int32_t> v;
+  void init(const std::vector<int32_t> &ix, const std::vector<int32_t> &iy) {
+    if (ix.empty()) return;
+    int32_t x1 = ix[0], y1 = iy[0];
+    x0 = ix[0];
+    y0 = iy[0];
+    for (size_t k = 0; k < ix.size(); k++) {
+      x0 = std::min(x0, ix[k]); x1 = std::max(x1, ix[k]);
+      y0 = std::min(y0, iy[k]); y1 = std::max(y1, iy[k]);
+    }
+    nx = x1 - x0 + 1;
+    ny = y1 - y0 + 1;
+    v.assign((size_t)nx * ny, -1);
+  }
+  void set(int32_t ix, int32_t iy, int32_t s) { v[(size_t)(ix - x0) * ny + (iy - y0)] = s; }
+  int32_t get(int32_t ix, int32_t iy) const {
+    ix -= x0;
+    iy -= y0;
+    if (ix < 0 || iy < 0 || ix >= nx || iy >= ny) return -1;
+    return v[(size_t)ix * ny + iy];
+  }
+};
+
+struct RowIn {
+  int32_t ix, iy;
+  double x, y, nu, R;
+};
+
+struct Tables {
+  std::vector<int32_t> hist_slot, hidx;
+  std::vector<int8_t> xf;
+  std::vector<float> feat;
+};
+
+// present(ix, iy) -> bool, slot_of(ix, iy) -> slot of the context that keeps the history.
+template <class Present, class SlotOf>
+static bool build_rows(const std::vector<RowIn> &rows, double delta, Present present,
+                       SlotOf slot_of, Tables &t) {
+  static const std::vector<std::pair<int, int>> off = canon_offsets();
+  std::unordered_map<int32_t, int32_t> hmap;
+  const size_t n = rows.size();
+  t.hidx.assign(n * kClOffsets, -1);
+  t.xf.assign(n, 0);
+  t.feat.assign(n * kClFeat, 0.0f);
+  for (size_t r = 0; r < n; r++) {
+    const RowIn &q = rows[r];
+    const double rr = std::sqrt(q.x * q.x + q.y * q.y);
+    const double nx = rr > 0 ? q.x / rr : 1.0, ny = rr > 0 ? q.y / rr : 0.0;
+    const int f = frame_code(nx, ny);
+    t.xf[r] = (int8_t)f;
+    t.feat[r * kClFeat + 0] = (float)((q.nu - 0.3) / 0.15);
+    t.feat[r * kClFeat + 1] = (float)std::max(std::fabs(nx), std::fabs(ny));
+    t.feat[r * kClFeat + 2] = (float)std::min(std::fabs(nx), std::fabs(ny));
+    t.feat[r * kClFeat + 3] = (float)((q.R - rr) / delta);
+    for (int j = 0; j < kClOffsets; j++) {
+      double px, py;
+      from_frame(f, (double)off[j].first, (double)off[j].second, px, py);
+      const int32_t jx = q.ix + (int32_t)std::lround(px), jy = q.iy + (int32_t)std::lround(py);
+      if (!present(jx, jy)) continue;
+      const int32_t s = slot_of(jx, jy);
+      if (s < 0) return false;
+      auto it = hmap.find(s);
+      int32_t h;
+      if (it == hmap.end()) {
+        h = (int32_t)t.hist_slot.size();
+        hmap.emplace(s, h);
+        t.hist_slot.push_back(s);
+      } else {
+        h = it->second;
+      }
+      t.hidx[r * kClOffsets + j] = h;
+    }
+  }
+  return true;
+}
+
+template <class T>
+static int32_t download(lps_ctx *c, std::vector<T> &h, const T *d, size_t n) {
+  h.resize(n);
+  if (n == 0) return LPS_OK;
+  CTX_TRY(c, hipMemcpyAsync(h.data(), d, sizeof(T) * n, hipMemcpyDeviceToHost, c->stream));
+  CTX_TRY(c, hipStreamSynchronize(c->stream));
+  return LPS_OK;
+}
+
+// Device side of a set of tables; clo may be null (history only).
+static int32_t make_attach(lps_ctx *c, const Tables &t, lps_closure *clo,
+                           const std::vector<int32_t> &row_slot,
+                           const std::vector<int32_t> &row_inst, int32_t n_inst,
+                           ClosureAttach **out) {
+  ClosureAttach *A = new ClosureAttach();
+  A->clo = clo;
+  A->n_rows = (int32_t)row_slot.size();
+  A->n_hist = (int32_t)t.hist_slot.size();
+  A->n_inst = n_inst;
+  const hipStream_t s = c->stream;
+  hipError_t e = A->upload(&A->hist_slot, t.hist_slot, s);
+  if (e == hipSuccess) e = A->upload(&A->hidx, t.hidx, s);
+  if (e == hipSuccess) e = A->upload(&A->xf, t.xf, s);
+  if (e == hipSuccess) e = A->upload(&A->feat, t.feat, s);
+  if (e == hipSuccess) e = A->upload(&A->row_slot, row_slot, s);
+  if (e == hipSuccess) e = A->upload(&A->row_inst, row_inst, s);
+  if (e == hipSuccess) e = A->upload(&A->gain, std::vector<float>(std::max(n_inst, 1), 1.0f), s);
+  for (int q = 0; q < 2 && e == hipSuccess; q++) {
+    e = A->alloc(&A->hist[q], (size_t)std::max(A->n_hist, 1) * kClRec);
+    if (e == hipSuccess) e = A->alloc(&A->mx[q], (size_t)std::max(A->n_hist, 1) * 2);
+  }
+  if (e == hipSuccess) e = A->alloc(&A->dbg, (size_t)std::max(A->n_rows, 1) * 8);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    delete A;
+    return c->hip_fail(e, "closure tables");
+  }
+  *out = A;
+  return LPS_OK;
+}
+
+// Lattice index of a coordinate on an origin-centred lattice.
+static inline int32_t lat(double x, double d) { return (int32_t)std::lround(x / d); }
+
+// ---------------------------------------------------------------------------------------------
+// Teacher-forced samples
+// ---------------------------------------------------------------------------------------------
+struct CollectArgs {
+  GatherSrc g;
+  int32_t n_rows, M;
+  const int32_t *row_slot, *cloud;  // truth slots; cloud n_rows x M (-1 skip)
+  const double *gu, *gv;            // n_rows x 8M (trunc's G_u, G_v rows)
+  const double *ux1, *uy1, *vx1, *vy1;                  // U^{n+1}, V^{n+1}
+  const double *ux, *uy, *uxm, *uym, *vx, *vy, *vxm, *vym;  // U^n, U^{n-1}, V^n, V^{n-1}
+  uint64_t seed, keep_thr;
+  int64_t step;  // n + 1
+  float amp_min;
+  double inv_amp_ref2;
+  __bf16 *dsx;
+  float *dsaux;
+  unsigned long long *count;
+  int64_t cap;
+};
+
+__device__ __forceinline__ double wsum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+__global__ void __launch_bounds__(256) k_collect(CollectArgs a) {
+  const int lane = threadIdx.x & 63;
+  const int32_t r = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (r >= a.n_rows) return;
+  const uint64_t h = dev_mix(a.seed ^ dev_mix(((uint64_t)a.step << 32) ^ (uint64_t)r));
+  if (h > a.keep_thr) return;
+  float su, sv;
+  row_scale(a.g, r, su, sv);
+  if (!(su >= a.amp_min) || su < 1e-30f) return;
+  // least-squares update on the truth state, as the solver's G_u / G_v rows
+  const double *gu0 = a.gu + (int64_t)r * 8 * a.M, *gu1 = gu0 + 4 * a.M;
+  const double *gv0 = a.gv + (int64_t)r * 8 * a.M, *gv1 = gv0 + 4 * a.M;
+  double Ux = 0.0, Uy = 0.0, Vx = 0.0, Vy = 0.0;
+  for (int base = 0; base < a.M; base += 64) {
+    const int jj = base + lane;
+    const int32_t k = jj < a.M ? a.cloud[(int64_t)r * a.M + jj] : -1;
+    double t0 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0;
+    if (k >= 0) {
+      const double ux = a.ux[k], uy = a.uy[k], uxm = a.uxm[k], uym = a.uym[k];
+      const double vx = a.vx[k], vy = a.vy[k], vxm = a.vxm[k], vym = a.vym[k];
+      t0 = gu0[4 * jj] * ux + gu0[4 * jj + 1] * uy + gu0[4 * jj + 2] * uxm + gu0[4 * jj + 3] * uym;
+      t1 = gu1[4 * jj] * ux + gu1[4 * jj + 1] * uy + gu1[4 * jj + 2] * uxm + gu1[4 * jj + 3] * uym;
+      t2 = gv0[4 * jj] * vx + gv0[4 * jj + 1] * vy + gv0[4 * jj + 2] * vxm + gv0[4 * jj + 3] * vym;
+      t3 = gv1[4 * jj] * vx + gv1[4 * jj + 1] * vy + gv1[4 * jj + 2] * vxm + gv1[4 * jj + 3] * vym;
+    }
+    Ux += wsum(t0);
+    Uy += wsum(t1);
+    Vx += wsum(t2);
+    Vy += wsum(t3);
+  }
+  unsigned long long idx = 0;
+  if (lane == 0) idx = atomicAdd(a.count, 1ull);
+  idx = ((unsigned long long)__shfl((unsigned)(idx >> 32), 0) << 32) | (unsigned)__shfl((unsigned)idx, 0);
+  if ((int64_t)idx >= a.cap) return;
+  const float iu = safe_inv(su), iv = safe_inv(sv);
+  for (int q = lane; q < kClIn / 8; q += 64)
+    *(bf16x8 *)(a.dsx + (int64_t)idx * kClIn + 8 * q) = gather8(a.g, r, 8 * q, iu, iv);
+  if (lane == 0) {
+    const int32_t i = a.row_slot[r];
+    const double dux = a.ux1[i] - Ux, duy = a.uy1[i] - Uy;
+    const double dvx = a.vx1[i] - Vx, dvy = a.vy1[i] - Vy;
+    const int f = a.g.xf[r];
+    // canonical components (to_frame in FP64)
+    const double pu = (f & 1) ? duy : dux, qu = (f & 1) ? dux : duy;
+    const double pv = (f & 1) ? dvy : dvx, qv = (f & 1) ? dvx : dvy;
+    float *o = a.dsaux + (int64_t)idx * 8;
+    o[0] = (float)(((f & 2) ? -pu : pu) * iu);
+    o[1] = (float)(((f & 4) ? -qu : qu) * iu);
+    o[2] = (float)(((f & 2) ? -pv : pv) * iv);
+    o[3] = (float)(((f & 4) ? -qv : qv) * iv);
+    o[4] = (float)((double)su * (double)su * a.inv_amp_ref2);
+    o[5] = su;
+    o[6] = sv;
+    o[7] = (float)r;
+  }
+}
+
+}  // namespace lps
+
+using namespace lps;
+
+extern "C" {
+
+int32_t lps_closure_attach(lps_ctx *c, lps_closure *clo) {
+  if (!clo) return c->fail(LPS_ERR_ARG, "closure is NULL");
+  if (!c->fast) return c->fail(LPS_ERR_ARG, "the learned closure needs the fast (tile) path");
+  if (clo->device != c->opt.device) return c->fail(LPS_ERR_ARG, "closure and context on different devices");
+  CTX_TRY(c, hipSetDevice(c->opt.device));
+  const int32_t B = c->n_inst;
+  std::vector<double> xs, ys;
+  std::vector<int8_t> kind;
+  std::vector<int32_t> aslot;
+  int32_t rc;
+  if ((rc = download(c, xs, c->x, c->n_slots)) || (rc = download(c, ys, c->y, c->n_slots)) ||
+      (rc = download(c, kind, c->kind, c->n_slots)) || (rc = download(c, aslot, c->abc_slot, c->n_abc)))
+    return rc;
+  std::vector<RowIn> rows;
+  std::vector<int32_t> row_slot, row_inst;
+  std::vector<Grid> grids(B);
+  int32_t a0 = 0;
+  for (int32_t i = 0; i < B; i++) {
+    const int64_t s0 = c->batch ? c->slot_off[i] : 0, s1 = c->batch ? c->slot_off[i + 1] : c->n_slots;
+    const int32_t na = c->batch ? c->abc_n[i] : c->n_abc;
+    const InstParams &P = c->inst_h[i];
+    const double nu = P.lam / (2.0 * (P.lam + P.mu));
+    std::vector<int32_t> ix, iy;
+    double R = 0.0;
+    for (int64_t k = s0; k < s1; k++)
+      if (kind[k]) {
+        ix.push_back(lat(xs[k], c->dx));
+        iy.push_back(lat(ys[k], c->dy));
+        R = std::max(R, std::sqrt(xs[k] * xs[k] + ys[k] * ys[k]));
+      }
+    Grid &G = grids[i];
+    G.init(ix, iy);
+    for (int64_t k = s0; k < s1; k++)
+      if (kind[k]) G.set(lat(xs[k], c->dx), lat(ys[k], c->dy), (int32_t)k);
+    for (int32_t r = a0; r < a0 + na; r++) {
+      const int32_t k = aslot[r];
+      rows.push_back(RowIn{lat(xs[k], c->dx), lat(ys[k], c->dy), xs[k], ys[k], nu, R});
+      row_slot.push_back(k);
+      row_inst.push_back(i);
+    }
+    a0 += na;
+  }
+  // rows of instance i only see instance i's lattice
+  Tables t;
+  size_t cur = 0;
+  int32_t cur_inst = 0;
+  auto present = [&](int32_t x, int32_t y) { return grids[cur_inst].get(x, y) >= 0; };
+  auto slot_of = [&](int32_t x, int32_t y) { return grids[cur_inst].get(x, y); };
+  // build instance by instance, keeping one history index space
+  {
+    Tables all;
+    std::unordered_map<int32_t, int32_t> hglob;
+    for (int32_t i = 0; i < B; i++) {
+      std::vector<RowIn> sub;
+      for (size_t r = cur; r < rows.size() && row_inst[r] == i; r++) sub.push_back(rows[r]);
+      cur_inst = i;
+      Tables ti;
+      if (!build_rows(sub, c->delta, present, slot_of, ti))
+        return c->fail(LPS_ERR_ARG, "closure tables: a lattice position without a slot");
+      for (size_t r = 0; r < sub.size(); r++) {
+        for (int j = 0; j < kClOffsets; j++) {
+          const int32_t h = ti.hidx[r * kClOffsets + j];
+          int32_t g = -1;
+          if (h >= 0) {
+            const int32_t s = ti.hist_slot[h];
+            auto it = hglob.find(s);
+            if (it == hglob.end()) {
+              g = (int32_t)all.hist_slot.size();
+              hglob.emplace(s, g);
+              all.hist_slot.push_back(s);
+            } else {
+              g = it->second;
+            }
+          }
+          all.hidx.push_back(g);
+        }
+        all.xf.push_back(ti.xf[r]);
+        for (int e = 0; e < kClFeat; e++) all.feat.push_back(ti.feat[r * kClFeat + e]);
+      }
+      cur += sub.size();
+    }
+    t = std::move(all);
+  }
+  ClosureAttach *A = nullptr;
+  if ((rc = make_attach(c, t, clo, row_slot, row_inst, B, &A))) return rc;
+  closure_free(c);
+  c->clo = A;
+  ctx_drop_graphs(c);
+  if (closure_reset_hook(c) != hipSuccess) return c->fail(LPS_ERR_HIP, "clearing the closure history");
+  CTX_TRY(c, hipStreamSynchronize(c->stream));
+  return LPS_OK;
+}
+
+int32_t lps_closure_detach(lps_ctx *c) {
+  CTX_TRY(c, hipStreamSynchronize(c->stream));
+  CTX_TRY(c, hipStreamSynchronize(c->stream2));
+  closure_free(c);
+  ctx_drop_graphs(c);
+  return LPS_OK;
+}
+
+int32_t lps_closure_set_gain(lps_ctx *c, const float *gain) {
+  if (!c->clo) return c->fail(LPS_ERR_ARG, "no closure attached");
+  CTX_TRY(c, hipMemcpyAsync(c->clo->gain, gain, sizeof(float) * c->clo->n_inst, hipMemcpyHostToDevice, c->stream));
+  CTX_TRY(c, hipStreamSynchronize(c->stream));
+  return LPS_OK;
+}
+
+int32_t lps_closure_inspect(lps_ctx *c, uint16_t *x, float *scale2, float *applied8) {
+  ClosureAttach *A = c->clo;
+  if (!A) return c->fail(LPS_ERR_ARG, "no closure attached");
+  if (c->step < 1) return c->fail(LPS_ERR_ARG, "no step completed since the attachment");
+  CTX_TRY(c, hipStreamSynchronize(c->stream));
+  CTX_TRY(c, hipStreamSynchronize(c->stream2));
+  const int par = (int)((c->step - 1) & 1);
+  const int32_t n = A->n_rows;
+  if (n == 0) return LPS_OK;
+  __bf16 *dx = nullptr;
+  float *ds = nullptr;
+  hipError_t e = hipMalloc((void **)&dx, sizeof(__bf16) * n * kClIn);
+  if (e == hipSuccess) e = hipMalloc((void **)&ds, sizeof(float) * n * 2);
+  if (e == hipSuccess) e = gather_rows(A->src(par), n, dx, ds, c->stream);
+  if (e == hipSuccess && x)
+    e = hipMemcpyAsync(x, dx, sizeof(__bf16) * n * kClIn, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess && scale2)
+    e = hipMemcpyAsync(scale2, ds, sizeof(float) * n * 2, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess && applied8)
+    e = hipMemcpyAsync(applied8, A->dbg, sizeof(float) * n * 8, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(dx);
+  (void)hipFree(ds);
+  if (e != hipSuccess) return c->hip_fail(e, "lps_closure_inspect");
+  return LPS_OK;
+}
+
+int32_t lps_closure_collect(lps_closure *clo, lps_ctx *c, const lps_problem *p, int32_t nsteps,
+                            int32_t first_step, int32_t sample_every, double keep, double amp_ref,
+                            double amp_min, uint64_t seed) {
+  if (!clo || !p) return c->fail(LPS_ERR_ARG, "closure and truncated problem are required");
+  if (!c->fast || c->batch) return c->fail(LPS_ERR_ARG, "the truth run needs a single fast-path context");
+  if (clo->device != c->opt.device) return c->fail(LPS_ERR_ARG, "closure and context on different devices");
+  if (clo->ds_cap <= 0) return c->fail(LPS_ERR_ARG, "reserve the sample store first");
+  if (nsteps < 0 || sample_every <= 0 || !(amp_ref > 0.0)) return c->fail(LPS_ERR_ARG, "bad sampling arguments");
+  const int32_t N = p->node_no, C = p->constrains_no, M = p->max_cloud_size;
+  if (N <= 0 || C <= 0 || M <= 0 || M > 64 || !p->nodes || !p->inf_nodes || !p->iclouds ||
+      !p->ci_nodes_u || !p->ci_nodes_v)
+    return c->fail(LPS_ERR_ARG, "the truncated problem needs nodes and ABC rows (cloud <= 64)");
+  if (p->dx != c->dx || p->dy != c->dy) return c->fail(LPS_ERR_ARG, "truncated and truth lattices differ");
+  CTX_TRY(c, hipSetDevice(c->opt.device));
+  // truth lattice
+  std::vector<double> xs, ys;
+  std::vector<int8_t> kind;
+  int32_t rc;
+  if ((rc = download(c, xs, c->x, c->n_slots)) || (rc = download(c, ys, c->y, c->n_slots)) ||
+      (rc = download(c, kind, c->kind, c->n_slots)))
+    return rc;
+  std::vector<int32_t> ix, iy;
+  for (int32_t k = 0; k < c->n_slots; k++)
+    if (kind[k]) { ix.push_back(lat(xs[k], c->dx)); iy.push_back(lat(ys[k], c->dy)); }
+  Grid truth;
+  truth.init(ix, iy);
+  for (int32_t k = 0; k < c->n_slots; k++)
+    if (kind[k]) truth.set(lat(xs[k], c->dx), lat(ys[k], c->dy), k);
+  // truncated lattice
+  std::vector<int32_t> tx(N), ty(N);
+  double R = 0.0;
+  for (int32_t i = 0; i < N; i++) {
+    const double x = p->nodes[2 * (int64_t)i], y = p->nodes[2 * (int64_t)i + 1];
+    tx[i] = lat(x, p->dx);
+    ty[i] = lat(y, p->dy);
+    R = std::max(R, std::sqrt(x * x + y * y));
+  }
+  Grid trunc;
+  trunc.init(tx, ty);
+  for (int32_t i = 0; i < N; i++) trunc.set(tx[i], ty[i], i);
+  std::vector<RowIn> rows;
+  std::vector<int32_t> row_slot, cloud((size_t)C * M, -1);
+  for (int32_t a = 0; a < C; a++) {
+    const int32_t node = p->inf_nodes[a];
+    if (node < 0 || node >= N) return c->fail(LPS_ERR_ARG, "infNodesIndex entry out of range");
+    const double x = p->nodes[2 * (int64_t)node], y = p->nodes[2 * (int64_t)node + 1];
+    rows.push_back(RowIn{tx[node], ty[node], x, y, p->nu, R});
+    const int32_t s = truth.get(tx[node], ty[node]);
+    if (s < 0) return c->fail(LPS_ERR_ARG, "an ABC node of the truncated problem is not in the truth domain");
+    row_slot.push_back(s);
+    for (int32_t j = 0; j < M; j++) {
+      const int32_t k = p->iclouds[(int64_t)a * M + j];
+      if (k < 0 || k >= N) continue;
+      const int32_t sk = truth.get(tx[k], ty[k]);
+      if (sk < 0) return c->fail(LPS_ERR_ARG, "a cloud node of the truncated problem is not in the truth domain");
+      cloud[(size_t)a * M + j] = sk;
+    }
+  }
+  Tables t;
+  auto present = [&](int32_t x, int32_t y) { return trunc.get(x, y) >= 0; };
+  auto slot_of = [&](int32_t x, int32_t y) { return truth.get(x, y); };
+  if (!build_rows(rows, c->delta, present, slot_of, t))
+    return c->fail(LPS_ERR_ARG, "a node of the truncated problem is not in the truth domain");
+  ClosureAttach *A = nullptr;
+  if ((rc = make_attach(c, t, nullptr, row_slot, std::vector<int32_t>(C, 0), 1, &A))) return rc;
+  int32_t *dcloud = nullptr;
+  double *dgu = nullptr, *dgv = nullptr;
+  const size_t gl = (size_t)C * 8 * M;
+  hipError_t e = A->upload(&dcloud, cloud, c->stream);
+  if (e == hipSuccess) e = A->alloc(&dgu, gl);
+  if (e == hipSuccess) e = A->alloc(&dgv, gl);
+  if (e == hipSuccess) e = hipMemcpyAsync(dgu, p->ci_nodes_u, sizeof(double) * gl, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(dgv, p->ci_nodes_v, sizeof(double) * gl, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) {
+    delete A;
+    return c->hip_fail(e, "closure collection tables");
+  }
+  // the history rides on the truth run's steps
+  ClosureAttach *saved = c->clo;
+  const int32_t saved_graph = c->opt.use_graph;
+  c->clo = A;
+  c->opt.use_graph = 0;
+  ctx_drop_graphs(c);
+  rc = closure_reset_hook(c) == hipSuccess ? LPS_OK : c->fail(LPS_ERR_HIP, "clearing the history");
+  CollectArgs ca{};
+  ca.n_rows = C;
+  ca.M = M;
+  ca.row_slot = A->row_slot;
+  ca.cloud = dcloud;
+  ca.gu = dgu;
+  ca.gv = dgv;
+  ca.seed = seed;
+  ca.keep_thr = keep >= 1.0 ? ~0ull : (uint64_t)(std::max(keep, 0.0) * 18446744073709551615.0);
+  ca.amp_min = (float)amp_min;
+  ca.inv_amp_ref2 = 1.0 / (amp_ref * amp_ref);
+  ca.dsx = clo->ds_x;
+  ca.dsaux = clo->ds_aux;
+  ca.count = clo->ds_count;
+  ca.cap = clo->ds_cap;
+  for (int32_t it = 0; it < nsteps && rc == LPS_OK; it++) {
+    const int64_t n = c->step;  // the step about to run
+    rc = lps_step(c, 1);
+    if (rc != LPS_OK) break;
+    const int64_t s1 = n + 1;
+    if (s1 < first_step || s1 % sample_every != 0) continue;
+    const int par = (int)(n & 1);
+    ca.g = A->src(par);
+    ca.step = s1;
+    const lps_ctx::Dev2 &U1 = c->U[s1 % 3], &U0 = c->U[n % 3], &Um = c->U[(n + 2) % 3];
+    const lps_ctx::Dev2 &V1 = c->V[s1 % 3], &V0 = c->V[n % 3], &Vm = c->V[(n + 2) % 3];
+    ca.ux1 = U1.x; ca.uy1 = U1.y; ca.vx1 = V1.x; ca.vy1 = V1.y;
+    ca.ux = U0.x; ca.uy = U0.y; ca.uxm = Um.x; ca.uym = Um.y;
+    ca.vx = V0.x; ca.vy = V0.y; ca.vxm = Vm.x; ca.vym = Vm.y;
+    hipLaunchKernelGGL(k_collect, dim3((C + 3) / 4), dim3(256), 0, c->stream, ca);
+    e = hipGetLastError();
+    if (e != hipSuccess) rc = c->hip_fail(e, "k_collect");
+  }
+  e = hipStreamSynchronize(c->stream);
+  if (rc == LPS_OK && e != hipSuccess) rc = c->hip_fail(e, "lps_closure_collect");
+  c->clo = saved;
+  c->opt.use_graph = saved_graph;
+  ctx_drop_graphs(c);
+  delete A;
+  return rc;
+}
+
+}  // extern "C"

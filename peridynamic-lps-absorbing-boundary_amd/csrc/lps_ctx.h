@@ -100,6 +100,17 @@ struct lps_ctx {
   StepState state(int64_t s) const;
 };
 
+namespace lps {
+// Learned ABC closure hooks (lps_closure.hip), active when ctx->clo is set. step: after the ABC
+// rows' G_u / G_v on their stream (at the end of the step when the context has no ABC rows):
+// records U^n, V^n into the history and adds the correction to U^{n+1}, V^{n+1} at the ABC nodes.
+hipError_t closure_step_hook(lps_ctx *c, const StepState &st, hipStream_t s);
+hipError_t closure_reset_hook(lps_ctx *c);  // clears the history (lps_reset)
+void closure_free(lps_ctx *c);
+// Drops the captured 30-step graphs (they hold the kernels they were captured with).
+void ctx_drop_graphs(lps_ctx *c);
+}  // namespace lps
+
 #define CTX_TRY(ctx, call)                                   \
   do {                                                       \
     hipError_t e_ = (call);                                  \

@@ -83,6 +83,34 @@ SIGNATURES = {
     "lps_last_error": (C.c_char_p, [C.c_void_p]),
     "lps_global_error": (C.c_char_p, []),
     "lps_destroy": (None, [C.c_void_p]),
+    # learned ABC closure
+    "lps_closure_param_count": (C.c_int64, []),
+    "lps_closure_create": (C.c_int32, [C.c_int32, C.c_uint64, C.POINTER(C.c_void_p)]),
+    "lps_closure_destroy": (None, [C.c_void_p]),
+    "lps_closure_error": (C.c_char_p, [C.c_void_p]),
+    "lps_closure_get_params": (C.c_int32, [C.c_void_p, C.c_void_p]),
+    "lps_closure_set_params": (C.c_int32, [C.c_void_p, C.c_void_p]),
+    "lps_closure_set_scales": (C.c_int32, [C.c_void_p, C.c_float, C.c_float]),
+    "lps_closure_get_scales": (C.c_int32, [C.c_void_p, C.c_void_p]),
+    "lps_closure_forward": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
+    "lps_closure_train_batch": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                            C.c_int32, C.c_float, C.c_void_p, C.c_void_p]),
+    "lps_closure_dataset_reserve": (C.c_int32, [C.c_void_p, C.c_int64]),
+    "lps_closure_dataset_count": (C.c_int64, [C.c_void_p]),
+    "lps_closure_dataset_clear": (C.c_int32, [C.c_void_p]),
+    "lps_closure_dataset_append": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]),
+    "lps_closure_dataset_read": (C.c_int32, [C.c_void_p, C.c_int64, C.c_int64, C.c_void_p,
+                                             C.c_void_p]),
+    "lps_closure_train": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, C.c_float, C.c_float,
+                                      C.c_int64, C.c_int64, C.c_uint64, C.c_void_p]),
+    "lps_closure_evaluate": (C.c_int32, [C.c_void_p, C.c_int64, C.c_int64, C.c_void_p]),
+    "lps_closure_collect": (C.c_int32, [C.c_void_p, C.c_void_p, C.POINTER(LpsProblem), C.c_int32,
+                                        C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double,
+                                        C.c_uint64]),
+    "lps_closure_attach": (C.c_int32, [C.c_void_p, C.c_void_p]),
+    "lps_closure_detach": (C.c_int32, [C.c_void_p]),
+    "lps_closure_set_gain": (C.c_int32, [C.c_void_p, C.c_void_p]),
+    "lps_closure_inspect": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
 }
 
 _lib = None
@@ -250,6 +278,29 @@ class Solver:
         self._check(self._L.lps_pass_times(self._h, _ptr(out)))
         return {"theta_ms": float(out[0]), "force_ms": float(out[1]), "abc_ms": float(out[2]),
                 "step_ms": float(out[3])}
+
+    # learned ABC closure (lps_amd.closure.Closure)
+    def attach_closure(self, clo):
+        """Apply the learned closure clo at every ABC row from the next step on."""
+        self._check(self._L.lps_closure_attach(self._h, clo._h))
+        self._closure = clo  # kept alive while attached
+
+    def detach_closure(self):
+        self._check(self._L.lps_closure_detach(self._h))
+        self._closure = None
+
+    def set_closure_gain(self, gain):
+        g = np.ascontiguousarray(gain, dtype=np.float32)
+        self._check(self._L.lps_closure_set_gain(self._h, _ptr(g)))
+
+    def closure_inspect(self, n_rows: int):
+        """Inputs (n_rows x 1600 BF16 bits), scales (n_rows x 2) and applied outputs (n_rows x 8)
+        of the last completed step."""
+        x = np.zeros((n_rows, 1600), dtype=np.uint16)
+        sc = np.zeros((n_rows, 2), dtype=np.float32)
+        ap = np.zeros((n_rows, 8), dtype=np.float32)
+        self._check(self._L.lps_closure_inspect(self._h, _ptr(x), _ptr(sc), _ptr(ap)))
+        return x, sc, ap
 
     def tile_info(self):
         """Tile-path layout: tiles, full tiles, partial tiles, nodes in full tiles, nodes (zeros on

@@ -1,0 +1,370 @@
+"""The learned ABC closure through the C ABI: the matrix-core network against a plain PyTorch FP32
+reference (forward, gradients, Adam), training on a known target, and its coupling to the time
+loop (a zero correction gives back the least-squares ABC bit for bit, the gathered inputs equal a
+numpy construction from the fetched fields, the applied correction is the network's output, and
+teacher-forced samples equal truth minus the least-squares update)."""
+import sys
+
+import numpy as np
+import pytest
+from conftest import load_golden, rel_l2
+
+from lps_amd import closure as cl, preprocess as pp, solver
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def bf16r(a):
+    return cl.bf16_to_f32(cl.bf16_bits(a))
+
+
+def random_params(seed, out_scale=0.05):
+    rng = np.random.default_rng(seed)
+    d = {}
+    for name, shp in cl.SHAPES:
+        fan_in = shp[1] if len(shp) == 2 else 1
+        if name.startswith("W"):
+            sd = np.sqrt(2.0 / fan_in) if name != "W4" else out_scale
+            d[name] = rng.normal(0.0, sd, shp)
+        else:
+            d[name] = rng.normal(0.0, 0.05, shp)
+    return cl.join_params(d)
+
+
+def torch_forward(params, x, with_grad=False):
+    """FP32 reference of the network with the kernel's roundings: BF16 weights W1..W3, BF16
+    activations (rounded after the ReLU, straight through for the gradient), FP32 output layer."""
+    import torch
+    P = cl.split_params(params)
+    T = {}
+    for k, v in P.items():
+        v = bf16r(v) if k in ("W1", "W2", "W3") else v
+        T[k] = torch.tensor(np.asarray(v, dtype=np.float32), requires_grad=with_grad)
+
+    def rnd(h):
+        return h + (torch.tensor(bf16r(h.detach().numpy())) - h).detach()
+
+    h = torch.tensor(x, dtype=torch.float32)
+    for i in (1, 2, 3):
+        h = rnd(torch.relu(h @ T[f"W{i}"].T + T[f"b{i}"]))
+    y = h @ T["W4"].T + T["b4"]
+    return y, T
+
+
+def test_forward_matches_torch_fp32():
+    say("closure forward vs torch")
+    n = 200  # not a multiple of the 32-row tile
+    rng = np.random.default_rng(1)
+    xb = cl.bf16_bits(rng.normal(0.0, 1.0, (n, cl.N_IN)).astype(np.float32))
+    p = random_params(2)
+    with cl.Closure(seed=3) as c:
+        c.set_params(p)
+        y = c.forward(xb)
+    yr, _ = torch_forward(p, cl.bf16_to_f32(xb))
+    err = rel_l2(y, yr.detach().numpy())
+    say("  rel L2", err)
+    assert err <= 2e-2, err
+
+
+def test_gradients_match_torch_autograd():
+    say("closure gradients vs torch autograd")
+    n = 256
+    rng = np.random.default_rng(4)
+    xb = cl.bf16_bits(rng.normal(0.0, 1.0, (n, cl.N_IN)).astype(np.float32))
+    t = rng.normal(0.0, 1.0, (n, 4)).astype(np.float32)
+    w = rng.uniform(0.5, 1.5, n).astype(np.float32)
+    p = random_params(5, out_scale=0.2)
+    with cl.Closure(seed=6) as c:
+        c.set_params(p)
+        c.set_scales(2.0, 0.5)
+        loss, g = c.train_batch(xb, t, w, lr=0.0, want_grads=True)
+        assert np.array_equal(c.get_params(), p)  # lr = 0 leaves the weights
+    import torch
+    y, T = torch_forward(p, cl.bf16_to_f32(xb), with_grad=True)
+    tt = torch.tensor(t) / torch.tensor([2.0, 2.0, 0.5, 0.5])
+    L = (torch.tensor(w)[:, None] * (y - tt) ** 2).sum() / n
+    L.backward()
+    Lf = float(L.detach())
+    assert abs(loss - Lf) <= 1e-2 * Lf, (loss, Lf)
+    G = cl.split_params(g)
+    for k in T:
+        err = rel_l2(G[k], T[k].grad.numpy())
+        say("  grad", k, err)
+        assert err <= 5e-2, (k, err)
+
+
+def test_adam_first_step():
+    """Adam's first step with bias correction is lr g / (|g| + eps)."""
+    n = 128
+    rng = np.random.default_rng(7)
+    xb = cl.bf16_bits(rng.normal(0.0, 1.0, (n, cl.N_IN)).astype(np.float32))
+    t = rng.normal(0.0, 1.0, (n, 4)).astype(np.float32)
+    w = np.ones(n, np.float32)
+    p = random_params(8, out_scale=0.2)
+    lr = 1e-3
+    with cl.Closure(seed=9) as c:
+        c.set_params(p)
+        _, g = c.train_batch(xb, t, w, lr=lr, want_grads=True)
+        p1 = c.get_params()
+    exp = p - lr * g / (np.abs(g) + 1e-8)
+    assert np.allclose(p1, exp, rtol=1e-5, atol=1e-8), np.max(np.abs(p1 - exp))
+
+
+def test_training_fits_a_known_map():
+    """Training on samples of a fixed linear map of the inputs drives the validation loss far below
+    the loss of a zero output."""
+    say("closure training")
+    rng = np.random.default_rng(10)
+    n = 8192
+    x = rng.normal(0.0, 1.0, (n, cl.N_IN)).astype(np.float32)
+    xb = cl.bf16_bits(x)
+    A = rng.normal(0.0, 1.0 / np.sqrt(cl.N_IN), (cl.N_IN, 4)).astype(np.float32)
+    aux = np.zeros((n, 8), np.float32)
+    aux[:, :4] = 0.5 * cl.bf16_to_f32(xb) @ A
+    aux[:, 4] = 1.0
+    with cl.Closure(seed=11) as c:
+        c.reserve(n)
+        c.append(xb, aux)
+        assert c.count() == n
+        xr, ar = c.read(100, 5)
+        assert np.array_equal(xr, xb[100:105]) and np.array_equal(ar, aux[100:105])
+        curve = c.train(600, 512, 2e-3, 1e-4, 0, 7168, seed=1)
+        loss, base, wsum = c.evaluate(7168, 1024)
+    say("  first/last batch loss", curve[0], curve[-1], " validation ratio", loss / base)
+    assert wsum == 1024.0
+    assert curve[-1] < 0.2 * curve[0]
+    assert loss / base < 0.2, loss / base
+
+
+# ---------------------------------------------------------------------------------------------
+# Coupling to the time loop
+# ---------------------------------------------------------------------------------------------
+def lattice_index(nodes, dx):
+    ij = np.rint(nodes / dx).astype(np.int64)
+    return {(int(a), int(b)): k for k, (a, b) in enumerate(ij)}, ij
+
+
+OFFSETS = [(x, y) for x in range(-4, 5) for y in range(-4, 5) if x * x + y * y <= 16]
+
+
+def frame_code(nx, ny):
+    if abs(nx) >= abs(ny):
+        return (2 if nx < 0 else 0) | (4 if ny < 0 else 0)
+    return 1 | (2 if ny < 0 else 0) | (4 if nx < 0 else 0)
+
+
+def to_frame(f, x, y):
+    a, b = (y, x) if f & 1 else (x, y)
+    return (-a if f & 2 else a), (-b if f & 4 else b)
+
+
+def from_frame(f, cx, cy):
+    a, b = (-cx if f & 2 else cx), (-cy if f & 4 else cy)
+    return (b, a) if f & 1 else (a, b)
+
+
+def expected_inputs(prob_rows, row_nodes, lut_present, lut_value, hist_u, hist_v, nu, R, delta):
+    """Inputs (BF16 bits) and scales of ABC rows from the history hist_u[h] (h = 0 newest), in the
+    layout of include/lps_amd.h. row_nodes: (x, y, ix, iy) of each row; lut_present/lut_value map a
+    lattice position to the node index of the domain the rows live in / of the history arrays."""
+    n = len(row_nodes)
+    X = np.zeros((n, cl.N_IN), np.float32)
+    S = np.zeros((n, 2), np.float32)
+    for r, (x, y, ix, iy) in enumerate(row_nodes):
+        rr = np.sqrt(x * x + y * y)
+        nx, ny = x / rr, y / rr
+        f = frame_code(nx, ny)
+        vals = np.zeros((49, 2, 8, 2), np.float32)
+        present = np.zeros(49, bool)
+        for j, (cx, cy) in enumerate(OFFSETS):
+            px, py = from_frame(f, cx, cy)
+            key = (ix + px, iy + py)
+            if key not in lut_present:
+                continue
+            k = lut_value[key]
+            present[j] = True
+            for p, H in enumerate((hist_u, hist_v)):
+                for h in range(8):
+                    a, b = to_frame(f, np.float32(H[h][2 * k]), np.float32(H[h][2 * k + 1]))
+                    vals[j, p, h] = (a, b)
+        su = np.float32(np.max(np.abs(vals[present, 0]))) if present.any() else np.float32(0)
+        sv = np.float32(np.max(np.abs(vals[present, 1]))) if present.any() else np.float32(0)
+        S[r] = (su, sv)
+        iu = np.float32(1) / su if su >= 1e-30 else np.float32(0)
+        iv = np.float32(1) / sv if sv >= 1e-30 else np.float32(0)
+        body = np.concatenate([vals[:, 0].reshape(49, 16) * iu, vals[:, 1].reshape(49, 16) * iv],
+                              axis=1).astype(np.float32)
+        X[r, :49 * 32] = body.reshape(-1)
+        X[r, 49 * 32:49 * 32 + 4] = np.array(
+            [(nu - 0.3) / 0.15, max(abs(nx), abs(ny)), min(abs(nx), abs(ny)), (R - rr) / delta],
+            dtype=np.float64).astype(np.float32)
+    return cl.bf16_bits(X), S
+
+
+def assert_inputs_equal(x, ex, sc, es):
+    """Gathered inputs equal the numpy construction bit for bit, except where a float32 history
+    value is subnormal (the device may flush those): such entries stay below 1e-6 after scaling."""
+    big = es >= 1e-30
+    assert np.array_equal(sc[big], es[big]), np.max(np.abs(sc - es))
+    assert np.all(sc[~big] < 1e-30)
+    xf, ef = cl.bf16_to_f32(x), cl.bf16_to_f32(ex)
+    off = (x != ex) & (np.abs(xf - ef) > 1e-6)
+    bad = np.argwhere(off)
+    assert bad.size == 0, (bad[:5], xf[tuple(bad[0])], ef[tuple(bad[0])])
+    return int(np.count_nonzero(x != ex))
+
+
+def with_wide_pulse(prob):
+    """The problem with a pulse that covers the whole domain (nonzero inputs at every row)."""
+    import dataclasses
+    return dataclasses.replace(prob, ic=pp.gaussian_ic(
+        prob.nodes, [pp.Pulse(1.0, -2.0, 20.0, 1e-3, (0.6, 0.8))]))
+
+
+def closure_with(seed, out_scale):
+    c = cl.Closure(seed=seed)
+    p = random_params(seed + 100, out_scale=out_scale)
+    if out_scale == 0.0:
+        d = cl.split_params(p)
+        d["W4"][:] = 0.0
+        d["b4"][:] = 0.0
+        p = cl.join_params(d)
+    c.set_params(p)
+    return c, p
+
+
+def test_zero_correction_is_the_least_squares_abc_bitwise():
+    say("closure with zero output vs plain LSQ")
+    prob, _ = load_golden("abc_r10_nu025")
+    c, _ = closure_with(20, 0.0)
+    with c, solver.Solver(prob) as a, solver.Solver(prob) as b:
+        a.attach_closure(c)
+        ra = a.run(120)
+        rb = b.run(120)
+        fa, fb = a.fetch(), b.fetch()
+        for k in ("u", "v", "a", "theta"):
+            assert np.array_equal(fa[k], fb[k]), k
+        assert [r["energy"] for r in ra] == [r["energy"] for r in rb]
+        a.reset()  # clears the history too
+        a.run(30)
+
+
+def test_gathered_inputs_equal_numpy_construction():
+    say("closure inputs vs numpy")
+    prob = with_wide_pulse(load_golden("abc_r12_nu033")[0])
+    dx, delta, nu = float(prob.data2[8]), float(prob.data2[7]), float(prob.data2[1])
+    lut, ij = lattice_index(prob.nodes, dx)
+    R = float(np.max(np.hypot(prob.nodes[:, 0], prob.nodes[:, 1])))
+    c, _ = closure_with(21, 0.0)
+    with c, solver.Solver(prob, use_graph=False) as s:
+        s.attach_closure(c)
+        hist = [s.fetch()]
+        for _ in range(12):
+            s.step(1)
+            hist.append(s.fetch())
+        x, sc, _ = s.closure_inspect(prob.C)
+    # the last completed step is n = 11: history h is the state before step 11 - h
+    hu = [hist[11 - h]["u"] for h in range(8)]
+    hv = [hist[11 - h]["v"] for h in range(8)]
+    rows = [(prob.nodes[a, 0], prob.nodes[a, 1], int(ij[a, 0]), int(ij[a, 1])) for a in prob.abc]
+    ex, es = expected_inputs(prob, rows, lut, lut, hu, hv, nu, R, delta)
+    assert np.all(es[:, 0] > 0)
+    inexact = assert_inputs_equal(x, ex, sc, es)
+    say("  entries off by subnormal flushing:", inexact)
+
+
+def test_applied_correction_is_the_network_output():
+    say("closure applied correction")
+    prob = with_wide_pulse(load_golden("abc_r10_nu025")[0])
+    c, p = closure_with(22, 0.0)
+    with c, solver.Solver(prob, use_graph=False) as a, solver.Solver(prob, use_graph=False) as b:
+        a.attach_closure(c)
+        a.step(40)
+        b.step(40)
+        c.set_params(random_params(23, out_scale=0.05))
+        c.set_scales(1e-2, 3e-2)
+        a.step(1)
+        b.step(1)
+        fa, fb = a.fetch(), b.fetch()
+        x, sc, ap = a.closure_inspect(prob.C)
+        y = c.forward(x)
+    assert np.array_equal(y, ap[:, :4])
+    assert np.array_equal(sc, ap[:, 4:6])
+    du = fa["u"] - fb["u"]
+    dv = fa["v"] - fb["v"]
+    mask = np.ones(prob.N, bool)
+    mask[prob.abc] = False
+    assert np.all(du.reshape(-1, 2)[mask] == 0) and np.all(dv.reshape(-1, 2)[mask] == 0)
+    for r, node in enumerate(prob.abc):
+        xq, yq = prob.nodes[node]
+        f = frame_code(xq / np.hypot(xq, yq), yq / np.hypot(xq, yq))
+        eu = np.array(from_frame(f, float(y[r, 0]), float(y[r, 1]))) * 1e-2 * float(sc[r, 0])
+        ev = np.array(from_frame(f, float(y[r, 2]), float(y[r, 3]))) * 3e-2 * float(sc[r, 1])
+        got_u = du[2 * node:2 * node + 2]
+        got_v = dv[2 * node:2 * node + 2]
+        ref_u = fb["u"][2 * node:2 * node + 2]
+        assert np.allclose(got_u, eu, rtol=1e-6, atol=1e-15 * np.abs(ref_u).max() + 1e-300), (r, got_u, eu)
+        assert np.allclose(got_v, ev, rtol=1e-6, atol=1e-12 * np.abs(fb["v"]).max() + 1e-300), (r, got_v, ev)
+
+
+def test_teacher_forced_samples():
+    """Samples collected on a larger domain equal truth's inputs at the truncated rows and truth's
+    U^{n+1}, V^{n+1} minus the truncated rows' G_u / G_v applied to truth's history."""
+    say("closure teacher forcing")
+    trunc, _ = load_golden("abc_r10_nu025")
+    dt, nu, delta = float(trunc.data2[6]), float(trunc.data2[1]), float(trunc.data2[7])
+    dx = float(trunc.data2[8])
+    truth = pp.make_problem(16.0, nu, pulses=[pp.Pulse(1.0, -2.0, 20.0, 1e-3, (0.6, 0.8))],
+                            steps=40, dt=dt, with_abc=False)
+    n_steps = 40
+    c = cl.Closure(seed=30)
+    with c, solver.Solver(truth, abc_enabled=False) as t1, \
+            solver.Solver(truth, abc_enabled=False, use_graph=False) as t2:
+        c.reserve(4 * trunc.C)
+        got = c.collect(t1, trunc, n_steps, first_step=n_steps, sample_every=n_steps, keep=1.0,
+                        amp_ref=1e-3, amp_min=0.0, seed=5)
+        assert got == trunc.C  # one sample per row at the single sampled step
+        xs, aux = c.read(0, got)
+        hist = [t2.fetch()]
+        for _ in range(n_steps):
+            t2.step(1)
+            hist.append(t2.fetch())
+    order = np.argsort(aux[:, 7])
+    xs, aux = xs[order], aux[order]
+    assert np.array_equal(aux[:, 7], np.arange(trunc.C))
+    lut_t, ij_t = lattice_index(truth.nodes, dx)
+    lut_c, ij_c = lattice_index(trunc.nodes, dx)
+    R = float(np.max(np.hypot(trunc.nodes[:, 0], trunc.nodes[:, 1])))
+    n = n_steps - 1  # sample of step n + 1 = 40
+    hu = [hist[n - h]["u"] for h in range(8)]
+    hv = [hist[n - h]["v"] for h in range(8)]
+    rows = [(trunc.nodes[a, 0], trunc.nodes[a, 1], int(ij_c[a, 0]), int(ij_c[a, 1]))
+            for a in trunc.abc]
+    ex, es = expected_inputs(trunc, rows, lut_c, lut_t, hu, hv, nu, R, delta)
+    assert_inputs_equal(xs, ex, aux[:, 5:7], es)
+    M = trunc.max_cloud
+    Uk, Um, Vk, Vm = hist[n]["u"], hist[n - 1]["u"], hist[n]["v"], hist[n - 1]["v"]
+    U1, V1 = hist[n + 1]["u"], hist[n + 1]["v"]
+    for r, a in enumerate(trunc.abc):
+        cloud = np.trunc(trunc.clouds[r].astype(np.float64) + 0.5).astype(np.int64)
+        tn = np.array([lut_t[(int(ij_c[k, 0]), int(ij_c[k, 1]))] for k in cloud])
+        hu4 = np.stack([Uk[2 * tn], Uk[2 * tn + 1], Um[2 * tn], Um[2 * tn + 1]], 1).ravel()
+        hv4 = np.stack([Vk[2 * tn], Vk[2 * tn + 1], Vm[2 * tn], Vm[2 * tn + 1]], 1).ravel()
+        lu = trunc.Gu[2 * r:2 * r + 2, :4 * M] @ hu4
+        lv = trunc.Gv[2 * r:2 * r + 2, :4 * M] @ hv4
+        at = lut_t[(int(ij_c[a, 0]), int(ij_c[a, 1]))]
+        du = U1[2 * at:2 * at + 2] - lu
+        dv = V1[2 * at:2 * at + 2] - lv
+        xq, yq = trunc.nodes[a]
+        f = frame_code(xq / np.hypot(xq, yq), yq / np.hypot(xq, yq))
+        eu = np.array(to_frame(f, du[0], du[1])) / float(es[r, 0])
+        ev = np.array(to_frame(f, dv[0], dv[1])) / float(es[r, 1])
+        scale_u = np.abs(U1[2 * at:2 * at + 2]).max() / float(es[r, 0]) + 1.0
+        assert np.allclose(aux[r, 0:2], eu, rtol=1e-5, atol=1e-9 * scale_u), (r, aux[r, :4], eu, ev)
+        assert np.allclose(aux[r, 2:4], ev, rtol=1e-5, atol=1e-9 * scale_u), (r, aux[r, :4], eu, ev)
+        assert np.isclose(aux[r, 4], (float(es[r, 0]) / 1e-3) ** 2, rtol=1e-6)
