@@ -91,6 +91,12 @@ int32_t lps_create(const lps_problem *problem, const lps_options *options, lps_c
  * rebuilding the horizons. */
 int32_t lps_reset(lps_ctx *ctx);
 
+/* New initial conditions (.icvalues / .icdvalues contents, 2 node_no each, the problem's node order)
+ * followed by lps_reset: the horizons and constants of lps_create are kept. No counterpart in the
+ * reference, which reads the initial conditions once per process (src/main.cpp:243-250, 624-634).
+ * Single-problem contexts only (LPS_ERR_ARG on a batch). */
+int32_t lps_set_initial(lps_ctx *ctx, const double *ic, const double *icd);
+
 /* Batches. No counterpart in the reference, which runs one job per process: many independent
  * problems (own radius, nu, dt, ABC operators, initial conditions; the same dx, dy and delta)
  * advanced together, every launch of a step covering all of them. Each problem is set up as
@@ -229,15 +235,17 @@ int32_t lps_closure_evaluate(lps_closure *clo, int64_t first, int64_t count, dou
 
 /* Teacher-forced samples from a reflection-free run. truth: a context on a larger domain of the
  * same origin-centred lattice (the fast path); trunc: the truncated problem whose ABC rows are
- * emulated (its nodes lie on truth's lattice). Runs truth for nsteps steps from its current state;
- * after each step n + 1 with n + 1 >= first_step and (n + 1) % sample_every == 0, every row of
- * trunc whose hash (seed, step, row) falls under keep and whose s_u >= amp_min appends one
- * sample: its inputs at step n from truth's history, and as target truth's U^{n+1}, V^{n+1} at
- * the row's node minus trunc's G_u / G_v applied to truth's U^n, U^{n-1}, V^n, V^{n-1}; weight
- * (s_u / amp_ref)^2. Samples beyond the capacity are dropped (the count still grows). */
+ * emulated (its nodes lie on truth's lattice). Runs truth for nsteps steps from its current state
+ * with an empty history; after each step n + 1 with n + 1 >= first_step and
+ * (n + 1) % sample_every == 0, every row of trunc whose hash (seed, step, row) falls under keep
+ * and whose s_u >= amp_min appends one sample: its inputs at step n from truth's history, and as
+ * target truth's U^{n+1}, V^{n+1} at the row's node minus trunc's G_u / G_v applied to truth's
+ * U^n, U^{n-1}, V^n, V^{n-1} (weight 1). A sample whose target exceeds y_max in magnitude (a wave
+ * front rising faster than its history shows) is skipped. Samples beyond the capacity are dropped
+ * (the count still grows). Errors are reported on truth. */
 int32_t lps_closure_collect(lps_closure *clo, lps_ctx *truth, const lps_problem *trunc,
                             int32_t nsteps, int32_t first_step, int32_t sample_every, double keep,
-                            double amp_ref, double amp_min, uint64_t seed);
+                            double amp_min, double y_max, uint64_t seed);
 
 /* Applies clo at every ABC row of ctx (fast path only; a batch uses each instance's nu and
  * radius) from the next step on; the history starts empty (zeros) and is cleared by lps_reset.
@@ -247,6 +255,14 @@ int32_t lps_closure_detach(lps_ctx *ctx);
 /* Factor on the correction per instance (count = lps_instance_count; 1 after attach; 0 gives the
  * least-squares ABC). */
 int32_t lps_closure_set_gain(lps_ctx *ctx, const float *gain);
+/* Fallback guard. At every output record (every 30 steps, lps_run / lps_run_batch) an instance
+ * whose energy rose by more than tol * E0 since the previous record is switched to the
+ * least-squares ABC (gain 0) for the rest of the run: absorbing boundaries take energy out, they do
+ * not add it. tol >= 0 sets the tolerance (0: no guard; 1e-3 after attach); a negative tol keeps
+ * it. fallback_step (count = lps_instance_count, may be NULL) receives the step of each
+ * instance's fallback, -1 where the closure is still on. lps_reset and lps_closure_set_gain put
+ * the gains back. */
+int32_t lps_closure_guard(lps_ctx *ctx, double tol, int64_t *fallback_step);
 /* Inputs and scales of ctx's ABC rows from the history of the last completed step (n_abc x 1600
  * BF16 bits, n_abc x 2), and the correction that step applied (n_abc x 8: 4 outputs, s_u, s_v,
  * 0, 0); any pointer may be NULL. */

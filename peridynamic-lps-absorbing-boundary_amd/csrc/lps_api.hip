@@ -591,6 +591,26 @@ int32_t lps_reset(lps_ctx *c) {
   return rc;
 }
 
+int32_t lps_set_initial(lps_ctx *c, const double *ic, const double *icd) {
+  if (c->batch) return c->fail(LPS_ERR_ARG, "lps_set_initial takes a single-problem context");
+  if (!ic || !icd) return c->fail(LPS_ERR_ARG, "ic and icd are required");
+  const int32_t NS = c->n_slots;
+  std::vector<double> ux(NS, 0.0), uy(NS, 0.0), vx(NS, 0.0), vy(NS, 0.0);
+  for (int32_t k = 0; k < NS; k++) {
+    const int32_t i = c->perm_h[k];
+    if (i < 0) continue;
+    ux[k] = ic[2 * (int64_t)i]; uy[k] = ic[2 * (int64_t)i + 1];
+    vx[k] = icd[2 * (int64_t)i]; vy[k] = icd[2 * (int64_t)i + 1];
+  }
+  const size_t bytes = sizeof(double) * NS;
+  CTX_TRY(c, hipMemcpyAsync(c->IC.x, ux.data(), bytes, hipMemcpyHostToDevice, c->stream));
+  CTX_TRY(c, hipMemcpyAsync(c->IC.y, uy.data(), bytes, hipMemcpyHostToDevice, c->stream));
+  CTX_TRY(c, hipMemcpyAsync(c->ICD.x, vx.data(), bytes, hipMemcpyHostToDevice, c->stream));
+  CTX_TRY(c, hipMemcpyAsync(c->ICD.y, vy.data(), bytes, hipMemcpyHostToDevice, c->stream));
+  CTX_TRY(c, hipStreamSynchronize(c->stream));
+  return lps_reset(c);
+}
+
 // A context with its device, streams and events, nothing else.
 static int32_t new_context(const lps_options *options, lps_ctx **out) {
   lps_ctx *c = new lps_ctx();
@@ -1158,6 +1178,7 @@ int32_t lps_run_batch(lps_ctx *c, int32_t nsteps, lps_diag *records, int32_t max
         if (records && nrec < max_records) records[(size_t)nrec * B + i] = d[i];
       }
       nrec++;
+      if (c->clo && (rc = closure_record_hook(c, d.data(), B)) != LPS_OK) return rc;
       for (int32_t i = 0; i < B; i++)
         if (std::isnan(d[i].energy) || std::isinf(d[i].energy) || std::isnan(d[i].max_u) ||
             std::isinf(d[i].max_u))
@@ -1209,6 +1230,7 @@ int32_t lps_run(lps_ctx *c, int32_t nsteps, lps_diag *records, int32_t max_recor
       d.de_rel = (c->E0 > 0.0) ? (d.energy - c->E0) / c->E0 : 0.0;
       if (records && nrec < max_records) records[nrec] = d;
       nrec++;
+      if (c->clo && (rc = closure_record_hook(c, &d, 1)) != LPS_OK) return rc;
       if (std::isnan(d.energy) || std::isinf(d.energy) || std::isnan(d.max_u) || std::isinf(d.max_u))
         return c->fail(LPS_ERR_NONFINITE, "Energy or displacement is NaN/Inf at step " +
                                               std::to_string(c->step));

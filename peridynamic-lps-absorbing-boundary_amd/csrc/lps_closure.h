@@ -50,12 +50,21 @@ constexpr int64_t kClParams = kOffB4 + kClOut;
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-// What the forward kernel reads of the network (device pointers).
+// What the forward kernels read of the network (device pointers).
 struct MlpDev {
-  const __bf16 *w1, *w2, *w3;  // BF16 copies of W1..W3, [out][in]
-  const float *p;              // FP32 master parameters (biases and the output layer are read here)
-  float sigma_u, sigma_v;      // output scales
+  const __bf16 *w1, *w2, *w3;     // BF16 copies of W1..W3, [out][in]
+  const __bf16 *w1p, *w2p, *w3p;  // the same in matrix-core fragment order (pack_index)
+  const float *p;                 // FP32 master parameters (biases and the output layer)
+  float sigma_u, sigma_v;         // output scales
 };
+
+// Fragment order of a [256 out][K in] BF16 weight: K step k (16 inputs) holds the B operands of
+// the 8 column tiles (32 outputs each) for the 64 lanes, 8 values per lane, contiguous (8 KB).
+__device__ __host__ __forceinline__ int64_t pack_index(int64_t o, int64_t i) {
+  const int64_t k = i >> 4, w = i & 15;
+  return (((k * 8 + (o >> 5)) * 64 + (o & 31) + 32 * (w >> 3)) << 3) + (w & 7);
+}
+constexpr int32_t kWideRows = 128;  // rows per workgroup of the wide forward kernel
 
 // Inputs of the ABC rows gathered from the per-node history (see the header comment).
 struct GatherSrc {
@@ -140,8 +149,10 @@ struct ApplyDst {
   double *ux1, *uy1, *vx1, *vy1;
   float *dbg;  // n x 8 when not null: the 4 outputs, s_u, s_v, and 0, 0 (tests)
 };
+// wide: 128 rows per workgroup (every wavefront all 256 columns, weights staged in LDS), for
+// large row counts; otherwise 32 rows per workgroup. Both give the same bits.
 hipError_t mlp_forward_apply(const MlpDev &m, const GatherSrc &g, int32_t n, const ApplyDst &d,
-                             hipStream_t s);
+                             bool wide, hipStream_t s);
 // Rows of gathered inputs written out (n x 1600 BF16) with their scales (n x 2).
 hipError_t gather_rows(const GatherSrc &g, int32_t n, __bf16 *x, float *scale, hipStream_t s);
 
@@ -163,6 +174,7 @@ struct lps_closure {
   float *p = nullptr, *g = nullptr, *m1 = nullptr, *m2 = nullptr;  // params, grads, Adam moments
   __bf16 *w1 = nullptr, *w2 = nullptr, *w3 = nullptr;               // [out][in] BF16 copies
   __bf16 *w2t = nullptr, *w3t = nullptr;                            // [in][out] BF16 copies
+  __bf16 *w1p = nullptr, *w2p = nullptr, *w3p = nullptr;            // fragment order
   float sigma_u = 1.0f, sigma_v = 1.0f;
   int64_t adam_t = 0;
   // samples: x (cap x 1600 BF16), aux (cap x 8: target 4, weight, s_u, s_v, 0)
@@ -180,7 +192,7 @@ struct lps_closure {
   std::vector<void *> allocs;
 
   lps::MlpDev dev() const {
-    return lps::MlpDev{w1, w2, w3, p, sigma_u, sigma_v};
+    return lps::MlpDev{w1, w2, w3, w1p, w2p, w3p, p, sigma_u, sigma_v};
   }
   template <class T>
   hipError_t alloc(T **q, size_t count) {

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdlib>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -27,6 +28,12 @@ struct ClosureAttach {
   float *gain = nullptr;                            // n_inst
   float *hist[2] = {nullptr, nullptr}, *mx[2] = {nullptr, nullptr};  // by step parity
   float *dbg = nullptr;                             // n_rows x 8, last applied correction
+  // per-instance gains: as set by the caller, and as in force (the guard zeroes them)
+  std::vector<float> gain_user, gain_now;
+  std::vector<double> e_prev;      // energy at the previous output record
+  std::vector<int64_t> off_step;   // step of the fallback to least squares, -1 none
+  double guard_tol = 1e-3;         // <= 0: no guard
+  int64_t wide_min = 16384;        // row count from which the wide forward kernel runs
   std::vector<void *> allocs;
   template <class T>
   hipError_t alloc(T **q, size_t count) {
@@ -93,11 +100,19 @@ hipError_t closure_step_hook(lps_ctx *c, const StepState &st, hipStream_t s) {
   }
   if (!A->clo || A->n_rows == 0) return hipSuccess;
   const ApplyDst d{A->row_slot, A->row_inst, A->gain, st.ux1, st.uy1, st.vx1, st.vy1, A->dbg};
-  return mlp_forward_apply(A->clo->dev(), A->src(par), A->n_rows, d, s);
+  return mlp_forward_apply(A->clo->dev(), A->src(par), A->n_rows, d, A->n_rows >= A->wide_min, s);
 }
 
 hipError_t closure_reset_hook(lps_ctx *c) {
   ClosureAttach *A = c->clo;
+  A->gain_now = A->gain_user;
+  A->e_prev.assign(A->n_inst, -1.0);
+  A->off_step.assign(A->n_inst, -1);
+  if (!A->gain_now.empty()) {
+    hipError_t e = hipMemcpyAsync(A->gain, A->gain_now.data(), sizeof(float) * A->n_inst,
+                                  hipMemcpyHostToDevice, c->stream);
+    if (e != hipSuccess) return e;
+  }
   const size_t hb = sizeof(float) * std::max<int64_t>(A->n_hist, 1);
   for (int q = 0; q < 2; q++) {
     hipError_t e = hipMemsetAsync(A->hist[q], 0, hb * kClRec, c->stream);
@@ -106,6 +121,28 @@ hipError_t closure_reset_hook(lps_ctx *c) {
   }
   if (A->dbg) return hipMemsetAsync(A->dbg, 0, sizeof(float) * 8 * std::max(A->n_rows, 1), c->stream);
   return hipSuccess;
+}
+
+int32_t closure_record_hook(lps_ctx *c, const lps_diag *d, int32_t count) {
+  ClosureAttach *A = c->clo;
+  if (!A->clo || count != A->n_inst) return LPS_OK;
+  bool changed = false;
+  for (int32_t i = 0; i < count; i++) {
+    const double E0 = c->batch ? c->E0_inst[i] : c->E0;
+    if (A->guard_tol > 0.0 && A->gain_now[i] != 0.0f && A->e_prev[i] >= 0.0 && E0 > 0.0 &&
+        !(d[i].energy <= A->e_prev[i] + A->guard_tol * E0)) {
+      A->gain_now[i] = 0.0f;
+      A->off_step[i] = c->step;
+      changed = true;
+    }
+    A->e_prev[i] = d[i].energy;
+  }
+  if (changed) {
+    CTX_TRY(c, hipMemcpyAsync(A->gain, A->gain_now.data(), sizeof(float) * A->n_inst,
+                              hipMemcpyHostToDevice, c->stream));
+    CTX_TRY(c, hipStreamSynchronize(c->stream));
+  }
+  return LPS_OK;
 }
 
 void closure_free(lps_ctx *c) {
@@ -231,6 +268,12 @@ static int32_t make_attach(lps_ctx *c, const Tables &t, lps_closure *clo,
   A->n_rows = (int32_t)row_slot.size();
   A->n_hist = (int32_t)t.hist_slot.size();
   A->n_inst = n_inst;
+  // LPS_CLOSURE_WIDE_MIN: row count from which the 128-row kernel runs (both give the same bits)
+  if (const char *w = std::getenv("LPS_CLOSURE_WIDE_MIN")) A->wide_min = std::atoll(w);
+  A->gain_user.assign(std::max(n_inst, 1), 1.0f);
+  A->gain_now = A->gain_user;
+  A->e_prev.assign(std::max(n_inst, 1), -1.0);
+  A->off_step.assign(std::max(n_inst, 1), -1);
   const hipStream_t s = c->stream;
   hipError_t e = A->upload(&A->hist_slot, t.hist_slot, s);
   if (e == hipSuccess) e = A->upload(&A->hidx, t.hidx, s);
@@ -269,7 +312,7 @@ struct CollectArgs {
   uint64_t seed, keep_thr;
   int64_t step;  // n + 1
   float amp_min;
-  double inv_amp_ref2;
+  double y_max;
   __bf16 *dsx;
   float *dsaux;
   unsigned long long *count;
@@ -312,27 +355,30 @@ __global__ void __launch_bounds__(256) k_collect(CollectArgs a) {
     Vx += wsum(t2);
     Vy += wsum(t3);
   }
+  // targets in the canonical frame, units of s_u / s_v (every lane holds the sums)
+  const float iu = safe_inv(su), iv = safe_inv(sv);
+  const int32_t i = a.row_slot[r];
+  const double dux = a.ux1[i] - Ux, duy = a.uy1[i] - Uy;
+  const double dvx = a.vx1[i] - Vx, dvy = a.vy1[i] - Vy;
+  const int f = a.g.xf[r];
+  const double pu = (f & 1) ? duy : dux, qu = (f & 1) ? dux : duy;
+  const double pv = (f & 1) ? dvy : dvx, qv = (f & 1) ? dvx : dvy;
+  const double y0 = ((f & 2) ? -pu : pu) * iu, y1 = ((f & 4) ? -qu : qu) * iu;
+  const double y2 = ((f & 2) ? -pv : pv) * iv, y3 = ((f & 4) ? -qv : qv) * iv;
+  if (!(fmax(fmax(fabs(y0), fabs(y1)), fmax(fabs(y2), fabs(y3))) <= a.y_max)) return;
   unsigned long long idx = 0;
   if (lane == 0) idx = atomicAdd(a.count, 1ull);
   idx = ((unsigned long long)__shfl((unsigned)(idx >> 32), 0) << 32) | (unsigned)__shfl((unsigned)idx, 0);
   if ((int64_t)idx >= a.cap) return;
-  const float iu = safe_inv(su), iv = safe_inv(sv);
   for (int q = lane; q < kClIn / 8; q += 64)
     *(bf16x8 *)(a.dsx + (int64_t)idx * kClIn + 8 * q) = gather8(a.g, r, 8 * q, iu, iv);
   if (lane == 0) {
-    const int32_t i = a.row_slot[r];
-    const double dux = a.ux1[i] - Ux, duy = a.uy1[i] - Uy;
-    const double dvx = a.vx1[i] - Vx, dvy = a.vy1[i] - Vy;
-    const int f = a.g.xf[r];
-    // canonical components (to_frame in FP64)
-    const double pu = (f & 1) ? duy : dux, qu = (f & 1) ? dux : duy;
-    const double pv = (f & 1) ? dvy : dvx, qv = (f & 1) ? dvx : dvy;
     float *o = a.dsaux + (int64_t)idx * 8;
-    o[0] = (float)(((f & 2) ? -pu : pu) * iu);
-    o[1] = (float)(((f & 4) ? -qu : qu) * iu);
-    o[2] = (float)(((f & 2) ? -pv : pv) * iv);
-    o[3] = (float)(((f & 4) ? -qv : qv) * iv);
-    o[4] = (float)((double)su * (double)su * a.inv_amp_ref2);
+    o[0] = (float)y0;
+    o[1] = (float)y1;
+    o[2] = (float)y2;
+    o[3] = (float)y3;
+    o[4] = 1.0f;
     o[5] = su;
     o[6] = sv;
     o[7] = (float)r;
@@ -447,9 +493,22 @@ int32_t lps_closure_detach(lps_ctx *c) {
 }
 
 int32_t lps_closure_set_gain(lps_ctx *c, const float *gain) {
-  if (!c->clo) return c->fail(LPS_ERR_ARG, "no closure attached");
-  CTX_TRY(c, hipMemcpyAsync(c->clo->gain, gain, sizeof(float) * c->clo->n_inst, hipMemcpyHostToDevice, c->stream));
+  ClosureAttach *A = c->clo;
+  if (!A) return c->fail(LPS_ERR_ARG, "no closure attached");
+  A->gain_user.assign(gain, gain + A->n_inst);
+  A->gain_now = A->gain_user;
+  A->off_step.assign(A->n_inst, -1);
+  CTX_TRY(c, hipMemcpyAsync(A->gain, A->gain_now.data(), sizeof(float) * A->n_inst, hipMemcpyHostToDevice, c->stream));
   CTX_TRY(c, hipStreamSynchronize(c->stream));
+  return LPS_OK;
+}
+
+int32_t lps_closure_guard(lps_ctx *c, double tol, int64_t *fallback_step) {
+  ClosureAttach *A = c->clo;
+  if (!A) return c->fail(LPS_ERR_ARG, "no closure attached");
+  if (tol >= 0.0) A->guard_tol = tol;
+  if (fallback_step)
+    for (int32_t i = 0; i < A->n_inst; i++) fallback_step[i] = A->off_step[i];
   return LPS_OK;
 }
 
@@ -481,13 +540,13 @@ int32_t lps_closure_inspect(lps_ctx *c, uint16_t *x, float *scale2, float *appli
 }
 
 int32_t lps_closure_collect(lps_closure *clo, lps_ctx *c, const lps_problem *p, int32_t nsteps,
-                            int32_t first_step, int32_t sample_every, double keep, double amp_ref,
-                            double amp_min, uint64_t seed) {
+                            int32_t first_step, int32_t sample_every, double keep, double amp_min,
+                            double y_max, uint64_t seed) {
   if (!clo || !p) return c->fail(LPS_ERR_ARG, "closure and truncated problem are required");
   if (!c->fast || c->batch) return c->fail(LPS_ERR_ARG, "the truth run needs a single fast-path context");
   if (clo->device != c->opt.device) return c->fail(LPS_ERR_ARG, "closure and context on different devices");
   if (clo->ds_cap <= 0) return c->fail(LPS_ERR_ARG, "reserve the sample store first");
-  if (nsteps < 0 || sample_every <= 0 || !(amp_ref > 0.0)) return c->fail(LPS_ERR_ARG, "bad sampling arguments");
+  if (nsteps < 0 || sample_every <= 0 || !(y_max > 0.0)) return c->fail(LPS_ERR_ARG, "bad sampling arguments");
   const int32_t N = p->node_no, C = p->constrains_no, M = p->max_cloud_size;
   if (N <= 0 || C <= 0 || M <= 0 || M > 64 || !p->nodes || !p->inf_nodes || !p->iclouds ||
       !p->ci_nodes_u || !p->ci_nodes_v)
@@ -575,7 +634,7 @@ int32_t lps_closure_collect(lps_closure *clo, lps_ctx *c, const lps_problem *p, 
   ca.seed = seed;
   ca.keep_thr = keep >= 1.0 ? ~0ull : (uint64_t)(std::max(keep, 0.0) * 18446744073709551615.0);
   ca.amp_min = (float)amp_min;
-  ca.inv_amp_ref2 = 1.0 / (amp_ref * amp_ref);
+  ca.y_max = y_max;
   ca.dsx = clo->ds_x;
   ca.dsaux = clo->ds_aux;
   ca.count = clo->ds_count;

@@ -105,7 +105,10 @@ namespace lps {
 // rows' G_u / G_v on their stream (at the end of the step when the context has no ABC rows):
 // records U^n, V^n into the history and adds the correction to U^{n+1}, V^{n+1} at the ABC nodes.
 hipError_t closure_step_hook(lps_ctx *c, const StepState &st, hipStream_t s);
-hipError_t closure_reset_hook(lps_ctx *c);  // clears the history (lps_reset)
+hipError_t closure_reset_hook(lps_ctx *c);  // clears the history and the fallbacks (lps_reset)
+// At each output record (count instances, de_rel set): an instance whose energy rose by more than
+// the guard tolerance times its E0 since the previous record falls back to the least-squares ABC.
+int32_t closure_record_hook(lps_ctx *c, const lps_diag *d, int32_t count);
 void closure_free(lps_ctx *c);
 // Drops the captured 30-step graphs (they hold the kernels they were captured with).
 void ctx_drop_graphs(lps_ctx *c);

@@ -75,6 +75,27 @@ __device__ __forceinline__ void dense_lds(const __bf16 (*hs)[kHsPitch], const __
   }
 }
 
+// Correction of one row added to the state (the gathered path's epilogue).
+__device__ __forceinline__ void apply_row(const MlpDev &m, const GatherSrc &g, const ApplyDst &d,
+                                          int32_t row, const float *y, float su, float sv) {
+  const int f = g.xf[row];
+  const double gain = d.gain[d.inst[row]];
+  double dux, duy, dvx, dvy;
+  from_frame(f, (double)y[0], (double)y[1], dux, duy);
+  from_frame(f, (double)y[2], (double)y[3], dvx, dvy);
+  const double ku = gain * m.sigma_u * (double)su, kv = gain * m.sigma_v * (double)sv;
+  const int32_t i = d.slot[row];
+  d.ux1[i] += ku * dux;
+  d.uy1[i] += ku * duy;
+  d.vx1[i] += kv * dvx;
+  d.vy1[i] += kv * dvy;
+  if (d.dbg) {
+    float *o = d.dbg + (int64_t)row * 8;
+    o[0] = y[0]; o[1] = y[1]; o[2] = y[2]; o[3] = y[3];
+    o[4] = su; o[5] = sv; o[6] = 0.0f; o[7] = 0.0f;
+  }
+}
+
 // The network over 32 rows per workgroup (4 wavefronts, 64 output columns each). kGather: inputs
 // gathered from the history (gather8) and the correction added to the state; else inputs from x.
 template <bool kGather>
@@ -168,23 +189,134 @@ __global__ void __launch_bounds__(256) k_mlp_fwd(MlpDev m, const __bf16 *x, Gath
       for (int c = 0; c < kClOut; c++) yo[(int64_t)row * kClOut + c] = y[c];
     }
   } else {
-    const int f = g.xf[row];
-    const double su = sc[r][2], sv = sc[r][3];
-    const double gain = d.gain[d.inst[row]];
-    double dux, duy, dvx, dvy;
-    from_frame(f, (double)y[0], (double)y[1], dux, duy);
-    from_frame(f, (double)y[2], (double)y[3], dvx, dvy);
-    const double ku = gain * m.sigma_u * su, kv = gain * m.sigma_v * sv;
-    const int32_t i = d.slot[row];
-    d.ux1[i] += ku * dux;
-    d.uy1[i] += ku * duy;
-    d.vx1[i] += kv * dvx;
-    d.vy1[i] += kv * dvy;
-    if (d.dbg) {
-      float *o = d.dbg + (int64_t)row * 8;
-      o[0] = y[0]; o[1] = y[1]; o[2] = y[2]; o[3] = y[3];
-      o[4] = (float)su; o[5] = (float)sv; o[6] = 0.0f; o[7] = 0.0f;
+    apply_row(m, g, d, row, y, sc[r][2], sc[r][3]);
+  }
+}
+
+// Fragments of K step k of a packed weight into LDS: 512 fragments, 2 per thread.
+__device__ __forceinline__ void stage_b(const __bf16 *wp, int k, bf16x8 (*bs)[64], int t) {
+  const bf16x8 *src = (const bf16x8 *)(wp + (int64_t)k * 4096);
+  bs[t >> 6][t & 63] = src[t];
+  bs[(t >> 6) + 4][t & 63] = src[t + 256];
+}
+
+// The network over 128 rows per workgroup: wavefront w owns rows 32 w .. 32 w + 31 and all 256
+// columns (8 accumulator tiles); each K step's weight fragments are staged once per workgroup in
+// LDS (double buffered). Same arithmetic order as k_mlp_fwd, so the same bits.
+template <bool kGather>
+__global__ void __launch_bounds__(256) k_mlp_fwd_wide(MlpDev m, const __bf16 *x, GatherSrc g,
+                                                      int32_t n, float *yo, ApplyDst d) {
+  __shared__ bf16x8 bs[2][8][64];
+  __shared__ __bf16 hs[kWideRows][kHsPitch];
+  __shared__ float sc[kWideRows][4];
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const int r0 = blockIdx.x * kWideRows;
+  if constexpr (kGather) {
+    const int r = t >> 1, j = t & 1;
+    float su = 0.0f, sv = 0.0f;
+    if (r0 + r < n) {
+      for (int q = j; q < kClOffsets; q += 2) {
+        const int32_t h = g.hidx[(int64_t)(r0 + r) * kClOffsets + q];
+        if (h >= 0) {
+          su = fmaxf(su, g.mx[2 * (int64_t)h]);
+          sv = fmaxf(sv, g.mx[2 * (int64_t)h + 1]);
+        }
+      }
     }
+    su = fmaxf(su, __shfl_xor(su, 1));
+    sv = fmaxf(sv, __shfl_xor(sv, 1));
+    if (j == 0) {
+      sc[r][0] = safe_inv(su);
+      sc[r][1] = safe_inv(sv);
+      sc[r][2] = su;
+      sc[r][3] = sv;
+    }
+  }
+  const int ar = lane & 31, ak = 8 * (lane >> 5);
+  const int lrow = 32 * wv + ar;  // this lane's A row inside the workgroup
+  f32x16 acc[8];
+  for (int layer = 0; layer < 3; layer++) {
+    const __bf16 *wp = layer == 0 ? m.w1p : layer == 1 ? m.w2p : m.w3p;
+    const int nk = (layer == 0 ? kClIn : kClHid) / 16;
+#pragma unroll
+    for (int q = 0; q < 8; q++) acc[q] = f32x16{};
+    stage_b(wp, 0, bs[0], t);
+    __syncthreads();
+    float iu = 0.0f, iv = 0.0f;
+    if constexpr (kGather) {
+      iu = sc[lrow][0];
+      iv = sc[lrow][1];
+    }
+    const int32_t row = r0 + lrow;
+    const bool live = row < n;
+    for (int k = 0; k < nk; k++) {
+      bf16x8 nb0, nb1;
+      const bool more = k + 1 < nk;
+      if (more) {
+        const bf16x8 *src = (const bf16x8 *)(wp + (int64_t)(k + 1) * 4096);
+        nb0 = src[t];
+        nb1 = src[t + 256];
+      }
+      bf16x8 a;
+      if (layer > 0) a = *(const bf16x8 *)&hs[lrow][16 * k + ak];
+      else if (!live) a = zero8();
+      else if constexpr (kGather) a = gather8(g, row, 16 * k + ak, iu, iv);
+      else a = *(const bf16x8 *)(x + (int64_t)row * kClIn + 16 * k + ak);
+#pragma unroll
+      for (int q = 0; q < 8; q++) acc[q] = mfma(a, bs[k & 1][q][lane], acc[q]);
+      if (more) {
+        bs[(k + 1) & 1][t >> 6][t & 63] = nb0;
+        bs[(k + 1) & 1][(t >> 6) + 4][t & 63] = nb1;
+      }
+      __syncthreads();
+    }
+    // bias + ReLU into LDS (this wavefront's own rows)
+    const float *bias = m.p + (layer == 0 ? kOffB1 : layer == 1 ? kOffB2 : kOffB3);
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const int col = 32 * q + ar;
+      const float b = bias[col];
+#pragma unroll
+      for (int e = 0; e < 16; e++)
+        hs[32 * wv + acc_row(e, lane)][col] = (__bf16)fmaxf(acc[q][e] + b, 0.0f);
+    }
+    __syncthreads();
+  }
+  // output layer (FP32): 2 threads per row; thread j forms the partial sums of the 32-wide slices
+  // 4 j .. 4 j + 3 and adds them in k_mlp_fwd's butterfly order
+  const int r = t >> 1, j = t & 1;
+  const float *w4 = m.p + kOffW4;
+  float y[kClOut];
+  {
+    float ps[4][kClOut];
+#pragma unroll
+    for (int jj = 0; jj < 4; jj++) {
+#pragma unroll
+      for (int c = 0; c < kClOut; c++) ps[jj][c] = 0.0f;
+      for (int q = 0; q < 32; q++) {
+        const int k = 32 * (4 * j + jj) + q;
+        const float hv = (float)hs[r][k];
+#pragma unroll
+        for (int c = 0; c < kClOut; c++) ps[jj][c] += hv * w4[c * kClHid + k];
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < kClOut; c++) {
+      y[c] = (ps[0][c] + ps[1][c]) + (ps[2][c] + ps[3][c]);
+      y[c] += __shfl_xor(y[c], 1);
+    }
+  }
+  const int32_t row = r0 + r;
+  if (j != 0 || row >= n) return;
+#pragma unroll
+  for (int c = 0; c < kClOut; c++) y[c] += m.p[kOffB4 + c];
+  if constexpr (!kGather) {
+    if (yo) {
+#pragma unroll
+      for (int c = 0; c < kClOut; c++) yo[(int64_t)row * kClOut + c] = y[c];
+    }
+  } else {
+    apply_row(m, g, d, row, y, sc[r][2], sc[r][3]);
   }
 }
 
@@ -197,8 +329,13 @@ hipError_t mlp_forward_dataset(const MlpDev &m, const __bf16 *x, int32_t n, __bf
 }
 
 hipError_t mlp_forward_apply(const MlpDev &m, const GatherSrc &g, int32_t n, const ApplyDst &d,
-                             hipStream_t s) {
+                             bool wide, hipStream_t s) {
   if (n <= 0) return hipSuccess;
+  if (wide) {
+    hipLaunchKernelGGL(k_mlp_fwd_wide<true>, dim3((n + kWideRows - 1) / kWideRows), dim3(256), 0,
+                       s, m, (const __bf16 *)nullptr, g, n, (float *)nullptr, d);
+    return hipGetLastError();
+  }
   hipLaunchKernelGGL(k_mlp_fwd<true>, dim3((n + kClRows - 1) / kClRows), dim3(256), 0, s, m,
                      (const __bf16 *)nullptr, g, n, (__bf16 *)nullptr, (__bf16 *)nullptr,
                      (__bf16 *)nullptr, (float *)nullptr, d);
@@ -335,7 +472,8 @@ __global__ void __launch_bounds__(1024) k_sum(const float *in, int32_t n, float 
 // Adam update (when lr != 0) and the BF16 copies of W1..W3 ([out][in], and [in][out] for W2, W3).
 __global__ void __launch_bounds__(256) k_adam(float *p, const float *g, float *m1, float *m2,
                                               float lr, float c1, float c2, __bf16 *w1, __bf16 *w2,
-                                              __bf16 *w3, __bf16 *w2t, __bf16 *w3t) {
+                                              __bf16 *w3, __bf16 *w2t, __bf16 *w3t, __bf16 *w1p,
+                                              __bf16 *w2p, __bf16 *w3p) {
   const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (q >= kClParams) return;
   float w = p[q];
@@ -350,14 +488,17 @@ __global__ void __launch_bounds__(256) k_adam(float *p, const float *g, float *m
   }
   if (q < kOffB1) {
     w1[q] = (__bf16)w;
+    w1p[pack_index(q / kClIn, q % kClIn)] = (__bf16)w;
   } else if (q >= kOffW2 && q < kOffB2) {
     const int64_t j = q - kOffW2;
     w2[j] = (__bf16)w;
     w2t[(j % kClHid) * kClHid + j / kClHid] = (__bf16)w;
+    w2p[pack_index(j / kClHid, j % kClHid)] = (__bf16)w;
   } else if (q >= kOffW3 && q < kOffB3) {
     const int64_t j = q - kOffW3;
     w3[j] = (__bf16)w;
     w3t[(j % kClHid) * kClHid + j / kClHid] = (__bf16)w;
+    w3p[pack_index(j / kClHid, j % kClHid)] = (__bf16)w;
   }
 }
 
@@ -420,7 +561,8 @@ int32_t repack(lps_closure *c, float lr) {
   const float c1 = c->adam_t > 0 ? 1.0f / (1.0f - std::pow(0.9f, (float)c->adam_t)) : 1.0f;
   const float c2 = c->adam_t > 0 ? 1.0f / (1.0f - std::pow(0.999f, (float)c->adam_t)) : 1.0f;
   hipLaunchKernelGGL(k_adam, blocks(kClParams, 256), dim3(256), 0, c->stream, c->p, c->g, c->m1,
-                     c->m2, lr, c1, c2, c->w1, c->w2, c->w3, c->w2t, c->w3t);
+                     c->m2, lr, c1, c2, c->w1, c->w2, c->w3, c->w2t, c->w3t, c->w1p, c->w2p,
+                     c->w3p);
   CLO_TRY(c, hipGetLastError());
   return LPS_OK;
 }
@@ -550,6 +692,9 @@ int32_t lps_closure_create(int32_t device, uint64_t seed, lps_closure **out) {
   if (e == hipSuccess) e = c->alloc(&c->w3, (size_t)kClHid * kClHid);
   if (e == hipSuccess) e = c->alloc(&c->w2t, (size_t)kClHid * kClHid);
   if (e == hipSuccess) e = c->alloc(&c->w3t, (size_t)kClHid * kClHid);
+  if (e == hipSuccess) e = c->alloc(&c->w1p, (size_t)kClHid * kClIn);
+  if (e == hipSuccess) e = c->alloc(&c->w2p, (size_t)kClHid * kClHid);
+  if (e == hipSuccess) e = c->alloc(&c->w3p, (size_t)kClHid * kClHid);
   if (e == hipSuccess) e = c->alloc(&c->ds_count, 1);
   if (e == hipSuccess) e = c->alloc(&c->loss_d, 1);
   if (e == hipSuccess) e = hipMemsetAsync(c->ds_count, 0, sizeof(unsigned long long), c->stream);

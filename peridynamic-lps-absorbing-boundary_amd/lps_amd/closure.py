@@ -137,13 +137,13 @@ class Closure:
         return float(out[0]), float(out[1]), float(out[2])
 
     def collect(self, truth, trunc_prob, nsteps: int, first_step: int = 8, sample_every: int = 1,
-                keep: float = 1.0, amp_ref: float = 1.0, amp_min: float = 0.0, seed: int = 0):
+                keep: float = 1.0, amp_min: float = 0.0, y_max: float = 1e30, seed: int = 0):
         """Teacher-forced samples: run `truth` (a Solver on a larger domain) for nsteps steps and
         append samples for the ABC rows of trunc_prob (a preprocess.Problem)."""
         keep_arrays = []
         p = _lps_problem(trunc_prob, keep_arrays)
         rc = self._L.lps_closure_collect(self._h, truth._h, C.byref(p), nsteps, first_step,
-                                         sample_every, float(keep), float(amp_ref), float(amp_min),
+                                         sample_every, float(keep), float(amp_min), float(y_max),
                                          seed)
         if rc < 0:  # the message sits on the truth context
             raise LpsError(rc, self._L.lps_last_error(truth._h).decode())

@@ -81,6 +81,23 @@ NU_SWEEP = {
     0.40: (17.0e-9, 0.7),
     0.45: (16.0e-9, 0.7),
 }
+# The same sweep at the midpoints (max ||G|| between 0.9985 and 0.9998, all stable): with NU_SWEEP
+# a 0.025 grid over [0.15, 0.45], the nu values of the learned-closure workloads.
+NU_SWEEP_MID = {
+    0.175: (17.0e-9, 0.7),
+    0.225: (17.5e-9, 1.0),
+    0.275: (17.5e-9, 0.9),
+    0.325: (17.0e-9, 0.7),
+    0.375: (17.5e-9, 0.9),
+    0.425: (16.0e-9, 0.7),
+}
+NU_GRID = tuple(sorted(list(NU_SWEEP) + list(NU_SWEEP_MID)))
+
+
+def sweep_pair(nu: float):
+    """(dt, delta_kappa factor) of a nu on the 0.025 grid."""
+    key = round(nu, 3)
+    return NU_SWEEP[key] if key in NU_SWEEP else NU_SWEEP_MID[key]
 
 
 @dataclass(frozen=True)

@@ -1,0 +1,312 @@
+"""Learned ABC closure: training data from reflection-free runs, training, and closed-loop evaluation
+against the least-squares ABC.
+
+The reference has no learned closure; its ABC is the least-squares G_u / G_v update
+(src/main.cpp:883-974, operators from preprocessor/main.cpp:415-637). Here a network (lps_amd.closure)
+adds a correction to that update at every ABC row, and the least-squares ABC stays both the
+fallback (gain 0) and the judge: a closure is kept for an instance only where its reflected energy
+is at or below the least-squares ABC's.
+
+Workload (truncated domain R = 125 mm, truth R = 375 mm):
+  * initial conditions: 1-4 Gaussian pulses, radius U(5, 40) mm, centres uniform in the disk of
+    radius 0.6 R, amplitude N(0, 1e-3), radial (P) or tangential (S) polarisation;
+  * nu uniform on the 0.025 grid over [0.15, 0.45] (configs.NU_GRID), each with the stability
+    sweep's dt (configs.sweep_pair);
+  * truth: the same pulses on an R = 375 mm domain without ABC. Its field at the truncated domain's
+    ABC rows is reflection-free until a wave from the pulses has gone out to 375 mm and back to the
+    rows' clouds (2 * 375 - 121 - max(|c| + r) mm at c_P, 10% margin);
+  * samples (teacher forcing): inputs from truth's history, target = truth's U^{n+1}, V^{n+1} minus
+    the truncated rows' G_u / G_v on truth's U^n, U^{n-1}, V^n, V^{n-1} (lps_closure_collect).
+Reflected energy of a run: E_int(T) / E0 = 1 + dE/E0 of the last output record
+(src/main.cpp:986-1043, E0 at step 30).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from dataclasses import dataclass, field, replace
+from typing import Dict, List, Optional
+
+import numpy as np
+
+from . import configs, preprocess as pp
+
+R_TRUNC = 125.0
+R_TRUTH = 375.0
+DX = pp.DX_DEFAULT
+DELTA = pp.DELTA_DEFAULT
+
+
+@dataclass
+class Run:
+    seed: int
+    nu: float
+    pulses: List[pp.Pulse]
+    d_max: float       # max over pulses of |centre| + radius (mm)
+    truth_steps: int   # reflection-free window of the truth run, capped at the waves' passage
+    dt: float
+    dk: float
+
+
+def wave_speeds(nu: float):
+    mc = pp.material(nu=nu)
+    return math.sqrt((mc.lam + 2.0 * mc.mu) / mc.rho), math.sqrt(mc.mu / mc.rho)
+
+
+def draw_runs(seed: int, count: int, R: float = R_TRUNC, centre_frac: float = 0.6) -> List[Run]:
+    rng = np.random.default_rng(seed)
+    runs = []
+    for k in range(count):
+        nu = configs.NU_GRID[int(rng.integers(len(configs.NU_GRID)))]
+        pulses = []
+        for _ in range(int(rng.integers(1, 5))):
+            r = rng.uniform(5.0, 40.0)
+            rho = centre_frac * R * math.sqrt(rng.uniform())
+            phi = rng.uniform(0.0, 2.0 * math.pi)
+            amp = rng.normal(0.0, 1e-3)
+            pol = "P" if rng.uniform() < 0.5 else "S"
+            pulses.append(pp.Pulse(rho * math.cos(phi), rho * math.sin(phi), r, amp, (1.0, 0.0),
+                                   polarization=pol))
+        d_max = max(math.hypot(p.cx, p.cy) + p.radius for p in pulses)
+        dt, dk = configs.sweep_pair(nu)
+        cP, cS = wave_speeds(nu)
+        band = R - 2.0 * DELTA  # innermost cloud node of the ABC rows
+        t_valid = 0.9 * (2.0 * R_TRUTH - band - d_max) / cP
+        t_pass = (R + d_max) / cS + 5e-6
+        runs.append(Run(seed=int(splitmix(seed, k)), nu=nu, pulses=pulses, d_max=d_max,
+                        truth_steps=int(min(t_valid, t_pass) / dt), dt=dt, dk=dk))
+    return runs
+
+
+def splitmix(a: int, b: int) -> int:
+    x = (a * 0x9E3779B97F4A7C15 + b + 0x632BE59BD9B4E019) & 0xFFFFFFFFFFFFFFFF
+    x = ((x ^ (x >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+    x = ((x ^ (x >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+    return x ^ (x >> 31)
+
+
+_TRUNC_CACHE: Dict[float, pp.Problem] = {}
+
+
+def truncated_problem(nu: float, workers: Optional[int] = None) -> pp.Problem:
+    """The R = 125 mm domain with its least-squares ABC operators for nu (zero initial state)."""
+    if nu not in _TRUNC_CACHE:
+        dt, dk = configs.sweep_pair(nu)
+        _TRUNC_CACHE[nu] = pp.make_problem(R_TRUNC, nu, pulses=[], steps=1, dt=dt, dk_factor=dk,
+                                           workers=workers)
+    return _TRUNC_CACHE[nu]
+
+
+def run_problem(run: Run, steps: int, workers: Optional[int] = None) -> pp.Problem:
+    base = truncated_problem(run.nu, workers)
+    return replace(base, ic=pp.gaussian_ic(base.nodes, run.pulses), steps=steps,
+                   info=dict(base.info, run_seed=run.seed))
+
+
+def log(*a):
+    print("[learned]", *a, flush=True)
+
+
+# ---------------------------------------------------------------------------------------------
+# Samples
+# ---------------------------------------------------------------------------------------------
+def collect(clo, runs: List[Run], keep: float, sample_every: int, amp_min_rel: float,
+            y_max: float = 1.0, workers: Optional[int] = None, device: int = 0) -> dict:
+    """Teacher-forced samples of every run appended to clo's store, grouped by nu."""
+    from .solver import Solver
+    stats = {"runs": 0, "truth_node_steps": 0, "truth_s": 0.0, "prep_s": 0.0}
+    by_nu: Dict[float, List[Run]] = {}
+    for r in runs:
+        by_nu.setdefault(r.nu, []).append(r)
+    for nu in sorted(by_nu):
+        t0 = time.time()
+        trunc = truncated_problem(nu, workers)
+        dt, dk = configs.sweep_pair(nu)
+        truth_p = pp.make_problem(R_TRUTH, nu, pulses=[], steps=1, dt=dt, with_abc=False)
+        stats["prep_s"] += time.time() - t0
+        with Solver(truth_p, abc_enabled=False, device=device) as truth:
+            for r in by_nu[nu]:
+                t1 = time.time()
+                ic = pp.gaussian_ic(truth_p.nodes, r.pulses)
+                amp = float(np.max(np.abs(ic)))
+                truth.set_initial(ic)
+                n = clo.collect(truth, trunc, r.truth_steps, first_step=8,
+                                sample_every=sample_every, keep=keep, amp_min=amp_min_rel * amp,
+                                y_max=y_max, seed=r.seed)
+                dtr = time.time() - t1
+                stats["runs"] += 1
+                stats["truth_node_steps"] += truth_p.N * r.truth_steps
+                stats["truth_s"] += dtr
+                log(f"nu {nu:.3f} run {stats['runs']}/{len(runs)}: {r.truth_steps} truth steps, "
+                    f"{len(r.pulses)} pulses, samples {n} ({dtr:.2f} s)")
+    return stats
+
+
+def target_scales(clo, first: int, count: int, chunk: int = 65536, pieces: int = 16):
+    """Weighted RMS of the u and v targets over a spread sample of [first, first + count)."""
+    su = sv = sw = 0.0
+    for k in range(pieces):
+        a = first + (count * k) // pieces
+        n = min(chunk, first + count - a)
+        if n <= 0:
+            continue
+        _, aux = clo.read(a, n)
+        w = aux[:, 4].astype(np.float64)
+        su += float(np.sum(w * (aux[:, 0] ** 2 + aux[:, 1] ** 2)))
+        sv += float(np.sum(w * (aux[:, 2] ** 2 + aux[:, 3] ** 2)))
+        sw += float(np.sum(w))
+    return math.sqrt(su / (2.0 * sw)), math.sqrt(sv / (2.0 * sw))
+
+
+def fit(clo, n_train: int, n_val: int, steps: int, batch: int, lr0: float, lr1: float,
+        seed: int = 0) -> dict:
+    s_u, s_v = target_scales(clo, 0, n_train)
+    clo.set_scales(s_u, s_v)
+    log(f"target scales sigma_u {s_u:.4g} sigma_v {s_v:.4g}")
+    t0 = time.time()
+    curve = clo.train(steps, batch, lr0, lr1, 0, n_train, seed=seed)
+    t_train = time.time() - t0
+    out = {"sigma": [s_u, s_v], "steps": steps, "batch": batch, "train_s": t_train,
+           "loss_first": float(np.mean(curve[:50])), "loss_last": float(np.mean(curve[-50:]))}
+    if n_val > 0:
+        l, b, _ = clo.evaluate(n_train, n_val)
+        out["val_ratio"] = l / b  # remaining fraction of the least-squares error (teacher forced)
+    l, b, _ = clo.evaluate(0, min(n_train, 1 << 20))
+    out["train_ratio"] = l / b
+    log(f"trained {steps} steps x {batch} in {t_train:.1f} s: loss {out['loss_first']:.4g} -> "
+        f"{out['loss_last']:.4g}; error left vs LSQ: train {out['train_ratio']:.3f}, "
+        f"validation {out.get('val_ratio', float('nan')):.3f}")
+    return out
+
+
+def save(clo, path: str, meta: dict):
+    s_u, s_v = clo.get_scales()
+    np.savez(path, params=clo.get_params(), sigma=np.array([s_u, s_v], np.float32),
+             meta=np.array(json.dumps(meta)))
+
+
+def load_into(clo, path: str) -> dict:
+    z = np.load(path, allow_pickle=False)
+    clo.set_params(z["params"])
+    clo.set_scales(float(z["sigma"][0]), float(z["sigma"][1]))
+    return json.loads(str(z["meta"]))
+
+
+# ---------------------------------------------------------------------------------------------
+# Closed loop
+# ---------------------------------------------------------------------------------------------
+def reflected(recs) -> float:
+    return 1.0 + recs[-1]["de_rel"]
+
+
+def evaluate(clo, runs: List[Run], steps: int, chunk: int = 64, device: int = 0,
+             workers: Optional[int] = None, guard: float = 1e-3) -> dict:
+    """Every run on the truncated domain with the least-squares ABC and with the closure (same
+    batch, reset between); per-instance reflected energy and the time of both loops."""
+    from .solver import Batch
+    res = {"lsq": [], "learned": [], "nu": [], "fallback_step": [], "t_lsq": 0.0,
+           "t_learned": 0.0, "node_steps": 0}
+    for a in range(0, len(runs), chunk):
+        part = runs[a:a + chunk]
+        probs = [run_problem(r, steps, workers) for r in part]
+        with Batch(probs, device=device) as b:
+            t0 = time.time()
+            rl = b.run(steps)
+            t1 = time.time()
+            b.reset()
+            b.attach_closure(clo)
+            b.closure_guard(guard)
+            t2 = time.time()
+            rn = b.run(steps)
+            t3 = time.time()
+            fb = b.closure_guard()
+        res["t_lsq"] += t1 - t0
+        res["t_learned"] += t3 - t2
+        res["node_steps"] += sum(p.N for p in probs) * steps
+        for i, r in enumerate(part):
+            res["lsq"].append(reflected(rl[i]))
+            res["learned"].append(reflected(rn[i]))
+            res["nu"].append(r.nu)
+            res["fallback_step"].append(int(fb[i]))
+        lq = np.array(res["lsq"][-len(part):])
+        ln = np.array(res["learned"][-len(part):])
+        log(f"eval runs {a + 1}-{a + len(part)}: reflected energy median LSQ {np.median(lq):.3e} "
+            f"learned {np.median(ln):.3e}; learned <= LSQ on {int(np.sum(ln <= lq))}/{len(part)}")
+    lq, ln = np.array(res["lsq"]), np.array(res["learned"])
+    res["summary"] = {
+        "runs": len(runs), "steps": steps,
+        "median_lsq": float(np.median(lq)), "median_learned": float(np.median(ln)),
+        "mean_lsq": float(np.mean(lq)), "mean_learned": float(np.mean(ln)),
+        "learned_le_lsq": int(np.sum(ln <= lq)),
+        "gate_lsq": int(np.sum(lq <= 1e-4)), "gate_learned": int(np.sum(ln <= 1e-4)),
+        "fallbacks": int(np.sum(np.array(res["fallback_step"]) >= 0)),
+        "overhead": res["t_learned"] / res["t_lsq"] - 1.0,
+    }
+    return res
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--train-runs", type=int, default=512)
+    ap.add_argument("--val-runs", type=int, default=32)
+    ap.add_argument("--eval-runs", type=int, default=64)
+    ap.add_argument("--eval-steps", type=int, default=6000)
+    ap.add_argument("--keep", type=float, default=0.01)
+    ap.add_argument("--sample-every", type=int, default=8)
+    ap.add_argument("--amp-min", type=float, default=1e-4)
+    ap.add_argument("--y-max", type=float, default=1.0)
+    ap.add_argument("--guard", type=float, default=1e-3)
+    ap.add_argument("--train-steps", type=int, default=20000)
+    ap.add_argument("--batch", type=int, default=4096)
+    ap.add_argument("--lr0", type=float, default=1e-3)
+    ap.add_argument("--lr1", type=float, default=1e-5)
+    ap.add_argument("--capacity", type=int, default=8_000_000)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--out", default="closure_r125.npz")
+    ap.add_argument("--report", default=None)
+    ap.add_argument("--load", default=None, help="skip training: load this parameter file")
+    ap.add_argument("--workers", type=int, default=None)
+    a = ap.parse_args(argv)
+    from .closure import Closure
+    report = {"args": vars(a)}
+    with Closure(seed=a.seed) as clo:
+        if a.load:
+            report["loaded"] = load_into(clo, a.load)
+        else:
+            train_runs = draw_runs(1000 + a.seed, a.train_runs)
+            val_runs = draw_runs(2000 + a.seed, a.val_runs)
+            clo.reserve(a.capacity)
+            t0 = time.time()
+            report["collect_train"] = collect(clo, train_runs, a.keep, a.sample_every, a.amp_min,
+                                              y_max=a.y_max, workers=a.workers)
+            n_train = min(clo.count(), a.capacity)
+            report["collect_val"] = collect(clo, val_runs, a.keep, a.sample_every, a.amp_min,
+                                            y_max=a.y_max, workers=a.workers)
+            n_all = min(clo.count(), a.capacity)
+            report["samples"] = {"train": n_train, "val": n_all - n_train,
+                                 "dropped": clo.count() - n_all}
+            report["collect_s"] = time.time() - t0
+            log(f"samples: train {n_train}, validation {n_all - n_train} "
+                f"({report['collect_s']:.0f} s)")
+            report["fit"] = fit(clo, n_train, n_all - n_train, a.train_steps, a.batch, a.lr0,
+                                a.lr1, a.seed)
+            save(clo, a.out, {"fit": report["fit"], "samples": report["samples"],
+                              "train_runs": a.train_runs, "seed": a.seed})
+            log("saved", a.out)
+        if a.eval_runs > 0:
+            ev = evaluate(clo, draw_runs(3000 + a.seed, a.eval_runs), a.eval_steps,
+                          workers=a.workers, guard=a.guard)
+            report["eval"] = ev
+            log("closed loop:", json.dumps(ev["summary"]))
+    if a.report:
+        with open(a.report, "w") as f:
+            json.dump(report, f, indent=1, default=float)
+    return report
+
+
+if __name__ == "__main__":
+    main()

@@ -63,6 +63,7 @@ SIGNATURES = {
     "lps_default_options": (None, [C.POINTER(LpsOptions)]),
     "lps_create": (C.c_int32, [C.POINTER(LpsProblem), C.POINTER(LpsOptions), C.POINTER(C.c_void_p)]),
     "lps_reset": (C.c_int32, [C.c_void_p]),
+    "lps_set_initial": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "lps_create_batch": (C.c_int32, [C.c_void_p, C.c_int32, C.POINTER(LpsOptions),
                                      C.POINTER(C.c_void_p)]),
     "lps_instance_count": (C.c_int32, [C.c_void_p]),
@@ -111,6 +112,7 @@ SIGNATURES = {
     "lps_closure_detach": (C.c_int32, [C.c_void_p]),
     "lps_closure_set_gain": (C.c_int32, [C.c_void_p, C.c_void_p]),
     "lps_closure_inspect": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "lps_closure_guard": (C.c_int32, [C.c_void_p, C.c_double, C.c_void_p]),
 }
 
 _lib = None
@@ -231,6 +233,13 @@ class Solver:
     def reset(self):
         self._check(self._L.lps_reset(self._h))
 
+    def set_initial(self, ic: np.ndarray, icd: Optional[np.ndarray] = None):
+        """New initial conditions (interleaved, problem node order), then back to step 0."""
+        ic = np.ascontiguousarray(ic, dtype=np.float64)
+        icd = np.zeros_like(ic) if icd is None else np.ascontiguousarray(icd, dtype=np.float64)
+        assert ic.size == 2 * self.N and icd.size == 2 * self.N
+        self._check(self._L.lps_set_initial(self._h, _ptr(ic), _ptr(icd)))
+
     def step(self, nsteps: int):
         self._check(self._L.lps_step(self._h, nsteps))
 
@@ -292,6 +301,14 @@ class Solver:
     def set_closure_gain(self, gain):
         g = np.ascontiguousarray(gain, dtype=np.float32)
         self._check(self._L.lps_closure_set_gain(self._h, _ptr(g)))
+
+    def closure_guard(self, tol: float = -1.0):
+        """Set the fallback guard's tolerance (tol >= 0) and return each instance's fallback step
+        (-1: the closure is still on)."""
+        n = getattr(self, "count", 1)
+        out = np.zeros(n, dtype=np.int64)
+        self._check(self._L.lps_closure_guard(self._h, float(tol), _ptr(out)))
+        return out
 
     def closure_inspect(self, n_rows: int):
         """Inputs (n_rows x 1600 BF16 bits), scales (n_rows x 2) and applied outputs (n_rows x 8)

@@ -242,13 +242,16 @@ def test_zero_correction_is_the_least_squares_abc_bitwise():
     say("closure with zero output vs plain LSQ")
     prob, _ = load_golden("abc_r10_nu025")
     c, _ = closure_with(20, 0.0)
-    with c, solver.Solver(prob) as a, solver.Solver(prob) as b:
+    # the same node kernels in both contexts (the autotuner may pick different ones per context)
+    with c, solver.Solver(prob, variant=3) as a, solver.Solver(prob, variant=3) as b:
         a.attach_closure(c)
         ra = a.run(120)
         rb = b.run(120)
         fa, fb = a.fetch(), b.fetch()
         for k in ("u", "v", "a", "theta"):
-            assert np.array_equal(fa[k], fb[k]), k
+            bad = np.nonzero(fa[k] != fb[k])[0]
+            assert bad.size == 0, (k, bad.size, bad[:8], np.max(np.abs(fa[k] - fb[k])),
+                                   np.isin(bad // (2 if k in "uva" else 1), prob.abc).sum())
         assert [r["energy"] for r in ra] == [r["energy"] for r in rb]
         a.reset()  # clears the history too
         a.run(30)
@@ -282,7 +285,8 @@ def test_applied_correction_is_the_network_output():
     say("closure applied correction")
     prob = with_wide_pulse(load_golden("abc_r10_nu025")[0])
     c, p = closure_with(22, 0.0)
-    with c, solver.Solver(prob, use_graph=False) as a, solver.Solver(prob, use_graph=False) as b:
+    with c, solver.Solver(prob, use_graph=False, variant=3) as a, \
+            solver.Solver(prob, use_graph=False, variant=3) as b:
         a.attach_closure(c)
         a.step(40)
         b.step(40)
@@ -327,7 +331,7 @@ def test_teacher_forced_samples():
             solver.Solver(truth, abc_enabled=False, use_graph=False) as t2:
         c.reserve(4 * trunc.C)
         got = c.collect(t1, trunc, n_steps, first_step=n_steps, sample_every=n_steps, keep=1.0,
-                        amp_ref=1e-3, amp_min=0.0, seed=5)
+                        amp_min=0.0, seed=5)
         assert got == trunc.C  # one sample per row at the single sampled step
         xs, aux = c.read(0, got)
         hist = [t2.fetch()]
@@ -367,4 +371,26 @@ def test_teacher_forced_samples():
         scale_u = np.abs(U1[2 * at:2 * at + 2]).max() / float(es[r, 0]) + 1.0
         assert np.allclose(aux[r, 0:2], eu, rtol=1e-5, atol=1e-9 * scale_u), (r, aux[r, :4], eu, ev)
         assert np.allclose(aux[r, 2:4], ev, rtol=1e-5, atol=1e-9 * scale_u), (r, aux[r, :4], eu, ev)
-        assert np.isclose(aux[r, 4], (float(es[r, 0]) / 1e-3) ** 2, rtol=1e-6)
+        assert aux[r, 4] == 1.0
+
+
+def test_wide_kernel_gives_the_same_bits(monkeypatch):
+    """The 128-row forward kernel (large row counts) and the 32-row kernel apply the same bits."""
+    say("closure wide vs narrow kernel")
+    prob = with_wide_pulse(load_golden("abc_r10_nu025")[0])
+    c, _ = closure_with(24, 0.0)
+    c.set_params(random_params(25, out_scale=0.05))
+    c.set_scales(1e-3, 1e-3)
+    out = []
+    with c:
+        for wide_min in ("0", "1000000000"):
+            monkeypatch.setenv("LPS_CLOSURE_WIDE_MIN", wide_min)
+            with solver.Solver(prob, variant=3) as s:
+                s.attach_closure(c)
+                s.run(60)
+                f = s.fetch()
+                out.append((f, s.closure_inspect(prob.C)[2]))
+    assert np.any(out[0][1][:, :4] != 0)
+    assert np.array_equal(out[0][1], out[1][1])
+    for k in ("u", "v", "a"):
+        assert np.array_equal(out[0][0][k], out[1][0][k]), k
