@@ -225,10 +225,12 @@ int32_t lps_closure_dataset_append(lps_closure *clo, const uint16_t *x, const fl
 int32_t lps_closure_dataset_read(lps_closure *clo, int64_t first, int64_t n, uint16_t *x,
                                  float *aux);
 /* nsteps Adam steps on batches of `batch` rows (a multiple of 64) drawn with replacement from
- * samples [first, first + count); the step size follows a cosine from lr0 to lr1. loss_curve
- * (nsteps floats, may be NULL) receives each step's batch loss. */
+ * samples [first, first + count); the step size follows a cosine from lr0 to lr1. input_noise > 0
+ * adds Gaussian noise of that standard deviation to the drawn history inputs (robustness to
+ * perturbed histories). loss_curve (nsteps floats, may be NULL) receives each step's batch loss. */
 int32_t lps_closure_train(lps_closure *clo, int32_t nsteps, int32_t batch, float lr0, float lr1,
-                          int64_t first, int64_t count, uint64_t seed, float *loss_curve);
+                          int64_t first, int64_t count, uint64_t seed, float input_noise,
+                          float *loss_curve);
 /* Over samples [first, first + count): out[0] = sum w |f - t/sigma|^2 (sigma units),
  * out[1] = sum w |t/sigma|^2 (the least-squares ABC alone), out[2] = sum w. */
 int32_t lps_closure_evaluate(lps_closure *clo, int64_t first, int64_t count, double *out3);
@@ -255,6 +257,10 @@ int32_t lps_closure_detach(lps_ctx *ctx);
 /* Factor on the correction per instance (count = lps_instance_count; 1 after attach; 0 gives the
  * least-squares ABC). */
 int32_t lps_closure_set_gain(lps_ctx *ctx, const float *gain);
+/* Amplitude gate per instance (count = lps_instance_count; 0 after attach): no correction at a
+ * row whose s_u (max |u| over its inputs) is below min_su, the range a closure was not trained on
+ * (lps_closure_collect's amp_min). */
+int32_t lps_closure_set_gate(lps_ctx *ctx, const float *min_su);
 /* Fallback guard. At every output record (every 30 steps, lps_run / lps_run_batch) an instance
  * whose energy rose by more than tol * E0 since the previous record is switched to the
  * least-squares ABC (gain 0) for the rest of the run: absorbing boundaries take energy out, they do

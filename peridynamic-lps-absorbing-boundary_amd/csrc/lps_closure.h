@@ -146,6 +146,7 @@ struct ApplyDst {
   const int32_t *slot;
   const int32_t *inst;
   const float *gain;
+  const float *gate;  // per instance: no correction where s_u < gate
   double *ux1, *uy1, *vx1, *vy1;
   float *dbg;  // n x 8 when not null: the 4 outputs, s_u, s_v, and 0, 0 (tests)
 };

@@ -26,6 +26,7 @@ struct ClosureAttach {
   float *feat = nullptr;         // n_rows x 4
   int32_t *row_slot = nullptr, *row_inst = nullptr;
   float *gain = nullptr;                            // n_inst
+  float *gate = nullptr;                            // n_inst: no correction where s_u < gate
   float *hist[2] = {nullptr, nullptr}, *mx[2] = {nullptr, nullptr};  // by step parity
   float *dbg = nullptr;                             // n_rows x 8, last applied correction
   // per-instance gains: as set by the caller, and as in force (the guard zeroes them)
@@ -99,7 +100,8 @@ hipError_t closure_step_hook(lps_ctx *c, const StepState &st, hipStream_t s) {
     if (e != hipSuccess) return e;
   }
   if (!A->clo || A->n_rows == 0) return hipSuccess;
-  const ApplyDst d{A->row_slot, A->row_inst, A->gain, st.ux1, st.uy1, st.vx1, st.vy1, A->dbg};
+  const ApplyDst d{A->row_slot, A->row_inst, A->gain, A->gate, st.ux1, st.uy1, st.vx1, st.vy1,
+                   A->dbg};
   return mlp_forward_apply(A->clo->dev(), A->src(par), A->n_rows, d, A->n_rows >= A->wide_min, s);
 }
 
@@ -282,6 +284,7 @@ static int32_t make_attach(lps_ctx *c, const Tables &t, lps_closure *clo,
   if (e == hipSuccess) e = A->upload(&A->row_slot, row_slot, s);
   if (e == hipSuccess) e = A->upload(&A->row_inst, row_inst, s);
   if (e == hipSuccess) e = A->upload(&A->gain, std::vector<float>(std::max(n_inst, 1), 1.0f), s);
+  if (e == hipSuccess) e = A->upload(&A->gate, std::vector<float>(std::max(n_inst, 1), 0.0f), s);
   for (int q = 0; q < 2 && e == hipSuccess; q++) {
     e = A->alloc(&A->hist[q], (size_t)std::max(A->n_hist, 1) * kClRec);
     if (e == hipSuccess) e = A->alloc(&A->mx[q], (size_t)std::max(A->n_hist, 1) * 2);
@@ -499,6 +502,14 @@ int32_t lps_closure_set_gain(lps_ctx *c, const float *gain) {
   A->gain_now = A->gain_user;
   A->off_step.assign(A->n_inst, -1);
   CTX_TRY(c, hipMemcpyAsync(A->gain, A->gain_now.data(), sizeof(float) * A->n_inst, hipMemcpyHostToDevice, c->stream));
+  CTX_TRY(c, hipStreamSynchronize(c->stream));
+  return LPS_OK;
+}
+
+int32_t lps_closure_set_gate(lps_ctx *c, const float *min_su) {
+  ClosureAttach *A = c->clo;
+  if (!A) return c->fail(LPS_ERR_ARG, "no closure attached");
+  CTX_TRY(c, hipMemcpyAsync(A->gate, min_su, sizeof(float) * A->n_inst, hipMemcpyHostToDevice, c->stream));
   CTX_TRY(c, hipStreamSynchronize(c->stream));
   return LPS_OK;
 }

@@ -79,7 +79,8 @@ __device__ __forceinline__ void dense_lds(const __bf16 (*hs)[kHsPitch], const __
 __device__ __forceinline__ void apply_row(const MlpDev &m, const GatherSrc &g, const ApplyDst &d,
                                           int32_t row, const float *y, float su, float sv) {
   const int f = g.xf[row];
-  const double gain = d.gain[d.inst[row]];
+  const int32_t in = d.inst[row];
+  const double gain = su >= d.gate[in] ? (double)d.gain[in] : 0.0;
   double dux, duy, dvx, dvy;
   from_frame(f, (double)y[0], (double)y[1], dux, duy);
   from_frame(f, (double)y[2], (double)y[3], dvx, dvy);
@@ -193,34 +194,46 @@ __global__ void __launch_bounds__(256) k_mlp_fwd(MlpDev m, const __bf16 *x, Gath
   }
 }
 
-// Fragments of K step k of a packed weight into LDS: 512 fragments, 2 per thread.
-__device__ __forceinline__ void stage_b(const __bf16 *wp, int k, bf16x8 (*bs)[64], int t) {
-  const bf16x8 *src = (const bf16x8 *)(wp + (int64_t)k * 4096);
-  bs[t >> 6][t & 63] = src[t];
-  bs[(t >> 6) + 4][t & 63] = src[t + 256];
-}
-
 // The network over 128 rows per workgroup: wavefront w owns rows 32 w .. 32 w + 31 and all 256
 // columns (8 accumulator tiles); each K step's weight fragments are staged once per workgroup in
-// LDS (double buffered). Same arithmetic order as k_mlp_fwd, so the same bits.
+// LDS (double buffered). The operands of K step k + kPf are loaded while step k computes (one
+// wavefront per SIMD: the prefetch hides the memory latency). Same arithmetic order as k_mlp_fwd,
+// so the same bits.
+constexpr int kPf = 4;
+
+struct RawG {  // 8 history values (4 (x, y) pairs) of one K step, FP32
+  float4 a, b;
+};
+
 template <bool kGather>
 __global__ void __launch_bounds__(256) k_mlp_fwd_wide(MlpDev m, const __bf16 *x, GatherSrc g,
                                                       int32_t n, float *yo, ApplyDst d) {
   __shared__ bf16x8 bs[2][8][64];
   __shared__ __bf16 hs[kWideRows][kHsPitch];
   __shared__ float sc[kWideRows][4];
+  // layer 1 keeps the rows' history indices where the activations go later
+  int32_t(*hi)[kClOffsets] = reinterpret_cast<int32_t(*)[kClOffsets]>(&hs[0][0]);
+  static_assert(sizeof(int32_t) * kWideRows * kClOffsets <= sizeof(__bf16) * kWideRows * kHsPitch,
+                "history indices fit in the activation buffer");
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   const int r0 = blockIdx.x * kWideRows;
+  const int ar = lane & 31, ak = 8 * (lane >> 5);
+  const int lrow = 32 * wv + ar;
+  const int32_t row = r0 + lrow;
+  const bool live = row < n;
   if constexpr (kGather) {
+    for (int q = t; q < kWideRows * kClOffsets; q += 256) {
+      const int rr = q / kClOffsets, jj = q % kClOffsets;
+      hi[rr][jj] = r0 + rr < n ? g.hidx[(int64_t)(r0 + rr) * kClOffsets + jj] : -1;
+    }
+    __syncthreads();
     const int r = t >> 1, j = t & 1;
     float su = 0.0f, sv = 0.0f;
-    if (r0 + r < n) {
-      for (int q = j; q < kClOffsets; q += 2) {
-        const int32_t h = g.hidx[(int64_t)(r0 + r) * kClOffsets + q];
-        if (h >= 0) {
-          su = fmaxf(su, g.mx[2 * (int64_t)h]);
-          sv = fmaxf(sv, g.mx[2 * (int64_t)h + 1]);
-        }
+    for (int q = j; q < kClOffsets; q += 2) {
+      const int32_t h = hi[r][q];
+      if (h >= 0) {
+        su = fmaxf(su, g.mx[2 * (int64_t)h]);
+        sv = fmaxf(sv, g.mx[2 * (int64_t)h + 1]);
       }
     }
     su = fmaxf(su, __shfl_xor(su, 1));
@@ -231,45 +244,92 @@ __global__ void __launch_bounds__(256) k_mlp_fwd_wide(MlpDev m, const __bf16 *x,
       sc[r][2] = su;
       sc[r][3] = sv;
     }
+    __syncthreads();
   }
-  const int ar = lane & 31, ak = 8 * (lane >> 5);
-  const int lrow = 32 * wv + ar;  // this lane's A row inside the workgroup
+  float iu = 0.0f, iv = 0.0f;
+  int f = 0;
+  if constexpr (kGather) {
+    iu = sc[lrow][0];
+    iv = sc[lrow][1];
+    f = live ? g.xf[row] : 0;
+  }
+  // loads of layer-1 inputs: history record (gathered) or the input row
+  auto issue_a = [&](int k, RawG &raw, bf16x8 &xr) {
+    if constexpr (kGather) {
+      raw.a = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      raw.b = raw.a;
+      const int j = k >> 1;
+      if (live && j < kClOffsets) {
+        const int32_t h = hi[lrow][j];
+        if (h >= 0) {
+          const float4 *q = (const float4 *)(g.hist + (int64_t)h * kClRec + 16 * (k & 1) + ak);
+          raw.a = q[0];
+          raw.b = q[1];
+        }
+      }
+    } else {
+      xr = live ? *(const bf16x8 *)(x + (int64_t)row * kClIn + 16 * k + ak) : zero8();
+    }
+  };
+  auto make_a = [&](int k, const RawG &raw, const bf16x8 &xr) -> bf16x8 {
+    if constexpr (kGather) {
+      bf16x8 out;
+      if ((k >> 1) == kClOffsets) {  // row features
+#pragma unroll
+        for (int e = 0; e < 8; e++) out[e] = (__bf16)0.0f;
+        if (live && 16 * (k & 1) + ak == 0) {
+#pragma unroll
+          for (int e = 0; e < kClFeat; e++) out[e] = (__bf16)g.feat[(int64_t)row * kClFeat + e];
+        }
+        return out;
+      }
+      const float inv = (k & 1) ? iv : iu;
+      float cx, cy;
+      to_frame(f, raw.a.x, raw.a.y, cx, cy); out[0] = (__bf16)(cx * inv); out[1] = (__bf16)(cy * inv);
+      to_frame(f, raw.a.z, raw.a.w, cx, cy); out[2] = (__bf16)(cx * inv); out[3] = (__bf16)(cy * inv);
+      to_frame(f, raw.b.x, raw.b.y, cx, cy); out[4] = (__bf16)(cx * inv); out[5] = (__bf16)(cy * inv);
+      to_frame(f, raw.b.z, raw.b.w, cx, cy); out[6] = (__bf16)(cx * inv); out[7] = (__bf16)(cy * inv);
+      return out;
+    } else {
+      return xr;
+    }
+  };
   f32x16 acc[8];
   for (int layer = 0; layer < 3; layer++) {
     const __bf16 *wp = layer == 0 ? m.w1p : layer == 1 ? m.w2p : m.w3p;
-    const int nk = (layer == 0 ? kClIn : kClHid) / 16;
+    const int nk = (layer == 0 ? kClIn : kClHid) / 16;  // multiples of kPf
 #pragma unroll
     for (int q = 0; q < 8; q++) acc[q] = f32x16{};
-    stage_b(wp, 0, bs[0], t);
-    __syncthreads();
-    float iu = 0.0f, iv = 0.0f;
-    if constexpr (kGather) {
-      iu = sc[lrow][0];
-      iv = sc[lrow][1];
-    }
-    const int32_t row = r0 + lrow;
-    const bool live = row < n;
-    for (int k = 0; k < nk; k++) {
-      bf16x8 nb0, nb1;
-      const bool more = k + 1 < nk;
-      if (more) {
-        const bf16x8 *src = (const bf16x8 *)(wp + (int64_t)(k + 1) * 4096);
-        nb0 = src[t];
-        nb1 = src[t + 256];
-      }
-      bf16x8 a;
-      if (layer > 0) a = *(const bf16x8 *)&hs[lrow][16 * k + ak];
-      else if (!live) a = zero8();
-      else if constexpr (kGather) a = gather8(g, row, 16 * k + ak, iu, iv);
-      else a = *(const bf16x8 *)(x + (int64_t)row * kClIn + 16 * k + ak);
+    RawG ra[kPf];
+    bf16x8 xa[kPf], rb0[kPf], rb1[kPf];
 #pragma unroll
-      for (int q = 0; q < 8; q++) acc[q] = mfma(a, bs[k & 1][q][lane], acc[q]);
-      if (more) {
-        bs[(k + 1) & 1][t >> 6][t & 63] = nb0;
-        bs[(k + 1) & 1][(t >> 6) + 4][t & 63] = nb1;
-      }
-      __syncthreads();
+    for (int dd = 0; dd < kPf; dd++) {
+      const bf16x8 *src = (const bf16x8 *)(wp + (int64_t)dd * 4096);
+      rb0[dd] = src[t];
+      rb1[dd] = src[t + 256];
+      if (layer == 0) issue_a(dd, ra[dd], xa[dd]);
     }
+    for (int k0 = 0; k0 < nk; k0 += kPf) {
+#pragma unroll
+      for (int dd = 0; dd < kPf; dd++) {
+        const int k = k0 + dd;
+        bs[k & 1][t >> 6][t & 63] = rb0[dd];
+        bs[k & 1][(t >> 6) + 4][t & 63] = rb1[dd];
+        bf16x8 a;
+        if (layer == 0) a = make_a(k, ra[dd], xa[dd]);
+        else a = *(const bf16x8 *)&hs[lrow][16 * k + ak];
+        if (k + kPf < nk) {
+          const bf16x8 *src = (const bf16x8 *)(wp + (int64_t)(k + kPf) * 4096);
+          rb0[dd] = src[t];
+          rb1[dd] = src[t + 256];
+          if (layer == 0) issue_a(k + kPf, ra[dd], xa[dd]);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 8; q++) acc[q] = mfma(a, bs[k & 1][q][lane], acc[q]);
+      }
+    }
+    __syncthreads();  // every wavefront is done with hs (history indices or the layer's input)
     // bias + ReLU into LDS (this wavefront's own rows)
     const float *bias = m.p + (layer == 0 ? kOffB1 : layer == 1 ? kOffB2 : kOffB3);
 #pragma unroll
@@ -306,17 +366,17 @@ __global__ void __launch_bounds__(256) k_mlp_fwd_wide(MlpDev m, const __bf16 *x,
       y[c] += __shfl_xor(y[c], 1);
     }
   }
-  const int32_t row = r0 + r;
-  if (j != 0 || row >= n) return;
+  const int32_t orow = r0 + r;
+  if (j != 0 || orow >= n) return;
 #pragma unroll
   for (int c = 0; c < kClOut; c++) y[c] += m.p[kOffB4 + c];
   if constexpr (!kGather) {
     if (yo) {
 #pragma unroll
-      for (int c = 0; c < kClOut; c++) yo[(int64_t)row * kClOut + c] = y[c];
+      for (int c = 0; c < kClOut; c++) yo[(int64_t)orow * kClOut + c] = y[c];
     }
   } else {
-    apply_row(m, g, d, row, y, sc[r][2], sc[r][3]);
+    apply_row(m, g, d, orow, y, sc[r][2], sc[r][3]);
   }
 }
 
@@ -503,16 +563,33 @@ __global__ void __launch_bounds__(256) k_adam(float *p, const float *g, float *m
 }
 
 // Batch rows drawn with replacement from samples [first, first + count).
+// noise > 0: Gaussian noise of that standard deviation added to the history inputs (the first
+// 49 x 32), so the network learns to be insensitive to perturbed histories.
 __global__ void __launch_bounds__(256) k_draw(const __bf16 *dsx, const float *dsaux, int64_t first,
                                               int64_t count, uint64_t seed, int32_t n, __bf16 *xb,
-                                              float *tgt, float *w) {
+                                              float *tgt, float *w, float noise) {
   const int lane = threadIdx.x & 63;
   const int32_t i = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (i >= n) return;
-  const int64_t src = first + (int64_t)(dev_mix(seed ^ (uint64_t)i * 0x2545F4914F6CDD1Dull) % (uint64_t)count);
+  const uint64_t hi = dev_mix(seed ^ (uint64_t)i * 0x2545F4914F6CDD1Dull);
+  const int64_t src = first + (int64_t)(hi % (uint64_t)count);
   const bf16x8 *a = (const bf16x8 *)(dsx + src * kClIn);
   bf16x8 *b = (bf16x8 *)(xb + (int64_t)i * kClIn);
-  for (int q = lane; q < kClIn / 8; q += 64) b[q] = a[q];
+  for (int q = lane; q < kClIn / 8; q += 64) {
+    bf16x8 v = a[q];
+    if (noise > 0.0f && q < kClOffsets * 4) {
+#pragma unroll
+      for (int e = 0; e < 8; e += 2) {
+        const uint64_t h = dev_mix(hi ^ ((uint64_t)(8 * q + e) << 40));
+        const float u1 = ((float)(h >> 40) + 0.5f) * (1.0f / 16777216.0f);
+        const float u2 = (float)((h >> 16) & 0xFFFFFF) * (1.0f / 16777216.0f);
+        const float rr = noise * sqrtf(-2.0f * logf(u1));
+        v[e] = (__bf16)((float)v[e] + rr * cosf(6.2831853f * u2));
+        v[e + 1] = (__bf16)((float)v[e + 1] + rr * sinf(6.2831853f * u2));
+      }
+    }
+    b[q] = v;
+  }
   if (lane < kClOut) tgt[(int64_t)i * kClOut + lane] = dsaux[src * 8 + lane];
   if (lane == kClOut) w[i] = dsaux[src * 8 + kClOut];
 }
@@ -856,7 +933,8 @@ int32_t lps_closure_dataset_read(lps_closure *c, int64_t first, int64_t n, uint1
 }
 
 int32_t lps_closure_train(lps_closure *c, int32_t nsteps, int32_t batch, float lr0, float lr1,
-                          int64_t first, int64_t count, uint64_t seed, float *loss_curve) {
+                          int64_t first, int64_t count, uint64_t seed, float input_noise,
+                          float *loss_curve) {
   const int64_t have = std::min<int64_t>(lps_closure_dataset_count(c), c->ds_cap);
   if (nsteps < 0 || count <= 0 || first < 0 || first + count > have)
     return c->fail(LPS_ERR_ARG, "training range out of bounds");
@@ -870,7 +948,7 @@ int32_t lps_closure_train(lps_closure *c, int32_t nsteps, int32_t batch, float l
     const float lr = (float)(lr1 + 0.5 * (lr0 - lr1) * (1.0 + std::cos(3.141592653589793 * f)));
     const uint64_t sd = splitmix64(seed ^ splitmix64((uint64_t)it + 1));
     hipLaunchKernelGGL(k_draw, blocks(batch, 4), dim3(256), 0, c->stream, c->ds_x, c->ds_aux, first,
-                       count, sd, batch, c->xb, c->tgt, c->wgt);
+                       count, sd, batch, c->xb, c->tgt, c->wgt, input_noise);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) rc = c->hip_fail(e, "k_draw");
     else rc = train_step(c, batch, lr, curve + it);

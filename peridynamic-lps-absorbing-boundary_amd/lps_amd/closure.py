@@ -124,10 +124,10 @@ class Closure:
         return x, a
 
     def train(self, nsteps: int, batch: int, lr0: float, lr1: float, first: int, count: int,
-              seed: int = 0) -> np.ndarray:
+              seed: int = 0, input_noise: float = 0.0) -> np.ndarray:
         curve = np.zeros(max(nsteps, 1), dtype=np.float32)
         self._check(self._L.lps_closure_train(self._h, nsteps, batch, lr0, lr1, first, count,
-                                              seed, _ptr(curve)))
+                                              seed, float(input_noise), _ptr(curve)))
         return curve[:nsteps]
 
     def evaluate(self, first: int, count: int):

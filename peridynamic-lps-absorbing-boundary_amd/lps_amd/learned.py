@@ -97,7 +97,7 @@ def truncated_problem(nu: float, workers: Optional[int] = None) -> pp.Problem:
     if nu not in _TRUNC_CACHE:
         dt, dk = configs.sweep_pair(nu)
         _TRUNC_CACHE[nu] = pp.make_problem(R_TRUNC, nu, pulses=[], steps=1, dt=dt, dk_factor=dk,
-                                           workers=workers)
+                                           workers=workers, robust_ls=True)
     return _TRUNC_CACHE[nu]
 
 
@@ -163,14 +163,14 @@ def target_scales(clo, first: int, count: int, chunk: int = 65536, pieces: int =
 
 
 def fit(clo, n_train: int, n_val: int, steps: int, batch: int, lr0: float, lr1: float,
-        seed: int = 0) -> dict:
+        seed: int = 0, noise: float = 0.0) -> dict:
     s_u, s_v = target_scales(clo, 0, n_train)
     clo.set_scales(s_u, s_v)
     log(f"target scales sigma_u {s_u:.4g} sigma_v {s_v:.4g}")
     t0 = time.time()
-    curve = clo.train(steps, batch, lr0, lr1, 0, n_train, seed=seed)
+    curve = clo.train(steps, batch, lr0, lr1, 0, n_train, seed=seed, input_noise=noise)
     t_train = time.time() - t0
-    out = {"sigma": [s_u, s_v], "steps": steps, "batch": batch, "train_s": t_train,
+    out = {"sigma": [s_u, s_v], "steps": steps, "batch": batch, "noise": noise, "train_s": t_train,
            "loss_first": float(np.mean(curve[:50])), "loss_last": float(np.mean(curve[-50:]))}
     if n_val > 0:
         l, b, _ = clo.evaluate(n_train, n_val)
@@ -203,13 +203,20 @@ def reflected(recs) -> float:
     return 1.0 + recs[-1]["de_rel"]
 
 
+def energy_rise(recs) -> float:
+    """Largest rise of the energy between consecutive output records, in units of E0."""
+    e = np.array([r["energy"] for r in recs])
+    return float(np.max(np.diff(e)) / e[0]) if len(e) > 1 and e[0] > 0 else 0.0
+
+
 def evaluate(clo, runs: List[Run], steps: int, chunk: int = 64, device: int = 0,
-             workers: Optional[int] = None, guard: float = 1e-3) -> dict:
+             workers: Optional[int] = None, guard: float = 1e-3, amp_min_rel: float = 0.0,
+             gain: float = 1.0) -> dict:
     """Every run on the truncated domain with the least-squares ABC and with the closure (same
     batch, reset between); per-instance reflected energy and the time of both loops."""
     from .solver import Batch
-    res = {"lsq": [], "learned": [], "nu": [], "fallback_step": [], "t_lsq": 0.0,
-           "t_learned": 0.0, "node_steps": 0}
+    res = {"lsq": [], "learned": [], "nu": [], "fallback_step": [], "lsq_rise": [],
+           "learned_rise": [], "t_lsq": 0.0, "t_learned": 0.0, "node_steps": 0, "gain": gain}
     for a in range(0, len(runs), chunk):
         part = runs[a:a + chunk]
         probs = [run_problem(r, steps, workers) for r in part]
@@ -220,6 +227,8 @@ def evaluate(clo, runs: List[Run], steps: int, chunk: int = 64, device: int = 0,
             b.reset()
             b.attach_closure(clo)
             b.closure_guard(guard)
+            b.set_closure_gain([gain] * len(probs))
+            b.set_closure_gate([amp_min_rel * float(np.max(np.abs(p.ic))) for p in probs])
             t2 = time.time()
             rn = b.run(steps)
             t3 = time.time()
@@ -232,6 +241,8 @@ def evaluate(clo, runs: List[Run], steps: int, chunk: int = 64, device: int = 0,
             res["learned"].append(reflected(rn[i]))
             res["nu"].append(r.nu)
             res["fallback_step"].append(int(fb[i]))
+            res["lsq_rise"].append(energy_rise(rl[i]))
+            res["learned_rise"].append(energy_rise(rn[i]))
         lq = np.array(res["lsq"][-len(part):])
         ln = np.array(res["learned"][-len(part):])
         log(f"eval runs {a + 1}-{a + len(part)}: reflected energy median LSQ {np.median(lq):.3e} "
@@ -244,9 +255,19 @@ def evaluate(clo, runs: List[Run], steps: int, chunk: int = 64, device: int = 0,
         "learned_le_lsq": int(np.sum(ln <= lq)),
         "gate_lsq": int(np.sum(lq <= 1e-4)), "gate_learned": int(np.sum(ln <= 1e-4)),
         "fallbacks": int(np.sum(np.array(res["fallback_step"]) >= 0)),
+        "lsq_max_rise": float(np.max(res["lsq_rise"])),
         "overhead": res["t_learned"] / res["t_lsq"] - 1.0,
     }
     return res
+
+
+def run_eval(clo, a, report, tag):
+    for gain in a.gains:
+        for gate in (a.gates or [a.amp_min]):
+            ev = evaluate(clo, draw_runs(3000 + a.seed, a.eval_runs), a.eval_steps,
+                          workers=a.workers, guard=a.guard, amp_min_rel=gate, gain=gain)
+            report[f"eval_{tag}_gain{gain:g}_gate{gate:g}"] = ev
+            log(f"closed loop, {tag}, gain {gain:g}, gate {gate:g}:", json.dumps(ev["summary"]))
 
 
 def main(argv=None):
@@ -260,6 +281,12 @@ def main(argv=None):
     ap.add_argument("--amp-min", type=float, default=1e-4)
     ap.add_argument("--y-max", type=float, default=1.0)
     ap.add_argument("--guard", type=float, default=1e-3)
+    ap.add_argument("--gains", type=float, nargs="+", default=[1.0])
+    ap.add_argument("--gates", type=float, nargs="+", default=None,
+                    help="amplitude gates (relative to the IC amplitude) of the evaluation; "
+                         "default the training's --amp-min")
+    ap.add_argument("--noise", type=float, nargs="+", default=[0.0],
+                    help="input-noise levels; one network trained and evaluated per level")
     ap.add_argument("--train-steps", type=int, default=20000)
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--lr0", type=float, default=1e-3)
@@ -276,6 +303,8 @@ def main(argv=None):
     with Closure(seed=a.seed) as clo:
         if a.load:
             report["loaded"] = load_into(clo, a.load)
+            if a.eval_runs > 0:
+                run_eval(clo, a, report, "loaded")
         else:
             train_runs = draw_runs(1000 + a.seed, a.train_runs)
             val_runs = draw_runs(2000 + a.seed, a.val_runs)
@@ -292,16 +321,18 @@ def main(argv=None):
             report["collect_s"] = time.time() - t0
             log(f"samples: train {n_train}, validation {n_all - n_train} "
                 f"({report['collect_s']:.0f} s)")
-            report["fit"] = fit(clo, n_train, n_all - n_train, a.train_steps, a.batch, a.lr0,
-                                a.lr1, a.seed)
-            save(clo, a.out, {"fit": report["fit"], "samples": report["samples"],
-                              "train_runs": a.train_runs, "seed": a.seed})
-            log("saved", a.out)
-        if a.eval_runs > 0:
-            ev = evaluate(clo, draw_runs(3000 + a.seed, a.eval_runs), a.eval_steps,
-                          workers=a.workers, guard=a.guard)
-            report["eval"] = ev
-            log("closed loop:", json.dumps(ev["summary"]))
+            init = clo.get_params()
+            for noise in a.noise:
+                clo.set_params(init)
+                tag = f"noise{noise:g}"
+                report["fit_" + tag] = fit(clo, n_train, n_all - n_train, a.train_steps, a.batch,
+                                           a.lr0, a.lr1, a.seed, noise)
+                out = a.out if len(a.noise) == 1 else a.out.replace(".npz", f"_{tag}.npz")
+                save(clo, out, {"fit": report["fit_" + tag], "samples": report["samples"],
+                                "train_runs": a.train_runs, "seed": a.seed})
+                log("saved", out)
+                if a.eval_runs > 0:
+                    run_eval(clo, a, report, tag)
     if a.report:
         with open(a.report, "w") as f:
             json.dump(report, f, indent=1, default=float)

@@ -256,7 +256,19 @@ def _dgelsd(AT: np.ndarray, B: np.ndarray, rcond: float = SVD_RCOND) -> np.ndarr
         # 769-772); the stability sweep then judges the resulting operator by its norm
         import warnings
         warnings.warn(f"dgelsd did not converge (info={info})")
+        if _ROBUST_LS[0]:
+            # same minimum-norm solution from the QR-iteration SVD (dgelss)
+            Bq = np.zeros((ldb, nrhs), dtype=np.float64, order="F")
+            Bq[:m] = B
+            out = lapack.dgelss(np.asfortranarray(AT), Bq, cond=rcond)
+            if out[-1] == 0:
+                return out[1][:n]
     return x[:n]
+
+
+# make_problem(robust_ls=True): a dgelsd that does not converge is solved again with dgelss
+# instead of keeping dgelsd's output (a deviation from the reference, for generated workloads).
+_ROBUST_LS = [False]
 
 
 def _local_frame(cloud_xy: np.ndarray, ax: float, ay: float, tau: float):
@@ -414,7 +426,12 @@ def abc_clouds(nodes: np.ndarray, abc_idx: np.ndarray, radius: float) -> List[np
 @dataclass
 class Pulse:
     """u = amp * dir * exp(-|x-c|^2 / (r^2/16)) for |x-c| < r. The reference pulse
-    (preprocessor/main.cpp:1280-1289) is Pulse(0, 0, 40, 1, (1, 1))."""
+    (preprocessor/main.cpp:1280-1289) is Pulse(0, 0, 40, 1, (1, 1)).
+
+    polarization "P": u = amp * (x - c) / a * exp(1/2 - |x-c|^2 / (2 a^2)), a = r / (4 sqrt 2): the
+    gradient of the Gaussian (curl-free, a pure dilatational pulse), |u| peaking at amp at
+    |x - c| = a. "S": the same rotated by 90 degrees (divergence-free, a pure shear pulse). Both are
+    smooth at the centre."""
     cx: float
     cy: float
     radius: float
@@ -435,9 +452,9 @@ def gaussian_ic(nodes: np.ndarray, pulses: Sequence[Pulse]) -> np.ndarray:
             ex = np.full(val.shape, p.direction[0])
             ey = np.full(val.shape, p.direction[1])
         else:
-            rr = np.sqrt(d2[inside])
-            rr[rr == 0.0] = 1.0
-            ex, ey = dxp[inside] / rr, dyp[inside] / rr
+            a = p.radius / (4.0 * math.sqrt(2.0))
+            k = math.exp(0.5) / a
+            ex, ey = k * dxp[inside], k * dyp[inside]
             if p.polarization == "S":
                 ex, ey = -ey, ex
         u[0::2][inside] += p.amp * ex * val
@@ -543,13 +560,22 @@ def make_problem(R: float, nu: float = MAT_NU, pulses: Optional[Sequence[Pulse]]
                  steps: Optional[int] = None, dt: Optional[float] = None,
                  dx: float = DX_DEFAULT, delta: Optional[float] = None,
                  workers: Optional[int] = None, with_abc: bool = True,
-                 dk_factor: Optional[float] = None) -> Problem:
+                 dk_factor: Optional[float] = None, robust_ls: bool = False) -> Problem:
     """Generate a full solver input set for a circular lattice domain of radius R.
 
     dt=None runs the reference's (dt, delta_kappa) sweep (preprocessor/main.cpp:1047-1174);
     otherwise dt and dk_factor (delta_kappa = dk_factor / sqrt(c_P), default 1) are taken as given,
     e.g. a sweep result computed once. steps=None takes ceil(210 us / dt)
-    (preprocessor/main.cpp:1177-1179)."""
+    (preprocessor/main.cpp:1177-1179). robust_ls=True re-solves a least-squares problem on which
+    dgelsd does not converge with dgelss (the reference keeps dgelsd's output)."""
+    _ROBUST_LS[0] = robust_ls
+    try:
+        return _make_problem(R, nu, pulses, steps, dt, dx, delta, workers, with_abc, dk_factor)
+    finally:
+        _ROBUST_LS[0] = False
+
+
+def _make_problem(R, nu, pulses, steps, dt, dx, delta, workers, with_abc, dk_factor) -> Problem:
     delta = 4.0 * dx if delta is None else delta
     workers = default_workers() if workers is None else workers
     mc = material(nu=nu, delta=delta, dx=dx, dy=dx)

@@ -103,7 +103,7 @@ SIGNATURES = {
     "lps_closure_dataset_read": (C.c_int32, [C.c_void_p, C.c_int64, C.c_int64, C.c_void_p,
                                              C.c_void_p]),
     "lps_closure_train": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, C.c_float, C.c_float,
-                                      C.c_int64, C.c_int64, C.c_uint64, C.c_void_p]),
+                                      C.c_int64, C.c_int64, C.c_uint64, C.c_float, C.c_void_p]),
     "lps_closure_evaluate": (C.c_int32, [C.c_void_p, C.c_int64, C.c_int64, C.c_void_p]),
     "lps_closure_collect": (C.c_int32, [C.c_void_p, C.c_void_p, C.POINTER(LpsProblem), C.c_int32,
                                         C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double,
@@ -113,6 +113,7 @@ SIGNATURES = {
     "lps_closure_set_gain": (C.c_int32, [C.c_void_p, C.c_void_p]),
     "lps_closure_inspect": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "lps_closure_guard": (C.c_int32, [C.c_void_p, C.c_double, C.c_void_p]),
+    "lps_closure_set_gate": (C.c_int32, [C.c_void_p, C.c_void_p]),
 }
 
 _lib = None
@@ -301,6 +302,10 @@ class Solver:
     def set_closure_gain(self, gain):
         g = np.ascontiguousarray(gain, dtype=np.float32)
         self._check(self._L.lps_closure_set_gain(self._h, _ptr(g)))
+
+    def set_closure_gate(self, min_su):
+        g = np.ascontiguousarray(min_su, dtype=np.float32)
+        self._check(self._L.lps_closure_set_gate(self._h, _ptr(g)))
 
     def closure_guard(self, tol: float = -1.0):
         """Set the fallback guard's tolerance (tol >= 0) and return each instance's fallback step
