@@ -65,7 +65,9 @@ typedef struct lps_options {
                             2 unrolled 68-bond stencil, 3 register-blocked 68-bond stencil (2 and
                             3 when the 68-bond stencil applies, else generic) */
   int32_t concurrent;    /* fast path: 1 runs the ABC rows' kernels (G_theta, G_u, G_v) on a
-                            second stream beside the tile kernels; 0 on the one stream */
+                            second stream beside the tile kernels; 2 also the partial tiles'
+                            kernels on a third stream beside the full tiles'; 0 on the one
+                            stream */
 } lps_options;
 
 /* One output record: the quantities src/main.cpp:986-1062 computes every 30 steps. */

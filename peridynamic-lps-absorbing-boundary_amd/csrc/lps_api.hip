@@ -141,6 +141,10 @@ void lps_destroy(lps_ctx *ctx) {
     if (ctx->ev_join[q]) (void)hipEventDestroy(ctx->ev_join[q]);
   }
   if (ctx->stream2) (void)hipStreamDestroy(ctx->stream2);
+  if (ctx->stream3) (void)hipStreamSynchronize(ctx->stream3);
+  for (hipEvent_t ev : ctx->ev_part)
+    if (ev) (void)hipEventDestroy(ev);
+  if (ctx->stream3) (void)hipStreamDestroy(ctx->stream3);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -316,6 +320,8 @@ static int32_t create_impl(lps_ctx *c, const lps_problem *p) {
   InstParams ip{};
   ip.dt = c->dt; ip.rho = c->rho; ip.lam = c->lam; ip.mu = c->mu; ip.kappa = c->kappa;
   ip.exclude_r = c->exclude_r;
+  ip.th_scale = (c->lam - c->mu) / c->mu;
+  ip.f_scale = c->mu;
 
   // ---- tile path: the stencil of a full horizon and the check against the CSR ----
   c->fast = false;
@@ -395,11 +401,8 @@ static int32_t create_impl(lps_ctx *c, const lps_problem *p) {
             if (aa * aa + bb * bb == kStencilR2[q]) { ra = aa; rb = bb; }
         const double ms[2] = {(double)ra, (double)rb};
         const double ps[3] = {(double)(ra * ra), (double)(ra * rb), (double)(rb * rb)};
-        for (int32_t z = 0; z < 2; z++) {
-          c->tile.wt[q][z] = C * c->tile.c1[q] * ms[z];
-          ip.wa[q][z] = (c->lam - c->mu) * C * c->tile.c1[q] * ms[z];
-        }
-        for (int32_t z = 0; z < 3; z++) ip.wb[q][z] = 8.0 * c->mu * C * c->tile.c2[q] * ps[z];
+        for (int32_t z = 0; z < 2; z++) c->tile.wt[q][z] = C * c->tile.c1[q] * ms[z];
+        for (int32_t z = 0; z < 3; z++) c->tile.wb[q][z] = 8.0 * C * c->tile.c2[q] * ps[z];
       }
     }
     std::vector<int32_t> tinst(c->tile.n_tiles, 0);
@@ -632,6 +635,9 @@ static int32_t new_context(const lps_options *options, lps_ctx **out) {
   // both streams at the default priority
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream3, hipStreamNonBlocking);
+  for (int q = 0; q < 3; q++)
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_part[q], hipEventDisableTiming);
   for (int q = 0; q < 2; q++) {
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_fork[q], hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_join[q], hipEventDisableTiming);
@@ -946,20 +952,30 @@ static int32_t choose_variant(lps_ctx *c) {
   } else if (c->opt.variant == 3) {
     th = fo = kBlocked;
   } else {
+    // two rounds of 3 warm-up and 10 timed steps per family, the minimum of the rounds; the
+    // dilatation family by its pass, the force family by the rest of the step (with the partial
+    // tiles on a third stream the force's share moves between the timed intervals)
     double best_th = 1e300, best_fo = 1e300;
+    double tth[3] = {1e300, 1e300, 1e300}, tfo[3] = {1e300, 1e300, 1e300};
+    for (int round = 0; round < 2; round++) {
+      for (int32_t m : {kGeneric, kUnrolled, kBlocked}) {
+        double t[4] = {0, 0, 0, 0};
+        c->tile.theta_kernel = c->tile.force_kernel = m;
+        int32_t rc = lps_set_timing(c, 1);
+        if (rc == LPS_OK) rc = lps_step(c, 3);
+        if (rc == LPS_OK) rc = lps_set_timing(c, 1);
+        if (rc == LPS_OK) rc = lps_step(c, 10);
+        if (rc == LPS_OK) rc = lps_pass_times(c, t);
+        if (rc == LPS_OK) rc = lps_set_timing(c, 0);
+        if (rc == LPS_OK) rc = reset_state(c);
+        if (rc != LPS_OK) return rc;
+        tth[m] = std::min(tth[m], t[0]);
+        tfo[m] = std::min(tfo[m], t[3] - t[0]);
+      }
+    }
     for (int32_t m : {kGeneric, kUnrolled, kBlocked}) {
-      double t[4] = {0, 0, 0, 0};
-      c->tile.theta_kernel = c->tile.force_kernel = m;
-      int32_t rc = lps_set_timing(c, 1);
-      if (rc == LPS_OK) rc = lps_step(c, 2);
-      if (rc == LPS_OK) rc = lps_set_timing(c, 1);
-      if (rc == LPS_OK) rc = lps_step(c, 6);
-      if (rc == LPS_OK) rc = lps_pass_times(c, t);
-      if (rc == LPS_OK) rc = lps_set_timing(c, 0);
-      if (rc == LPS_OK) rc = reset_state(c);
-      if (rc != LPS_OK) return rc;
-      if (t[0] < best_th) { best_th = t[0]; th = m; }
-      if (t[1] + t[2] < best_fo) { best_fo = t[1] + t[2]; fo = m; }
+      if (tth[m] < best_th) { best_th = tth[m]; th = m; }
+      if (tfo[m] < best_fo) { best_fo = tfo[m]; fo = m; }
     }
   }
   c->tile.theta_kernel = th;
@@ -987,8 +1003,18 @@ static hipError_t step_once(lps_ctx *c, bool record_events) {
   // ABC prediction for G_u / G_v (ev_join[1])
   const bool abc = c->fast && a.n_abc > 0;
   const bool fork = abc && c->opt.concurrent;
-  if (fork) {
+  // concurrent == 2: the partial tiles' kernels on a third stream, beside the full tiles' kernels
+  const bool split = c->fast && c->opt.concurrent == 2 && a.tile.n_part > 0 &&
+                     a.tile.theta_kernel == kBlocked && a.tile.force_kernel != kGeneric;
+  if (fork || split) {
     if ((e = hipEventRecord(c->ev_fork[0], c->stream)) != hipSuccess) return e;
+  }
+  if (split) {
+    if ((e = hipStreamWaitEvent(c->stream3, c->ev_fork[0], 0)) != hipSuccess) return e;
+    if ((e = launch_theta_part(a, st, c->stream3)) != hipSuccess) return e;
+    if ((e = hipEventRecord(c->ev_part[0], c->stream3)) != hipSuccess) return e;
+  }
+  if (fork) {
     if ((e = hipStreamWaitEvent(c->stream2, c->ev_fork[0], 0)) != hipSuccess) return e;
     if ((e = launch_abc_theta(a, st, c->stream2)) != hipSuccess) return e;
     if ((e = hipEventRecord(c->ev_join[0], c->stream2)) != hipSuccess) return e;
@@ -998,15 +1024,23 @@ static hipError_t step_once(lps_ctx *c, bool record_events) {
   } else if (abc && (e = launch_abc_theta(a, st, c->stream)) != hipSuccess) {
     return e;
   }
-  if ((e = launch_theta(a, st, c->fast, c->stream)) != hipSuccess) return e;
+  if ((e = launch_theta(a, st, c->fast, c->stream, !split)) != hipSuccess) return e;
+  if (split && (e = hipStreamWaitEvent(c->stream, c->ev_part[0], 0)) != hipSuccess) return e;
   if (fork && (e = hipStreamWaitEvent(c->stream, c->ev_join[0], 0)) != hipSuccess) return e;
   if (a.n_abc > 0 && a.theta_method == LPS_THETA_IDW && (e = launch_idw(a, st, c->stream)) != hipSuccess)
     return e;
+  if (split) {  // every theta is written: the partial tiles' force may start
+    if ((e = hipEventRecord(c->ev_part[1], c->stream)) != hipSuccess) return e;
+    if ((e = hipStreamWaitEvent(c->stream3, c->ev_part[1], 0)) != hipSuccess) return e;
+    if ((e = launch_force_part(a, st, c->stream3)) != hipSuccess) return e;
+    if ((e = hipEventRecord(c->ev_part[2], c->stream3)) != hipSuccess) return e;
+  }
   if (record_events && (e = hipEventRecord(ev[1], c->stream)) != hipSuccess) return e;
   if ((e = launch_force(a, st, c->fast, c->stream)) != hipSuccess) return e;
   if (record_events && (e = hipEventRecord(ev[2], c->stream)) != hipSuccess) return e;
   if (c->fast) {
-    if ((e = launch_force_part(a, st, c->stream)) != hipSuccess) return e;
+    if (!split && (e = launch_force_part(a, st, c->stream)) != hipSuccess) return e;
+    if (split && (e = hipStreamWaitEvent(c->stream, c->ev_part[2], 0)) != hipSuccess) return e;
     if (fork && (e = hipStreamWaitEvent(c->stream, c->ev_join[1], 0)) != hipSuccess) return e;
     if (abc && !fork && (e = launch_abc_uv(a, st, c->stream)) != hipSuccess) return e;
     if (abc && !fork && c->clo && (e = closure_step_hook(c, st, c->stream)) != hipSuccess) return e;

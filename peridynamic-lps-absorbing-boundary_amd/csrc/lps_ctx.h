@@ -28,8 +28,10 @@ struct lps_ctx {
   using StepState = lps::StepState;
   lps_options opt{};
   hipStream_t stream = nullptr;
-  hipStream_t stream2 = nullptr;  // the ABC rows' force kernel beside the node force kernel
+  hipStream_t stream2 = nullptr;  // the ABC rows' kernels beside the tile kernels
+  hipStream_t stream3 = nullptr;  // the partial tiles' kernels beside the full tiles' kernels
   hipEvent_t ev_fork[2] = {nullptr, nullptr}, ev_join[2] = {nullptr, nullptr};
+  hipEvent_t ev_part[3] = {nullptr, nullptr, nullptr};  // partial theta done, theta done, partial force done
   std::string err, path;
   int32_t n = 0, n_slots = 0, n_abc = 0, M = 0;
   double E = 0, nu = 0, rho = 0, dt = 0, delta = 0, dx = 0, dy = 0;

@@ -882,11 +882,13 @@ struct ThetaMasked {
 };
 
 // Full-tile force: F_i = sum_k (lambda - mu) C c1_k theta_j o_k + 8 mu C c2_k ((u_j - u_i) . o_k) o_k.
+// Weights wt (theta_j terms, theta scaled by the instance's th_scale in the window) and wb
+// (bond-extension terms); the sum is scaled by the instance's f_scale.
 struct ForceFull {
   const double2 *su;
   const double *sth;
   int32_t c0;
-  const InstParams &T;  // the instance's folded weights wa, wb
+  const TileArgs &T;    // the class tables (kernel arguments)
   double2 ui[kRB];
   double fx[kRB], fy[kRB];
   double2 u;
@@ -903,17 +905,20 @@ struct ForceFull {
     const double dux = u.x - ui[R].x, duy = u.y - ui[R].y;
     if constexpr (OX != 0) {
       constexpr int za = rep_slot(cls, iabs(OX)), zb = prod_slot(cls, OX * OX);
-      fx[R] = fma(OX > 0 ? T.wa[cls][za] : -T.wa[cls][za], th, fx[R]);
+      const double wa = T.wt[cls][za];
+      fx[R] = fma(OX > 0 ? wa : -wa, th, fx[R]);
       fx[R] = fma(T.wb[cls][zb], dux, fx[R]);
     }
     if constexpr (OY != 0) {
       constexpr int za = rep_slot(cls, iabs(OY)), zb = prod_slot(cls, OY * OY);
-      fy[R] = fma(OY > 0 ? T.wa[cls][za] : -T.wa[cls][za], th, fy[R]);
+      const double wa = T.wt[cls][za];
+      fy[R] = fma(OY > 0 ? wa : -wa, th, fy[R]);
       fy[R] = fma(T.wb[cls][zb], duy, fy[R]);
     }
     if constexpr (OX != 0 && OY != 0) {
       constexpr int z = prod_slot(cls, iabs(OX * OY));
-      const double w = ((OX > 0) == (OY > 0)) ? T.wb[cls][z] : -T.wb[cls][z];
+      const double wb = T.wb[cls][z];
+      const double w = ((OX > 0) == (OY > 0)) ? wb : -wb;
       fx[R] = fma(w, duy, fx[R]);
       fy[R] = fma(w, dux, fy[R]);
     }
@@ -1055,14 +1060,14 @@ __global__ void __launch_bounds__(64, 3) k_force_rb(StepArgs a, StepState st) {
     vi[r] = make_double2(st.vx[i], st.vy[i]);
     ai[r] = make_double2(st.ax[i], st.ay[i]);
   }
+  const double ratio = a.inst[T.tile_inst[t]].th_scale;
   fill_window68(nb, lane, [&](WinCell w) {
     su[w.cell] = make_double2(st.px[w.slot], st.py[w.slot]);
-    sth[w.cell] = st.theta[w.slot];
+    sth[w.cell] = ratio * st.theta[w.slot];
   }, std::make_integer_sequence<int, 9>{});
   __syncthreads();
   const int32_t c0 = (lx0 + 4) * kW68 + ly + 4;
-  const InstParams &I = a.inst[T.tile_inst[t]];
-  ForceFull P{su, sth, c0, I, {}, {}, {}, {}, 0.0};
+  ForceFull P{su, sth, c0, T, {}, {}, {}, {}, 0.0};
 #pragma unroll
   for (int r = 0; r < kRB; r++) {
     P.ui[r] = su[c0 + r * kW68];
@@ -1070,10 +1075,13 @@ __global__ void __launch_bounds__(64, 3) k_force_rb(StepArgs a, StepState st) {
     P.fy[r] = 0.0;
   }
   rb_sweep(P, std::make_integer_sequence<int, kRB + 8>{});
+  const InstParams &I = a.inst[T.tile_inst[t]];
+  const double scale = I.f_scale;
 #pragma unroll
   for (int r = 0; r < kRB; r++) {
     const int32_t i = t * (kTile * kTile) + (lx0 + r) * kTile + ly;
-    if (kd[r] != 0) node_update(I, st, i, kd[r], P.ui[r], vi[r], ai[r], P.fx[r], P.fy[r]);
+    if (kd[r] != 0)
+      node_update(I, st, i, kd[r], P.ui[r], vi[r], ai[r], scale * P.fx[r], scale * P.fy[r]);
   }
 }
 
@@ -1501,14 +1509,22 @@ hipError_t launch_predict(const StepArgs &a, const double *ux, const double *uy,
 
 static inline int32_t list_blocks(int32_t count) { return (count + 7) / 8 * 8; }
 
-hipError_t launch_theta(const StepArgs &a, const StepState &st, bool fast, hipStream_t s) {
+hipError_t launch_theta_part(const StepArgs &a, const StepState &st, hipStream_t s) {
+  const TileArgs &T = a.tile;
+  if (T.theta_kernel != kBlocked || T.n_part == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_theta_rbm, dim3(list_blocks(T.n_part)), dim3(64), 0, s, a, st);
+  return hipGetLastError();
+}
+
+hipError_t launch_theta(const StepArgs &a, const StepState &st, bool fast, hipStream_t s,
+                        bool with_part) {
   if (fast) {
     const TileArgs &T = a.tile;
     if (T.theta_kernel == kBlocked) {
       if (T.n_full > 0)
         hipLaunchKernelGGL(k_theta_rb, dim3(list_blocks(T.n_full)), dim3(64), 0, s, a, st);
       hipError_t e = hipGetLastError();
-      if (e != hipSuccess || T.n_part == 0) return e;
+      if (e != hipSuccess || T.n_part == 0 || !with_part) return e;
       hipLaunchKernelGGL(k_theta_rbm, dim3(list_blocks(T.n_part)), dim3(64), 0, s, a, st);
     } else if (T.theta_kernel == kUnrolled) {
       hipLaunchKernelGGL(k_theta_t68, dim3(tile_blocks(a)), dim3(kBlock), 0, s, a, st,

@@ -185,7 +185,7 @@ def _lps_options(L, theta_method, abc_enabled, exact, use_graph, device, horizon
     o.device = device
     o.horizon_radius = horizon_radius
     o.variant = variant
-    o.concurrent = 1 if concurrent else 0
+    o.concurrent = int(concurrent)  # 0 serial, 1 ABC rows beside the tiles, 2 also partial tiles
     return o
 
 
