@@ -138,7 +138,7 @@ def bench_batch(args, configs, solver):
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"{w.name}: {w.description}", "instances": len(probs),
                    "nodes": n_nodes, "abc_nodes": n_abc, "kernels": path,
-                   "abc": "least-squares G_u/G_v/G_theta (the learned closure is not built)",
+                   "abc": "least-squares G_u/G_v/G_theta (learned closure: --workload cfg4)",
                    "streams": "serial" if args.serial else "concurrent",
                    "untimed_node_steps_per_s": n_nodes * steps / untimed_s,
                    "reflected_energy_final_max": max(refl) if refl else None,
@@ -149,12 +149,67 @@ def bench_batch(args, configs, solver):
     print(json.dumps(out), flush=True)
 
 
+def bench_learned(args):
+    """Learned ABC closure against the least-squares ABC on held-out runs of the R = 125 mm domain
+    (1-4 smooth P/S Gaussian pulses, nu on the 0.025 grid), all advanced in one batch: the
+    least-squares batch, then the same batch with the closure (amplitude gate at the training's
+    1e-4, energy guard 1e-3 E0 per output record with per-instance fallback). value = node-steps/s
+    of the batch with the closure; the least-squares batch's rate and both reflected energies
+    (E_int(T)/E0 at the last output record) are in config."""
+    from lps_amd import closure as cl, learned as L
+    steps = args.steps if args.steps is not None else 6000
+    runs = L.draw_runs(3000, args.runs)
+    workers = args.workers or min(16, os.cpu_count() or 1)
+    path = os.path.join(ROOT, "peridynamic-lps-absorbing-boundary_amd", "lps_amd", "data",
+                        "closure_r125.npz")
+    with cl.Closure() as clo:
+        meta = L.load_into(clo, path)
+        log(f"[bench] closure {path}: trained on {meta.get('train_runs')} runs, teacher-forced "
+            f"error left vs LSQ {meta['fit'].get('val_ratio')}")
+        ev = L.evaluate(clo, runs, steps, chunk=args.runs, workers=workers, guard=1e-3,
+                        amp_min_rel=1e-4, gain=1.0)
+    sm = ev["summary"]
+    n_rows = sum(L.truncated_problem(r.nu).C for r in runs)
+    v_l = ev["node_steps"] / ev["t_learned"]
+    v_q = ev["node_steps"] / ev["t_lsq"]
+    # the closure's share of a step, from the two loops' wall times (history shift + network)
+    t_clo = (ev["t_learned"] - ev["t_lsq"]) / steps
+    flops = n_rows * 2.0 * (1600 * 256 + 2 * 256 * 256 + 256 * 4)
+    log(f"[bench] {len(runs)} runs x {steps} steps: LSQ {v_q:.4e}, learned {v_l:.4e} node-steps/s; "
+        f"closure {t_clo * 1e3:.3f} ms/step for {n_rows} rows")
+    out = {
+        "metric": "node-steps/s at reflected dE/E0\u22641e-4: learned vs LSQ ABC, 1 MI355X",
+        "value": v_l, "unit": "node-steps/s", "n_gpus": 1, "steps": steps, "warmup": 0,
+        "ms_per_step": ev["t_learned"] / steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64 (state), bf16 MFMA / f32 accumulation (closure)",
+        "data": "synthetic",
+        "config": {"workload": f"cfg4: {len(runs)} held-out runs, R=125 mm, 1-4 P/S pulses, nu on "
+                               f"the 0.025 grid, {steps} steps, one batch",
+                   "abc_rows": n_rows, "lsq_node_steps_per_s": v_q,
+                   "closure_overhead": sm["overhead"],
+                   "reflected_median_lsq": sm["median_lsq"],
+                   "reflected_median_learned": sm["median_learned"],
+                   "learned_le_lsq": sm["learned_le_lsq"], "fallbacks": sm["fallbacks"],
+                   "gate_reflected_le_1e-4": {"lsq": sm["gate_lsq"], "learned": sm["gate_learned"]},
+                   "closure_weights": meta},
+        "roofline": {"bound": "mfma", "achieved": round(flops / t_clo / 1e12, 2), "peak": 2500.0,
+                     "unit": "TFLOP/s", "frac": round(flops / t_clo / 1e12 / 2500.0, 4),
+                     "traffic": None, "kernel": "k_mlp_fwd_wide<true> (+ k_hist_shift)",
+                     "flops_per_row": flops / n_rows,
+                     "note": "closure time = (learned - LSQ loop wall time) / steps"},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out, default=float), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=60)
-    ap.add_argument("--workload", default="cfg2")
+    ap.add_argument("--workload", default="cfg2",
+                    help="cfg2 (default), cfg1, cfg3 (64-domain batch), cfg4 (learned vs LSQ ABC)")
+    ap.add_argument("--runs", type=int, default=64, help="cfg4: held-out runs (one batch)")
     ap.add_argument("--exact", action="store_true")
     ap.add_argument("--variant", type=int, default=0,
                     help="fast-path node kernels: 0 autotune, 1 generic tile, 2 unrolled stencil, "
@@ -188,6 +243,9 @@ def main():
         sys.exit(2)
 
     from lps_amd import configs, solver
+    if args.workload == "cfg4":
+        bench_learned(args)
+        return
     if args.workload in configs.BATCH_WORKLOADS:
         bench_batch(args, configs, solver)
         return
@@ -263,7 +321,7 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"{w.name}: {w.description}", "nodes": prob.N, "abc_nodes": prob.C,
                    "bonds": nnz, "dt": prob.dt, "exact": bool(args.exact), "kernels": path,
-                   "abc": "least-squares G_u/G_v/G_theta (the learned closure is not built)",
+                   "abc": "least-squares G_u/G_v/G_theta (learned closure: --workload cfg4)",
                    "streams": "serial" if args.serial else "concurrent",
                    "untimed_node_steps_per_s": prob.N * steps / untimed_s,
                    "reflected_energy_final": refl,

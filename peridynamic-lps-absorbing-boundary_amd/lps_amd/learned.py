@@ -108,7 +108,7 @@ def run_problem(run: Run, steps: int, workers: Optional[int] = None) -> pp.Probl
 
 
 def log(*a):
-    print("[learned]", *a, flush=True)
+    print("[learned]", *a, file=sys.stderr, flush=True)
 
 
 # ---------------------------------------------------------------------------------------------
