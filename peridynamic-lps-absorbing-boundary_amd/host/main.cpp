@@ -2,7 +2,7 @@
 // time loop on an MI355X through the lps_amd C ABI.
 //
 //   PD2_LPS_ABC <prefix> <jobname> [--theta-method=gtheta|idw] [--vtk=on|off] [--exact]
-//                                  [--no-graph] [--device=N] [--abc=lsq]
+//                                  [--no-graph] [--device=N] [--abc=lsq|learned=<weights.bin>]
 //   PD2_LPS_ABC --batch=<manifest> [options]
 //
 // Same positional arguments, input files, .energy/.verify outputs (byte format of
@@ -13,6 +13,12 @@
 // --batch runs many jobs in one process (no counterpart in the reference): the manifest holds one
 // "<prefix> <jobname>" per line ('#' starts a comment); the jobs, which must share dx, dy, delta and
 // the step count, are advanced together (lps_create_batch) and each gets its own outputs.
+//
+// --abc=learned=<weights.bin> (no counterpart in the reference) adds the learned closure to the
+// least-squares ABC: weights.bin holds lps_closure_param_count() float32 parameters then sigma_u,
+// sigma_v (little endian; `python -m lps_amd.learned --export-bin` writes it). The closure acts where
+// a row's max |u| reaches 1e-4 of the job's largest initial displacement, and a job whose energy
+// rises falls back to the least-squares ABC (lps_closure_guard).
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -160,7 +166,53 @@ struct Job {
   }
 };
 
-int run_single(Job &job, const lps_options &opt, bool vtk) {
+// The learned closure of --abc=learned=<path>, attached to a context; destroyed with the holder.
+struct Learned {
+  lps_closure *clo = nullptr;
+  ~Learned() { lps_closure_destroy(clo); }
+  // 0 on success, else prints the FATAL message and returns 1.
+  int attach(lps_ctx *ctx, const std::string &path, int32_t device,
+             const std::vector<float> &gate) {
+    const int64_t np = lps_closure_param_count();
+    std::ifstream f(path, std::ios::binary);
+    std::vector<float> w(np + 2);
+    if (!f || !f.read(reinterpret_cast<char *>(w.data()), sizeof(float) * w.size()) ||
+        f.peek() != std::char_traits<char>::eof()) {
+      std::cerr << "FATAL: " << path << " is not a closure weight file (" << np + 2
+                << " float32 values)" << std::endl;
+      return 1;
+    }
+    if (lps_closure_create(device, 0, &clo) != LPS_OK ||
+        lps_closure_set_params(clo, w.data()) != LPS_OK ||
+        lps_closure_set_scales(clo, w[np], w[np + 1]) != LPS_OK) {
+      std::cerr << "FATAL: learned closure setup failed: " << lps_closure_error(clo) << std::endl;
+      return 1;
+    }
+    if (lps_closure_attach(ctx, clo) != LPS_OK || lps_closure_set_gate(ctx, gate.data()) != LPS_OK) {
+      std::cerr << "FATAL: " << lps_last_error(ctx) << std::endl;
+      return 1;
+    }
+    std::cout << "Learned ABC closure " << path << " attached." << std::endl;
+    return 0;
+  }
+  void report(lps_ctx *ctx, const std::vector<std::string> &names) {
+    if (!clo) return;
+    std::vector<int64_t> fb(names.size(), -1);
+    lps_closure_guard(ctx, -1.0, fb.data());
+    for (size_t i = 0; i < names.size(); i++)
+      if (fb[i] >= 0)
+        std::cout << "[" << names[i] << "] learned closure fell back to the least-squares ABC at "
+                  << "step " << fb[i] << " (energy rise)" << std::endl;
+  }
+};
+
+float gate_of(const std::vector<double> &ic) {
+  double m = 0.0;
+  for (double x : ic) m = std::max(m, std::fabs(x));
+  return (float)(1e-4 * m);
+}
+
+int run_single(Job &job, const lps_options &opt, bool vtk, const std::string &learned) {
   std::cout << "Reading data... " << std::flush;
   if (job.read(opt)) return 1;
   std::cout << "finished." << std::endl;
@@ -186,6 +238,11 @@ int run_single(Job &job, const lps_options &opt, bool vtk) {
   std::cout << std::endl << "Setup on the GPU (horizons, m_i, theta history) took " << timer.toc()
             << " s." << std::endl;
   std::cout << "  m_ref      = " << lps_m_ref(ctx) << std::endl;
+  Learned closure;
+  if (!learned.empty() && closure.attach(ctx, learned, opt.device, {gate_of(job.ic)})) {
+    lps_destroy(ctx);
+    return 1;
+  }
   if (vtk) {
     lps_fetch(ctx, job.u.data(), nullptr, nullptr, nullptr, nullptr);
     lpsio::write_vtk_vector(job.name + job.prefix + "-0.vtk", job.nodes, job.u, "displacement",
@@ -217,11 +274,13 @@ int run_single(Job &job, const lps_options &opt, bool vtk) {
     }
   }
   std::cout << std::endl << "Time marching took " << timer.toc() << " seconds." << std::endl;
+  closure.report(ctx, {job.name});
   lps_destroy(ctx);
   return 0;
 }
 
-int run_batch(const std::string &manifest, const lps_options &opt, bool vtk) {
+int run_batch(const std::string &manifest, const lps_options &opt, bool vtk,
+              const std::string &learned) {
   std::ifstream mf(manifest);
   if (!mf) {
     std::cerr << "FATAL: Cannot read " << manifest << std::endl;
@@ -265,6 +324,17 @@ int run_batch(const std::string &manifest, const lps_options &opt, bool vtk) {
   }
   std::cout << "Setup on the GPU took " << timer.toc() << " s: " << lps_describe(ctx) << std::endl;
   const int32_t B = (int32_t)jobs.size();
+  Learned closure;
+  std::vector<std::string> names;
+  for (Job &j : jobs) names.push_back(j.name);
+  if (!learned.empty()) {
+    std::vector<float> gate;
+    for (Job &j : jobs) gate.push_back(gate_of(j.ic));
+    if (closure.attach(ctx, learned, opt.device, gate)) {
+      lps_destroy(ctx);
+      return 1;
+    }
+  }
   for (int32_t i = 0; i < B; i++) {
     Job &j = jobs[i];
     if (vtk) {
@@ -306,6 +376,7 @@ int run_batch(const std::string &manifest, const lps_options &opt, bool vtk) {
     }
   }
   std::cout << std::endl << "Time marching took " << timer.toc() << " seconds." << std::endl;
+  closure.report(ctx, names);
   lps_destroy(ctx);
   return 0;
 }
@@ -322,7 +393,7 @@ int main(int argc, char *argv[]) {
             << std::endl;
 
   std::vector<std::string> pos;
-  std::string manifest;
+  std::string manifest, learned;
   lps_options opt;
   lps_default_options(&opt);
   bool vtk = true;
@@ -337,16 +408,18 @@ int main(int argc, char *argv[]) {
     else if (a == "--no-graph") opt.use_graph = 0;
     else if (a.rfind("--device=", 0) == 0) opt.device = std::atoi(a.c_str() + 9);
     else if (a.rfind("--batch=", 0) == 0) manifest = a.substr(8);
-    else if (a == "--abc=lsq") {}
-    else if (a == "--abc=learned") {
-      std::cerr << "FATAL: the learned ABC closure is not part of this build" << std::endl;
-      return 1;
-    } else {
+    else if (a == "--abc=lsq") learned.clear();
+    else if (a.rfind("--abc=learned=", 0) == 0) learned = a.substr(14);
+    else {
       std::cerr << "unknown option " << a << std::endl;
       return 1;
     }
   }
-  if (!manifest.empty()) return run_batch(manifest, opt, vtk);
+  if (!learned.empty() && opt.exact) {
+    std::cerr << "FATAL: the learned closure runs on the fast path only (drop --exact)" << std::endl;
+    return 1;
+  }
+  if (!manifest.empty()) return run_batch(manifest, opt, vtk, learned);
   if (pos.size() != 2) {
     std::cout << "Usage: " << argv[0] << " <prefix> <jobname>" << std::endl;
     return 0;
@@ -354,5 +427,5 @@ int main(int argc, char *argv[]) {
   Job job;
   job.prefix = pos[0];
   job.name = pos[1];
-  return run_single(job, opt, vtk);
+  return run_single(job, opt, vtk, learned);
 }

@@ -189,6 +189,13 @@ def save(clo, path: str, meta: dict):
              meta=np.array(json.dumps(meta)))
 
 
+def export_bin(params: np.ndarray, sigma, path: str):
+    """The weight file of PD2_LPS_ABC --abc=learned=<path>: the float32 parameters, then sigma_u,
+    sigma_v, little endian."""
+    out = np.concatenate([np.asarray(params, np.float32).ravel(), np.asarray(sigma, np.float32)])
+    out.astype("<f4").tofile(path)
+
+
 def load_into(clo, path: str) -> dict:
     z = np.load(path, allow_pickle=False)
     clo.set_params(z["params"])
@@ -297,7 +304,13 @@ def main(argv=None):
     ap.add_argument("--report", default=None)
     ap.add_argument("--load", default=None, help="skip training: load this parameter file")
     ap.add_argument("--workers", type=int, default=None)
+    ap.add_argument("--export-bin", default=None, metavar="BIN",
+                    help="with --load: write the weight file of PD2_LPS_ABC --abc=learned=BIN and stop")
     a = ap.parse_args(argv)
+    if a.export_bin:
+        z = np.load(a.load, allow_pickle=False)
+        export_bin(z["params"], z["sigma"], a.export_bin)
+        return {"exported": a.export_bin}
     from .closure import Closure
     report = {"args": vars(a)}
     with Closure(seed=a.seed) as clo:

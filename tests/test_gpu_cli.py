@@ -80,3 +80,31 @@ def test_cli_batch_manifest_runs_several_jobs_in_one_process(tmp_path):
                             1e-10, prob, name)
         assert_series_close(files.read_series(job + "_out.verify"), ref["verify"][:n], VERIFY_KEYS,
                             1e-10, prob, name)
+
+
+def test_cli_learned_closure(tmp_path):
+    """--abc=learned=<weights.bin>: a closure whose output layer is zero gives the least-squares
+    ABC's outputs (to 1e-12: each run autotunes its own node kernels; the bitwise identity is tested
+    through the C ABI with the kernels fixed); a trained closure runs; a malformed file is refused."""
+    print("cli learned", flush=True)
+    from lps_amd import closure as cl, learned
+    prob, _ = load_golden("abc_r10_nu025")
+    prob.steps = 120
+    d = cl.split_params(np.random.default_rng(3).normal(0.0, 0.05, 542468).astype(np.float32))
+    d["W4"][:] = 0.0
+    d["b4"][:] = 0.0
+    learned.export_bin(cl.join_params(d), [0.03, 0.03], str(tmp_path / "zero.bin"))
+    job, r = run_cli(tmp_path, prob, "--vtk=off")
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    lsq = files.read_series(job + "_out.energy")
+    job, r = run_cli(tmp_path, prob, "--vtk=off", "--abc=learned=zero.bin")
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert "closure" in r.stdout
+    assert_series_close(files.read_series(job + "_out.energy"), lsq, ENERGY_KEYS, 1e-12, prob)
+    z = np.load(os.path.join(PKG, "lps_amd", "data", "closure_r125.npz"), allow_pickle=False)
+    learned.export_bin(z["params"], z["sigma"], str(tmp_path / "trained.bin"))
+    job, r = run_cli(tmp_path, prob, "--vtk=off", "--abc=learned=trained.bin")
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    (tmp_path / "bad.bin").write_bytes(b"1234")
+    job, r = run_cli(tmp_path, prob, "--vtk=off", "--abc=learned=bad.bin")
+    assert r.returncode == 1 and "not a closure weight file" in r.stderr
