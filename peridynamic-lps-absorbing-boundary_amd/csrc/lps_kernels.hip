@@ -853,35 +853,6 @@ struct ThetaFull {
   }
 };
 
-// Partial-tile dilatation, the reference's bond term: theta_i = c2m_i sum_k c1_k (u_j - u_i) . o_k
-// over the cells that hold a node (mask 1).
-struct ThetaMasked {
-  const double2 *su;
-  const double *sm;
-  int32_t c0;
-  const TileArgs &T;
-  double2 ui[kRB];
-  double acc[kRB];
-  double2 u;
-  double msk;
-  template <int D, int OY>
-  __device__ __forceinline__ void cell() {
-    u = su[c0 + D * kW68 + OY];
-    msk = sm[c0 + D * kW68 + OY];
-    rb_bonds<D, OY>(*this, std::make_integer_sequence<int, kRB>{});
-  }
-  template <int OX, int OY, int R>
-  __device__ __forceinline__ void bond() {
-    constexpr int cls = stencil_class(OX * OX + OY * OY);
-    acc[R] = fma(T.c1[cls] * msk, dot_int(u.x - ui[R].x, u.y - ui[R].y, OX, OY), acc[R]);
-  }
-  __device__ __forceinline__ void end_row() {
-#pragma unroll
-    for (int r = 0; r < kRB; r++) pin(acc[r]);
-    __builtin_amdgcn_sched_barrier(0);
-  }
-};
-
 // Full-tile force: F_i = sum_k (lambda - mu) C c1_k theta_j o_k + 8 mu C c2_k ((u_j - u_i) . o_k) o_k.
 // Weights wt (theta_j terms, theta scaled by the instance's th_scale in the window) and wb
 // (bond-extension terms); the sum is scaled by the instance's f_scale.
@@ -934,42 +905,6 @@ struct ForceFull {
   }
 };
 
-// Partial-tile force, the reference's bond term with ab = ((lambda - mu) c2m theta, 4 mu c2m):
-// s = (ab_i.x + ab_j.x) c1 + (ab_i.y + ab_j.y) c2 ((u_j - u_i) . o), skipped where ab_j.y == 0
-// (no node, or m_j <= 0; src/main.cpp:852).
-struct ForceMasked {
-  const double2 *su;
-  const double2 *sab;
-  int32_t c0;
-  const TileArgs &T;
-  double2 ui[kRB], abi[kRB];
-  double fx[kRB], fy[kRB];
-  double2 u, ab;
-  template <int D, int OY>
-  __device__ __forceinline__ void cell() {
-    u = su[c0 + D * kW68 + OY];
-    ab = sab[c0 + D * kW68 + OY];
-    rb_bonds<D, OY>(*this, std::make_integer_sequence<int, kRB>{});
-  }
-  template <int OX, int OY, int R>
-  __device__ __forceinline__ void bond() {
-    constexpr int cls = stencil_class(OX * OX + OY * OY);
-    const double tt = dot_int(u.x - ui[R].x, u.y - ui[R].y, OX, OY);
-    double sp = (abi[R].x + ab.x) * T.c1[cls] + (abi[R].y + ab.y) * tt * T.c2[cls];
-    sp = (ab.y != 0.0) ? sp : 0.0;
-    if constexpr (OX != 0) fx[R] = fma(sp, (double)OX, fx[R]);
-    if constexpr (OY != 0) fy[R] = fma(sp, (double)OY, fy[R]);
-  }
-  __device__ __forceinline__ void end_row() {
-#pragma unroll
-    for (int r = 0; r < kRB; r++) {
-      pin(fx[r]);
-      pin(fy[r]);
-    }
-    __builtin_amdgcn_sched_barrier(0);
-  }
-};
-
 // Dilatation of the full tiles; one wavefront per tile (full_list order).
 __global__ void __launch_bounds__(64, 4) k_theta_rb(StepArgs a, StepState st) {
   __shared__ double2 su[kCells68];
@@ -995,46 +930,6 @@ __global__ void __launch_bounds__(64, 4) k_theta_rb(StepArgs a, StepState st) {
 #pragma unroll
   for (int r = 0; r < kRB; r++)
     if (kd[r] == 1) st.theta[t * (kTile * kTile) + (lx0 + r) * kTile + ly] = P.acc[r];
-}
-
-// Dilatation of the partial tiles; one wavefront per tile (part_list order).
-__global__ void __launch_bounds__(64, 4) k_theta_rbm(StepArgs a, StepState st) {
-  __shared__ double2 su[kCells68];
-  __shared__ double sm[kCells68];
-  const TileArgs &T = a.tile;
-  const int32_t q = tile_of_block(blockIdx.x, T.n_part);
-  if (q >= T.n_part) return;
-  const int32_t t = T.part_list[q];
-  int32_t nb[9];
-#pragma unroll
-  for (int k = 0; k < 9; k++) nb[k] = T.tile_nbr[9 * t + k];
-  const int32_t lane = threadIdx.x;
-  const int32_t ly = lane & 15, lx0 = (lane >> 4) * kRB;
-  int8_t kd[kRB];
-  double c2[kRB];
-#pragma unroll
-  for (int r = 0; r < kRB; r++) {
-    const int32_t i = t * (kTile * kTile) + (lx0 + r) * kTile + ly;
-    kd[r] = a.kind[i];
-    c2[r] = a.c2m[i];
-  }
-  fill_window68(nb, lane, [&](WinCell w) {
-    const bool ok = w.tile >= 0 && a.kind[w.slot] != 0;
-    su[w.cell] = ok ? make_double2(st.px[w.slot], st.py[w.slot]) : make_double2(0.0, 0.0);
-    sm[w.cell] = ok ? 1.0 : 0.0;
-  }, std::make_integer_sequence<int, 9>{});
-  __syncthreads();
-  const int32_t c0 = (lx0 + 4) * kW68 + ly + 4;
-  ThetaMasked P{su, sm, c0, T, {}, {}, {}, 0.0};
-#pragma unroll
-  for (int r = 0; r < kRB; r++) {
-    P.ui[r] = su[c0 + r * kW68];
-    P.acc[r] = 0.0;
-  }
-  rb_sweep(P, std::make_integer_sequence<int, kRB + 8>{});
-#pragma unroll
-  for (int r = 0; r < kRB; r++)
-    if (kd[r] == 1) st.theta[t * (kTile * kTile) + (lx0 + r) * kTile + ly] = c2[r] * P.acc[r];
 }
 
 // Force pass of the full tiles with the fused Verlet corrector and predictor; one wavefront per
@@ -1083,60 +978,6 @@ __global__ void __launch_bounds__(64, 3) k_force_rb(StepArgs a, StepState st) {
     const int32_t i = t * (kTile * kTile) + (lx0 + r) * kTile + ly;
     if (kd[r] != 0)
       node_update(I, st, i, kd[r], P.ui[r], vi[r], ai[r], scale * P.fx[r], scale * P.fy[r]);
-  }
-}
-
-// Force pass of the partial tiles; one wavefront per tile (part_list order).
-__global__ void __launch_bounds__(64, 3) k_force_rbm(StepArgs a, StepState st) {
-  __shared__ double2 su[kCells68];
-  __shared__ double2 sab[kCells68];
-  const TileArgs &T = a.tile;
-  const int32_t q = tile_of_block(blockIdx.x, T.n_part);
-  if (q >= T.n_part) return;
-  const int32_t t = T.part_list[q];
-  int32_t nb[9];
-#pragma unroll
-  for (int k = 0; k < 9; k++) nb[k] = T.tile_nbr[9 * t + k];
-  const int32_t lane = threadIdx.x;
-  const int32_t ly = lane & 15, lx0 = (lane >> 4) * kRB;
-  const InstParams &P = a.inst[T.tile_inst[t]];
-  const double lm = P.lam - P.mu, mu4 = 4.0 * P.mu;
-  int8_t kd[kRB];
-  double2 vi[kRB], ai[kRB];
-#pragma unroll
-  for (int r = 0; r < kRB; r++) {
-    const int32_t i = t * (kTile * kTile) + (lx0 + r) * kTile + ly;
-    kd[r] = a.kind[i];
-    vi[r] = make_double2(st.vx[i], st.vy[i]);
-    ai[r] = make_double2(st.ax[i], st.ay[i]);
-  }
-  fill_window68(nb, lane, [&](WinCell w) {
-    double2 u = make_double2(0.0, 0.0), ab = make_double2(0.0, 0.0);
-    if (w.tile >= 0 && a.kind[w.slot] != 0) {
-      const double c2 = a.c2m[w.slot];  // 0 when m <= 0
-      u = make_double2(st.px[w.slot], st.py[w.slot]);
-      ab = make_double2(lm * c2 * st.theta[w.slot], mu4 * c2);
-    }
-    su[w.cell] = u;
-    sab[w.cell] = ab;
-  }, std::make_integer_sequence<int, 9>{});
-  __syncthreads();
-  const int32_t c0 = (lx0 + 4) * kW68 + ly + 4;
-  ForceMasked Q{su, sab, c0, T, {}, {}, {}, {}, {}, {}};
-#pragma unroll
-  for (int r = 0; r < kRB; r++) {
-    Q.ui[r] = su[c0 + r * kW68];
-    Q.abi[r] = sab[c0 + r * kW68];
-    Q.fx[r] = 0.0;
-    Q.fy[r] = 0.0;
-  }
-  rb_sweep(Q, std::make_integer_sequence<int, kRB + 8>{});
-#pragma unroll
-  for (int r = 0; r < kRB; r++) {
-    const int32_t i = t * (kTile * kTile) + (lx0 + r) * kTile + ly;
-    const bool on = Q.abi[r].y != 0.0;  // a node with m <= 0 gets no force
-    if (kd[r] != 0)
-      node_update(P, st, i, kd[r], Q.ui[r], vi[r], ai[r], on ? Q.fx[r] : 0.0, on ? Q.fy[r] : 0.0);
   }
 }
 
