@@ -99,6 +99,17 @@ int32_t lps_reset(lps_ctx *ctx);
  * Single-problem contexts only (LPS_ERR_ARG on a batch). */
 int32_t lps_set_initial(lps_ctx *ctx, const double *ic, const double *icd);
 
+/* Checkpoint and restart. No counterpart in the reference, whose time loop (src/main.cpp:716-1166)
+ * always starts from the initial conditions. lps_checkpoint_save writes the state the next step
+ * reads (U^n, U^{n-1}, V^n, V^{n-1}, A^n, the prediction, theta^n, theta^{n-1}), the step count
+ * and E0 of every instance to a binary file. lps_checkpoint_load restores such a file into a
+ * context created from the same problem(s) and options; stepping on then gives the same bits as
+ * the uninterrupted run. A file of another problem, layout or option set is refused (LPS_ERR_ARG)
+ * before the context is touched. Neither works with a learned closure attached (LPS_ERR_ARG).
+ * Both block until the device is done. */
+int32_t lps_checkpoint_save(lps_ctx *ctx, const char *path);
+int32_t lps_checkpoint_load(lps_ctx *ctx, const char *path);
+
 /* Batches. No counterpart in the reference, which runs one job per process: many independent
  * problems (own radius, nu, dt, ABC operators, initial conditions; the same dx, dy and delta)
  * advanced together, every launch of a step covering all of them. Each problem is set up as

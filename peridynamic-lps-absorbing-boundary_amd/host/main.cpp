@@ -4,6 +4,7 @@
 //   PD2_LPS_ABC <prefix> <jobname> [--theta-method=gtheta|idw] [--vtk=on|off] [--exact]
 //                                  [--no-graph] [--device=N] [--abc=lsq|learned=<weights.bin>]
 //   PD2_LPS_ABC --batch=<manifest> [options]
+//   restart options: [--checkpoint=<file>] [--restart=<file>] [--stop-at=<step>]
 //
 // Same positional arguments, input files, .energy/.verify outputs (byte format of
 // src/main.cpp:694-707, 1106-1123) and exit codes (0 on success or on the usage message, 1 on a FATAL
@@ -19,6 +20,12 @@
 // sigma_v (little endian; `python -m lps_amd.learned --export-bin` writes it). The closure acts where
 // a row's max |u| reaches 1e-4 of the job's largest initial displacement, and a job whose energy
 // rises falls back to the least-squares ABC (lps_closure_guard).
+//
+// Checkpoint and restart (no counterpart in the reference): --stop-at=<step> ends the time loop at
+// that step, --checkpoint=<file> writes the state after the last step (lps_checkpoint_save), and
+// --restart=<file> resumes from such a file (lps_checkpoint_load) and runs on to StepNo, appending
+// its records to the .energy/.verify files of the run it continues. A run split this way writes the
+// same files, bit for bit, as the run that was never stopped.
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -123,11 +130,15 @@ struct Job {
     return p;
   }
 
-  void open_outputs() {
-    energy.open(name + prefix + ".energy", std::ios::out | std::ios::trunc);
-    energy << "# t  E_total  E_kinetic  E_potential  P_x  P_y  max_u  max_theta" << std::endl;
-    verify.open(name + prefix + ".verify", std::ios::out | std::ios::trunc);
-    verify << "# t  E_total  dE/E0  P_x  P_y  max_theta  max_u  max_v  max_a" << std::endl;
+  // append: continue the files of the run a restart resumes (no header)
+  void open_outputs(bool append = false) {
+    const auto mode = append ? std::ios::out | std::ios::app : std::ios::out | std::ios::trunc;
+    energy.open(name + prefix + ".energy", mode);
+    if (!append)
+      energy << "# t  E_total  E_kinetic  E_potential  P_x  P_y  max_u  max_theta" << std::endl;
+    verify.open(name + prefix + ".verify", mode);
+    if (!append)
+      verify << "# t  E_total  dE/E0  P_x  P_y  max_theta  max_u  max_v  max_a" << std::endl;
   }
 
   void write_vtk(int64_t s) {
@@ -206,13 +217,43 @@ struct Learned {
   }
 };
 
+// --checkpoint, --restart, --stop-at
+struct Restart {
+  std::string save, load;
+  int64_t stop_at = -1;
+  // The step the loop ends at.
+  int64_t end(int64_t step_no) const { return stop_at >= 0 ? std::min(stop_at, step_no) : step_no; }
+  // 0 on success, else prints the FATAL message and returns 1.
+  int resume(lps_ctx *ctx) const {
+    if (load.empty()) return 0;
+    if (lps_checkpoint_load(ctx, load.c_str()) != LPS_OK) {
+      std::cerr << "FATAL: " << lps_last_error(ctx) << std::endl;
+      return 1;
+    }
+    std::cout << "Restarted from " << load << " at step " << lps_step_index(ctx) << "."
+              << std::endl;
+    return 0;
+  }
+  int finish(lps_ctx *ctx) const {
+    if (save.empty()) return 0;
+    if (lps_checkpoint_save(ctx, save.c_str()) != LPS_OK) {
+      std::cerr << "FATAL: " << lps_last_error(ctx) << std::endl;
+      return 1;
+    }
+    std::cout << "Checkpoint of step " << lps_step_index(ctx) << " written to " << save << "."
+              << std::endl;
+    return 0;
+  }
+};
+
 float gate_of(const std::vector<double> &ic) {
   double m = 0.0;
   for (double x : ic) m = std::max(m, std::fabs(x));
   return (float)(1e-4 * m);
 }
 
-int run_single(Job &job, const lps_options &opt, bool vtk, const std::string &learned) {
+int run_single(Job &job, const lps_options &opt, bool vtk, const std::string &learned,
+               const Restart &rs) {
   std::cout << "Reading data... " << std::flush;
   if (job.read(opt)) return 1;
   std::cout << "finished." << std::endl;
@@ -243,21 +284,25 @@ int run_single(Job &job, const lps_options &opt, bool vtk, const std::string &le
     lps_destroy(ctx);
     return 1;
   }
-  if (vtk) {
+  if (rs.resume(ctx)) {
+    lps_destroy(ctx);
+    return 1;
+  }
+  if (vtk && rs.load.empty()) {
     lps_fetch(ctx, job.u.data(), nullptr, nullptr, nullptr, nullptr);
     lpsio::write_vtk_vector(job.name + job.prefix + "-0.vtk", job.nodes, job.u, "displacement",
                             job.node_no, job.exclude_r);
   }
-  job.open_outputs();
-  std::cout << std::endl << "Time marching started (" << job.step_no << " steps):" << std::endl;
+  job.open_outputs(!rs.load.empty());
+  const int64_t end = rs.end(job.step_no);
+  std::cout << std::endl << "Time marching started (" << end - lps_step_index(ctx) << " steps):"
+            << std::endl;
   timer.tic();
-  int32_t done = 0;
-  while (done < job.step_no) {
-    const int32_t chunk =
-        std::min<int32_t>(30 - (int32_t)(lps_step_index(ctx) % 30), job.step_no - done);
+  while (lps_step_index(ctx) < end) {
+    const int64_t s0 = lps_step_index(ctx);
+    const int32_t chunk = (int32_t)std::min<int64_t>(30 - s0 % 30, end - s0);
     lps_diag d;
     rc = lps_run(ctx, chunk, &d, 1);
-    done += chunk;
     if (rc < 0 && rc != LPS_ERR_NONFINITE) {
       std::cerr << "FATAL: " << lps_last_error(ctx) << std::endl;
       lps_destroy(ctx);
@@ -275,12 +320,13 @@ int run_single(Job &job, const lps_options &opt, bool vtk, const std::string &le
   }
   std::cout << std::endl << "Time marching took " << timer.toc() << " seconds." << std::endl;
   closure.report(ctx, {job.name});
+  rc = rs.finish(ctx);
   lps_destroy(ctx);
-  return 0;
+  return rc;
 }
 
 int run_batch(const std::string &manifest, const lps_options &opt, bool vtk,
-              const std::string &learned) {
+              const std::string &learned, const Restart &rs) {
   std::ifstream mf(manifest);
   if (!mf) {
     std::cerr << "FATAL: Cannot read " << manifest << std::endl;
@@ -335,25 +381,28 @@ int run_batch(const std::string &manifest, const lps_options &opt, bool vtk,
       return 1;
     }
   }
+  if (rs.resume(ctx)) {
+    lps_destroy(ctx);
+    return 1;
+  }
   for (int32_t i = 0; i < B; i++) {
     Job &j = jobs[i];
-    if (vtk) {
+    if (vtk && rs.load.empty()) {
       lps_fetch_instance(ctx, i, j.u.data(), nullptr, nullptr, nullptr, nullptr);
       lpsio::write_vtk_vector(j.name + j.prefix + "-0.vtk", j.nodes, j.u, "displacement",
                               j.node_no, j.exclude_r);
     }
-    j.open_outputs();
+    j.open_outputs(!rs.load.empty());
   }
-  const int32_t steps = jobs[0].step_no;
-  std::cout << std::endl << "Time marching started (" << steps << " steps, " << B << " jobs):"
-            << std::endl;
+  const int64_t end = rs.end(jobs[0].step_no);
+  std::cout << std::endl << "Time marching started (" << end - lps_step_index(ctx) << " steps, "
+            << B << " jobs):" << std::endl;
   timer.tic();
   std::vector<lps_diag> d(B);
-  int32_t done = 0;
-  while (done < steps) {
-    const int32_t chunk = std::min<int32_t>(30 - (int32_t)(lps_step_index(ctx) % 30), steps - done);
+  while (lps_step_index(ctx) < end) {
+    const int64_t s0 = lps_step_index(ctx);
+    const int32_t chunk = (int32_t)std::min<int64_t>(30 - s0 % 30, end - s0);
     rc = lps_run_batch(ctx, chunk, d.data(), 1);
-    done += chunk;
     if (rc < 0 && rc != LPS_ERR_NONFINITE) {
       std::cerr << "FATAL: " << lps_last_error(ctx) << std::endl;
       lps_destroy(ctx);
@@ -377,8 +426,9 @@ int run_batch(const std::string &manifest, const lps_options &opt, bool vtk,
   }
   std::cout << std::endl << "Time marching took " << timer.toc() << " seconds." << std::endl;
   closure.report(ctx, names);
+  rc = rs.finish(ctx);
   lps_destroy(ctx);
-  return 0;
+  return rc;
 }
 
 }  // namespace
@@ -394,6 +444,7 @@ int main(int argc, char *argv[]) {
 
   std::vector<std::string> pos;
   std::string manifest, learned;
+  Restart rs;
   lps_options opt;
   lps_default_options(&opt);
   bool vtk = true;
@@ -410,6 +461,9 @@ int main(int argc, char *argv[]) {
     else if (a.rfind("--batch=", 0) == 0) manifest = a.substr(8);
     else if (a == "--abc=lsq") learned.clear();
     else if (a.rfind("--abc=learned=", 0) == 0) learned = a.substr(14);
+    else if (a.rfind("--checkpoint=", 0) == 0) rs.save = a.substr(13);
+    else if (a.rfind("--restart=", 0) == 0) rs.load = a.substr(10);
+    else if (a.rfind("--stop-at=", 0) == 0) rs.stop_at = std::atoll(a.c_str() + 10);
     else {
       std::cerr << "unknown option " << a << std::endl;
       return 1;
@@ -419,7 +473,12 @@ int main(int argc, char *argv[]) {
     std::cerr << "FATAL: the learned closure runs on the fast path only (drop --exact)" << std::endl;
     return 1;
   }
-  if (!manifest.empty()) return run_batch(manifest, opt, vtk, learned);
+  if (!learned.empty() && !(rs.save.empty() && rs.load.empty())) {
+    std::cerr << "FATAL: checkpoints do not hold the learned closure's state (drop --abc=learned)"
+              << std::endl;
+    return 1;
+  }
+  if (!manifest.empty()) return run_batch(manifest, opt, vtk, learned, rs);
   if (pos.size() != 2) {
     std::cout << "Usage: " << argv[0] << " <prefix> <jobname>" << std::endl;
     return 0;
@@ -427,5 +486,5 @@ int main(int argc, char *argv[]) {
   Job job;
   job.prefix = pos[0];
   job.name = pos[1];
-  return run_single(job, opt, vtk, learned);
+  return run_single(job, opt, vtk, learned, rs);
 }

@@ -64,6 +64,8 @@ SIGNATURES = {
     "lps_create": (C.c_int32, [C.POINTER(LpsProblem), C.POINTER(LpsOptions), C.POINTER(C.c_void_p)]),
     "lps_reset": (C.c_int32, [C.c_void_p]),
     "lps_set_initial": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "lps_checkpoint_save": (C.c_int32, [C.c_void_p, C.c_char_p]),
+    "lps_checkpoint_load": (C.c_int32, [C.c_void_p, C.c_char_p]),
     "lps_create_batch": (C.c_int32, [C.c_void_p, C.c_int32, C.POINTER(LpsOptions),
                                      C.POINTER(C.c_void_p)]),
     "lps_instance_count": (C.c_int32, [C.c_void_p]),
@@ -240,6 +242,14 @@ class Solver:
         icd = np.zeros_like(ic) if icd is None else np.ascontiguousarray(icd, dtype=np.float64)
         assert ic.size == 2 * self.N and icd.size == 2 * self.N
         self._check(self._L.lps_set_initial(self._h, _ptr(ic), _ptr(icd)))
+
+    def checkpoint(self, path: str):
+        """Writes the time-loop state, step count and E0 to path (lps_checkpoint_save)."""
+        self._check(self._L.lps_checkpoint_save(self._h, os.fsencode(path)))
+
+    def restore(self, path: str):
+        """Restores a checkpoint of the same problem and options (lps_checkpoint_load)."""
+        self._check(self._L.lps_checkpoint_load(self._h, os.fsencode(path)))
 
     def step(self, nsteps: int):
         self._check(self._L.lps_step(self._h, nsteps))

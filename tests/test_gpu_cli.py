@@ -108,3 +108,33 @@ def test_cli_learned_closure(tmp_path):
     (tmp_path / "bad.bin").write_bytes(b"1234")
     job, r = run_cli(tmp_path, prob, "--vtk=off", "--abc=learned=bad.bin")
     assert r.returncode == 1 and "not a closure weight file" in r.stderr
+
+
+def test_cli_checkpoint_restart_writes_the_same_files(tmp_path):
+    """--stop-at + --checkpoint, then --restart: the .energy/.verify files equal, byte for byte,
+    those of the run that was never stopped (--exact: the same kernels in every process)."""
+    print("cli checkpoint", flush=True)
+    prob, _ = load_golden("abc_r12_nu033")
+    job = str(tmp_path / "job")
+    files.write_job(prob, job, exact_text=True)
+
+    def cli(*opts):
+        r = subprocess.run([EXE, *opts], cwd=tmp_path, capture_output=True, text=True, timeout=240)
+        assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+        return r
+
+    cli("_full", "job", "--vtk=off", "--exact")
+    cli("_split", "job", "--vtk=off", "--exact", "--stop-at=150", "--checkpoint=job.ckpt")
+    r = cli("_split", "job", "--vtk=off", "--exact", "--restart=job.ckpt")
+    assert "at step 150" in r.stdout
+    for ext in (".energy", ".verify"):
+        full = open(job + "_full" + ext).read()
+        split = open(job + "_split" + ext).read()
+        assert len(full.splitlines()) == prob.steps // 30 + 1
+        assert split == full, ext
+    # a checkpoint of another problem is refused with exit code 1
+    other, _ = load_golden("abc_r10_nu025")
+    files.write_job(other, str(tmp_path / "other"), exact_text=True)
+    r = subprocess.run([EXE, "_x", "other", "--vtk=off", "--exact", "--restart=job.ckpt"],
+                       cwd=tmp_path, capture_output=True, text=True, timeout=240)
+    assert r.returncode == 1 and "another problem" in r.stderr, r.stderr[-2000:]
