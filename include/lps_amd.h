@@ -50,6 +50,9 @@ typedef struct lps_problem {
   const double *ci_nodes_u; /* 2 constrains_no x 4 max_cloud_size (.ciNodesU) */
   const double *ci_nodes_v; /* 2 constrains_no x 4 max_cloud_size (.ciNodesV) */
   const double *ci_nodes_theta; /* constrains_no x 2 max_cloud_size (.ciNodesTheta); NULL for IDW */
+  int32_t prenotch_no;      /* .data1[2] PrenotchNo (0: no notch) */
+  const double *prenotches; /* prenotch_no x 4 (x0 y0 x1 y1), row-major (.prenotch): every bond
+                               whose segment meets one of them is cut (src/main.cpp:386-402) */
 } lps_problem;
 
 typedef struct lps_options {

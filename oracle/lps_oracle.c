@@ -17,6 +17,9 @@
  *                          plane-wave known answer F = M^LPS u (preprocessor/main.cpp:238-246).
  *   time loop, ABC, energy the reference solver binary (src/main.cpp compiled from its sources)
  *                          on generated cases: .energy/.verify series and final VTK fields.
+ *   prenotch visibility    the reference solver on a notched case (series and fields); the
+ *                          intersection test against exact rational orientation on random and
+ *                          degenerate (collinear, touching) segments.
  */
 #include "lps_oracle.h"
 
@@ -230,6 +233,60 @@ static inline void force_node(const oracle_ctx *c, const double *u, const double
 }
 
 /* ------------------------------------------------------------------------------------------- */
+/* Prenotch visibility (src/main.cpp:119-140, 386-402)                                          */
+/* ------------------------------------------------------------------------------------------- */
+/* Side of point (px, py) relative to the directed segment (x1, y1) -> (x2, y2): the sign of the
+ * cross product; on the segment's line, -1 before its start, +1 past its end, 0 on it
+ * (src/main.cpp:119-132). */
+static int side_of(double x1, double y1, double x2, double y2, double px, double py) {
+  const double dx = x2 - x1, dy = y2 - y1;
+  double qx = px - x1, qy = py - y1;
+  double c = qx * dy - qy * dx;
+  if (c == 0.0) {
+    c = qx * dx + qy * dy;
+    if (c > 0.0) {
+      qx -= dx;
+      qy -= dy;
+      c = qx * dx + qy * dy;
+      if (c < 0.0) c = 0.0;
+    }
+  }
+  return (c > 0.0) - (c < 0.0);
+}
+
+/* src/main.cpp:134-140: each segment's end points lie on both sides of (or on) the other's. */
+int oracle_segments_intersect(double x1, double y1, double x2, double y2, double x3, double y3,
+                              double x4, double y4) {
+  return side_of(x1, y1, x2, y2, x3, y3) * side_of(x1, y1, x2, y2, x4, y4) <= 0 &&
+         side_of(x3, y3, x4, y4, x1, y1) * side_of(x3, y3, x4, y4, x2, y2) <= 0;
+}
+
+/* Drops every bond i -> j (self excluded) that a prenotch segment crosses, keeping the order of
+ * the rest (src/main.cpp:386-402 marks them -1; every later loop skips -1 entries, which is the
+ * same as removing them). Returns the new nnz. */
+static int64_t cut_prenotch_bonds(int32_t N, const double *nodes, int32_t n_notch,
+                                  const double *notch, int64_t *hp, int32_t *hc) {
+  int64_t out = 0;
+  for (int32_t i = 0; i < N; i++) {
+    const int64_t b = hp[i], e = hp[i + 1];
+    hp[i] = out;
+    for (int64_t q = b; q < e; q++) {
+      const int32_t j = hc[q];
+      int cut = 0;
+      if (q > b)
+        for (int32_t k = 0; k < n_notch && !cut; k++)
+          cut = oracle_segments_intersect(nodes[2 * (int64_t)i], nodes[2 * (int64_t)i + 1],
+                                          nodes[2 * (int64_t)j], nodes[2 * (int64_t)j + 1],
+                                          notch[4 * k], notch[4 * k + 1], notch[4 * k + 2],
+                                          notch[4 * k + 3]);
+      if (!cut) hc[out++] = j;
+    }
+  }
+  hp[N] = out;
+  return out;
+}
+
+/* ------------------------------------------------------------------------------------------- */
 /* Setup (src/main.cpp:346-356, 408-486, 518-535, 598-684)                                     */
 /* ------------------------------------------------------------------------------------------- */
 static void *dup(const void *src, size_t bytes) {
@@ -265,6 +322,8 @@ oracle_ctx *oracle_create(const oracle_problem *p, int32_t threads) {
   if (nnz < 0) { oracle_destroy(c); return NULL; }
   c->hc = (int32_t *)malloc(sizeof(int32_t) * (size_t)(nnz > 0 ? nnz : 1));
   oracle_build_horizons(N, c->nodes, search_r, 128, c->hp, c->hc);
+  if (p->n_notch > 0 && p->notch)
+    cut_prenotch_bonds(N, c->nodes, p->n_notch, p->notch, c->hp, c->hc);
 
   /* Lame constants (src/main.cpp:215-217) */
   c->Lambda = p->E * p->nu / ((1.0 + p->nu) * (1.0 - 2.0 * p->nu));

@@ -28,6 +28,8 @@ typedef struct oracle_problem {
   const double *Gth;          /* C x 2M (NULL when theta_method == 0) */
   int32_t theta_method;       /* 1: G_theta (src/main.cpp:38 default), 0: IDW */
   int32_t abc_enabled;        /* 1: ABCS_STEP == 2, 0: ABCS_STEP == -1 */
+  int32_t n_notch;            /* prenotch segments (.data1[2]) */
+  const double *notch;        /* n_notch x 4: x0 y0 x1 y1 (.prenotch) */
 } oracle_problem;
 
 /* One output record, the quantities of src/main.cpp:986-1062. */
@@ -45,6 +47,11 @@ typedef struct oracle_ctx oracle_ctx;
  * than max_len entries. row_ptr has N+1 entries; cols is filled only when non-NULL. */
 int64_t oracle_build_horizons(int32_t N, const double *nodes, double r, int32_t max_len,
                               int64_t *row_ptr, int32_t *cols);
+
+/* Prenotch visibility (src/main.cpp:119-140, 386-402): 1 when the segment (x1,y1)-(x2,y2)
+ * intersects or touches the segment (x3,y3)-(x4,y4). */
+int oracle_segments_intersect(double x1, double y1, double x2, double y2, double x3, double y3,
+                              double x4, double y4);
 
 oracle_ctx *oracle_create(const oracle_problem *p, int32_t threads);
 void oracle_destroy(oracle_ctx *c);

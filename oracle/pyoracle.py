@@ -18,7 +18,8 @@ class OracleProblem(C.Structure):
                 ("nodes", C.c_void_p), ("bctypes", C.c_void_p), ("ic", C.c_void_p),
                 ("icd", C.c_void_p), ("abc", C.c_void_p), ("clouds", C.c_void_p),
                 ("Gu", C.c_void_p), ("Gv", C.c_void_p), ("Gth", C.c_void_p),
-                ("theta_method", C.c_int32), ("abc_enabled", C.c_int32)]
+                ("theta_method", C.c_int32), ("abc_enabled", C.c_int32),
+                ("n_notch", C.c_int32), ("notch", C.c_void_p)]
 
 
 class OracleDiag(C.Structure):
@@ -58,6 +59,8 @@ def lib():
         L.oracle_horizons.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.oracle_dilatation.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.oracle_force.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.oracle_segments_intersect.restype = C.c_int
+        L.oracle_segments_intersect.argtypes = [C.c_double] * 8
         _lib = L
     return _lib
 
@@ -76,6 +79,12 @@ def build_horizons(nodes: np.ndarray, r: float, max_len: int = 128):
     cols = np.zeros(max(nnz, 1), dtype=np.int32)
     lib().oracle_build_horizons(N, _ptr(nodes), r, max_len, _ptr(rp), _ptr(cols))
     return rp, cols[:nnz]
+
+
+def segments_intersect(a, b, c, d) -> bool:
+    """Prenotch visibility test of segment a-b against segment c-d (points as (x, y))."""
+    return bool(lib().oracle_segments_intersect(float(a[0]), float(a[1]), float(b[0]), float(b[1]),
+                                                float(c[0]), float(c[1]), float(d[0]), float(d[1])))
 
 
 def diag_to_dict(d: OracleDiag) -> dict:
@@ -113,6 +122,10 @@ class Oracle:
             p.Gth = _ptr(arr(prob.Gth, np.float64)) if prob.Gth.size else None
         p.theta_method = theta_method
         p.abc_enabled = 1 if (abc_enabled and Cn > 0) else 0
+        pre = np.asarray(getattr(prob, "prenotch", np.zeros((0, 4))), dtype=np.float64)
+        if pre.shape[0] > 0:
+            p.n_notch = pre.shape[0]
+            p.notch = _ptr(arr(pre.reshape(-1, 4), np.float64))
         self.N = N
         self.dt = float(d2[6])
         self._c = lib().oracle_create(C.byref(p), threads)

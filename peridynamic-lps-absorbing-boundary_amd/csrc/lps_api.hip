@@ -162,6 +162,8 @@ static int32_t create_impl(lps_ctx *c, const lps_problem *p) {
   if (c->opt.theta_method != LPS_THETA_GTHETA && c->opt.theta_method != LPS_THETA_IDW)
     return c->fail(LPS_ERR_ARG, "theta_method must be LPS_THETA_GTHETA or LPS_THETA_IDW");
   if (!(p->dx > 0.0) || !(p->dy > 0.0)) return c->fail(LPS_ERR_ARG, "dx and dy must be positive");
+  if (p->prenotch_no < 0 || (p->prenotch_no > 0 && !p->prenotches))
+    return c->fail(LPS_ERR_ARG, "prenotch_no must be >= 0, with prenotches set when positive");
   c->n = N;
   c->M = abc_on ? M : 0;
   c->E = p->E; c->nu = p->nu; c->rho = p->rho; c->dt = p->dt;
@@ -275,8 +277,12 @@ static int32_t create_impl(lps_ctx *c, const lps_problem *p) {
 
   // ---- horizons ----
   bool overflow = false;
-  CTX_TRY(c, build_horizons(NS, c->x, c->y, c->kind, c->perm_d, search_r, kMaxHorizon, &c->hb,
-                            &overflow, s));
+  double *d_notch = nullptr;
+  if (p->prenotch_no > 0)
+    CTX_TRY(c, c->upload(&d_notch, std::vector<double>(p->prenotches,
+                                                       p->prenotches + 4 * (size_t)p->prenotch_no)));
+  CTX_TRY(c, build_horizons(NS, c->x, c->y, c->kind, c->perm_d, search_r, kMaxHorizon, d_notch,
+                            p->prenotch_no, &c->hb, &overflow, s));
   if (overflow) return c->fail(LPS_ERR_HORIZON, "found_node_no > max_node_no; increase max_node_no.");
 
   // ---- m_i on the device, m_ref and the ABC override on the host (src/main.cpp:408-480) ----

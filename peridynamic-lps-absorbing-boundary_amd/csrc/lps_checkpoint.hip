@@ -48,6 +48,7 @@ uint64_t fingerprint(const lps_ctx *c) {
   f.value(c->opt.exact);
   f.value(c->opt.theta_method);
   f.value(c->opt.abc_enabled);
+  f.value(c->hb.nnz);  // the bonds, after any prenotch cut
   f.bytes(c->perm_h.data(), c->perm_h.size() * sizeof(int32_t));
   for (const lps::InstParams &p : c->inst_h) {
     f.value(p.dt);

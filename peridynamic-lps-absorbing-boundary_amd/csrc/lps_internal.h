@@ -186,10 +186,12 @@ struct HorizonBuild {
 hipError_t morton_order(int32_t n, const double *xo, const double *yo, const int8_t *seg,
                         const int32_t *seg_rank, double cell, int32_t *perm, hipStream_t s);
 // xs, ys: slot coordinates; kind: slot kind (0 = hole); perm: slot -> original index (ties of
-// equal distance are ranked by it). *overflow is set when a horizon exceeds max_len.
+// equal distance are ranked by it). *overflow is set when a horizon exceeds max_len (before the
+// notches cut any bond, as in the reference). notch: n_notch x 4 device array of prenotch
+// segments (x0, y0, x1, y1); the bonds they cut are left out of the rows.
 hipError_t build_horizons(int32_t n_slots, const double *xs, const double *ys, const int8_t *kind,
-                          const int32_t *perm, double r, int32_t max_len, HorizonBuild *hb,
-                          bool *overflow, hipStream_t s);
+                          const int32_t *perm, double r, int32_t max_len, const double *notch,
+                          int32_t n_notch, HorizonBuild *hb, bool *overflow, hipStream_t s);
 void free_horizons(HorizonBuild *hb);
 
 }  // namespace lps

@@ -14,7 +14,10 @@
 //      (no contraction, correctly rounded sqrt) as the reference, so the sets are identical;
 //   4. each row is ranked by (distance, original id) - one wavefront per row, ranks by counting -
 //      which puts self first;
-//   5. the CSR is repacked as SELL-64 without self for the step kernels.
+//   5. prenotch visibility (src/main.cpp:386-402): bonds whose segment meets a notch segment are
+//      dropped from the sorted rows, the rest keep their order (the reference marks them -1 and
+//      every later loop skips them, which is the same);
+//   6. the CSR is repacked as SELL-64 without self for the step kernels.
 #include <hipcub/hipcub.hpp>
 
 #include "lps_internal.h"
@@ -134,6 +137,59 @@ __global__ void k_search(int32_t n, const double *xn, const double *yn, const in
     len_or_ptr[k] = cnt;
     if (cnt > max_len) atomicExch(overflow, 1);
   }
+}
+
+// Side of (px, py) relative to the directed segment (x1, y1) -> (x2, y2): the sign of the cross
+// product; on the segment's line -1 before its start, +1 past its end, 0 on it
+// (src/main.cpp:119-132, the same FP64 operations without contraction).
+__device__ __forceinline__ int notch_side(double x1, double y1, double x2, double y2, double px,
+                                          double py) {
+#pragma clang fp contract(off)
+  const double dx = x2 - x1, dy = y2 - y1;
+  double qx = px - x1, qy = py - y1;
+  double c = qx * dy - qy * dx;
+  if (c == 0.0) {
+    c = qx * dx + qy * dy;
+    if (c > 0.0) {
+      qx -= dx;
+      qy -= dy;
+      c = qx * dx + qy * dy;
+      if (c < 0.0) c = 0.0;
+    }
+  }
+  return (c > 0.0) - (c < 0.0);
+}
+
+// The bond i -> j meets (or touches) notch segment k (src/main.cpp:134-140).
+__device__ __forceinline__ bool bond_cut(double xi, double yi, double xj, double yj,
+                                         const double *nt) {
+  return notch_side(xi, yi, xj, yj, nt[0], nt[1]) * notch_side(xi, yi, xj, yj, nt[2], nt[3]) <= 0 &&
+         notch_side(nt[0], nt[1], nt[2], nt[3], xi, yi) *
+                 notch_side(nt[0], nt[1], nt[2], nt[3], xj, yj) <= 0;
+}
+
+// Rows with the notched bonds dropped: FILL = false writes each row's new length, FILL = true
+// copies the kept entries in order to their new place. One thread per row; self (entry 0) stays.
+template <bool FILL>
+__global__ void k_notch_rows(int32_t n, const double *xn, const double *yn, const int64_t *row_ptr,
+                             const int32_t *col, const double *notch, int32_t n_notch,
+                             int64_t *len_or_ptr, int32_t *col_out) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t b = row_ptr[i], e = row_ptr[i + 1];
+  const double xi = xn[i], yi = yn[i];
+  int64_t out = FILL ? len_or_ptr[i] : 0;
+  int32_t kept = 0;
+  for (int64_t q = b; q < e; q++) {
+    const int32_t j = col[q];
+    bool cut = false;
+    if (q > b)
+      for (int32_t k = 0; k < n_notch && !cut; k++) cut = bond_cut(xi, yi, xn[j], yn[j], notch + 4 * k);
+    if (cut) continue;
+    if (FILL) col_out[out + kept] = j;
+    kept++;
+  }
+  if (!FILL) len_or_ptr[i] = kept;
 }
 
 // Rank each row by (distance, original id): one wavefront per row, ranks by counting.
@@ -262,8 +318,8 @@ hipError_t morton_order(int32_t n, const double *xo, const double *yo, const int
 }
 
 hipError_t build_horizons(int32_t n, const double *xn, const double *yn, const int8_t *kind,
-                          const int32_t *perm, double r, int32_t max_len, HorizonBuild *hb,
-                          bool *overflow, hipStream_t s) {
+                          const int32_t *perm, double r, int32_t max_len, const double *notch,
+                          int32_t n_notch, HorizonBuild *hb, bool *overflow, hipStream_t s) {
   *overflow = false;
   Scratch sc;
   // bounding box of the nodes (holes excluded)
@@ -352,6 +408,36 @@ hipError_t build_horizons(int32_t n, const double *xn, const double *yn, const i
   hipLaunchKernelGGL(k_sort_rows, dim3(grid_of(n, 4)), dim3(256), 0, s, n, hb->row_ptr, cand_col,
                      cand_dist, perm, hb->col);
   LPS_TRY(hipGetLastError());
+
+  // prenotch visibility: count the kept bonds per row, scan, copy them in order
+  if (n_notch > 0) {
+    int64_t *klen, *kptr;
+    int32_t *kcol;
+    LPS_TRY(sc.get(&klen, n + 1));
+    LPS_TRY(hipMemsetAsync(klen, 0, sizeof(int64_t) * (n + 1), s));
+    hipLaunchKernelGGL(k_notch_rows<false>, dim3(grid_of(n)), dim3(256), 0, s, n, xn, yn,
+                       hb->row_ptr, hb->col, notch, n_notch, klen, nullptr);
+    LPS_TRY(hipGetLastError());
+    LPS_TRY(hipMalloc((void **)&kptr, sizeof(int64_t) * (n + 1)));
+    tmp_bytes = 0;
+    LPS_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, klen, kptr, n + 1, s));
+    char *tmp5;
+    LPS_TRY(sc.get(&tmp5, tmp_bytes));
+    LPS_TRY(hipcub::DeviceScan::ExclusiveSum(tmp5, tmp_bytes, klen, kptr, n + 1, s));
+    int64_t knnz = 0;
+    LPS_TRY(hipMemcpyAsync(&knnz, kptr + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    LPS_TRY(hipStreamSynchronize(s));
+    LPS_TRY(hipMalloc((void **)&kcol, sizeof(int32_t) * std::max<int64_t>(knnz, 1)));
+    hipLaunchKernelGGL(k_notch_rows<true>, dim3(grid_of(n)), dim3(256), 0, s, n, xn, yn,
+                       hb->row_ptr, hb->col, notch, n_notch, kptr, kcol);
+    LPS_TRY(hipGetLastError());
+    LPS_TRY(hipStreamSynchronize(s));
+    (void)hipFree(hb->row_ptr);
+    (void)hipFree(hb->col);
+    hb->row_ptr = kptr;
+    hb->col = kcol;
+    hb->nnz = knnz;
+  }
 
   // SELL-64 without self
   const int32_t nslices = (n + kSlice - 1) / kSlice;

@@ -60,7 +60,7 @@ struct Job {
   std::string prefix, name;
   int32_t node_no = 0, step_no = 0, constrains_no = 0, max_cloud = 0;
   std::vector<int32_t> data1;
-  std::vector<double> data2, nodes, bcvalues, icd, ic, Gu, Gv, Gth;
+  std::vector<double> data2, nodes, bcvalues, icd, ic, Gu, Gv, Gth, notches;
   std::vector<int32_t> bctypes, iclouds, infnodes;
   double exclude_r = -1.0;
   std::ofstream energy, verify;
@@ -75,11 +75,10 @@ struct Job {
     const int32_t prenotch_no = data1[2];
     constrains_no = data1[4];
     max_cloud = data1[5];
-    if (prenotch_no > 0) {
-      std::cerr << "FATAL: prenotch bond cutting is not supported by this build" << std::endl;
-      return 1;
-    }
     const int64_t N = node_no;
+    // prenotch segments, read only when the count is positive (src/main.cpp:251-254)
+    if (prenotch_no > 0 && !lpsio::read_doubles(J + ".prenotch", 4 * (int64_t)prenotch_no, notches))
+      return fatal_read(J, ".prenotch");
     if (!lpsio::read_doubles(J + ".nodes", 2 * N, nodes)) return fatal_read(J, ".nodes");
     if (!lpsio::read_ints(J + ".bctypes", 2 * N, bctypes)) return fatal_read(J, ".bctypes");
     if (!lpsio::read_doubles(J + ".bcvalues", 2 * N, bcvalues)) return fatal_read(J, ".bcvalues");
@@ -127,6 +126,8 @@ struct Job {
     p.ci_nodes_u = Gu.empty() ? nullptr : Gu.data();
     p.ci_nodes_v = Gv.empty() ? nullptr : Gv.data();
     p.ci_nodes_theta = Gth.empty() ? nullptr : Gth.data();
+    p.prenotch_no = (int32_t)(notches.size() / 4);
+    p.prenotches = notches.empty() ? nullptr : notches.data();
     return p;
   }
 

@@ -40,7 +40,8 @@ class LpsProblem(C.Structure):
                 ("nodes", C.c_void_p), ("bctypes", C.c_void_p), ("icvalues", C.c_void_p),
                 ("icdvalues", C.c_void_p), ("inf_nodes", C.c_void_p), ("iclouds", C.c_void_p),
                 ("ci_nodes_u", C.c_void_p), ("ci_nodes_v", C.c_void_p),
-                ("ci_nodes_theta", C.c_void_p)]
+                ("ci_nodes_theta", C.c_void_p), ("prenotch_no", C.c_int32),
+                ("prenotches", C.c_void_p)]
 
 
 class LpsOptions(C.Structure):
@@ -173,6 +174,10 @@ def _lps_problem(prob, keep: List[np.ndarray]) -> "LpsProblem":
         p.ci_nodes_v = _ptr(arr(prob.Gv, np.float64))
         if prob.Gth is not None and prob.Gth.size:
             p.ci_nodes_theta = _ptr(arr(prob.Gth, np.float64))
+    pre = np.asarray(getattr(prob, "prenotch", np.zeros((0, 4))), dtype=np.float64)
+    if pre.shape[0] > 0:
+        p.prenotch_no = pre.shape[0]
+        p.prenotches = _ptr(arr(pre.reshape(-1, 4), np.float64))
     return p
 
 

@@ -32,6 +32,8 @@ def load_golden(name):
                    ic=z["ic"], icd=z["icd"], abc=z["abc"].astype(np.int64),
                    clouds=z["clouds"].astype(np.int64), Gu=Gu, Gv=Gv, Gth=Gth,
                    info={"golden": name})
+    if "prenotch" in z:
+        prob.prenotch = z["prenotch"]
     ref = {k: z[k] for k in ("energy", "verify", "vtk_points", "vtk_u", "vtk_v", "vtk_theta")}
     ref["vtk_step"] = int(z["vtk_step"])
     return prob, ref

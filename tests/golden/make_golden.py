@@ -1,13 +1,16 @@
 """Writes the golden fixtures of tests/golden/ from the reference's own programs.
 
 Run in a container where the reference tree and its builds exist (`make -C oracle`):
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [fixture ...]      (default: all)
 
 Every fixture is data: inputs and the outputs the reference produced for them.
   abc_*.npz      a generated job (ABC operators rounded to float32 and written with 17 digits, so
                  the text the reference reads and the arrays stored here are the same doubles) and
                  the reference solver's .energy / .verify series plus the fields of its last VTK
                  snapshot (oracle/_ref/PD2_LPS_ABC, built from src/main.cpp).
+  notch_r12_nu025.npz  a generated job with two prenotch segments (one crossing the lattice at
+                 generic angles, one along a lattice line with end points on nodes) and the
+                 reference solver's outputs for it (src/main.cpp:386-402 cuts the bonds).
   noabc_r30.npz  the input of tests/generate_test_data.cpp (run as built from the reference source)
                  and the reference solver's outputs for it.
   kdtree_sets.npz  neighbour lists of the reference's build_clouds
@@ -56,6 +59,8 @@ def save_case(name: str, q: pp.Problem, energy, verify, last, u, v, th, g32: boo
                   ic=q.ic, icd=q.icd, abc=q.abc.astype(np.int32), clouds=q.clouds.astype(np.int32),
                   energy=energy, verify=verify, vtk_step=np.int64(last), vtk_points=u["points"],
                   vtk_u=u["vectors"], vtk_v=v["vectors"], vtk_theta=th["scalars"])
+    if q.prenotch.shape[0] > 0:
+        arrays.update(prenotch=q.prenotch)
     if q.C > 0:
         if g32:  # the float32 values, exact in double
             assert np.array_equal(q.Gu.astype(np.float32).astype(np.float64), q.Gu)
@@ -67,8 +72,10 @@ def save_case(name: str, q: pp.Problem, energy, verify, last, u, v, th, g32: boo
     print(name, q.N, q.C, q.steps, os.path.getsize(os.path.join(OUT, name + ".npz")))
 
 
-def abc_case(name, R, nu, steps, pulse):
+def abc_case(name, R, nu, steps, pulse, prenotch=None):
     p = pp.make_problem(R, nu, steps=steps, pulses=[pulse], workers=8)
+    if prenotch is not None:
+        p.prenotch = np.asarray(prenotch, dtype=np.float64).reshape(-1, 4)
     p.Gu = p.Gu.astype(np.float32).astype(np.float64)
     p.Gv = p.Gv.astype(np.float32).astype(np.float64)
     p.Gth = p.Gth.astype(np.float32).astype(np.float64)
@@ -133,7 +140,16 @@ if __name__ == "__main__":
     for f in (REF_BIN, REF_GEN, REF_KD):
         if not os.path.exists(f):
             sys.exit(f"{f} missing: run `make -C oracle` where the reference tree is present")
-    kdtree_sets()
-    noabc_case()
-    abc_case("abc_r10_nu025", 10.0, 0.25, 600, pp.Pulse(3.0, -2.0, 4.0, 1.0, (1.0, 1.0)))
-    abc_case("abc_r12_nu033", 12.0, 0.33, 300, pp.Pulse(3.6, -2.4, 4.8, 1.0, (1.0, 1.0)))
+    cases = {
+        "kdtree_sets": kdtree_sets,
+        "noabc_r30": noabc_case,
+        "abc_r10_nu025": lambda: abc_case("abc_r10_nu025", 10.0, 0.25, 600,
+                                          pp.Pulse(3.0, -2.0, 4.0, 1.0, (1.0, 1.0))),
+        "abc_r12_nu033": lambda: abc_case("abc_r12_nu033", 12.0, 0.33, 300,
+                                          pp.Pulse(3.6, -2.4, 4.8, 1.0, (1.0, 1.0))),
+        "notch_r12_nu025": lambda: abc_case(
+            "notch_r12_nu025", 12.0, 0.25, 450, pp.Pulse(-1.5, 3.5, 4.0, 1.0, (1.0, 1.0)),
+            prenotch=[[-4.1, 0.13, 3.7, 1.9], [0.0, -3.0, 0.0, -6.0]]),
+    }
+    for name in sys.argv[1:] or list(cases):
+        cases[name]()

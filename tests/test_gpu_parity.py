@@ -11,7 +11,7 @@ from lps_amd import preprocess as pp, solver
 
 pytestmark = pytest.mark.gpu
 
-CASES = ["noabc_r30", "abc_r10_nu025", "abc_r12_nu033"]
+CASES = ["noabc_r30", "abc_r10_nu025", "abc_r12_nu033", "notch_r12_nu025"]
 
 
 def say(*a):

@@ -7,7 +7,7 @@ from conftest import (ENERGY_KEYS, VERIFY_KEYS, assert_series_close, load_golden
 
 import pyoracle
 
-CASES = ["noabc_r30", "abc_r10_nu025", "abc_r12_nu033"]
+CASES = ["noabc_r30", "abc_r10_nu025", "abc_r12_nu033", "notch_r12_nu025"]
 
 
 @pytest.fixture(scope="module", params=CASES)
@@ -50,3 +50,16 @@ def test_horizon_sets_match_reference_kdtree(name):
         assert np.array_equal(np.sort(mine), np.sort(theirs)), i
         d = np.hypot(*(nodes[mine] - nodes[i]).T)
         assert np.all(np.diff(d) >= 0)
+
+
+def test_prenotch_fixture_is_not_vacuous():
+    """The notched case's two segments cut bonds (src/main.cpp:386-402), and without them the
+    oracle's series would miss the reference's by far more than the tolerance."""
+    prob, ref = load_golden("notch_r12_nu025")
+    assert prob.prenotch.shape == (2, 4)
+    cut = pyoracle.Oracle(prob, threads=4)
+    prob.prenotch = np.zeros((0, 4))
+    whole = pyoracle.Oracle(prob, threads=4)
+    assert whole.horizons()[0][-1] - cut.horizons()[0][-1] > 1000
+    recs = whole.run(prob.steps)
+    assert rel_l2(records_table(recs, ["energy"])[:, 0], ref["energy"][:, 1]) > 1e-2

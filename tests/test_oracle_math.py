@@ -208,3 +208,57 @@ def test_idw_abc_dilatation_of_an_isotropic_strain():
     is_abc[abc] = True
     assert np.allclose(th, 2 * eps, rtol=1e-12, atol=0)
     assert np.max(np.abs(direct[is_abc] / (2 * eps) - 1)) > 1e-2
+
+
+def _exact_intersect(a, b, c, d):
+    """Closed-segment intersection in exact rational arithmetic, by another method than the
+    oracle's: solve a + t (b - a) = c + s (d - c) for t, s in [0, 1]; parallel segments intersect
+    when they are collinear and their projections on the common line overlap."""
+    from fractions import Fraction as F
+    a, b, c, d = ([F(v) for v in p] for p in (a, b, c, d))
+    r = (b[0] - a[0], b[1] - a[1])
+    q = (d[0] - c[0], d[1] - c[1])
+    w = (c[0] - a[0], c[1] - a[1])
+    den = r[0] * q[1] - r[1] * q[0]
+    if den != 0:
+        t = (w[0] * q[1] - w[1] * q[0]) / den
+        s = (w[0] * r[1] - w[1] * r[0]) / den
+        return 0 <= t <= 1 and 0 <= s <= 1
+    if w[0] * r[1] - w[1] * r[0] != 0 or (r == (0, 0) and q == (0, 0) and w != (0, 0)):
+        return False  # parallel on different lines, or two distinct points
+    # collinear: overlap of the projections on the line's dominant axis
+    k = 0 if max(abs(r[0]), abs(q[0]), abs(w[0])) > 0 else 1
+    lo1, hi1 = sorted((a[k], b[k]))
+    lo2, hi2 = sorted((c[k], d[k]))
+    return max(lo1, lo2) <= min(hi1, hi2)
+
+
+def test_prenotch_intersection_exact_on_grid_points():
+    """On coordinates that are multiples of 0.5 every product in the test is exact, so the
+    visibility test (src/main.cpp:119-140) must equal the exact predicate, including collinear,
+    touching and overlapping segments."""
+    rng = np.random.default_rng(7)
+    pts = rng.integers(-4, 5, size=(20000, 8)) * 0.5
+    # force many degenerate configurations: shared end points and collinear pieces
+    pts[::7, 4:6] = pts[::7, 0:2]
+    pts[1::7, 4:6] = 2 * pts[1::7, 2:4] - pts[1::7, 0:2]
+    pts[2::7, 6:8] = (pts[2::7, 0:2] + pts[2::7, 2:4]) / 2
+    pts[3::7, 1] = pts[3::7, 3] = pts[3::7, 5] = pts[3::7, 7] = 1.0
+    agree, hits = 0, 0
+    for row in pts:
+        a, b, c, d = row[0:2], row[2:4], row[4:6], row[6:8]
+        if np.array_equal(a, b) or np.array_equal(c, d):
+            continue  # zero-length segments: not a bond or a notch
+        want = _exact_intersect(a, b, c, d)
+        assert pyoracle.segments_intersect(a, b, c, d) == want, (a, b, c, d, want)
+        agree += 1
+        hits += want
+    assert agree > 15000 and 0.2 < hits / agree < 0.8
+
+
+def test_prenotch_intersection_random_segments():
+    rng = np.random.default_rng(8)
+    for row in rng.uniform(-3.0, 3.0, size=(5000, 8)):
+        a, b, c, d = row[0:2], row[2:4], row[4:6], row[6:8]
+        assert pyoracle.segments_intersect(a, b, c, d) == _exact_intersect(a, b, c, d)
+        assert pyoracle.segments_intersect(b, a, d, c) == _exact_intersect(a, b, c, d)
