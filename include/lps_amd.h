@@ -290,6 +290,50 @@ int32_t lps_closure_guard(lps_ctx *ctx, double tol, int64_t *fallback_step);
  * 0, 0); any pointer may be NULL. */
 int32_t lps_closure_inspect(lps_ctx *ctx, uint16_t *x, float *scale2, float *applied8);
 
+/* ------------------------------------------------------------------------------------------------
+ * Least-squares ABC operators on the GPU: the per-row solves of the reference's preprocessor
+ * (G_u / G_v preprocessor/main.cpp:415-637, G_theta :657-785, the dgelsd solve :368-402) for every
+ * ABC row at once. The caller gives the wave modes (the dispersion relation, :199-356) and the rows'
+ * clouds (:850-902); the errors of these calls are reported by lps_global_error(). */
+typedef struct lps_abc_modes {
+  int32_t count;           /* wave modes, in the order of the system's rows (4 rows per mode) */
+  const double *a1, *a2;   /* wave vector (count each) */
+  const double *om_p, *om_s; /* P and S angular frequencies */
+  const double *g_p, *g_s; /* P and S polarisations, count x 2 */
+  const double *s_r, *s_i; /* dilatation factor S, real and imaginary parts, count x 2 */
+} lps_abc_modes;
+
+typedef struct lps_abc_rows {
+  int32_t count;           /* ABC rows */
+  int32_t max_cloud;       /* cloud slots per row (maxCloudSize) */
+  const double *xy;        /* count x 2: the ABC node */
+  const double *tau;       /* count: its outward angle */
+  const int32_t *cloud_len; /* count: cloud size, 1 .. max_cloud */
+  const double *cloud_xy;  /* count x max_cloud x 2: the cloud nodes (the first cloud_len valid) */
+} lps_abc_rows;
+
+/* G_u, G_v (2 count x 4 max_cloud each, the .ciNodesU / .ciNodesV layout) and, when gth is not
+ * NULL, G_theta (count x 2 max_cloud, .ciNodesTheta) for time step dt; theta_coeff = 2 / m_ref.
+ * Minimum-norm least squares with the singular values s_i <= rcond s_1 dropped, as dgelsd
+ * (rcond 1e-14 in the reference). rank (count x 3, may be NULL) receives the number of singular
+ * values kept for G_u, G_v, G_theta, or -1 where the SVD did not converge. */
+int32_t lps_abc_operators(int32_t device, const lps_abc_rows *rows, const lps_abc_modes *modes,
+                          double dt, double theta_coeff, double rcond, double *gu, double *gv,
+                          double *gth, int32_t *rank);
+/* The least-squares systems of rows first .. first + count - 1 for one operator (kind 0 G_u,
+ * 1 G_v, 2 G_theta): A (count x m x n, each column-major, m = 4 modes, n = 4 or 2 max_cloud) and
+ * the right-hand sides B (count x m x nrhs, nrhs 2 or 1). */
+int32_t lps_abc_system(int32_t device, const lps_abc_rows *rows, const lps_abc_modes *modes,
+                       double dt, double theta_coeff, int32_t kind, int32_t first, int32_t count,
+                       double *A, double *B);
+/* The batched solver under lps_abc_operators on caller systems: for each of batch problems,
+ * X (n x nrhs) = minimum-norm least-squares solution of A (m x n) X = B (m x nrhs), all
+ * column-major and contiguous per problem, singular values s_i <= rcond s_1 dropped. rank (batch,
+ * may be NULL) as in lps_abc_operators. */
+int32_t lps_lstsq_batched(int32_t device, int32_t m, int32_t n, int32_t nrhs, int32_t batch,
+                          const double *A, const double *B, double rcond, double *X,
+                          int32_t *rank);
+
 #ifdef __cplusplus
 }
 #endif

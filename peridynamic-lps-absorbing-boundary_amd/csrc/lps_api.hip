@@ -22,9 +22,9 @@
 
 using namespace lps;
 
-namespace {
-thread_local std::string g_create_error;
-}  // namespace
+namespace lps {
+thread_local std::string g_create_error;  // message of the last failed call without a context
+}  // namespace lps
 
 StepArgs lps_ctx::args() const {
   StepArgs a{};

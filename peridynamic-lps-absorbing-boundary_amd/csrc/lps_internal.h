@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <string>
 
 namespace lps {
 
@@ -193,5 +194,8 @@ hipError_t build_horizons(int32_t n_slots, const double *xs, const double *ys, c
                           const int32_t *perm, double r, int32_t max_len, const double *notch,
                           int32_t n_notch, HorizonBuild *hb, bool *overflow, hipStream_t s);
 void free_horizons(HorizonBuild *hb);
+
+// Message of the last failed call that has no context to hold it (lps_global_error).
+extern thread_local std::string g_create_error;
 
 }  // namespace lps

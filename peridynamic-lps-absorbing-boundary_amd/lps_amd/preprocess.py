@@ -560,22 +560,43 @@ def make_problem(R: float, nu: float = MAT_NU, pulses: Optional[Sequence[Pulse]]
                  steps: Optional[int] = None, dt: Optional[float] = None,
                  dx: float = DX_DEFAULT, delta: Optional[float] = None,
                  workers: Optional[int] = None, with_abc: bool = True,
-                 dk_factor: Optional[float] = None, robust_ls: bool = False) -> Problem:
+                 dk_factor: Optional[float] = None, robust_ls: bool = False,
+                 ls: str = "cpu", device: int = 0) -> Problem:
     """Generate a full solver input set for a circular lattice domain of radius R.
 
     dt=None runs the reference's (dt, delta_kappa) sweep (preprocessor/main.cpp:1047-1174);
     otherwise dt and dk_factor (delta_kappa = dk_factor / sqrt(c_P), default 1) are taken as given,
     e.g. a sweep result computed once. steps=None takes ceil(210 us / dt)
     (preprocessor/main.cpp:1177-1179). robust_ls=True re-solves a least-squares problem on which
-    dgelsd does not converge with dgelss (the reference keeps dgelsd's output)."""
+    dgelsd does not converge with dgelss (the reference keeps dgelsd's output). ls="gpu" builds
+    the operators (and the sweep's) on the GPU (lps_amd.abc_gpu) instead of with LAPACK."""
+    if ls not in ("cpu", "gpu"):
+        raise ValueError("ls must be 'cpu' or 'gpu'")
     _ROBUST_LS[0] = robust_ls
     try:
-        return _make_problem(R, nu, pulses, steps, dt, dx, delta, workers, with_abc, dk_factor)
+        return _make_problem(R, nu, pulses, steps, dt, dx, delta, workers, with_abc, dk_factor,
+                             ls, device)
     finally:
         _ROBUST_LS[0] = False
 
 
-def _make_problem(R, nu, pulses, steps, dt, dx, delta, workers, with_abc, dk_factor) -> Problem:
+def _sweep_gpu(nodes, abc_idx, tau, clouds, M, sample, sweep_modes, m_ref, device):
+    """max ||G|| over the sample rows for every (delta_kappa, dt) of the sweep, on the GPU."""
+    from . import abc_gpu
+    sub = np.asarray(sample, dtype=np.int64)
+    cl = [clouds[i] for i in sub]
+    res = []
+    for idk in range(len(DK_FACTORS)):
+        for dtv in DT_VALUES:
+            Gu, Gv, _, _ = abc_gpu.operators(nodes, abc_idx[sub], tau[sub], cl, M, sweep_modes[idk],
+                                             dtv, m_ref, with_theta=False, device=device)
+            res.append(max(g_norm(Gu[2 * i:2 * i + 2], Gv[2 * i:2 * i + 2])
+                           for i in range(len(sub))))
+    return res
+
+
+def _make_problem(R, nu, pulses, steps, dt, dx, delta, workers, with_abc, dk_factor, ls="cpu",
+                  device=0) -> Problem:
     delta = 4.0 * dx if delta is None else delta
     workers = default_workers() if workers is None else workers
     mc = material(nu=nu, delta=delta, dx=dx, dy=dx)
@@ -602,7 +623,11 @@ def _make_problem(R, nu, pulses, steps, dt, dx, delta, workers, with_abc, dk_fac
             _W["sweep_modes"] = [compute_all_modes(mc, tg, dk0 * f).sorted_by_kappa()
                                  for f in DK_FACTORS]
             jobs = [(idk, dtv) for idk in range(len(DK_FACTORS)) for dtv in DT_VALUES]
-            res = _pool_map(_w_sweep, jobs, workers)
+            if ls == "gpu":
+                res = _sweep_gpu(nodes, abc_idx, tau, clouds, M, sample, _W["sweep_modes"],
+                                 mc.m_ref, device)
+            else:
+                res = _pool_map(_w_sweep, jobs, workers)
             best = None
             for (idk, dtv), mg in zip(jobs, res):
                 if mg <= 1.0 and (best is None or mg > best[2]):
@@ -623,14 +648,21 @@ def _make_problem(R, nu, pulses, steps, dt, dx, delta, workers, with_abc, dk_fac
         modes = compute_all_modes(mc, tg, dk).sorted_by_kappa()
         _W.update(modes=modes, dt=dt)
         C = len(abc_idx)
-        uv = _pool_map(_w_uv, list(range(C)), workers)
-        th = _pool_map(_w_theta, list(range(C)), workers)
-        Gu = np.zeros((2 * C, 4 * M))
-        Gv = np.zeros((2 * C, 4 * M))
-        for i, (gu, gv) in enumerate(uv):
-            Gu[2 * i:2 * i + 2] = gu
-            Gv[2 * i:2 * i + 2] = gv
-        Gth = np.stack(th) if C else np.zeros((0, 2 * M))
+        if ls == "gpu":
+            from . import abc_gpu
+            Gu, Gv, Gth, rank = abc_gpu.operators(nodes, abc_idx, tau, clouds, M, modes, dt,
+                                                  mc.m_ref, device=device)
+            info["ls"] = "gpu"
+            info["ls_unconverged"] = int(np.sum(rank < 0))
+        else:
+            uv = _pool_map(_w_uv, list(range(C)), workers)
+            th = _pool_map(_w_theta, list(range(C)), workers)
+            Gu = np.zeros((2 * C, 4 * M))
+            Gv = np.zeros((2 * C, 4 * M))
+            for i, (gu, gv) in enumerate(uv):
+                Gu[2 * i:2 * i + 2] = gu
+                Gv[2 * i:2 * i + 2] = gv
+            Gth = np.stack(th) if C else np.zeros((0, 2 * M))
         cl = -np.ones((C, M), dtype=np.int64)
         for i, c in enumerate(clouds):
             cl[i, :len(c)] = c

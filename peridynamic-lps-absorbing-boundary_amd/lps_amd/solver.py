@@ -44,6 +44,16 @@ class LpsProblem(C.Structure):
                 ("prenotches", C.c_void_p)]
 
 
+class LpsAbcModes(C.Structure):
+    _fields_ = [("count", C.c_int32)] + [(n, C.c_void_p) for n in (
+        "a1", "a2", "om_p", "om_s", "g_p", "g_s", "s_r", "s_i")]
+
+
+class LpsAbcRows(C.Structure):
+    _fields_ = [("count", C.c_int32), ("max_cloud", C.c_int32), ("xy", C.c_void_p),
+                ("tau", C.c_void_p), ("cloud_len", C.c_void_p), ("cloud_xy", C.c_void_p)]
+
+
 class LpsOptions(C.Structure):
     _fields_ = [("theta_method", C.c_int32), ("abc_enabled", C.c_int32), ("exact", C.c_int32),
                 ("use_graph", C.c_int32), ("device", C.c_int32), ("horizon_radius", C.c_double),
@@ -65,6 +75,13 @@ SIGNATURES = {
     "lps_create": (C.c_int32, [C.POINTER(LpsProblem), C.POINTER(LpsOptions), C.POINTER(C.c_void_p)]),
     "lps_reset": (C.c_int32, [C.c_void_p]),
     "lps_set_initial": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "lps_abc_operators": (C.c_int32, [C.c_int32, C.POINTER(LpsAbcRows), C.POINTER(LpsAbcModes),
+                                      C.c_double, C.c_double, C.c_double] + [C.c_void_p] * 4),
+    "lps_abc_system": (C.c_int32, [C.c_int32, C.POINTER(LpsAbcRows), C.POINTER(LpsAbcModes),
+                                   C.c_double, C.c_double, C.c_int32, C.c_int32, C.c_int32,
+                                   C.c_void_p, C.c_void_p]),
+    "lps_lstsq_batched": (C.c_int32, [C.c_int32] * 5 + [C.c_void_p, C.c_void_p, C.c_double,
+                                                        C.c_void_p, C.c_void_p]),
     "lps_checkpoint_save": (C.c_int32, [C.c_void_p, C.c_char_p]),
     "lps_checkpoint_load": (C.c_int32, [C.c_void_p, C.c_char_p]),
     "lps_create_batch": (C.c_int32, [C.c_void_p, C.c_int32, C.POINTER(LpsOptions),
