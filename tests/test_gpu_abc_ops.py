@@ -26,12 +26,13 @@ def _orth(rng, n, k):
     return q
 
 
-@pytest.mark.parametrize("m,n", [(30, 12), (12, 30), (224, 196)])
-def test_lstsq_known_minimum_norm_solutions(m, n):
+@pytest.mark.parametrize("m,n,low", [(30, 12, 0), (12, 30, 0), (224, 196, 0), (224, 196, 24)])
+def test_lstsq_known_minimum_norm_solutions(m, n, low):
     """A = U diag(s) V' with prescribed singular values, some below rcond s_1: the solution is
-    V diag(1 / s_i, s_i > rcond s_1) U' B."""
-    print("lstsq", m, n, flush=True)
-    rng = np.random.default_rng(m * 1000 + n)
+    V diag(1 / s_i, s_i > rcond s_1) U' B. low > 0: numerical rank low (the range-compressed path
+    of large systems); otherwise most singular values are kept (the full SVD)."""
+    print("lstsq", m, n, low, flush=True)
+    rng = np.random.default_rng(m * 1000 + n + low)
     k = min(m, n)
     batch, nrhs = 5, 2
     A = np.zeros((batch, m, n))
@@ -40,6 +41,9 @@ def test_lstsq_known_minimum_norm_solutions(m, n):
     want_rank = np.zeros(batch, dtype=np.int32)
     for p in range(batch):
         s = np.logspace(0, -8, k) if p % 2 == 0 else np.logspace(0, -20, k)
+        if low:
+            s = np.concatenate([np.logspace(0, -8 if p % 2 == 0 else -12, low),
+                                np.full(k - low, 1e-18)])
         s[(s > 1e-16) & (s < 1e-12)] = 1e-18  # none near the cut: rounding of A moves them
         U, V = _orth(rng, m, k), _orth(rng, n, k)
         A[p] = (U * s) @ V.T
