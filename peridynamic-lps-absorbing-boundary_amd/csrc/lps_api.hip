@@ -130,6 +130,8 @@ void lps_destroy(lps_ctx *ctx) {
   if (ctx->stream2) (void)hipStreamSynchronize(ctx->stream2);
   closure_free(ctx);
   if (ctx->graph30) (void)hipGraphExecDestroy(ctx->graph30);
+  for (hipGraphExec_t g : ctx->graph30t)
+    if (g) (void)hipGraphExecDestroy(g);
   for (hipEvent_t e : ctx->ev) (void)hipEventDestroy(e);
   for (void *p : ctx->allocs) (void)hipFree(p);
   free_horizons(&ctx->hb);
@@ -995,14 +997,20 @@ static int32_t choose_variant(lps_ctx *c) {
 
 constexpr int kEv = 4;  // events per timed step
 
+// A timing event on the context's stream; under capture it becomes an event-record node of the
+// graph (hipEventRecordExternal), so a replay records it like a direct launch would.
+static hipError_t record_timing(lps_ctx *c, hipEvent_t e) {
+  return hipEventRecordWithFlags(e, c->stream, c->capturing ? hipEventRecordExternal : 0);
+}
+
 static hipError_t step_once(lps_ctx *c, bool record_events) {
   const StepArgs a = c->args();
   const StepState st = c->state(c->step);
   hipEvent_t *ev = nullptr;
   hipError_t e;
   if (record_events) {
-    ev = &c->ev[kEv * (size_t)c->ev_used];
-    if ((e = hipEventRecord(ev[0], c->stream)) != hipSuccess) return e;
+    ev = &c->ev[kEv * (30 * (size_t)c->ev_set + c->ev_used)];
+    if ((e = record_timing(c, ev[0])) != hipSuccess) return e;
   }
   // fast path: the ABC rows' G_theta and G_u / G_v read only the previous steps' state and run on
   // the second stream beside the tile passes; the force pass waits for G_theta (ev_join[0]), the
@@ -1041,9 +1049,9 @@ static hipError_t step_once(lps_ctx *c, bool record_events) {
     if ((e = launch_force_part(a, st, c->stream3)) != hipSuccess) return e;
     if ((e = hipEventRecord(c->ev_part[2], c->stream3)) != hipSuccess) return e;
   }
-  if (record_events && (e = hipEventRecord(ev[1], c->stream)) != hipSuccess) return e;
+  if (record_events && (e = record_timing(c, ev[1])) != hipSuccess) return e;
   if ((e = launch_force(a, st, c->fast, c->stream)) != hipSuccess) return e;
-  if (record_events && (e = hipEventRecord(ev[2], c->stream)) != hipSuccess) return e;
+  if (record_events && (e = record_timing(c, ev[2])) != hipSuccess) return e;
   if (c->fast) {
     if (!split && (e = launch_force_part(a, st, c->stream)) != hipSuccess) return e;
     if (split && (e = hipStreamWaitEvent(c->stream, c->ev_part[2], 0)) != hipSuccess) return e;
@@ -1054,27 +1062,48 @@ static hipError_t step_once(lps_ctx *c, bool record_events) {
   }
   if (c->clo && !abc && (e = closure_step_hook(c, st, c->stream)) != hipSuccess) return e;
   if (record_events) {
-    if ((e = hipEventRecord(ev[3], c->stream)) != hipSuccess) return e;
+    if ((e = record_timing(c, ev[3])) != hipSuccess) return e;
     c->ev_used++;
   }
   c->step++;
   return hipSuccess;
 }
 
-static int32_t flush_events(lps_ctx *c) {
-  if (c->ev_used == 0) return LPS_OK;
-  CTX_TRY(c, hipEventSynchronize(c->ev[kEv * (size_t)(c->ev_used - 1) + kEv - 1]));
-  for (int32_t k = 0; k < c->ev_used; k++) {
-    float t[kEv - 1] = {0, 0, 0}, tall = 0;
-    hipEvent_t *e = &c->ev[kEv * (size_t)k];
-    for (int q = 0; q < kEv - 1; q++) CTX_TRY(c, hipEventElapsedTime(&t[q], e[q], e[q + 1]));
-    CTX_TRY(c, hipEventElapsedTime(&tall, e[0], e[kEv - 1]));
-    for (int q = 0; q < kEv - 1; q++) c->t_pass[q] += t[q];
-    c->t_pass[kEv - 1] += tall;
+// Reads the times of `used` steps of event set `set`. Events recorded by a graph the runtime cannot
+// time switch timing mode to separate launches (the samples are dropped).
+static int32_t flush_set(lps_ctx *c, int32_t set, int32_t used, bool from_graph) {
+  if (used == 0) return LPS_OK;
+  hipEvent_t *e0 = &c->ev[kEv * 30 * (size_t)set];
+  CTX_TRY(c, hipEventSynchronize(e0[kEv * (used - 1) + kEv - 1]));
+  std::vector<float> t((size_t)used * kEv);
+  for (int32_t k = 0; k < used; k++) {
+    hipEvent_t *e = e0 + kEv * k;
+    for (int q = 0; q < kEv; q++) {
+      hipError_t r = q < kEv - 1 ? hipEventElapsedTime(&t[kEv * k + q], e[q], e[q + 1])
+                                 : hipEventElapsedTime(&t[kEv * k + q], e[0], e[kEv - 1]);
+      if (r != hipSuccess && from_graph) {
+        (void)hipGetLastError();
+        c->graph_timing = false;
+        return LPS_OK;
+      }
+      if (r != hipSuccess) return c->hip_fail(r, "hipEventElapsedTime");
+    }
+  }
+  for (int32_t k = 0; k < used; k++) {
+    for (int q = 0; q < kEv; q++) c->t_pass[q] += t[kEv * k + q];
     c->t_count++;
   }
-  c->ev_used = 0;
   return LPS_OK;
+}
+
+// Reads every recorded step's times: the pending replay's set, then the steps launched one by one.
+static int32_t flush_events(lps_ctx *c) {
+  const int32_t prev = c->ev_prev_used, used = c->ev_used;
+  c->ev_prev_used = 0;
+  c->ev_used = 0;
+  int32_t rc = flush_set(c, 1 - c->ev_set, prev, true);
+  if (rc != LPS_OK) return rc;
+  return flush_set(c, c->ev_set, used, false);
 }
 
 int32_t lps_set_timing(lps_ctx *c, int32_t on) {
@@ -1084,7 +1113,7 @@ int32_t lps_set_timing(lps_ctx *c, int32_t on) {
   for (double &t : c->t_pass) t = 0.0;
   c->t_count = 0;
   if (c->timing && c->ev.empty()) {
-    c->ev.resize(kEv * 30);
+    c->ev.resize(2 * kEv * 30);
     for (auto &e : c->ev) CTX_TRY(c, hipEventCreate(&e));
   }
   return LPS_OK;
@@ -1100,13 +1129,18 @@ int32_t lps_pass_times(lps_ctx *c, double *out4) {
 
 // 30 steps from a step that is a multiple of 30 (the buffer rotation repeats every 6 steps),
 // captured once and replayed.
-static int32_t build_graph30(lps_ctx *c) {
-  hipGraphExec_t *out = &c->graph30;
+// timed: with the timing events of event set c->ev_set inside, into graph30t[c->ev_set].
+static int32_t build_graph30(lps_ctx *c, bool timed) {
+  hipGraphExec_t *out = timed ? &c->graph30t[c->ev_set] : &c->graph30;
   hipGraph_t g = nullptr;
   CTX_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   const int64_t s0 = c->step;
   hipError_t err = hipSuccess;
-  for (int k = 0; k < 30 && err == hipSuccess; k++) err = step_once(c, false);
+  c->capturing = true;
+  c->ev_used = 0;
+  for (int k = 0; k < 30 && err == hipSuccess; k++) err = step_once(c, timed);
+  c->capturing = false;
+  c->ev_used = 0;
   c->step = s0;
   hipError_t e2 = hipStreamEndCapture(c->stream, &g);
   if (err != hipSuccess) return c->hip_fail(err, "capture of the step loop");
@@ -1122,6 +1156,10 @@ static int32_t build_graph30(lps_ctx *c) {
 
 static void drop_graphs(lps_ctx *c) {
   if (c->graph30) (void)hipGraphExecDestroy(c->graph30);
+  for (hipGraphExec_t &g : c->graph30t) {
+    if (g) (void)hipGraphExecDestroy(g);
+    g = nullptr;
+  }
   c->graph30 = nullptr;
 }
 
@@ -1130,13 +1168,33 @@ int32_t lps_step(lps_ctx *c, int32_t nsteps) {
   int32_t done = 0;
   while (done < nsteps) {
     const int32_t left = nsteps - done;
-    // timed steps run as separate launches: HIP cannot time events recorded inside a graph
-    if (c->opt.use_graph && !c->timing && c->step % 30 == 0 && left >= 30) {
-      if (!c->graph30) {
-        int32_t rc = build_graph30(c);
+    if (c->opt.use_graph && c->step % 30 == 0 && left >= 30 && (!c->timing || c->graph_timing)) {
+      if (c->timing) {
+        // steps launched one by one are read first; then this replay records its 30 steps into
+        // event set k while the previous replay's set is read
+        int32_t rc = flush_set(c, c->ev_set, c->ev_used, false);
+        c->ev_used = 0;
         if (rc != LPS_OK) return rc;
+        const int32_t k = c->ev_set;
+        if (!c->graph30t[k] && (rc = build_graph30(c, true)) != LPS_OK) return rc;
+        CTX_TRY(c, hipGraphLaunch(c->graph30t[k], c->stream));
+        rc = flush_set(c, 1 - k, c->ev_prev_used, true);
+        if (rc != LPS_OK) return rc;
+        c->ev_prev_used = 30;
+        c->ev_set = 1 - k;
+        if (!c->graph_timing) {  // this runtime cannot time graph events: separate launches
+          c->ev_prev_used = 0;
+          c->step += 30;
+          done += 30;
+          continue;
+        }
+      } else {
+        if (!c->graph30) {
+          int32_t rc = build_graph30(c, false);
+          if (rc != LPS_OK) return rc;
+        }
+        CTX_TRY(c, hipGraphLaunch(c->graph30, c->stream));
       }
-      CTX_TRY(c, hipGraphLaunch(c->graph30, c->stream));
       c->step += 30;
       done += 30;
       continue;

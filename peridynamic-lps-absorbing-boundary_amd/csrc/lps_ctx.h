@@ -68,10 +68,16 @@ struct lps_ctx {
   int32_t n_partials = 0;
   // graph of 30 steps from a step that is a multiple of 30
   hipGraphExec_t graph30 = nullptr;
-  // timing
+  // the same with the timing events inside (external event-record nodes), one per event set:
+  // timing mode alternates them, reading one set's times while the other replays, as long as
+  // graph_timing holds (cleared when the runtime cannot time such events)
+  hipGraphExec_t graph30t[2] = {nullptr, nullptr};
+  bool graph_timing = true, capturing = false;
+  // timing: 2 sets of 30 steps x 4 events. ev_set receives the next steps (ev_used so far); the
+  // other set may hold a replay's events not yet read (ev_prev_used steps)
   bool timing = false;
-  std::vector<hipEvent_t> ev;  // 4 per timed step
-  int32_t ev_used = 0;
+  std::vector<hipEvent_t> ev;
+  int32_t ev_set = 0, ev_used = 0, ev_prev_used = 0;
   double t_pass[4] = {0, 0, 0, 0};  // dilatation, force (nodes), force (ABC rows), whole step
   int64_t t_count = 0;
   // learned ABC closure applied after the G_u / G_v rows (null: least-squares ABC only)
