@@ -435,6 +435,9 @@ static int32_t create_impl(lps_ctx *c, const lps_problem *p) {
     c->tile.part_list = d_pl;
     c->tile.n_full = (int32_t)full_list.size();
     c->tile.n_part = (int32_t)part_list.size();
+    c->abc_in_part = true;  // every ABC node in a partial tile (its horizon reaches past the rim)
+    for (int32_t r = 0; r < c->n_abc; r++)
+      if (full[abc_slot[r] / (kTile * kTile)]) c->abc_in_part = false;
     CTX_TRY(c, c->upload(&d_delta, dl));
     CTX_TRY(c, c->upload(&d_hx, thx));
     CTX_TRY(c, c->upload(&d_hy, thy));
@@ -721,6 +724,8 @@ static int32_t merge_batch(lps_ctx *c, const std::vector<lps_ctx *> &sub) {
   c->batch = true;
   c->fast = true;
   c->stencil68_ok = true;
+  c->abc_in_part = true;
+  for (const lps_ctx *q : sub) c->abc_in_part = c->abc_in_part && q->abc_in_part;
   c->n_inst = B;
   // scalars shared by the instances (the fast kernels take the material from InstParams)
   c->E = s0.E; c->nu = s0.nu; c->rho = s0.rho; c->dt = s0.dt;
@@ -1052,7 +1057,15 @@ static hipError_t step_once(lps_ctx *c, bool record_events) {
   if (record_events && (e = record_timing(c, ev[1])) != hipSuccess) return e;
   if ((e = launch_force(a, st, c->fast, c->stream)) != hipSuccess) return e;
   if (record_events && (e = record_timing(c, ev[2])) != hipSuccess) return e;
-  if (c->fast) {
+  // the ABC nodes' prediction folded into the partial tiles' force pass (every ABC node lies in a
+  // partial tile), which then waits for G_u / G_v; not with a closure attached, whose long ABC
+  // branch the partial tiles would wait for
+  const bool fold = abc && !split && !c->clo && c->abc_in_part && a.tile.force_kernel != kGeneric;
+  if (c->fast && fold) {
+    if (fork && (e = hipStreamWaitEvent(c->stream, c->ev_join[1], 0)) != hipSuccess) return e;
+    if (!fork && (e = launch_abc_uv(a, st, c->stream)) != hipSuccess) return e;
+    if ((e = launch_force_part(a, st, c->stream, true)) != hipSuccess) return e;
+  } else if (c->fast) {
     if (!split && (e = launch_force_part(a, st, c->stream)) != hipSuccess) return e;
     if (split && (e = hipStreamWaitEvent(c->stream, c->ev_part[2], 0)) != hipSuccess) return e;
     if (fork && (e = hipStreamWaitEvent(c->stream, c->ev_join[1], 0)) != hipSuccess) return e;

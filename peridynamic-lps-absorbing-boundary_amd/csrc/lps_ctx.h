@@ -38,6 +38,7 @@ struct lps_ctx {
   double lam = 0, mu = 0, kappa = 0, m_ref = 0, exclude_r = -1, E0 = -1;
   bool fast = false;
   bool stencil68_ok = false;  // the unrolled 68-bond kernels apply to this problem
+  bool abc_in_part = false;   // tile path: every ABC node lies in a partial tile
   int64_t n_full_nodes = 0;   // nodes in the full tiles
   // instances: one for lps_create, several for lps_create_batch
   std::vector<InstParams> inst_h;

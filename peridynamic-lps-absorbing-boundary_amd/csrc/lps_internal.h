@@ -138,7 +138,10 @@ hipError_t launch_idw(const StepArgs &a, const StepState &st, hipStream_t s);
 // (force at every node, ABC nodes included, with the Verlet epilogue), launch_force_part = the
 // partial tiles when the selected kernel takes only the full ones.
 hipError_t launch_force(const StepArgs &a, const StepState &st, bool fast, hipStream_t s);
-hipError_t launch_force_part(const StepArgs &a, const StepState &st, hipStream_t s);
+// predict_abc: also the ABC nodes' next prediction from U^{n+1}, V^{n+1} (G_u / G_v done) and the new
+// acceleration, as k_abc_predict (all ABC nodes must lie in partial tiles).
+hipError_t launch_force_part(const StepArgs &a, const StepState &st, hipStream_t s,
+                             bool predict_abc = false);
 // Fast path, ABC rows. launch_abc_theta (G_theta / direct IDW value) reads only the theta history
 // and the prediction, launch_abc_uv (G_u, G_v) only the previous steps' state: both may run on a
 // second stream beside the tile passes; the force pass needs abc_theta, launch_abc_predict (the

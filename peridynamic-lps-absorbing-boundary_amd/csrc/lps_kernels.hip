@@ -666,7 +666,9 @@ __device__ __forceinline__ void node_update(const InstParams &P, const StepState
 // Unrolled force kernel over the tiles of list. FULL: the full tiles, no masks (every window cell
 // holds a node with m == m_ref); otherwise a bond to a cell without a node or with m <= 0 is
 // skipped, and a node with m <= 0 gets no force (src/main.cpp:852).
-template <bool FULL>
+// PRED: at ABC nodes also the next prediction, as k_abc_predict (G_u / G_v have written
+// U^{n+1}, V^{n+1} there).
+template <bool FULL, bool PRED = false>
 __global__ void __launch_bounds__(kBlock, FULL ? 5 : 2) k_force_t68(StepArgs a, StepState st,
                                                                     const int32_t *list,
                                                                     int32_t count) {
@@ -704,6 +706,13 @@ __global__ void __launch_bounds__(kBlock, FULL ? 5 : 2) k_force_t68(StepArgs a, 
   if (FULL || abi.y != 0.0) force_sum68<FULL>(sh.fo.u, sh.fo.ab, c, ui, abi, T, &F0, &F1);
   node_update(P, st, i, kd, ui, make_double2(st.vx[i], st.vy[i]), make_double2(st.ax[i], st.ay[i]),
               F0, F1);
+  if (PRED && kd == 2) {
+    const double dt = P.dt, h = 0.5 * dt;
+    const double inv_rho = 1.0 / P.rho;
+    const double a1x = F0 * inv_rho, a1y = F1 * inv_rho;
+    st.pnx[i] = st.ux1[i] + dt * (st.vx1[i] + h * a1x);
+    st.pny[i] = st.uy1[i] + dt * (st.vy1[i] + h * a1y);
+  }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1417,11 +1426,16 @@ hipError_t launch_force(const StepArgs &a, const StepState &st, bool fast, hipSt
   return hipGetLastError();
 }
 
-hipError_t launch_force_part(const StepArgs &a, const StepState &st, hipStream_t s) {
+hipError_t launch_force_part(const StepArgs &a, const StepState &st, hipStream_t s,
+                             bool predict_abc) {
   if (a.tile.force_kernel == kGeneric || a.tile.n_part == 0) return hipSuccess;
   // both families take the partial tiles one node per thread (see launch_theta_part)
-  hipLaunchKernelGGL(k_force_t68<false>, dim3(list_blocks(a.tile.n_part)), dim3(kBlock), 0, s, a,
-                     st, a.tile.part_list, a.tile.n_part);
+  if (predict_abc)
+    hipLaunchKernelGGL((k_force_t68<false, true>), dim3(list_blocks(a.tile.n_part)), dim3(kBlock),
+                       0, s, a, st, a.tile.part_list, a.tile.n_part);
+  else
+    hipLaunchKernelGGL(k_force_t68<false>, dim3(list_blocks(a.tile.n_part)), dim3(kBlock), 0, s, a,
+                       st, a.tile.part_list, a.tile.n_part);
   return hipGetLastError();
 }
 
