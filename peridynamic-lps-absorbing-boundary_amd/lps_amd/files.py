@@ -117,6 +117,19 @@ def read_job(job: str) -> Problem:
                    prenotch=pre, vinodes=vin, info={"job": job})
 
 
+ENERGY_COLUMNS = ("t", "energy", "energy_k", "energy_p", "px", "py", "max_u", "theta_max",
+                  "ub_rms", "vb_rms")
+VERIFY_COLUMNS = ("t", "energy", "de_rel", "px", "py", "theta_max", "max_u", "max_v", "max_a")
+
+
+def write_series(path: str, records: List[Dict[str, float]], kind: str) -> None:
+    """Output records (solver.Solver.run dicts) as a .energy or .verify file: header, then one
+    row per record, precision 16, as src/main.cpp:694-707, 1106-1123 writes them."""
+    header, cols = (ENERGY_HEADER, ENERGY_COLUMNS) if kind == "energy" else \
+        (VERIFY_HEADER, VERIFY_COLUMNS)
+    _write_lines(path, [header] + [" ".join(fmt16(r[c]) for c in cols) for r in records])
+
+
 def read_series(path: str) -> np.ndarray:
     """Rows of a .energy or .verify file (header skipped) as a float array."""
     rows = []
