@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <fstream>
 
 namespace lpsio {
@@ -62,19 +63,50 @@ static bool excluded(const std::vector<double> &nodes, int32_t i, double exclude
   return std::sqrt(x * x + y * y) > exclude_r;
 }
 
+// Big-endian output of the BINARY encoding.
+static void put_be(std::ofstream &file, double v) {
+  uint64_t b;
+  std::memcpy(&b, &v, 8);
+  unsigned char c[8];
+  for (int k = 0; k < 8; k++) c[k] = (unsigned char)(b >> (56 - 8 * k));
+  file.write(reinterpret_cast<const char *>(c), 8);
+}
+static void put_be(std::ofstream &file, int32_t v) {
+  const uint32_t b = (uint32_t)v;
+  unsigned char c[4];
+  for (int k = 0; k < 4; k++) c[k] = (unsigned char)(b >> (24 - 8 * k));
+  file.write(reinterpret_cast<const char *>(c), 4);
+}
+
 static void vtk_points(std::ofstream &file, const std::vector<double> &nodes, int32_t n,
-                       double exclude_r, int32_t count) {
+                       double exclude_r, int32_t count, VtkFormat fmt) {
+  const bool bin = fmt == VtkFormat::Binary;
   file << "# vtk DataFile Version 3.0" << std::endl;
   file << "PD2_LPS_ABC output" << std::endl;
-  file << "ASCII" << std::endl;
+  file << (bin ? "BINARY" : "ASCII") << std::endl;
   file << "DATASET POLYDATA" << std::endl;
   file << "POINTS " << count << " double" << std::endl;
   for (int32_t i = 0; i < n; i++) {
     if (excluded(nodes, i, exclude_r)) continue;
-    file << nodes[2 * (size_t)i] << " " << nodes[2 * (size_t)i + 1] << " 0.0" << std::endl;
+    if (bin) {
+      put_be(file, nodes[2 * (size_t)i]);
+      put_be(file, nodes[2 * (size_t)i + 1]);
+      put_be(file, 0.0);
+    } else {
+      file << nodes[2 * (size_t)i] << " " << nodes[2 * (size_t)i + 1] << " 0.0" << std::endl;
+    }
   }
+  if (bin) file << std::endl;
   file << "VERTICES " << count << " " << 2 * count << std::endl;
-  for (int32_t i = 0; i < count; i++) file << "1 " << i << std::endl;
+  for (int32_t i = 0; i < count; i++) {
+    if (bin) {
+      put_be(file, (int32_t)1);
+      put_be(file, i);
+    } else {
+      file << "1 " << i << std::endl;
+    }
+  }
+  if (bin) file << std::endl;
   file << "POINT_DATA " << count << std::endl;
 }
 
@@ -85,44 +117,62 @@ static int32_t count_points(const std::vector<double> &nodes, int32_t n, double 
   return count;
 }
 
+static std::ofstream open_vtk(const std::string &path, VtkFormat fmt) {
+  std::ofstream file(path.c_str(), fmt == VtkFormat::Binary ? std::ios::out | std::ios::binary
+                                                            : std::ios::out);
+  file.precision(12);
+  return file;
+}
+
 bool write_vtk_vector(const std::string &path, const std::vector<double> &nodes,
                       const std::vector<double> &vec, const char *name, int32_t n,
-                      double exclude_r) {
+                      double exclude_r, VtkFormat fmt) {
   const int32_t count = count_points(nodes, n, exclude_r);
-  std::ofstream file(path.c_str());
+  std::ofstream file = open_vtk(path, fmt);
   if (!file) return false;
-  file.precision(12);
-  vtk_points(file, nodes, n, exclude_r, count);
+  const bool bin = fmt == VtkFormat::Binary;
+  vtk_points(file, nodes, n, exclude_r, count, fmt);
   file << "VECTORS " << name << " double" << std::endl;
   for (int32_t i = 0; i < n; i++) {
     if (excluded(nodes, i, exclude_r)) continue;
-    file << vec[2 * (size_t)i] << " " << vec[2 * (size_t)i + 1] << " 0.0" << std::endl;
+    if (bin) {
+      put_be(file, vec[2 * (size_t)i]);
+      put_be(file, vec[2 * (size_t)i + 1]);
+      put_be(file, 0.0);
+    } else {
+      file << vec[2 * (size_t)i] << " " << vec[2 * (size_t)i + 1] << " 0.0" << std::endl;
+    }
   }
+  if (bin) file << std::endl;
   file << "SCALARS " << name << "_magnitude double 1" << std::endl;
   file << "LOOKUP_TABLE default" << std::endl;
   for (int32_t i = 0; i < n; i++) {
     if (excluded(nodes, i, exclude_r)) continue;
     const double a = vec[2 * (size_t)i], b = vec[2 * (size_t)i + 1];
-    file << std::sqrt(a * a + b * b) << std::endl;
+    if (bin) put_be(file, std::sqrt(a * a + b * b));
+    else file << std::sqrt(a * a + b * b) << std::endl;
   }
-  return true;
+  if (bin) file << std::endl;
+  return (bool)file;
 }
 
 bool write_vtk_scalar(const std::string &path, const std::vector<double> &nodes,
                       const std::vector<double> &vec, const char *name, int32_t n,
-                      double exclude_r) {
+                      double exclude_r, VtkFormat fmt) {
   const int32_t count = count_points(nodes, n, exclude_r);
-  std::ofstream file(path.c_str());
+  std::ofstream file = open_vtk(path, fmt);
   if (!file) return false;
-  file.precision(12);
-  vtk_points(file, nodes, n, exclude_r, count);
+  const bool bin = fmt == VtkFormat::Binary;
+  vtk_points(file, nodes, n, exclude_r, count, fmt);
   file << "SCALARS " << name << " double 1" << std::endl;
   file << "LOOKUP_TABLE default" << std::endl;
   for (int32_t i = 0; i < n; i++) {
     if (excluded(nodes, i, exclude_r)) continue;
-    file << vec[(size_t)i] << std::endl;
+    if (bin) put_be(file, vec[(size_t)i]);
+    else file << vec[(size_t)i] << std::endl;
   }
-  return true;
+  if (bin) file << std::endl;
+  return (bool)file;
 }
 
 }  // namespace lpsio

@@ -15,13 +15,16 @@ bool read_ints(const std::string &path, int64_t count, std::vector<int32_t> &out
 // Index arrays written as floats: read as double, converted with (int)(x + 0.5) (utils.h:61, :76).
 bool read_rounded_ints(const std::string &path, int64_t count, std::vector<int32_t> &out);
 
-// src/helpers/utils.h:94-167 and :170-224 (precision 12, points with r > exclude_r dropped when
-// exclude_r > 0).
+// Legacy VTK POLYDATA encodings: the reference's ASCII (precision 12), or BINARY (big-endian
+// doubles and int32, the same values unrounded; no counterpart in the reference).
+enum class VtkFormat { Ascii, Binary };
+
+// src/helpers/utils.h:94-167 and :170-224 (points with r > exclude_r dropped when exclude_r > 0).
 bool write_vtk_vector(const std::string &path, const std::vector<double> &nodes,
                       const std::vector<double> &vec, const char *name, int32_t n,
-                      double exclude_r);
+                      double exclude_r, VtkFormat fmt = VtkFormat::Ascii);
 bool write_vtk_scalar(const std::string &path, const std::vector<double> &nodes,
                       const std::vector<double> &vec, const char *name, int32_t n,
-                      double exclude_r);
+                      double exclude_r, VtkFormat fmt = VtkFormat::Ascii);
 
 }  // namespace lpsio

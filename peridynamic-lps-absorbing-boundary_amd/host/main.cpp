@@ -1,7 +1,7 @@
 // PD2_LPS_ABC - drop-in replacement of the reference solver's executable (src/main.cpp), running the
 // time loop on an MI355X through the lps_amd C ABI.
 //
-//   PD2_LPS_ABC <prefix> <jobname> [--theta-method=gtheta|idw] [--vtk=on|off] [--exact]
+//   PD2_LPS_ABC <prefix> <jobname> [--theta-method=gtheta|idw] [--vtk=on|off|binary] [--exact]
 //                                  [--no-graph] [--device=N] [--abc=lsq|learned=<weights.bin>]
 //   PD2_LPS_ABC --batch=<manifest> [options]
 //   restart options: [--checkpoint=<file>] [--restart=<file>] [--stop-at=<step>]
@@ -10,6 +10,10 @@
 // src/main.cpp:694-707, 1106-1123) and exit codes (0 on success or on the usage message, 1 on a FATAL
 // condition) as the reference. VTK snapshots (src/main.cpp:689-691, 1091-1103) are written unless
 // --vtk=off. The options select the reference's compile-time variants at run time.
+//
+// VTK (no counterpart in the reference for these two): --vtk=binary writes legacy BINARY VTK (the
+// same fields unrounded, big-endian) instead of ASCII; --vtk-threads=N (default 3, 0 = inline)
+// formats and writes the snapshots on N background threads while the GPU goes on stepping.
 //
 // --batch runs many jobs in one process (no counterpart in the reference): the manifest holds one
 // "<prefix> <jobname>" per line ('#' starts a comment); the jobs, which must share dx, dy, delta and
@@ -26,6 +30,7 @@
 // --restart=<file> resumes from such a file (lps_checkpoint_load) and runs on to StepNo, appending
 // its records to the .energy/.verify files of the run it continues. A run split this way writes the
 // same files, bit for bit, as the run that was never stopped.
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -33,12 +38,14 @@
 #include <fstream>
 #include <iomanip>
 #include <iostream>
+#include <memory>
 #include <sstream>
 #include <string>
 #include <vector>
 
 #include "job_io.h"
 #include "lps_amd.h"
+#include "vtk_async.h"
 
 namespace {
 
@@ -142,14 +149,28 @@ struct Job {
       verify << "# t  E_total  dE/E0  P_x  P_y  max_theta  max_u  max_v  max_a" << std::endl;
   }
 
-  void write_vtk(int64_t s) {
-    const std::string ss = std::to_string(s);
-    lpsio::write_vtk_vector(name + prefix + "-" + ss + ".vtk", nodes, u, "displacement", node_no,
-                            exclude_r);
-    lpsio::write_vtk_vector(name + prefix + "-vel-" + ss + ".vtk", nodes, v, "velocity", node_no,
-                            exclude_r);
-    lpsio::write_vtk_scalar(name + prefix + "-theta-" + ss + ".vtk", nodes, th, "dilatation",
-                            node_no, exclude_r);
+  // The three snapshots of step s (src/main.cpp:1090-1103) from copies of u, v, theta, queued on
+  // the writer; nodes must outlive the writer's wait.
+  void write_vtk(int64_t s, lpsio::AsyncWriter &w, lpsio::VtkFormat fmt) {
+    const std::string base = name + prefix + "-", ss = std::to_string(s);
+    auto cu = std::make_shared<std::vector<double>>(u);
+    auto cv = std::make_shared<std::vector<double>>(v);
+    auto ct = std::make_shared<std::vector<double>>(th);
+    const int32_t n = node_no;
+    const double er = exclude_r;
+    const std::vector<double> *nd = &nodes;
+    w.submit([=] { return lpsio::write_vtk_vector(base + ss + ".vtk", *nd, *cu, "displacement", n, er, fmt); });
+    w.submit([=] { return lpsio::write_vtk_vector(base + "vel-" + ss + ".vtk", *nd, *cv, "velocity", n, er, fmt); });
+    w.submit([=] { return lpsio::write_vtk_scalar(base + "theta-" + ss + ".vtk", *nd, *ct, "dilatation", n, er, fmt); });
+  }
+  // The initial displacement snapshot (src/main.cpp:689-691).
+  void write_vtk0(lpsio::AsyncWriter &w, lpsio::VtkFormat fmt) {
+    auto cu = std::make_shared<std::vector<double>>(u);
+    const std::string path = name + prefix + "-0.vtk";
+    const int32_t n = node_no;
+    const double er = exclude_r;
+    const std::vector<double> *nd = &nodes;
+    w.submit([=] { return lpsio::write_vtk_vector(path, *nd, *cu, "displacement", n, er, fmt); });
   }
 
   // One output record (src/main.cpp:1065-1123); returns 1 on the NaN/Inf FATAL (:1126-1133).
@@ -218,6 +239,19 @@ struct Learned {
   }
 };
 
+// --vtk, --vtk-threads
+struct Vtk {
+  bool on = true;
+  lpsio::VtkFormat fmt = lpsio::VtkFormat::Ascii;
+  int threads = 3;
+};
+
+// Waits for the queued snapshots; a failed write is reported, as the reference does not check.
+void finish_vtk(lpsio::AsyncWriter &w) {
+  const int failed = w.wait();
+  if (failed > 0) std::cerr << "WARNING: " << failed << " VTK file(s) could not be written" << std::endl;
+}
+
 // --checkpoint, --restart, --stop-at
 struct Restart {
   std::string save, load;
@@ -253,8 +287,9 @@ float gate_of(const std::vector<double> &ic) {
   return (float)(1e-4 * m);
 }
 
-int run_single(Job &job, const lps_options &opt, bool vtk, const std::string &learned,
+int run_single(Job &job, const lps_options &opt, const Vtk &vtk, const std::string &learned,
                const Restart &rs) {
+  lpsio::AsyncWriter writer(vtk.on ? vtk.threads : 0);
   std::cout << "Reading data... " << std::flush;
   if (job.read(opt)) return 1;
   std::cout << "finished." << std::endl;
@@ -289,10 +324,9 @@ int run_single(Job &job, const lps_options &opt, bool vtk, const std::string &le
     lps_destroy(ctx);
     return 1;
   }
-  if (vtk && rs.load.empty()) {
+  if (vtk.on && rs.load.empty()) {
     lps_fetch(ctx, job.u.data(), nullptr, nullptr, nullptr, nullptr);
-    lpsio::write_vtk_vector(job.name + job.prefix + "-0.vtk", job.nodes, job.u, "displacement",
-                            job.node_no, job.exclude_r);
+    job.write_vtk0(writer, vtk.fmt);
   }
   job.open_outputs(!rs.load.empty());
   const int64_t end = rs.end(job.step_no);
@@ -310,15 +344,16 @@ int run_single(Job &job, const lps_options &opt, bool vtk, const std::string &le
       return 1;
     }
     if (rc == 0) continue;
-    if (vtk) {
+    if (vtk.on) {
       lps_fetch(ctx, job.u.data(), job.v.data(), nullptr, job.th.data(), nullptr);
-      job.write_vtk(d.step);
+      job.write_vtk(d.step, writer, vtk.fmt);
     }
     if (job.write_record(d, "")) {
       lps_destroy(ctx);
       return 1;
     }
   }
+  finish_vtk(writer);
   std::cout << std::endl << "Time marching took " << timer.toc() << " seconds." << std::endl;
   closure.report(ctx, {job.name});
   rc = rs.finish(ctx);
@@ -326,7 +361,7 @@ int run_single(Job &job, const lps_options &opt, bool vtk, const std::string &le
   return rc;
 }
 
-int run_batch(const std::string &manifest, const lps_options &opt, bool vtk,
+int run_batch(const std::string &manifest, const lps_options &opt, const Vtk &vtk,
               const std::string &learned, const Restart &rs) {
   std::ifstream mf(manifest);
   if (!mf) {
@@ -386,12 +421,12 @@ int run_batch(const std::string &manifest, const lps_options &opt, bool vtk,
     lps_destroy(ctx);
     return 1;
   }
+  lpsio::AsyncWriter writer(vtk.on ? vtk.threads : 0);
   for (int32_t i = 0; i < B; i++) {
     Job &j = jobs[i];
-    if (vtk && rs.load.empty()) {
+    if (vtk.on && rs.load.empty()) {
       lps_fetch_instance(ctx, i, j.u.data(), nullptr, nullptr, nullptr, nullptr);
-      lpsio::write_vtk_vector(j.name + j.prefix + "-0.vtk", j.nodes, j.u, "displacement",
-                              j.node_no, j.exclude_r);
+      j.write_vtk0(writer, vtk.fmt);
     }
     j.open_outputs(!rs.load.empty());
   }
@@ -413,9 +448,9 @@ int run_batch(const std::string &manifest, const lps_options &opt, bool vtk,
     int fatal = 0;
     for (int32_t i = 0; i < B; i++) {
       Job &j = jobs[i];
-      if (vtk) {
+      if (vtk.on) {
         lps_fetch_instance(ctx, i, j.u.data(), j.v.data(), nullptr, j.th.data(), nullptr);
-        j.write_vtk(d[i].step);
+        j.write_vtk(d[i].step, writer, vtk.fmt);
       }
       const std::string tag = "[" + j.name + "] ";
       fatal |= j.write_record(d[i], tag.c_str());
@@ -425,6 +460,7 @@ int run_batch(const std::string &manifest, const lps_options &opt, bool vtk,
       return 1;
     }
   }
+  finish_vtk(writer);
   std::cout << std::endl << "Time marching took " << timer.toc() << " seconds." << std::endl;
   closure.report(ctx, names);
   rc = rs.finish(ctx);
@@ -448,14 +484,16 @@ int main(int argc, char *argv[]) {
   Restart rs;
   lps_options opt;
   lps_default_options(&opt);
-  bool vtk = true;
+  Vtk vtk;
   for (int i = 1; i < argc; i++) {
     std::string a = argv[i];
     if (a.rfind("--", 0) != 0) { pos.push_back(a); continue; }
     if (a == "--theta-method=gtheta") opt.theta_method = LPS_THETA_GTHETA;
     else if (a == "--theta-method=idw") opt.theta_method = LPS_THETA_IDW;
-    else if (a == "--vtk=on") vtk = true;
-    else if (a == "--vtk=off") vtk = false;
+    else if (a == "--vtk=on") vtk.on = true, vtk.fmt = lpsio::VtkFormat::Ascii;
+    else if (a == "--vtk=off") vtk.on = false;
+    else if (a == "--vtk=binary") vtk.on = true, vtk.fmt = lpsio::VtkFormat::Binary;
+    else if (a.rfind("--vtk-threads=", 0) == 0) vtk.threads = std::max(0, std::atoi(a.c_str() + 14));
     else if (a == "--exact") opt.exact = 1;
     else if (a == "--no-graph") opt.use_graph = 0;
     else if (a.rfind("--device=", 0) == 0) opt.device = std::atoi(a.c_str() + 9);

@@ -142,10 +142,13 @@ def read_series(path: str) -> np.ndarray:
 
 
 def read_vtk_field(path: str) -> Dict[str, np.ndarray]:
-    """Points and the first data block of a legacy-ASCII VTK file written by the solver
-    (src/helpers/utils.h:94-224)."""
-    with open(path) as f:
-        toks = f.read().split()
+    """Points and the first data block of a legacy VTK file written by the solver
+    (src/helpers/utils.h:94-224): ASCII, or BINARY (big-endian doubles, int32 vertices)."""
+    with open(path, "rb") as f:
+        raw = f.read()
+    if raw.split(b"\n", 3)[2].strip() == b"BINARY":
+        return _read_vtk_binary(raw)
+    toks = raw.decode().split()
     out: Dict[str, np.ndarray] = {}
     i = toks.index("POINTS")
     n = int(toks[i + 1])
@@ -157,4 +160,25 @@ def read_vtk_field(path: str) -> Dict[str, np.ndarray]:
     else:
         j = toks.index("LOOKUP_TABLE")
         out["scalars"] = np.array(toks[j + 2:j + 2 + n], dtype=np.float64)
+    return out
+
+
+def _read_vtk_binary(raw: bytes) -> Dict[str, np.ndarray]:
+    def after(key: bytes, start: int = 0):
+        k = raw.index(key, start)
+        e = raw.index(b"\n", k)
+        return raw[k:e].split(), e + 1
+
+    out: Dict[str, np.ndarray] = {}
+    head, p = after(b"POINTS ")
+    n = int(head[1])
+    out["points"] = np.frombuffer(raw, ">f8", 3 * n, p).reshape(n, 3)[:, :2].astype(np.float64)
+    head, p = after(b"VERTICES ", p + 24 * n)
+    head, p = after(b"POINT_DATA ", p + 4 * int(head[2]))
+    if raw.startswith(b"VECTORS", p):
+        head, p = after(b"VECTORS ", p)
+        out["vectors"] = np.frombuffer(raw, ">f8", 3 * n, p).reshape(n, 3)[:, :2].astype(np.float64)
+    else:
+        head, p = after(b"LOOKUP_TABLE ", p)
+        out["scalars"] = np.frombuffer(raw, ">f8", n, p).astype(np.float64)
     return out

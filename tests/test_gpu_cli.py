@@ -138,3 +138,31 @@ def test_cli_checkpoint_restart_writes_the_same_files(tmp_path):
     r = subprocess.run([EXE, "_x", "other", "--vtk=off", "--exact", "--restart=job.ckpt"],
                        cwd=tmp_path, capture_output=True, text=True, timeout=240)
     assert r.returncode == 1 and "another problem" in r.stderr, r.stderr[-2000:]
+
+
+def test_cli_vtk_async_and_binary(tmp_path):
+    """Snapshots written on background threads equal the inline ones byte for byte; --vtk=binary
+    holds the same points and fields unrounded (the ASCII values within their 12 digits)."""
+    print("cli vtk", flush=True)
+    prob, ref = load_golden("abc_r10_nu025")
+    prob.steps = 120
+    job = str(tmp_path / "job")
+    files.write_job(prob, job, exact_text=True)
+
+    def cli(*opts):
+        r = subprocess.run([EXE, *opts], cwd=tmp_path, capture_output=True, text=True, timeout=240)
+        assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+
+    # --exact: the same kernels in every process (the default autotunes per process)
+    cli("_sync", "job", "--exact", "--vtk-threads=0")
+    cli("_async", "job", "--exact", "--vtk-threads=4")
+    cli("_bin", "job", "--exact", "--vtk=binary")
+    names = ["-0.vtk"] + [f"{k}-{s}.vtk" for s in range(30, 121, 30) for k in ("", "-vel", "-theta")]
+    for nm in names:
+        a = open(f"{job}_sync{nm}", "rb").read()
+        assert a == open(f"{job}_async{nm}", "rb").read(), nm
+        asc = files.read_vtk_field(f"{job}_sync{nm}")
+        b = files.read_vtk_field(f"{job}_bin{nm}")
+        assert np.array_equal(asc["points"], b["points"])
+        key = "vectors" if "vectors" in b else "scalars"
+        assert np.all(np.abs(asc[key] - b[key]) <= 5e-12 * np.abs(b[key]) + 1e-300), nm
