@@ -914,10 +914,51 @@ struct ForceFull {
   }
 };
 
-// Dilatation of the full tiles; one wavefront per tile (full_list order).
+// Dilatation of a quarter of a partial tile (tile rows 4 b .. 4 b + 3, one node per lane) with the
+// masked per-bond sum of k_theta_t68 (the same arithmetic): window rows 4 b .. 4 b + 11 (12 x 24
+// cells) in LDS, u in su[0, 288), the cell masks as bytes from su[288] on.
+__device__ __forceinline__ void theta_part_quarter(const StepArgs &a, const StepState &st,
+                                                   double2 *su, int32_t t, int32_t b) {
+  const TileArgs &T = a.tile;
+  const int32_t *nbr = T.tile_nbr + 9 * t;
+  const int32_t lane = threadIdx.x;
+  int8_t *ok = reinterpret_cast<int8_t *>(su + 12 * kW68);
+  for (int32_t c = lane; c < 12 * kW68; c += 64) {
+    const int32_t s = window_slot68(nbr, 4 * b * kW68 + c);
+    const bool on = s >= 0 && a.kind[s] != 0;
+    su[c] = on ? make_double2(st.px[s], st.py[s]) : make_double2(0.0, 0.0);
+    ok[c] = on;
+  }
+  __syncthreads();
+  const int32_t i = t * (kTile * kTile) + (4 * b + lane / kTile) * kTile + lane % kTile;
+  if (a.kind[i] != 1) return;
+  const int32_t c = (lane / kTile + 4) * kW68 + (lane % kTile + 4);
+  const double2 ui = su[c];
+  constexpr Stencil68 S = make_stencil68();
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};  // bond k feeds chain k % 4
+#pragma unroll
+  for (int k = 0; k < 68; k++) {
+    const Off68 o = S.o[k];
+    const int32_t cj = c + o.ox * kW68 + o.oy;
+    const double2 uj = su[cj];
+    const double term = T.c1[o.cls] * dot_int(uj.x - ui.x, uj.y - ui.y, o.ox, o.oy);
+    acc[k % 4] += ok[cj] ? term : 0.0;
+  }
+  st.theta[i] = a.c2m[i] * ((acc[0] + acc[1]) + (acc[2] + acc[3]));
+}
+
+// Dilatation of the full tiles, one wavefront per tile (full_list order); blocks past the full
+// tiles' (rounded up to a multiple of 8) take the partial tiles, four wavefronts per tile, so they
+// fill the tail of the full tiles' launch instead of following it.
 __global__ void __launch_bounds__(64, 4) k_theta_rb(StepArgs a, StepState st) {
   __shared__ double2 su[kCells68];
   const TileArgs &T = a.tile;
+  const int32_t nfb = (T.n_full + 7) / 8 * 8;
+  if ((int32_t)blockIdx.x >= nfb) {
+    const int32_t pb = blockIdx.x - nfb;
+    if ((pb >> 2) < T.n_part) theta_part_quarter(a, st, su, T.part_list[pb >> 2], pb & 3);
+    return;
+  }
   const int32_t q = tile_of_block(blockIdx.x, T.n_full);
   if (q >= T.n_full) return;
   const int32_t t = T.full_list[q];
@@ -1376,11 +1417,10 @@ hipError_t launch_theta(const StepArgs &a, const StepState &st, bool fast, hipSt
   if (fast) {
     const TileArgs &T = a.tile;
     if (T.theta_kernel == kBlocked) {
-      if (T.n_full > 0)
-        hipLaunchKernelGGL(k_theta_rb, dim3(list_blocks(T.n_full)), dim3(64), 0, s, a, st);
-      hipError_t e = hipGetLastError();
-      if (e != hipSuccess || T.n_part == 0 || !with_part) return e;
-      return launch_theta_part(a, st, s);
+      // the partial tiles in the same launch (4 blocks each) unless another stream takes them
+      const int32_t nb = list_blocks(T.n_full) + (with_part ? 4 * T.n_part : 0);
+      if (nb > 0) hipLaunchKernelGGL(k_theta_rb, dim3(nb), dim3(64), 0, s, a, st);
+      return hipGetLastError();
     } else if (T.theta_kernel == kUnrolled) {
       hipLaunchKernelGGL(k_theta_t68, dim3(tile_blocks(a)), dim3(kBlock), 0, s, a, st,
                          (const int32_t *)nullptr, T.n_tiles);
