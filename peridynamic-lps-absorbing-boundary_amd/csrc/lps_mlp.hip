@@ -195,9 +195,8 @@ __global__ void __launch_bounds__(256) k_mlp_fwd(MlpDev m, const __bf16 *x, Gath
 }
 
 // The network over 128 rows per workgroup: wavefront w owns rows 32 w .. 32 w + 31 and all 256
-// columns (8 accumulator tiles). The weight fragments of kPf K steps at a time are staged once per
-// workgroup in LDS (double buffered by group), so one barrier serves 8 kPf matrix-core operations
-// of each wavefront. The operands of the next group are loaded while the current one computes (one
+// columns (8 accumulator tiles); each K step's weight fragments are staged once per workgroup in
+// LDS (double buffered). The operands of K step k + kPf are loaded while step k computes (one
 // wavefront per SIMD: the prefetch hides the memory latency). Same arithmetic order as k_mlp_fwd,
 // so the same bits.
 constexpr int kPf = 4;
@@ -209,7 +208,7 @@ struct RawG {  // 8 history values (4 (x, y) pairs) of one K step, FP32
 template <bool kGather>
 __global__ void __launch_bounds__(256) k_mlp_fwd_wide(MlpDev m, const __bf16 *x, GatherSrc g,
                                                       int32_t n, float *yo, ApplyDst d) {
-  __shared__ bf16x8 bs[2][kPf][8][64];
+  __shared__ bf16x8 bs[2][8][64];
   __shared__ __bf16 hs[kWideRows][kHsPitch];
   __shared__ float sc[kWideRows][4];
   // layer 1 keeps the rows' history indices where the activations go later
@@ -311,27 +310,23 @@ __global__ void __launch_bounds__(256) k_mlp_fwd_wide(MlpDev m, const __bf16 *x,
       if (layer == 0) issue_a(dd, ra[dd], xa[dd]);
     }
     for (int k0 = 0; k0 < nk; k0 += kPf) {
-      const int buf = (k0 / kPf) & 1;
-      bf16x8 a[kPf];
 #pragma unroll
       for (int dd = 0; dd < kPf; dd++) {
         const int k = k0 + dd;
-        bs[buf][dd][t >> 6][t & 63] = rb0[dd];
-        bs[buf][dd][(t >> 6) + 4][t & 63] = rb1[dd];
-        if (layer == 0) a[dd] = make_a(k, ra[dd], xa[dd]);
-        else a[dd] = *(const bf16x8 *)&hs[lrow][16 * k + ak];
+        bs[k & 1][t >> 6][t & 63] = rb0[dd];
+        bs[k & 1][(t >> 6) + 4][t & 63] = rb1[dd];
+        bf16x8 a;
+        if (layer == 0) a = make_a(k, ra[dd], xa[dd]);
+        else a = *(const bf16x8 *)&hs[lrow][16 * k + ak];
         if (k + kPf < nk) {
           const bf16x8 *src = (const bf16x8 *)(wp + (int64_t)(k + kPf) * 4096);
           rb0[dd] = src[t];
           rb1[dd] = src[t + 256];
           if (layer == 0) issue_a(k + kPf, ra[dd], xa[dd]);
         }
-      }
-      __syncthreads();
+        __syncthreads();
 #pragma unroll
-      for (int dd = 0; dd < kPf; dd++) {
-#pragma unroll
-        for (int q = 0; q < 8; q++) acc[q] = mfma(a[dd], bs[buf][dd][q][lane], acc[q]);
+        for (int q = 0; q < 8; q++) acc[q] = mfma(a, bs[k & 1][q][lane], acc[q]);
       }
     }
     __syncthreads();  // every wavefront is done with hs (history indices or the layer's input)
