@@ -10,7 +10,7 @@ plus the output block every 30 steps, with inputs resident on the GPU and no VTK
 
 Printed: one JSON line with the metric, the roofline of the force pass's main kernel (its average
 duration from HIP events on its stream inside the timed region, against its compulsory HBM bytes,
-121 per node) and the CPU oracle timed on a bounded sample of the same workload on the host's cores.
+105 per node) and the CPU oracle timed on a bounded sample of the same workload on the host's cores.
 
   python bench.py [--steps K] [--warmup W] [--workload cfg2|cfg1] [--no-cpu-baseline]
 """
@@ -48,14 +48,15 @@ def measured_traffic(kernel, workload):
 def force_roofline(path, ti, n_nodes, pt, workload):
     """Roofline of the force pass's main kernel (the longest launch of a step). Its compulsory HBM
     traffic, each element once: reads u, theta, v, a (8 B each component) and kind (1 B), writes
-    a, v, u and the next prediction: 121 B per node. The halo reads of the LDS-staged windows are
-    L2 hits and not counted. A CSR-gather formulation of the same pass would move 68 B per node plus
+    a, v and the next prediction: 105 B per node (the tile kernels leave U^{n+1} of an interior
+    node in the prediction ring; the exact kernel also writes u: 121 B). The halo reads of the
+    LDS-staged windows are L2 hits and not counted. A CSR-gather formulation of the same pass would move 68 B per node plus
     4 B per bond (coordinates, m, row pointer, column index; csr_gather_bytes)."""
     fk = path.split("force ")[1].split(" ")[0].rstrip(",") if "force " in path else "exact"
     kernel = {"blocked": "k_force_rb", "unrolled": "k_force_t68<true>",
               "generic": "k_force_tile", "exact": "k_force_exact"}.get(fk, fk)
     units = ti["full_tile_nodes"] if fk in ("blocked", "unrolled") else n_nodes
-    per_unit = 121
+    per_unit = 121 if fk == "exact" else 105
     force_bytes = per_unit * units
     achieved = force_bytes / (pt["force_ms"] * 1e-3) / 1e9
     traffic = measured_traffic(kernel, workload)

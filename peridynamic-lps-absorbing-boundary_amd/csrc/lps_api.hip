@@ -44,7 +44,7 @@ StepArgs lps_ctx::args() const {
 
 StepState lps_ctx::state(int64_t s) const {
   StepState st{};
-  const Dev2 &Pc = P[s % 2], &Pn = P[(s + 1) % 2];
+  const Dev2 &Pc = P[s % 6], &Pn = P[(s + 1) % 6];
   const Dev2 &Uc = U[s % 3], &Um = U[(s + 2) % 3], &Un = U[(s + 1) % 3];
   const Dev2 &Vc = V[s % 3], &Vm = V[(s + 2) % 3], &Vn = V[(s + 1) % 3];
   const Dev2 &Ac = A[s % 2], &An = A[(s + 1) % 2];
@@ -511,6 +511,8 @@ static int32_t create_impl(lps_ctx *c, const lps_problem *p) {
   }
   for (int b = 0; b < 2; b++) {
     CTX_TRY(c, c->alloc(&c->A[b].x, NS)); CTX_TRY(c, c->alloc(&c->A[b].y, NS));
+  }
+  for (int b = 0; b < 6; b++) {
     CTX_TRY(c, c->alloc(&c->P[b].x, NS)); CTX_TRY(c, c->alloc(&c->P[b].y, NS));
   }
   {
@@ -534,6 +536,21 @@ static int32_t create_impl(lps_ctx *c, const lps_problem *p) {
   CTX_TRY(c, c->alloc(&c->dout, 1));
   CTX_TRY(c, hipHostMalloc((void **)&c->hout, sizeof(DiagPartial)));
   return reset_state(c);
+}
+
+// The prediction ring at step 0 (P[0], the prediction of step 1, written by the caller): U^0 = IC
+// in P[5], U^{-1} = U^{-2} = 0 in P[4], P[3].
+static hipError_t reset_ring(lps_ctx *c) {
+  const size_t bytes = sizeof(double) * c->n_slots;
+  hipError_t e = hipSuccess;
+  for (int b = 1; b < 5 && e == hipSuccess; b++) {
+    e = hipMemsetAsync(c->P[b].x, 0, bytes, c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(c->P[b].y, 0, bytes, c->stream);
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(c->P[5].x, c->IC.x, bytes, hipMemcpyDeviceToDevice, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(c->P[5].y, c->IC.y, bytes, hipMemcpyDeviceToDevice, c->stream);
+  c->u_synced = 3;
+  return e;
 }
 
 // Initial state (src/main.cpp:624-634), theta history (:664-682) and the prediction of step 1.
@@ -560,6 +577,7 @@ static int32_t reset_state(lps_ctx *c) {
     CTX_TRY(c, hipMemcpyAsync(c->TH[0], c->TH0, bytes, hipMemcpyDeviceToDevice, s));
     CTX_TRY(c, hipMemcpyAsync(c->P[0].x, c->P0.x, bytes, hipMemcpyDeviceToDevice, s));
     CTX_TRY(c, hipMemcpyAsync(c->P[0].y, c->P0.y, bytes, hipMemcpyDeviceToDevice, s));
+    CTX_TRY(c, reset_ring(c));
     CTX_TRY(c, hipStreamSynchronize(s));
     c->step = 0;
     c->E0 = -1.0;
@@ -588,6 +606,7 @@ static int32_t reset_state(lps_ctx *c) {
   // prediction of step 1 (later ones come from the force pass)
   CTX_TRY(c, launch_predict(a, c->U[0].x, c->U[0].y, c->V[0].x, c->V[0].y, c->A[0].x, c->A[0].y,
                             c->P[0].x, c->P[0].y, s));
+  CTX_TRY(c, reset_ring(c));
   CTX_TRY(c, hipStreamSynchronize(s));
   c->step = 0;
   c->E0 = -1.0;
@@ -895,6 +914,8 @@ static int32_t merge_batch(lps_ctx *c, const std::vector<lps_ctx *> &sub) {
   }
   for (int b = 0; b < 2; b++) {
     CTX_TRY(c, c->alloc(&c->A[b].x, NS)); CTX_TRY(c, c->alloc(&c->A[b].y, NS));
+  }
+  for (int b = 0; b < 6; b++) {
     CTX_TRY(c, c->alloc(&c->P[b].x, NS)); CTX_TRY(c, c->alloc(&c->P[b].y, NS));
   }
   CTX_TRY(c, c->alloc(&c->partials, NT));
@@ -1037,7 +1058,7 @@ static hipError_t step_once(lps_ctx *c, bool record_events) {
     if ((e = hipStreamWaitEvent(c->stream2, c->ev_fork[0], 0)) != hipSuccess) return e;
     if ((e = launch_abc_theta(a, st, c->stream2)) != hipSuccess) return e;
     if ((e = hipEventRecord(c->ev_join[0], c->stream2)) != hipSuccess) return e;
-    if ((e = launch_abc_uv(a, st, c->stream2)) != hipSuccess) return e;
+    if ((e = launch_abc_uv(a, st, c->ring(c->step), c->stream2)) != hipSuccess) return e;
     if (c->clo && (e = closure_step_hook(c, st, c->stream2)) != hipSuccess) return e;
     if ((e = hipEventRecord(c->ev_join[1], c->stream2)) != hipSuccess) return e;
   } else if (abc && (e = launch_abc_theta(a, st, c->stream)) != hipSuccess) {
@@ -1063,13 +1084,13 @@ static hipError_t step_once(lps_ctx *c, bool record_events) {
   const bool fold = abc && !split && !c->clo && c->abc_in_part && a.tile.force_kernel != kGeneric;
   if (c->fast && fold) {
     if (fork && (e = hipStreamWaitEvent(c->stream, c->ev_join[1], 0)) != hipSuccess) return e;
-    if (!fork && (e = launch_abc_uv(a, st, c->stream)) != hipSuccess) return e;
+    if (!fork && (e = launch_abc_uv(a, st, c->ring(c->step), c->stream)) != hipSuccess) return e;
     if ((e = launch_force_part(a, st, c->stream, true)) != hipSuccess) return e;
   } else if (c->fast) {
     if (!split && (e = launch_force_part(a, st, c->stream)) != hipSuccess) return e;
     if (split && (e = hipStreamWaitEvent(c->stream, c->ev_part[2], 0)) != hipSuccess) return e;
     if (fork && (e = hipStreamWaitEvent(c->stream, c->ev_join[1], 0)) != hipSuccess) return e;
-    if (abc && !fork && (e = launch_abc_uv(a, st, c->stream)) != hipSuccess) return e;
+    if (abc && !fork && (e = launch_abc_uv(a, st, c->ring(c->step), c->stream)) != hipSuccess) return e;
     if (abc && !fork && c->clo && (e = closure_step_hook(c, st, c->stream)) != hipSuccess) return e;
     if (abc && (e = launch_abc_predict(a, st, c->stream)) != hipSuccess) return e;
   }
@@ -1079,6 +1100,7 @@ static hipError_t step_once(lps_ctx *c, bool record_events) {
     c->ev_used++;
   }
   c->step++;
+  c->u_synced = 0;
   return hipSuccess;
 }
 
@@ -1198,6 +1220,7 @@ int32_t lps_step(lps_ctx *c, int32_t nsteps) {
         if (!c->graph_timing) {  // this runtime cannot time graph events: separate launches
           c->ev_prev_used = 0;
           c->step += 30;
+          c->u_synced = 0;
           done += 30;
           continue;
         }
@@ -1209,6 +1232,7 @@ int32_t lps_step(lps_ctx *c, int32_t nsteps) {
         CTX_TRY(c, hipGraphLaunch(c->graph30, c->stream));
       }
       c->step += 30;
+      c->u_synced = 0;
       done += 30;
       continue;
     }
@@ -1249,6 +1273,8 @@ int32_t lps_diagnostics_batch(lps_ctx *c, lps_diag *out) {
   const int64_t s = c->step;
   const StepArgs a = c->args();
   if (!c->fast) return c->fail(LPS_ERR_ARG, "per-instance diagnostics need the tile path");
+  int32_t rc = ctx_sync_u(c, 1);
+  if (rc != LPS_OK) return rc;
   const Dev2 &U = c->U[s % 3], &V = c->V[s % 3], &A = c->A[s % 2];
   const double *th = c->TH[s % 3];
   if (c->batch) {
@@ -1310,6 +1336,8 @@ int32_t lps_diagnostics(lps_ctx *c, lps_diag *out) {
   if (c->batch) return c->fail(LPS_ERR_ARG, "a batched context reports per instance: use lps_diagnostics_batch");
   const int64_t s = c->step;
   const StepArgs a = c->args();
+  int32_t rc = ctx_sync_u(c, 1);
+  if (rc != LPS_OK) return rc;
   const Dev2 &U = c->U[s % 3], &V = c->V[s % 3], &A = c->A[s % 2];
   const double *th = c->TH[s % 3];
   if (c->fast)
@@ -1383,6 +1411,7 @@ static int32_t fetch_range(lps_ctx *c, int64_t s0, int64_t s1, int64_t node0, do
     return LPS_OK;
   };
   int32_t rc;
+  if (u && (rc = ctx_sync_u(c, 1)) != LPS_OK) return rc;
   if ((rc = pull2(c->U[s % 3], u)) != LPS_OK) return rc;
   if ((rc = pull2(c->V[s % 3], v)) != LPS_OK) return rc;
   if ((rc = pull2(c->A[s % 2], a)) != LPS_OK) return rc;
@@ -1432,3 +1461,14 @@ int64_t lps_horizons(lps_ctx *c, int64_t *row_ptr, int32_t *cols) {
 }  // extern "C"
 
 void lps::ctx_drop_graphs(lps_ctx *c) { drop_graphs(c); }
+
+int32_t lps::ctx_sync_u(lps_ctx *c, int32_t depth) {
+  if (!c->fast || c->u_synced >= depth) return LPS_OK;
+  const StepArgs a = c->args();
+  for (int32_t j = c->u_synced; j < depth; j++) {
+    const Dev2 &R = c->ring_u(c->step, j), &U = c->U[(c->step + 3 - j) % 3];
+    CTX_TRY(c, launch_u_from_ring(a, R.x, R.y, U.x, U.y, c->stream));
+  }
+  c->u_synced = depth;
+  return LPS_OK;
+}

@@ -63,7 +63,7 @@ uint64_t fingerprint(const lps_ctx *c) {
 void state_arrays(const lps_ctx *c, int64_t s, double *arr[kArrays]) {
   const lps::Dev2 &Uc = c->U[s % 3], &Um = c->U[(s + 2) % 3];
   const lps::Dev2 &Vc = c->V[s % 3], &Vm = c->V[(s + 2) % 3];
-  const lps::Dev2 &Ac = c->A[s % 2], &Pc = c->P[s % 2];
+  const lps::Dev2 &Ac = c->A[s % 2], &Pc = c->P[s % 6];
   double *a[kArrays] = {Uc.x, Uc.y, Um.x, Um.y, Vc.x, Vc.y, Vm.x, Vm.y,
                         Ac.x, Ac.y, Pc.x, Pc.y, c->TH[s % 3], c->TH[(s + 2) % 3]};
   for (int k = 0; k < kArrays; k++) arr[k] = a[k];
@@ -94,6 +94,8 @@ int32_t lps_checkpoint_save(lps_ctx *c, const char *path) {
   if (c->clo)
     return c->fail(LPS_ERR_ARG, "a context with a learned closure attached cannot be checkpointed");
   CTX_TRY(c, hipSetDevice(c->opt.device));
+  int32_t rc = lps::ctx_sync_u(c, 2);  // U^n, U^{n-1} of every node
+  if (rc != LPS_OK) return rc;
   CTX_TRY(c, hipStreamSynchronize(c->stream));
   if (c->stream2) CTX_TRY(c, hipStreamSynchronize(c->stream2));
   if (c->stream3) CTX_TRY(c, hipStreamSynchronize(c->stream3));
@@ -152,6 +154,13 @@ int32_t lps_checkpoint_load(lps_ctx *c, const char *path) {
   for (int k = 0; k < kArrays; k++)
     CTX_TRY(c, hipMemcpy(arr[k], buf.data() + (size_t)k * c->n_slots,
                          sizeof(double) * c->n_slots, hipMemcpyHostToDevice));
+  // the interior nodes' U^n, U^{n-1} back into the prediction ring
+  for (int j = 0; j < 2; j++) {
+    const lps::Dev2 &R = c->ring_u(h.step, j), &U = c->U[(h.step + 3 - j) % 3];
+    CTX_TRY(c, hipMemcpy(R.x, U.x, sizeof(double) * c->n_slots, hipMemcpyDeviceToDevice));
+    CTX_TRY(c, hipMemcpy(R.y, U.y, sizeof(double) * c->n_slots, hipMemcpyDeviceToDevice));
+  }
+  c->u_synced = 2;
   c->step = h.step;
   if (c->batch) {
     c->E0_inst = e0;

@@ -56,13 +56,20 @@ struct ClosureAttach {
 };
 
 // History shift of step n: record = (U^n, V^n, the 7 steps before), and its max |u|, max |v|.
+// U^n of an interior node (kind 1) comes from the prediction ring (rux, ruy).
 __global__ void __launch_bounds__(256) k_hist_shift(int32_t n_hist, const int32_t *slot,
-                                                    const double *ux, const double *uy,
-                                                    const double *vx, const double *vy,
-                                                    const float *prev, float *cur, float *mx) {
+                                                    const int8_t *kind, const double *rux,
+                                                    const double *ruy, const double *ux,
+                                                    const double *uy, const double *vx,
+                                                    const double *vy, const float *prev,
+                                                    float *cur, float *mx) {
   const int32_t b = blockIdx.x * 256 + threadIdx.x;
   if (b >= n_hist) return;
   const int32_t i = slot[b];
+  if (kind[i] == 1) {
+    ux = rux;
+    uy = ruy;
+  }
   const float4 *p = (const float4 *)(prev + (int64_t)b * kClRec);
   float4 *q = (float4 *)(cur + (int64_t)b * kClRec);
   float mu = 0.0f, mv = 0.0f;
@@ -94,7 +101,8 @@ hipError_t closure_step_hook(lps_ctx *c, const StepState &st, hipStream_t s) {
   const int par = (int)(c->step & 1);
   if (A->n_hist > 0) {
     hipLaunchKernelGGL(k_hist_shift, dim3((A->n_hist + 255) / 256), dim3(256), 0, s, A->n_hist,
-                       A->hist_slot, st.ux, st.uy, st.vx, st.vy, A->hist[par ^ 1], A->hist[par],
+                       A->hist_slot, c->kind, c->ring_u(c->step, 0).x, c->ring_u(c->step, 0).y,
+                       st.ux, st.uy, st.vx, st.vy, A->hist[par ^ 1], A->hist[par],
                        A->mx[par]);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
@@ -659,6 +667,8 @@ int32_t lps_closure_collect(lps_closure *clo, lps_ctx *c, const lps_problem *p, 
     const int par = (int)(n & 1);
     ca.g = A->src(par);
     ca.step = s1;
+    // U^{s1}, U^{s1-1}, U^{s1-2} of every node
+    if ((rc = ctx_sync_u(c, 3)) != LPS_OK) break;
     const lps_ctx::Dev2 &U1 = c->U[s1 % 3], &U0 = c->U[n % 3], &Um = c->U[(n + 2) % 3];
     const lps_ctx::Dev2 &V1 = c->V[s1 % 3], &V0 = c->V[n % 3], &Vm = c->V[(n + 2) % 3];
     ca.ux1 = U1.x; ca.uy1 = U1.y; ca.vx1 = V1.x; ca.vy1 = V1.y;

@@ -26,6 +26,7 @@ struct lps_ctx {
   using TileArgs = lps::TileArgs;
   using StepArgs = lps::StepArgs;
   using StepState = lps::StepState;
+  using RingU = lps::RingU;
   lps_options opt{};
   hipStream_t stream = nullptr;
   hipStream_t stream2 = nullptr;  // the ABC rows' kernels beside the tile kernels
@@ -63,7 +64,14 @@ struct lps_ctx {
   int32_t *cloud = nullptr;
   double *gu = nullptr, *gv = nullptr, *gth = nullptr;
   TileArgs tile{};
-  Dev2 U[3], V[3], A[2], P[2], IC, ICD;
+  // Prediction ring: step s reads P[s % 6] (U^{s+1} before the ABC overwrite) and writes
+  // P[(s + 1) % 6]. At the interior nodes (kind 1) U^{s+1} equals that prediction and the fast
+  // path does not store it again: U^{s-j} of those nodes is ring_u(s, j) = P[(s + 5 - j) % 6]
+  // (j <= 3 while step s runs). U[] holds the ABC nodes' values, and the interior nodes' only
+  // after sync_u copies them in; u_synced: U^{s-j} holds every node for j < u_synced.
+  Dev2 U[3], V[3], A[2], P[6], IC, ICD;
+  int32_t u_synced = 3;
+  const Dev2 &ring_u(int64_t s, int j) const { return P[(s + 5 - j) % 6]; }
   double *TH[3] = {nullptr, nullptr, nullptr};
   DiagPartial *partials = nullptr, *dout = nullptr, *hout = nullptr;
   int32_t n_partials = 0;
@@ -107,6 +115,10 @@ struct lps_ctx {
   }
   StepArgs args() const;
   StepState state(int64_t s) const;
+  RingU ring(int64_t s) const {
+    const Dev2 &a = ring_u(s, 0), &b = ring_u(s, 1);
+    return RingU{a.x, a.y, b.x, b.y};
+  }
 };
 
 namespace lps {
@@ -121,6 +133,9 @@ int32_t closure_record_hook(lps_ctx *c, const lps_diag *d, int32_t count);
 void closure_free(lps_ctx *c);
 // Drops the captured 30-step graphs (they hold the kernels they were captured with).
 void ctx_drop_graphs(lps_ctx *c);
+// Fast path: copies the interior nodes' U^{s-j}, j < depth (<= 3), from the prediction ring into
+// U (s = ctx->step), on ctx->stream, where not already done since the last step.
+int32_t ctx_sync_u(lps_ctx *c, int32_t depth);
 }  // namespace lps
 
 #define CTX_TRY(ctx, call)                                   \

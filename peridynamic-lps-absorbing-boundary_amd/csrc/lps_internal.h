@@ -116,6 +116,13 @@ struct StepState {
   double *pnx, *pny;               // next prediction U^{n+2} (fused predictor)
 };
 
+// Fast path: U^n and U^{n-1} at the interior nodes (kind 1). The corrector leaves U^{n+1} equal
+// to the prediction there, so those values stay only in the ring of earlier predictions; the
+// ABC nodes' values are in StepState.ux / uxm.
+struct RingU {
+  const double *ux, *uy, *uxm, *uym;
+};
+
 // Per-block partial sums of the output block.
 struct DiagPartial {
   double energy, ek, ep, px, py, max_u, max_v, max_a, th_max, th_min, ub, vb;
@@ -147,8 +154,11 @@ hipError_t launch_force_part(const StepArgs &a, const StepState &st, hipStream_t
 // second stream beside the tile passes; the force pass needs abc_theta, launch_abc_predict (the
 // next prediction at the ABC nodes) needs abc_uv and the force pass.
 hipError_t launch_abc_theta(const StepArgs &a, const StepState &st, hipStream_t s);
-hipError_t launch_abc_uv(const StepArgs &a, const StepState &st, hipStream_t s);
+hipError_t launch_abc_uv(const StepArgs &a, const StepState &st, const RingU &r, hipStream_t s);
 hipError_t launch_abc_predict(const StepArgs &a, const StepState &st, hipStream_t s);
+// U at the interior nodes (kind 1) copied from the prediction ring: ux[i] = px[i] there.
+hipError_t launch_u_from_ring(const StepArgs &a, const double *px, const double *py, double *ux,
+                              double *uy, hipStream_t s);
 hipError_t launch_theta_field(const StepArgs &a, const double *ux, const double *uy, double *theta,
                               hipStream_t s);
 hipError_t launch_weighted_volume(const StepArgs &a, double *m, hipStream_t s);

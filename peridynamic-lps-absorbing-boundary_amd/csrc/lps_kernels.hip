@@ -646,6 +646,8 @@ __device__ __forceinline__ void force_sum68(const double2 *su, const double2 *sa
 // Force-pass epilogue of node slot i: acceleration, velocity-Verlet corrector and the next step's
 // prediction (src/main.cpp:874, 922-925, 725-728). At an ABC node (kind 2) only the acceleration:
 // G_u / G_v replace its displacement and velocity (k_abc_uv) and k_abc_predict then predicts it.
+// U^{n+1} of an interior node is its prediction ui (src/main.cpp:922 keeps u): it is not stored
+// again; it stays in the prediction ring (RingU), from which launch_u_from_ring fills U on demand.
 // vi, ai: V^n and A^n of the node (read by the caller, possibly early); P: its instance.
 __device__ __forceinline__ void node_update(const InstParams &P, const StepState &st, int32_t i,
                                             int8_t kd, double2 ui, double2 vi, double2 ai,
@@ -657,7 +659,6 @@ __device__ __forceinline__ void node_update(const InstParams &P, const StepState
   const double dt = P.dt, h = 0.5 * dt;
   const double v1x = vi.x + h * (ai.x + a1x);
   const double v1y = vi.y + h * (ai.y + a1y);
-  st.ux1[i] = ui.x; st.uy1[i] = ui.y;
   st.vx1[i] = v1x; st.vy1[i] = v1y;
   st.pnx[i] = ui.x + dt * (v1x + h * a1x);
   st.pny[i] = ui.y + dt * (v1y + h * a1y);
@@ -1042,8 +1043,9 @@ __global__ void __launch_bounds__(kBlock) k_abc_theta(StepArgs a, StepState st) 
 
 // Fast path, ABC rows: U^{n+1} = G_u [U^n, U^{n-1}] and V^{n+1} = G_v [V^n, V^{n-1}] over the
 // row's cloud (src/main.cpp:883-916, 933-974), one wavefront per row, lane k on cloud member k.
-// The tile kernels compute the force at the ABC nodes like at any node.
-__global__ void __launch_bounds__(kBlock) k_abc_uv(StepArgs a, StepState st) {
+// The tile kernels compute the force at the ABC nodes like at any node. U of an interior cloud
+// member comes from the prediction ring r, of an ABC member from U.
+__global__ void __launch_bounds__(kBlock) k_abc_uv(StepArgs a, StepState st, RingU r) {
   const int32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int32_t row = blockIdx.x * kWavesPerBlock + wid;
   if (row >= a.n_abc) return;
@@ -1058,7 +1060,9 @@ __global__ void __launch_bounds__(kBlock) k_abc_uv(StepArgs a, StepState st) {
     if (k >= 0) {
       const double4 g0 = *(const double4 *)(gu0 + 4 * jj), g1 = *(const double4 *)(gu1 + 4 * jj);
       const double4 h0 = *(const double4 *)(gv0 + 4 * jj), h1 = *(const double4 *)(gv1 + 4 * jj);
-      const double ux = st.ux[k], uy = st.uy[k], uxm = st.uxm[k], uym = st.uym[k];
+      const bool in = a.kind[k] == 1;
+      const double ux = (in ? r.ux : st.ux)[k], uy = (in ? r.uy : st.uy)[k];
+      const double uxm = (in ? r.uxm : st.uxm)[k], uym = (in ? r.uym : st.uym)[k];
       const double vx = st.vx[k], vy = st.vy[k], vxm = st.vxm[k], vym = st.vym[k];
       t0 = g0.x * ux + g0.y * uy + g0.z * uxm + g0.w * uym;
       t1 = g1.x * ux + g1.y * uy + g1.z * uxm + g1.w * uym;
@@ -1479,9 +1483,26 @@ hipError_t launch_force_part(const StepArgs &a, const StepState &st, hipStream_t
   return hipGetLastError();
 }
 
-hipError_t launch_abc_uv(const StepArgs &a, const StepState &st, hipStream_t s) {
+hipError_t launch_abc_uv(const StepArgs &a, const StepState &st, const RingU &r, hipStream_t s) {
   if (a.n_abc == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_abc_uv, dim3(abc_blocks(a)), dim3(kBlock), 0, s, a, st);
+  hipLaunchKernelGGL(k_abc_uv, dim3(abc_blocks(a)), dim3(kBlock), 0, s, a, st, r);
+  return hipGetLastError();
+}
+
+__global__ void __launch_bounds__(kBlock) k_u_from_ring(int32_t n, const int8_t *kind,
+                                                        const double *px, const double *py,
+                                                        double *ux, double *uy) {
+  const int32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n || kind[i] != 1) return;
+  ux[i] = px[i];
+  uy[i] = py[i];
+}
+
+hipError_t launch_u_from_ring(const StepArgs &a, const double *px, const double *py, double *ux,
+                              double *uy, hipStream_t s) {
+  if (a.n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_u_from_ring, dim3(blocks_for(a.n)), dim3(kBlock), 0, s, a.n, a.kind, px, py,
+                     ux, uy);
   return hipGetLastError();
 }
 

@@ -175,3 +175,24 @@ def test_timing_mode_reports_pass_times_and_changes_no_result(graph, r40_problem
         assert np.array_equal(plain[k], timed[k]), k
     assert all(pt[k] > 0 for k in ("theta_ms", "force_ms", "step_ms")), pt
     assert pt["step_ms"] >= pt["theta_ms"] + pt["force_ms"], pt
+
+
+@pytest.mark.parametrize("name", ["abc_r12_nu033", "noabc_r30"])
+def test_fields_after_any_step_count_match_oracle(name):
+    """The tile path keeps U^{n+1} of the interior nodes only in its six-entry prediction ring and
+    copies it into U before a fetch or an output record. Step counts that land on every ring
+    phase, with a 30-step graph replay in between, give the oracle's fields and energy."""
+    say("ring phases", name)
+    prob, _ = load_golden(name)
+    o = pyoracle.Oracle(prob)
+    with solver.Solver(prob) as s:
+        assert s.path.startswith("tile"), s.path
+        for k in (1, 1, 3, 1, 7, 17, 31, 2):
+            s.step(k)
+            o.run(k)
+            assert s.step_index == o.step
+            f, of = s.fetch(), o.fetch()
+            for key in ("u", "v", "a", "theta"):
+                assert rel_l2(f[key], of[key]) <= 1e-10, (s.step_index, key)
+            e, oe = s.diagnostics()["energy"], o.diagnostics()["energy"]
+            assert abs(e - oe) <= 1e-10 * abs(oe), (s.step_index, e, oe)
