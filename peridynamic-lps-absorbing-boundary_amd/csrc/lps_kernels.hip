@@ -1205,18 +1205,53 @@ __device__ __forceinline__ void diag_identity(DiagPartial &d) {
   d.max_u_orig = INT64_MAX; d.max_v_orig = INT64_MAX;
 }
 
-__device__ void block_reduce_diag(DiagPartial &d, DiagPartial *sh) {
-  sh[threadIdx.x] = d;
-  __syncthreads();
-  for (int s = kBlock / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) {
-      DiagPartial t = sh[threadIdx.x];
-      diag_merge(t, sh[threadIdx.x + s]);
-      sh[threadIdx.x] = t;
-    }
-    __syncthreads();
+// Butterfly over the wavefront: each round merges a lane's partial with its partner's, lane l ^ off
+// (sums commute, the maxima break ties by the lower node id), so a fixed pattern of merges.
+__device__ __forceinline__ void wave_reduce_diag(DiagPartial &d) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    DiagPartial o;
+    o.energy = __shfl_xor(d.energy, off);
+    o.ek = __shfl_xor(d.ek, off);
+    o.ep = __shfl_xor(d.ep, off);
+    o.px = __shfl_xor(d.px, off);
+    o.py = __shfl_xor(d.py, off);
+    o.max_u = __shfl_xor(d.max_u, off);
+    o.max_v = __shfl_xor(d.max_v, off);
+    o.max_a = __shfl_xor(d.max_a, off);
+    o.th_max = __shfl_xor(d.th_max, off);
+    o.th_min = __shfl_xor(d.th_min, off);
+    o.ub = __shfl_xor(d.ub, off);
+    o.vb = __shfl_xor(d.vb, off);
+    o.max_u_orig = __shfl_xor(d.max_u_orig, off);
+    o.max_v_orig = __shfl_xor(d.max_v_orig, off);
+    diag_merge(d, o);
   }
+}
+
+// Block reduction: the wavefronts' results (lane 0 of each) merged in wavefront order. Every
+// thread returns the block's result. sh: kWavesPerBlock entries.
+__device__ void block_reduce_diag(DiagPartial &d, DiagPartial *sh) {
+  wave_reduce_diag(d);
+  const int32_t wid = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) sh[wid] = d;
+  __syncthreads();
   d = sh[0];
+  for (int w = 1; w < kWavesPerBlock; w++) diag_merge(d, sh[w]);
+}
+
+// Thread-strided merge of p[0, np) in a fixed order (thread j: j, j + kBlock, ...), four loads in
+// flight per thread.
+__device__ __forceinline__ void merge_strided(DiagPartial &d, const DiagPartial *p, int32_t np) {
+  int32_t t = threadIdx.x;
+  for (; t + 3 * kBlock < np; t += 4 * kBlock) {
+    const DiagPartial p0 = p[t], p1 = p[t + kBlock], p2 = p[t + 2 * kBlock], p3 = p[t + 3 * kBlock];
+    diag_merge(d, p0);
+    diag_merge(d, p1);
+    diag_merge(d, p2);
+    diag_merge(d, p3);
+  }
+  for (; t < np; t += kBlock) diag_merge(d, p[t]);
 }
 
 __global__ void __launch_bounds__(kBlock) k_diag(StepArgs a, const double *ux, const double *uy,
@@ -1225,7 +1260,7 @@ __global__ void __launch_bounds__(kBlock) k_diag(StepArgs a, const double *ux, c
                                                  const double *theta, const int32_t *perm,
                                                  double exclude_r, int32_t per_block,
                                                  DiagPartial *partials) {
-  __shared__ DiagPartial sh[kBlock];
+  __shared__ DiagPartial sh[kWavesPerBlock];
   DiagPartial d;
   diag_identity(d);
   const int32_t base = blockIdx.x * per_block;
@@ -1293,7 +1328,7 @@ __global__ void __launch_bounds__(kBlock) k_diag_tile(StepArgs a, const double *
                                                       const double *ay, const double *theta,
                                                       const int32_t *perm,
                                                       DiagPartial *partials) {
-  __shared__ DiagPartial sh[kBlock];
+  __shared__ DiagPartial sh[kWavesPerBlock];
   __shared__ double2 su[kMaxWin * kMaxWin];
   __shared__ int8_t sok[kMaxWin * kMaxWin];
   const TileArgs &T = a.tile;
@@ -1364,10 +1399,10 @@ __global__ void __launch_bounds__(kBlock) k_diag_tile(StepArgs a, const double *
 
 __global__ void __launch_bounds__(kBlock) k_diag_final(const DiagPartial *partials, int32_t np,
                                                        DiagPartial *out) {
-  __shared__ DiagPartial sh[kBlock];
+  __shared__ DiagPartial sh[kWavesPerBlock];
   DiagPartial d;
   diag_identity(d);
-  for (int32_t t = threadIdx.x; t < np; t += kBlock) diag_merge(d, partials[t]);
+  merge_strided(d, partials, np);
   block_reduce_diag(d, sh);
   if (threadIdx.x == 0) *out = d;
 }
@@ -1376,11 +1411,11 @@ __global__ void __launch_bounds__(kBlock) k_diag_final(const DiagPartial *partia
 __global__ void __launch_bounds__(kBlock) k_diag_final_inst(const DiagPartial *partials,
                                                             const int32_t *tile_off,
                                                             DiagPartial *out) {
-  __shared__ DiagPartial sh[kBlock];
+  __shared__ DiagPartial sh[kWavesPerBlock];
   const int32_t i = blockIdx.x, t0 = tile_off[i], np = tile_off[i + 1] - t0;
   DiagPartial d;
   diag_identity(d);
-  for (int32_t t = threadIdx.x; t < np; t += kBlock) diag_merge(d, partials[t0 + t]);
+  merge_strided(d, partials + t0, np);
   block_reduce_diag(d, sh);
   if (threadIdx.x == 0) out[i] = d;
 }
