@@ -169,8 +169,10 @@ double lps_m_ref(const lps_ctx *ctx);
  * kernel (exact path: nodes and ABC rows; fast path: the selected tile kernel), out[2] the rest of
  * the step (fast path: the partial tiles' force kernel, the wait for the ABC rows' G_u / G_v, which
  * run on a second stream beside the tile kernels, and the ABC prediction), out[3] whole step.
- * Timing is enabled with lps_set_timing(ctx, 1); timed steps run as separate launches, without
- * the HIP graph (HIP cannot time events recorded inside a graph). */
+ * lps_set_timing(ctx, 1) records four events per step, lps_set_timing(ctx, 2) only the two around
+ * the force pass's main kernel (out[1]; the others read 0), 0 switches timing off. Timed steps
+ * replay 30-step graphs with the events as external event-record nodes; where the runtime cannot
+ * time such events, they run as separate launches. */
 int32_t lps_set_timing(lps_ctx *ctx, int32_t on);
 int32_t lps_pass_times(lps_ctx *ctx, double *out4);
 

@@ -1036,7 +1036,7 @@ static hipError_t step_once(lps_ctx *c, bool record_events) {
   hipError_t e;
   if (record_events) {
     ev = &c->ev[kEv * (30 * (size_t)c->ev_set + c->ev_used)];
-    if ((e = record_timing(c, ev[0])) != hipSuccess) return e;
+    if (!c->timing_force_only && (e = record_timing(c, ev[0])) != hipSuccess) return e;
   }
   // fast path: the ABC rows' G_theta and G_u / G_v read only the previous steps' state and run on
   // the second stream beside the tile passes; the force pass waits for G_theta (ev_join[0]), the
@@ -1096,7 +1096,7 @@ static hipError_t step_once(lps_ctx *c, bool record_events) {
   }
   if (c->clo && !abc && (e = closure_step_hook(c, st, c->stream)) != hipSuccess) return e;
   if (record_events) {
-    if ((e = record_timing(c, ev[3])) != hipSuccess) return e;
+    if (!c->timing_force_only && (e = record_timing(c, ev[3])) != hipSuccess) return e;
     c->ev_used++;
   }
   c->step++;
@@ -1109,11 +1109,13 @@ static hipError_t step_once(lps_ctx *c, bool record_events) {
 static int32_t flush_set(lps_ctx *c, int32_t set, int32_t used, bool from_graph) {
   if (used == 0) return LPS_OK;
   hipEvent_t *e0 = &c->ev[kEv * 30 * (size_t)set];
-  CTX_TRY(c, hipEventSynchronize(e0[kEv * (used - 1) + kEv - 1]));
-  std::vector<float> t((size_t)used * kEv);
+  const bool fo = c->timing_force_only;  // events 1 and 2 only
+  CTX_TRY(c, hipEventSynchronize(e0[kEv * (used - 1) + (fo ? 2 : kEv - 1)]));
+  std::vector<float> t((size_t)used * kEv, 0.0f);
   for (int32_t k = 0; k < used; k++) {
     hipEvent_t *e = e0 + kEv * k;
     for (int q = 0; q < kEv; q++) {
+      if (fo && q != 1) continue;
       hipError_t r = q < kEv - 1 ? hipEventElapsedTime(&t[kEv * k + q], e[q], e[q + 1])
                                  : hipEventElapsedTime(&t[kEv * k + q], e[0], e[kEv - 1]);
       if (r != hipSuccess && from_graph) {
@@ -1144,6 +1146,14 @@ static int32_t flush_events(lps_ctx *c) {
 int32_t lps_set_timing(lps_ctx *c, int32_t on) {
   int32_t rc = flush_events(c);
   if (rc != LPS_OK) return rc;
+  const bool fo = on == 2;
+  if (fo != c->timing_force_only) {  // the timed graphs hold the other mode's event nodes
+    for (hipGraphExec_t &g : c->graph30t) {
+      if (g) (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+  }
+  c->timing_force_only = fo;
   c->timing = on != 0;
   for (double &t : c->t_pass) t = 0.0;
   c->t_count = 0;

@@ -312,8 +312,10 @@ class Solver:
     def m_ref(self) -> float:
         return self._L.lps_m_ref(self._h)
 
-    def set_timing(self, on: bool):
-        self._check(self._L.lps_set_timing(self._h, 1 if on else 0))
+    def set_timing(self, on):
+        """True / 1: the four pass events per step; 2: only the force kernel's two; False / 0: off."""
+        mode = 2 if (not isinstance(on, bool) and on == 2) else (1 if on else 0)
+        self._check(self._L.lps_set_timing(self._h, mode))
 
     def pass_times(self):
         out = np.zeros(4)

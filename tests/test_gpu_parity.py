@@ -158,8 +158,9 @@ def test_tile_path_matches_oracle_on_a_larger_domain(variant, r40_problem):
 
 @pytest.mark.parametrize("graph", [True, False])
 def test_timing_mode_reports_pass_times_and_changes_no_result(graph, r40_problem):
-    """lps_set_timing: timed steps run as separate launches with per-pass events (also when graphs
-    are on); the state afterwards is the same bits as an untimed run."""
+    """lps_set_timing: per-pass events (mode 1) or the force kernel's two events only (mode 2),
+    inside the 30-step graphs when graphs are on; the state afterwards is the same bits as an
+    untimed run, and switching modes in between changes nothing."""
     say("timing", graph)
     prob = r40_problem
     with solver.Solver(prob, use_graph=graph) as s:
@@ -171,10 +172,22 @@ def test_timing_mode_reports_pass_times_and_changes_no_result(graph, r40_problem
         pt = s.pass_times()
         s.set_timing(False)
         timed = s.fetch()
+        s.reset()
+        s.set_timing(2)
+        s.run(30)
+        s.set_timing(True)
+        s.run(1)
+        s.set_timing(2)
+        s.run(29)
+        pf = s.pass_times()
+        s.set_timing(False)
+        force_only = s.fetch()
     for k in ("u", "v", "theta"):
         assert np.array_equal(plain[k], timed[k]), k
+        assert np.array_equal(plain[k], force_only[k]), k
     assert all(pt[k] > 0 for k in ("theta_ms", "force_ms", "step_ms")), pt
     assert pt["step_ms"] >= pt["theta_ms"] + pt["force_ms"], pt
+    assert pf["force_ms"] > 0 and pf["theta_ms"] == pf["abc_ms"] == pf["step_ms"] == 0.0, pf
 
 
 @pytest.mark.parametrize("name", ["abc_r12_nu033", "noabc_r30"])
