@@ -1380,12 +1380,21 @@ __global__ void __launch_bounds__(kBlock) k_diag_tile(StepArgs a, const double *
       d.ek = ek;
       const int32_t c = (threadIdx.x / kTile + T.halo) * W + (threadIdx.x % kTile + T.halo);
       double acc = 0.0;
-      for (int32_t k = 0; k < T.nt; k++) {
-        const int32_t cj = c + T.delta[k];
-        const double hx = T.hx[k], hy = T.hy[k], g2 = T.g2[k], xin = T.xi[k];
-        const double2 uj = su[cj];
-        const double ed = (uj.x - uxi) * hx + (uj.y - uyi) * hy - th * xin * 0.5;
-        acc += sok[cj] ? ed * ed * g2 : 0.0;
+      if (T.tile_full[t]) {  // every window cell holds a node: no mask
+        const double thh = 0.5 * th;
+        for (int32_t k = 0; k < T.nt; k++) {
+          const double2 uj = su[c + T.delta[k]];
+          const double ed = fma(uj.y - uyi, T.hy[k], (uj.x - uxi) * T.hx[k]) - thh * T.xi[k];
+          acc = fma(ed * T.g2[k], ed, acc);
+        }
+      } else {
+        for (int32_t k = 0; k < T.nt; k++) {
+          const int32_t cj = c + T.delta[k];
+          const double hx = T.hx[k], hy = T.hy[k], g2 = T.g2[k], xin = T.xi[k];
+          const double2 uj = su[cj];
+          const double ed = (uj.x - uxi) * hx + (uj.y - uyi) * hy - th * xin * 0.5;
+          acc += sok[cj] ? ed * ed * g2 : 0.0;
+        }
       }
       const double W_i = 0.5 * P.kappa * th * th + (4.0 * P.mu / a.m[i]) * acc;
       const double ep = W_i * a.dx * a.dy;
