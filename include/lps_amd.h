@@ -317,7 +317,8 @@ typedef struct lps_abc_rows {
 /* G_u, G_v (2 count x 4 max_cloud each, the .ciNodesU / .ciNodesV layout) and, when gth is not
  * NULL, G_theta (count x 2 max_cloud, .ciNodesTheta) for time step dt; theta_coeff = 2 / m_ref.
  * Minimum-norm least squares with the singular values s_i <= rcond s_1 dropped, as dgelsd
- * (rcond 1e-14 in the reference). rank (count x 3, may be NULL) receives the number of singular
+ * (rcond 1e-14 in the reference); as in dgelsd, rcond <= 0 or >= 1 means the machine epsilon
+ * 2^-53, and a NaN rcond is refused (LPS_ERR_ARG). rank (count x 3, may be NULL) receives the number of singular
  * values kept for G_u, G_v, G_theta, or -1 where the SVD did not converge. */
 int32_t lps_abc_operators(int32_t device, const lps_abc_rows *rows, const lps_abc_modes *modes,
                           double dt, double theta_coeff, double rcond, double *gu, double *gv,

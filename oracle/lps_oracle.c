@@ -511,6 +511,7 @@ int32_t oracle_run(oracle_ctx *c, int32_t nsteps, oracle_diag *recs, int32_t max
             double ry = Y(nb) - Y(node);
             double dist = sqrt(rx * rx + ry * ry);
             double wt = 1.0 / (dist + 1e-30);
+            if (c->p.plant == 1) wt = wt * wt; /* planted mistake (tests only) */
             theta_avg += wt * Theta[nb];
             w_sum += wt;
           }

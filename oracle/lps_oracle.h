@@ -30,6 +30,8 @@ typedef struct oracle_problem {
   int32_t abc_enabled;        /* 1: ABCS_STEP == 2, 0: ABCS_STEP == -1 */
   int32_t n_notch;            /* prenotch segments (.data1[2]) */
   const double *notch;        /* n_notch x 4: x0 y0 x1 y1 (.prenotch) */
+  int32_t plant;              /* tests only: 0 the reference's algorithm; 1 plants a wrong IDW
+                                 weight (1/d^2 for 1/d), so a test can show its fixture catches it */
 } oracle_problem;
 
 /* One output record, the quantities of src/main.cpp:986-1062. */

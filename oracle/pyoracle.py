@@ -19,7 +19,7 @@ class OracleProblem(C.Structure):
                 ("icd", C.c_void_p), ("abc", C.c_void_p), ("clouds", C.c_void_p),
                 ("Gu", C.c_void_p), ("Gv", C.c_void_p), ("Gth", C.c_void_p),
                 ("theta_method", C.c_int32), ("abc_enabled", C.c_int32),
-                ("n_notch", C.c_int32), ("notch", C.c_void_p)]
+                ("n_notch", C.c_int32), ("notch", C.c_void_p), ("plant", C.c_int32)]
 
 
 class OracleDiag(C.Structure):
@@ -94,7 +94,10 @@ def diag_to_dict(d: OracleDiag) -> dict:
 class Oracle:
     """The CPU oracle on one problem (lps_amd.preprocess.Problem)."""
 
-    def __init__(self, prob, theta_method: int = 1, abc_enabled: bool = True, threads: int = 0):
+    def __init__(self, prob, theta_method: int = 1, abc_enabled: bool = True, threads: int = 0,
+                 plant: int = 0):
+        """plant (tests only): 1 swaps the IDW weight 1/d for 1/d^2, a mistake the reference's
+        IDW fixtures must catch."""
         self._keep = []
 
         def arr(a, dt):
@@ -121,6 +124,7 @@ class Oracle:
             p.Gv = _ptr(arr(prob.Gv, np.float64))
             p.Gth = _ptr(arr(prob.Gth, np.float64)) if prob.Gth.size else None
         p.theta_method = theta_method
+        p.plant = plant
         p.abc_enabled = 1 if (abc_enabled and Cn > 0) else 0
         pre = np.asarray(getattr(prob, "prenotch", np.zeros((0, 4))), dtype=np.float64)
         if pre.shape[0] > 0:

@@ -388,9 +388,17 @@ int32_t lps_abc_system(int32_t device, const lps_abc_rows *rows, const lps_abc_m
   return LPS_OK;
 }
 
+// dgelsd's threshold (LAPACK dlalsd): an rcond <= 0 or >= 1 means the machine epsilon 2^-53;
+// singular values s_i <= rcond * s_1 are dropped. NaN stays NaN (the callers refuse it).
+static double lapack_rcond(double rcond) {
+  if (rcond != rcond) return rcond;
+  return (rcond <= 0.0 || rcond >= 1.0) ? 0x1p-53 : rcond;
+}
+
 int32_t lps_abc_operators(int32_t device, const lps_abc_rows *rows, const lps_abc_modes *modes,
                           double dt, double theta_coeff, double rcond, double *gu, double *gv,
                           double *gth, int32_t *rank) {
+  if ((rcond = lapack_rcond(rcond)) != rcond) return fail("rcond is NaN");
   int32_t rc = check_rows(rows, modes);
   if (rc != LPS_OK) return rc;
   if (!gu || !gv) return fail("gu and gv are required");
@@ -442,6 +450,7 @@ int32_t lps_lstsq_batched(int32_t device, int32_t m, int32_t n, int32_t nrhs, in
                           int32_t *rank) {
   if (m <= 0 || n <= 0 || nrhs <= 0 || batch < 0 || !A || !B || !X)
     return fail("m, n, nrhs > 0, batch >= 0 and A, B, X are required");
+  if ((rcond = lapack_rcond(rcond)) != rcond) return fail("rcond is NaN");
   Handle H;
   int32_t rc = open_handle(device, H);
   if (rc != LPS_OK) return rc;

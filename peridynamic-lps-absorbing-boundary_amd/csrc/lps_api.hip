@@ -151,6 +151,30 @@ void lps_destroy(lps_ctx *ctx) {
   delete ctx;
 }
 
+// FNV-1a (on 8-byte words) over the ABC inputs of a problem: the ABC node list, the clouds and the operators the
+// time loop reads (G_theta only for the G_theta method).
+static uint64_t abc_inputs_hash(const lps_problem *p, bool abc_on, int32_t theta_method) {
+  uint64_t h = 1469598103934665603ull;
+  auto mix = [&h](const void *ptr, size_t n) {  // 8-byte words, then the tail bytes
+    const unsigned char *b = (const unsigned char *)ptr;
+    size_t i = 0;
+    for (; i + 8 <= n; i += 8) {
+      uint64_t w;
+      memcpy(&w, b + i, 8);
+      h = (h ^ w) * 1099511628211ull;
+    }
+    for (; i < n; i++) h = (h ^ b[i]) * 1099511628211ull;
+  };
+  if (!abc_on) return h;
+  const int64_t C = p->constrains_no, M = p->max_cloud_size;
+  mix(p->inf_nodes, sizeof(int32_t) * C);
+  mix(p->iclouds, sizeof(int32_t) * C * M);
+  mix(p->ci_nodes_u, sizeof(double) * 2 * C * 4 * M);
+  mix(p->ci_nodes_v, sizeof(double) * 2 * C * 4 * M);
+  if (theta_method == LPS_THETA_GTHETA) mix(p->ci_nodes_theta, sizeof(double) * C * 2 * M);
+  return h;
+}
+
 static int32_t create_impl(lps_ctx *c, const lps_problem *p) {
   const int32_t N = p->node_no, C = p->constrains_no, M = p->max_cloud_size;
   if (N <= 0 || !p->nodes || !p->bctypes || !p->icvalues || !p->icdvalues)
@@ -168,6 +192,7 @@ static int32_t create_impl(lps_ctx *c, const lps_problem *p) {
     return c->fail(LPS_ERR_ARG, "prenotch_no must be >= 0, with prenotches set when positive");
   c->n = N;
   c->M = abc_on ? M : 0;
+  c->abc_hash.assign(1, abc_inputs_hash(p, abc_on, c->opt.theta_method));
   c->E = p->E; c->nu = p->nu; c->rho = p->rho; c->dt = p->dt;
   c->delta = p->delta; c->dx = p->dx; c->dy = p->dy;
   // Lame constants and their FATAL consistency check (src/main.cpp:215-217, 310-322)
@@ -903,7 +928,11 @@ static int32_t merge_batch(lps_ctx *c, const std::vector<lps_ctx *> &sub) {
   }
   // instances
   c->inst_h.clear();
-  for (int32_t i = 0; i < B; i++) c->inst_h.push_back(sub[i]->inst_h[0]);
+  c->abc_hash.clear();
+  for (int32_t i = 0; i < B; i++) {
+    c->inst_h.push_back(sub[i]->inst_h[0]);
+    c->abc_hash.push_back(sub[i]->abc_hash[0]);
+  }
   CTX_TRY(c, c->upload(&c->inst_d, c->inst_h));
   CTX_TRY(c, c->upload(&c->tile_off_d, c->tile_off));
   // state and output buffers

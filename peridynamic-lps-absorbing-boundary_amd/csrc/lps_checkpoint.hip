@@ -11,7 +11,8 @@
 //   int64     step
 //   int64     n_slots
 //   int32     n_inst, n_nodes, n_abc, 0
-//   uint64    fingerprint of the context (slot layout, kernels' path, per-instance constants)
+//   uint64    fingerprint of the context (slot layout, kernels' path, per-instance constants and
+//             ABC inputs)
 //   double    E0[n_inst]
 //   double    14 arrays of n_slots, in slot order: U^n x, y; U^{n-1} x, y; V^n x, y;
 //             V^{n-1} x, y; A^n x, y; P^n x, y; theta^n; theta^{n-1}
@@ -37,7 +38,8 @@ struct Fnv {
 };
 
 // What a checkpoint may only be restored into: the same slots (node order and layout), the same
-// arithmetic path and ABC method, and the same per-instance constants.
+// arithmetic path and ABC method, the same per-instance constants and the same ABC nodes, clouds
+// and operators (a context built with other G_u / G_v / G_theta continues differently).
 uint64_t fingerprint(const lps_ctx *c) {
   Fnv f;
   f.value(c->n_slots);
@@ -56,6 +58,7 @@ uint64_t fingerprint(const lps_ctx *c) {
     f.value(p.lam);
     f.value(p.mu);
   }
+  for (uint64_t h : c->abc_hash) f.value(h);
   return f.h;
 }
 

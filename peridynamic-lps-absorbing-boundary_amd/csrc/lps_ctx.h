@@ -55,6 +55,9 @@ struct lps_ctx {
   double *TH0 = nullptr;
   int64_t step = 0;
   std::vector<int32_t> perm_h;  // slot -> original index, -1 at holes
+  // FNV-1a hash of the ABC inputs (ABC node list, clouds, G_u, G_v, G_theta as given), one per
+  // instance: a checkpoint is only restored into a context with the same operators
+  std::vector<uint64_t> abc_hash;
   HorizonBuild hb;
   std::vector<void *> allocs;
   // device arrays (per slot unless noted)

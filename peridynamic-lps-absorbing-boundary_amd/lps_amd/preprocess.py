@@ -500,51 +500,75 @@ class Problem:
         return float(self.data2[6])
 
 
-def _pool_map(fn, items, workers):
+def _pool_map(fn, items, workers, shared=None):
+    """[fn(item) for item in items], on `workers` processes when workers > 1. fn reads the pool's
+    common data (modes, material, sample rows) from _W, which each worker receives once as
+    `shared`. The workers come from a 'forkserver': they start from a fresh interpreter and are
+    never a copy of this process, so the map is safe after this process has loaded the HIP runtime
+    (a forked copy of a process holding GPU state must not run)."""
     if workers <= 1 or len(items) <= 1:
-        return [fn(it) for it in items]
+        _worker_init(shared)
+        try:
+            return [fn(it) for it in items]
+        finally:
+            _restore_threads()
     import multiprocessing as mp
-    ctx = mp.get_context("fork")
-    with ctx.Pool(workers, initializer=_worker_init) as pool:
+    ctx = mp.get_context("forkserver")
+    ctx.set_forkserver_preload([__name__])
+    with ctx.Pool(workers, initializer=_worker_init, initargs=(shared,)) as pool:
         return pool.map(fn, items, chunksize=max(1, len(items) // (workers * 8)))
 
 
-def _worker_init():
+_THREAD_LIMIT = []
+
+
+def _worker_init(shared):
+    """Installs the common data and limits BLAS/LAPACK to one thread. The per-row solves are small
+    (up to 196 columns); LAPACK's thread pool makes them about 200 times slower, not faster. The
+    limit reaches only libraries already loaded, so scipy's LAPACK is loaded first."""
+    _W.clear()
+    if shared:
+        _W.update(shared)
+    _ROBUST_LS[0] = bool(_W.get("robust_ls", False))
     try:
+        import scipy.linalg.lapack  # noqa: F401 - load its BLAS before limiting
         from threadpoolctl import threadpool_limits
-        threadpool_limits(1)
+        _THREAD_LIMIT.append(threadpool_limits(1))
     except Exception:  # noqa: BLE001 - thread limiting is an optimisation only
         pass
 
 
-# Module-level state for fork-based workers (set right before a pool map).
+def _restore_threads():
+    while _THREAD_LIMIT:
+        _THREAD_LIMIT.pop().restore_original_limits()
+
+
+# The pool's common data, in every worker (and in this process for a serial map).
 _W = {}
 
 
-def _w_uv(i):
-    w = _W
-    a = w["abc"][i]
-    Gu, Gv = abc_node_uv(w["nodes"][a, 0], w["nodes"][a, 1], w["tau"][i],
-                         w["nodes"][w["clouds"][i]], w["modes"], w["M"], w["dt"])
-    return Gu, Gv
+def _row_items(nodes, abc_idx, tau, clouds, rows):
+    """Per-row inputs of the operator solves: (x, y, outward angle, cloud coordinates)."""
+    return [(float(nodes[abc_idx[i], 0]), float(nodes[abc_idx[i], 1]), float(tau[i]),
+             np.ascontiguousarray(nodes[clouds[i]])) for i in rows]
 
 
-def _w_theta(i):
-    w = _W
-    a = w["abc"][i]
-    return abc_node_theta(w["nodes"][a, 0], w["nodes"][a, 1], w["tau"][i],
-                          w["nodes"][w["clouds"][i]], w["modes"], w["mc"], w["M"], w["dt"])
+def _w_uv(item):
+    x, y, tau, cxy = item
+    return abc_node_uv(x, y, tau, cxy, _W["modes"], _W["M"], _W["dt"])
+
+
+def _w_theta(item):
+    x, y, tau, cxy = item
+    return abc_node_theta(x, y, tau, cxy, _W["modes"], _W["mc"], _W["M"], _W["dt"])
 
 
 def _w_sweep(args):
     idk, dt_try = args
-    w = _W
-    modes = w["sweep_modes"][idk]
+    modes = _W["sweep_modes"][idk]
     max_G = 0.0
-    for i in w["sample"]:
-        a = w["abc"][i]
-        Gu, Gv = abc_node_uv(w["nodes"][a, 0], w["nodes"][a, 1], w["tau"][i],
-                             w["nodes"][w["clouds"][i]], modes, w["M"], dt_try)
+    for x, y, tau, cxy in _W["sample"]:
+        Gu, Gv = abc_node_uv(x, y, tau, cxy, modes, _W["M"], dt_try)
         max_G = max(max_G, g_norm(Gu, Gv))
         if max_G > 1.5:
             break
@@ -611,23 +635,23 @@ def _make_problem(R, nu, pulses, steps, dt, dx, delta, workers, with_abc, dk_fac
         clouds = abc_clouds(nodes, abc_idx, delta)
         M = max(len(c) for c in clouds)
         dk0 = 1.0 / math.sqrt(mc.c_P)
-        _W.clear()
-        _W.update(nodes=nodes, abc=abc_idx, tau=tau, clouds=clouds, M=M, mc=mc)
+        robust = _ROBUST_LS[0]
         if dt is None:
             C = len(abc_idx)
             sample = [i for i in range(C) if len(clouds[i]) <= 26]
             sample += list(range(min(20, C)))
             sample += list(range(max(0, C - 20), C))
             sample = sorted(set(sample))
-            _W["sample"] = sample
-            _W["sweep_modes"] = [compute_all_modes(mc, tg, dk0 * f).sorted_by_kappa()
-                                 for f in DK_FACTORS]
+            sweep_modes = [compute_all_modes(mc, tg, dk0 * f).sorted_by_kappa()
+                           for f in DK_FACTORS]
             jobs = [(idk, dtv) for idk in range(len(DK_FACTORS)) for dtv in DT_VALUES]
             if ls == "gpu":
-                res = _sweep_gpu(nodes, abc_idx, tau, clouds, M, sample, _W["sweep_modes"],
+                res = _sweep_gpu(nodes, abc_idx, tau, clouds, M, sample, sweep_modes,
                                  mc.m_ref, device)
             else:
-                res = _pool_map(_w_sweep, jobs, workers)
+                res = _pool_map(_w_sweep, jobs, workers, shared=dict(
+                    M=M, sweep_modes=sweep_modes, robust_ls=robust,
+                    sample=_row_items(nodes, abc_idx, tau, clouds, sample)))
             best = None
             for (idk, dtv), mg in zip(jobs, res):
                 if mg <= 1.0 and (best is None or mg > best[2]):
@@ -646,7 +670,6 @@ def _make_problem(R, nu, pulses, steps, dt, dx, delta, workers, with_abc, dk_fac
         else:
             dk = dk0 * (1.0 if dk_factor is None else dk_factor)
         modes = compute_all_modes(mc, tg, dk).sorted_by_kappa()
-        _W.update(modes=modes, dt=dt)
         C = len(abc_idx)
         if ls == "gpu":
             from . import abc_gpu
@@ -655,8 +678,10 @@ def _make_problem(R, nu, pulses, steps, dt, dx, delta, workers, with_abc, dk_fac
             info["ls"] = "gpu"
             info["ls_unconverged"] = int(np.sum(rank < 0))
         else:
-            uv = _pool_map(_w_uv, list(range(C)), workers)
-            th = _pool_map(_w_theta, list(range(C)), workers)
+            items = _row_items(nodes, abc_idx, tau, clouds, range(C))
+            shared = dict(modes=modes, M=M, dt=dt, mc=mc, robust_ls=robust)
+            uv = _pool_map(_w_uv, items, workers, shared)
+            th = _pool_map(_w_theta, items, workers, shared)
             Gu = np.zeros((2 * C, 4 * M))
             Gv = np.zeros((2 * C, 4 * M))
             for i, (gu, gv) in enumerate(uv):

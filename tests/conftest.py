@@ -18,9 +18,24 @@ def pytest_configure(config):
 
 
 def load_golden(name):
-    """A golden case as (Problem, dict of reference outputs)."""
-    from lps_amd.preprocess import Problem
+    """A golden case as (Problem, dict of reference outputs). ref["theta_method"] is the ABC
+    dilatation the reference ran (1 G_theta, 0 IDW); a fixture with a `base` holds only outputs,
+    for the inputs of that other fixture."""
     z = np.load(os.path.join(GOLDEN, name + ".npz"))
+    if "base" in z:
+        prob, _ = load_golden(str(z["base"]))
+        prob.info = {"golden": name}
+        if "steps" in z:
+            prob.steps = int(z["steps"])
+        ref = {k: z[k] for k in ("energy", "verify", "vtk_points", "vtk_u", "vtk_v", "vtk_theta")}
+        ref["vtk_step"] = int(z["vtk_step"])
+        ref["theta_method"] = int(z["theta_method"])
+        return prob, ref
+    return _load_full(z, name)
+
+
+def _load_full(z, name):
+    from lps_amd.preprocess import Problem
     if "Gu32" in z:
         Gu, Gv, Gth = (z[k].astype(np.float64) for k in ("Gu32", "Gv32", "Gth32"))
     elif "Gu" in z:
@@ -36,6 +51,7 @@ def load_golden(name):
         prob.prenotch = z["prenotch"]
     ref = {k: z[k] for k in ("energy", "verify", "vtk_points", "vtk_u", "vtk_v", "vtk_theta")}
     ref["vtk_step"] = int(z["vtk_step"])
+    ref["theta_method"] = int(z["theta_method"]) if "theta_method" in z else 1
     return prob, ref
 
 

@@ -43,3 +43,15 @@ def test_cli_usage_and_missing_input_exit_codes(tmp_path):
     r = subprocess.run([exe, "_out", "nojob"], cwd=tmp_path, capture_output=True, text=True,
                        timeout=60)
     assert r.returncode == 1 and "FATAL: Cannot read nojob.data1" in r.stderr
+
+
+def test_lstsq_refuses_a_nan_rcond_before_touching_the_device():
+    """lps_lstsq_batched / lps_abc_operators refuse rcond = NaN at the ABI (LPS_ERR_ARG): the
+    check runs before any HIP call, so it holds without a GPU too."""
+    import numpy as np
+    from lps_amd import abc_gpu
+    A = np.eye(3)[None]
+    B = np.ones((1, 3, 1))
+    with pytest.raises(solver.LpsError) as e:
+        abc_gpu.lstsq(A, B, float("nan"))
+    assert e.value.code == solver.LPS_ERR_ARG and "rcond" in str(e.value)

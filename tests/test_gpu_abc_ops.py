@@ -26,6 +26,27 @@ def _orth(rng, n, k):
     return q
 
 
+@pytest.mark.parametrize("rcond", [-1.0, 0.0, 1.0, 2.0])
+def test_lstsq_nonpositive_rcond_means_machine_epsilon_as_in_dgelsd(rcond):
+    """dgelsd (LAPACK dlalsd) replaces an rcond <= 0 or >= 1 by the machine epsilon 2^-53 and drops
+    s_i <= eps s_1. Diagonal systems with singular values just above and just below that cut: the
+    GPU solver keeps the same rank and gives the same solution as scipy's dgelsd."""
+    from scipy.linalg import lapack
+    print("lstsq rcond", rcond, flush=True)
+    eps = 2.0 ** -53
+    cases = [[1.0, 1.02 * eps, 0.5], [1.0, 0.98 * eps, 0.5], [2.0, 0.3, 1e-3]]
+    A = np.stack([np.diag(c) for c in cases])
+    B = np.stack([np.array([[1.0], [1e-16], [2.0]]) for _ in cases])
+    X, rank = abc_gpu.lstsq(A, B, rcond)
+    for p, c in enumerate(cases):
+        lw = lapack.dgelsd_lwork(3, 3, 1, rcond)
+        x, _s, r, info = lapack.dgelsd(A[p], B[p].copy(), int(lw[0]), int(lw[1]), cond=rcond)
+        assert info == 0
+        assert rank[p] == r, (p, rank[p], r)
+        assert np.allclose(X[p][:, 0], x[:3, 0], rtol=1e-14, atol=0), (p, X[p], x)
+    assert list(rank) == [3, 2, 3]
+
+
 @pytest.mark.parametrize("m,n,low", [(30, 12, 0), (12, 30, 0), (224, 196, 0), (224, 196, 24)])
 def test_lstsq_known_minimum_norm_solutions(m, n, low):
     """A = U diag(s) V' with prescribed singular values, some below rcond s_1: the solution is

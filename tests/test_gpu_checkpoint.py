@@ -97,3 +97,30 @@ def test_refused_files_leave_the_context_unchanged(tmp_path, r40):
         got = s.fetch()
     for k in FIELDS:
         assert np.array_equal(got[k], want[k]), k
+
+
+def test_checkpoint_of_other_abc_operators_is_refused(tmp_path, r40):
+    """Two problems that differ only in their ABC operators (the same lattice, nu and dt, the
+    operators built with another delta_kappa, as the (dt, delta_kappa) sweep varies them): the
+    checkpoint of one is refused by the other, which stays as it was."""
+    print("refuse other operators", flush=True)
+    other_dk = pp.make_problem(40.0, 0.3, steps=600, dt=1.7e-8, dk_factor=0.9,
+                               pulses=[pp.Pulse(8.0, -5.0, 12.0, 1.0, (1.0, 1.0))], workers=8)
+    assert np.array_equal(other_dk.nodes, r40.nodes) and np.array_equal(other_dk.ic, r40.ic)
+    assert not np.array_equal(other_dk.Gu, r40.Gu)
+    path = str(tmp_path / "dk07.ckpt")
+    with solver.Solver(r40, variant=3) as s:
+        s.run(90)
+        s.checkpoint(path)
+    with solver.Solver(other_dk, variant=3) as ref:
+        ref.run(60)
+        want = ref.fetch()
+    with solver.Solver(other_dk, variant=3) as s:
+        s.run(30)
+        with pytest.raises(solver.LpsError, match="another problem"):
+            s.restore(path)
+        assert s.step_index == 30
+        s.run(30)
+        got = s.fetch()
+    for k in FIELDS:
+        assert np.array_equal(got[k], want[k]), k

@@ -66,9 +66,16 @@ def test_forward_matches_torch_fp32():
         c.set_params(p)
         y = c.forward(xb)
     yr, _ = torch_forward(p, cl.bf16_to_f32(xb))
-    err = rel_l2(y, yr.detach().numpy())
-    say("  rel L2", err)
-    assert err <= 2e-2, err
+    yr = yr.detach().numpy()
+    err = rel_l2(y, yr)
+    # per row: the reference applies the same BF16 roundings, so rows differ only where FP32
+    # accumulation order flips an activation's BF16 rounding; a wrong row or column would show as
+    # rows far off
+    row = np.linalg.norm(y - yr, axis=1) / np.maximum(np.linalg.norm(yr, axis=1), 1e-30)
+    say("  rel L2", err, "row max", row.max(), "rows > 1e-3", int((row > 1e-3).sum()))
+    assert err <= 3e-3, err
+    assert (row > 1e-3).sum() <= max(2, n // 50), np.sort(row)[-10:]
+    assert row.max() <= 2e-2, row.max()
 
 
 def test_gradients_match_torch_autograd():
@@ -95,7 +102,7 @@ def test_gradients_match_torch_autograd():
     for k in T:
         err = rel_l2(G[k], T[k].grad.numpy())
         say("  grad", k, err)
-        assert err <= 5e-2, (k, err)
+        assert err <= 1e-2, (k, err)
 
 
 def test_adam_first_step():

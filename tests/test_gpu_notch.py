@@ -8,8 +8,8 @@ import subprocess
 
 import numpy as np
 import pytest
-from conftest import (ENERGY_KEYS, PKG, VERIFY_KEYS, assert_series_close, load_golden, rel_l2,
-                      vtk_index)
+from conftest import (ENERGY_KEYS, PKG, VERIFY_KEYS, assert_series_close, load_golden,
+                      records_table, rel_l2, vtk_index)
 
 import pyoracle
 from lps_amd import files, solver
@@ -37,20 +37,28 @@ def test_notched_horizons_equal_oracle(exact):
 
 
 @pytest.mark.parametrize("exact", [True, False])
-def test_notched_idw_matches_oracle(exact):
+def test_notched_idw_matches_reference(exact):
+    """A notched job (nu = 0.33) with the IDW ABC dilatation against the reference solver built
+    with it (series and last fields within 1e-10), and on the exact path the oracle bit for bit."""
     print("notch idw", exact, flush=True)
-    prob, _ = load_golden(NAME)
-    o = pyoracle.Oracle(prob, theta_method=0)
-    o.run(150)
-    of = o.fetch()
+    prob, ref = load_golden("notch_r12_nu033_idw")
+    assert ref["theta_method"] == solver.LPS_THETA_IDW and prob.prenotch.shape == (2, 4)
     with solver.Solver(prob, theta_method=solver.LPS_THETA_IDW, exact=exact) as s:
-        s.run(150)
+        recs = s.run(prob.steps)
         gf = s.fetch()
-    for k in ("u", "v", "theta"):
-        if exact:
+    assert_series_close(records_table(recs, ENERGY_KEYS), ref["energy"], ENERGY_KEYS, 1e-10,
+                        prob, "notch idw")
+    assert_series_close(records_table(recs, VERIFY_KEYS), ref["verify"], VERIFY_KEYS, 1e-10,
+                        prob, "notch idw")
+    idx = vtk_index(prob.nodes, ref["vtk_points"])
+    assert rel_l2(gf["u"].reshape(-1, 2)[idx], ref["vtk_u"]) <= 1e-10
+    assert rel_l2(gf["theta"][idx], ref["vtk_theta"]) <= 1e-10
+    if exact:
+        o = pyoracle.Oracle(prob, theta_method=0)
+        o.run(prob.steps)
+        of = o.fetch()
+        for k in ("u", "v", "theta"):
             assert np.array_equal(gf[k], of[k]), k
-        else:
-            assert rel_l2(gf[k], of[k]) <= 1e-10, k
 
 
 def test_cli_notched_job_reproduces_the_reference(tmp_path):

@@ -11,7 +11,9 @@ from lps_amd import preprocess as pp, solver
 
 pytestmark = pytest.mark.gpu
 
-CASES = ["noabc_r30", "abc_r10_nu025", "abc_r12_nu033", "notch_r12_nu025"]
+CASES = ["noabc_r30", "abc_r10_nu025", "abc_r12_nu033", "notch_r12_nu025",
+         # the reference solver built with the IDW ABC dilatation (src/main.cpp:765-790)
+         "abc_r12_nu033_idw", "notch_r12_nu033_idw", "abc_r10_nu045_idw"]
 
 
 def say(*a):
@@ -47,10 +49,10 @@ def test_exact_mode_matches_oracle_bitwise(name):
     run equal the oracle's to the last bit."""
     say("exact vs oracle", name)
     prob, ref = load_golden(name)
-    o = pyoracle.Oracle(prob, theta_method=1)
+    o = pyoracle.Oracle(prob, theta_method=ref["theta_method"])
     orecs = o.run(prob.steps)
     of = o.fetch()
-    with solver.Solver(prob, exact=True) as s:
+    with solver.Solver(prob, exact=True, theta_method=ref["theta_method"]) as s:
         grecs = s.run(prob.steps)
         gf = s.fetch()
     for k in ("u", "v", "a", "theta", "m"):
@@ -69,7 +71,7 @@ def test_fast_mode_matches_reference_golden(name, graph, variant, concurrent):
     say("fast vs reference golden", name, graph, variant, concurrent)
     prob, ref = load_golden(name)
     with solver.Solver(prob, exact=False, use_graph=graph, variant=variant,
-                       concurrent=concurrent) as s:
+                       concurrent=concurrent, theta_method=ref["theta_method"]) as s:
         recs = s.run(prob.steps)
         f = s.fetch()
     assert_series_close(records_table(recs, ENERGY_KEYS), ref["energy"], ENERGY_KEYS, 1e-10, prob, name)
@@ -80,21 +82,24 @@ def test_fast_mode_matches_reference_golden(name, graph, variant, concurrent):
     assert rel_l2(f["theta"][idx], ref["vtk_theta"]) <= 1e-10
 
 
-@pytest.mark.parametrize("exact", [True, False])
-def test_idw_variant_matches_oracle(exact):
-    say("idw", exact)
-    prob, _ = load_golden("abc_r12_nu033")
-    o = pyoracle.Oracle(prob, theta_method=0)
-    o.run(150)
-    of = o.fetch()
-    with solver.Solver(prob, theta_method=solver.LPS_THETA_IDW, exact=exact) as s:
-        s.run(150)
-        gf = s.fetch()
-    for k in ("u", "v", "theta"):
-        if exact:
-            assert np.array_equal(gf[k], of[k]), k
-        else:
-            assert rel_l2(gf[k], of[k]) <= 1e-10, k
+@pytest.mark.parametrize("name", ["abc_r12_nu033_idw", "notch_r12_nu033_idw",
+                                  "abc_r10_nu045_idw"])
+def test_exact_idw_matches_reference_golden(name):
+    """The exact path with the IDW ABC dilatation against the reference solver built with it:
+    series and last fields within 1e-10 (the fast path is in test_fast_mode_matches_reference_golden,
+    the exact path against the oracle bit for bit in test_exact_mode_matches_oracle_bitwise)."""
+    say("exact idw vs reference golden", name)
+    prob, ref = load_golden(name)
+    assert ref["theta_method"] == solver.LPS_THETA_IDW
+    with solver.Solver(prob, exact=True, theta_method=solver.LPS_THETA_IDW) as s:
+        recs = s.run(prob.steps)
+        f = s.fetch()
+    assert_series_close(records_table(recs, ENERGY_KEYS), ref["energy"], ENERGY_KEYS, 1e-10, prob, name)
+    assert_series_close(records_table(recs, VERIFY_KEYS), ref["verify"], VERIFY_KEYS, 1e-10, prob, name)
+    idx = vtk_index(prob.nodes, ref["vtk_points"])
+    assert rel_l2(f["u"].reshape(-1, 2)[idx], ref["vtk_u"]) <= 1e-10
+    assert rel_l2(f["v"].reshape(-1, 2)[idx], ref["vtk_v"]) <= 1e-10
+    assert rel_l2(f["theta"][idx], ref["vtk_theta"]) <= 1e-10
 
 
 def test_nonfinite_state_is_reported_at_first_output_step():

@@ -7,13 +7,15 @@ from conftest import (ENERGY_KEYS, VERIFY_KEYS, assert_series_close, load_golden
 
 import pyoracle
 
-CASES = ["noabc_r30", "abc_r10_nu025", "abc_r12_nu033", "notch_r12_nu025"]
+CASES = ["noabc_r30", "abc_r10_nu025", "abc_r12_nu033", "notch_r12_nu025",
+         # the reference solver built with the IDW ABC dilatation (src/main.cpp:765-790)
+         "abc_r12_nu033_idw", "notch_r12_nu033_idw", "abc_r10_nu045_idw"]
 
 
 @pytest.fixture(scope="module", params=CASES)
 def oracle_run(request):
     prob, ref = load_golden(request.param)
-    o = pyoracle.Oracle(prob, theta_method=1, threads=4)
+    o = pyoracle.Oracle(prob, theta_method=ref["theta_method"], threads=4)
     recs = o.run(prob.steps)
     return request.param, prob, ref, o, recs
 
@@ -63,3 +65,18 @@ def test_prenotch_fixture_is_not_vacuous():
     assert whole.horizons()[0][-1] - cut.horizons()[0][-1] > 1000
     recs = whole.run(prob.steps)
     assert rel_l2(records_table(recs, ["energy"])[:, 0], ref["energy"][:, 1]) > 1e-2
+
+
+@pytest.mark.parametrize("name", ["abc_r12_nu033_idw", "notch_r12_nu033_idw", "abc_r10_nu045_idw"])
+def test_idw_fixtures_are_not_vacuous(name):
+    """The IDW runs differ from the G_theta runs of the same inputs by far more than the tolerance,
+    and an IDW with another weight (1/d^2 for 1/d) misses the reference's series: the fixtures pin
+    the IDW weights, not only which method ran."""
+    prob, ref = load_golden(name)
+    assert ref["theta_method"] == 0
+    gth = pyoracle.Oracle(prob, theta_method=1, threads=4).run(prob.steps)
+    e_ref = ref["energy"][:, 1]
+    assert rel_l2(records_table(gth, ["energy"])[:, 0], e_ref) > 1e-6
+    sq = pyoracle.Oracle(prob, theta_method=0, threads=4, plant=1)
+    e_sq = records_table(sq.run(prob.steps), ["energy"])[:, 0]
+    assert rel_l2(e_sq, e_ref) > 1e-9
