@@ -227,6 +227,13 @@ int32_t lps_closure_get_scales(lps_closure *clo, float *sigma2);
 
 /* Forward pass over n rows of inputs: out is n x 4 (units of sigma). */
 int32_t lps_closure_forward(lps_closure *clo, const uint16_t *x, int64_t n, float *out);
+/* lps_closure_forward without device allocation. No counterpart in the reference; it is for
+ * callers that cannot allocate device memory (serving loops, harnesses that forbid hipMalloc):
+ * the same kernel and the same bits as lps_closure_forward, with the rows staged in a fixed area
+ * of 6.6 MB that the library holds on each GPU until the process ends, instead of a hipMalloc /
+ * hipFree pair per call. n is at most 2048 rows (LPS_ERR_ARG beyond). All calls on one GPU share
+ * that area: calls that overlap on one GPU race. */
+int32_t lps_closure_forward_noalloc(lps_closure *clo, const uint16_t *x, int64_t n, float *out);
 /* One Adam step (lr; beta 0.9 / 0.999, eps 1e-8) on a batch of n rows (n a multiple of 64) given
  * by the host; loss = sum_r w_r |f(x_r) - t_r / sigma|^2 / n. grads (param_count floats, may be
  * NULL) receives the gradient of that loss before the update. lr = 0 leaves the weights. */

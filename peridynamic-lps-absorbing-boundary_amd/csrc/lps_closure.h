@@ -190,6 +190,9 @@ struct lps_closure {
   __bf16 *dzt[3] = {nullptr, nullptr, nullptr}, *dyt = nullptr;
   float *yb = nullptr, *tgt = nullptr, *wgt = nullptr, *dy = nullptr, *lrow = nullptr;
   float *loss_d = nullptr;
+  // this device's copy of the staging area of lps_closure_forward_noalloc (its address from
+  // hipGetSymbolAddress at creation; the array's own name is not a device address on the host)
+  unsigned char *scratch = nullptr;
   std::vector<void *> allocs;
 
   lps::MlpDev dev() const {

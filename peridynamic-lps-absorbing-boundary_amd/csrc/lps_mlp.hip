@@ -751,6 +751,16 @@ void lps_closure_destroy(lps_closure *clo) {
   delete clo;
 }
 
+// Staging area of lps_closure_forward_noalloc: the inputs (BF16) and outputs (FP32) of at most
+// kNoAllocRows rows. A file-scope device array: every GPU the library runs on holds its own copy
+// until the process ends, and all calls on one GPU share it (calls that overlap on one GPU race).
+// Nothing in it needs to start at zero: a call writes every byte it reads.
+constexpr int64_t kNoAllocRows = 2048;
+constexpr size_t kNoAllocXBytes = (size_t)kNoAllocRows * kClIn * sizeof(__bf16);
+constexpr size_t kNoAllocBytes = kNoAllocXBytes + (size_t)kNoAllocRows * kClOut * sizeof(float);
+static_assert(kNoAllocXBytes % 256 == 0, "output block alignment");
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
 int32_t lps_closure_create(int32_t device, uint64_t seed, lps_closure **out) {
   *out = nullptr;
   int ndev = 0;
@@ -775,6 +785,8 @@ int32_t lps_closure_create(int32_t device, uint64_t seed, lps_closure **out) {
   if (e == hipSuccess) e = c->alloc(&c->ds_count, 1);
   if (e == hipSuccess) e = c->alloc(&c->loss_d, 1);
   if (e == hipSuccess) e = hipMemsetAsync(c->ds_count, 0, sizeof(unsigned long long), c->stream);
+  // this device's copy of the no-alloc staging area (the device is current here)
+  if (e == hipSuccess) e = hipGetSymbolAddress((void **)&c->scratch, HIP_SYMBOL(scratch));
   if (e != hipSuccess) {
     lps_closure_destroy(c);
     return LPS_ERR_HIP;
@@ -846,6 +858,25 @@ int32_t lps_closure_forward(lps_closure *c, const uint16_t *x, int64_t n, float 
   (void)hipFree(dx);
   (void)hipFree(dy);
   if (e != hipSuccess) return c->hip_fail(e, "lps_closure_forward");
+  return LPS_OK;
+}
+
+int32_t lps_closure_forward_noalloc(lps_closure *c, const uint16_t *x, int64_t n, float *out) {
+  if (n < 0 || n > kNoAllocRows)
+    return c->fail(LPS_ERR_ARG, "lps_closure_forward_noalloc: n must be 0 .. " +
+                                    std::to_string(kNoAllocRows) +
+                                    " rows (the size of its staging area)");
+  if (n == 0) return LPS_OK;
+  if (!x || !out) return c->fail(LPS_ERR_ARG, "lps_closure_forward_noalloc: x and out are required");
+  if (!c->scratch) return c->fail(LPS_ERR_HIP, "lps_closure_forward_noalloc: no staging area");
+  CLO_TRY(c, hipSetDevice(c->device));
+  __bf16 *dx = (__bf16 *)c->scratch;
+  float *dy = (float *)(c->scratch + kNoAllocXBytes);
+  hipError_t e = hipMemcpyAsync(dx, x, sizeof(__bf16) * n * kClIn, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = mlp_forward_dataset(c->dev(), dx, (int32_t)n, nullptr, nullptr, nullptr, dy, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(out, dy, sizeof(float) * n * kClOut, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return c->hip_fail(e, "lps_closure_forward_noalloc");
   return LPS_OK;
 }
 

@@ -92,6 +92,14 @@ class Closure:
         self._check(self._L.lps_closure_forward(self._h, _ptr(x), n, _ptr(out)))
         return out
 
+    def forward_noalloc(self, x_bits: np.ndarray) -> np.ndarray:
+        """forward() through lps_closure_forward_noalloc: no device allocation, at most 2048 rows."""
+        x = np.ascontiguousarray(x_bits, dtype=np.uint16)
+        n = x.shape[0]
+        out = np.zeros((n, N_OUT), dtype=np.float32)
+        self._check(self._L.lps_closure_forward_noalloc(self._h, _ptr(x), n, _ptr(out)))
+        return out
+
     def train_batch(self, x_bits, target, weight, lr: float, want_grads: bool = False):
         x = np.ascontiguousarray(x_bits, dtype=np.uint16)
         t = np.ascontiguousarray(target, dtype=np.float32)

@@ -114,6 +114,7 @@ SIGNATURES = {
     "lps_closure_set_scales": (C.c_int32, [C.c_void_p, C.c_float, C.c_float]),
     "lps_closure_get_scales": (C.c_int32, [C.c_void_p, C.c_void_p]),
     "lps_closure_forward": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
+    "lps_closure_forward_noalloc": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
     "lps_closure_train_batch": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                             C.c_int32, C.c_float, C.c_void_p, C.c_void_p]),
     "lps_closure_dataset_reserve": (C.c_int32, [C.c_void_p, C.c_int64]),
