@@ -398,7 +398,8 @@ static double lapack_rcond(double rcond) {
 int32_t lps_abc_operators(int32_t device, const lps_abc_rows *rows, const lps_abc_modes *modes,
                           double dt, double theta_coeff, double rcond, double *gu, double *gv,
                           double *gth, int32_t *rank) {
-  if ((rcond = lapack_rcond(rcond)) != rcond) return fail("rcond is NaN");
+  rcond = lapack_rcond(rcond);
+  if (rcond != rcond) return fail("rcond is NaN");
   int32_t rc = check_rows(rows, modes);
   if (rc != LPS_OK) return rc;
   if (!gu || !gv) return fail("gu and gv are required");
@@ -450,7 +451,8 @@ int32_t lps_lstsq_batched(int32_t device, int32_t m, int32_t n, int32_t nrhs, in
                           int32_t *rank) {
   if (m <= 0 || n <= 0 || nrhs <= 0 || batch < 0 || !A || !B || !X)
     return fail("m, n, nrhs > 0, batch >= 0 and A, B, X are required");
-  if ((rcond = lapack_rcond(rcond)) != rcond) return fail("rcond is NaN");
+  rcond = lapack_rcond(rcond);
+  if (rcond != rcond) return fail("rcond is NaN");
   Handle H;
   int32_t rc = open_handle(device, H);
   if (rc != LPS_OK) return rc;

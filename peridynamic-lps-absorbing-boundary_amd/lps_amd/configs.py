@@ -109,6 +109,7 @@ class BatchWorkload:
     steps: int
     seed: int
     description: str
+    nu_grid: bool = False  # nu ~ U(0.15, 0.45) snapped to NU_GRID (else drawn from NU_SET)
 
 
 BATCH_WORKLOADS = {
@@ -118,6 +119,11 @@ BATCH_WORKLOADS = {
     "cfg3": BatchWorkload("cfg3", 64, 50.0, 250.0, 3000, 0,
                           "64 domains, R ~ U(50,250) mm, nu in {0.15..0.45}, 1-3 pulses, LSQ ABC "
                           "(G_theta), 3000 steps"),
+    # 256 instances, R ~ U(50, 500) mm snapped to dx, nu ~ U(0.15, 0.45) on the 0.025 grid (each
+    # with its sweep pair), pulses as in cfg3; 5000 steps. About 3e8 nodes and 3.5e6 ABC rows.
+    "cfg5": BatchWorkload("cfg5", 256, 50.0, 500.0, 5000, 5,
+                          "256 domains, R ~ U(50,500) mm, nu ~ U(0.15,0.45) on the 0.025 grid, "
+                          "1-3 pulses, LSQ ABC (G_theta), 5000 steps", nu_grid=True),
 }
 
 
@@ -131,7 +137,10 @@ def batch_specs(name: str, count: Optional[int] = None):
     specs = []
     for _ in range(count or w.count):
         R = round(rng.uniform(w.r_min, w.r_max) / dx) * dx
-        nu = NU_SET[int(rng.integers(len(NU_SET)))]
+        if w.nu_grid:
+            nu = min(NU_GRID, key=lambda g: abs(g - rng.uniform(0.15, 0.45)))
+        else:
+            nu = NU_SET[int(rng.integers(len(NU_SET)))]
         pulses = []
         for _ in range(int(rng.integers(1, 4))):
             r = rng.uniform(10.0, 40.0)
@@ -141,19 +150,24 @@ def batch_specs(name: str, count: Optional[int] = None):
             amp = rng.normal(0.0, 1e-3)
             pulses.append(pp.Pulse(rho * math.cos(phi), rho * math.sin(phi), r, amp,
                                    (math.cos(ang), math.sin(ang))))
-        dt, dk = NU_SWEEP[nu]
+        dt, dk = sweep_pair(nu)
         specs.append(dict(R=R, nu=nu, pulses=pulses, dt=dt, dk_factor=dk))
     return specs
 
 
 def make_batch(name: str, workers: Optional[int] = None, steps: Optional[int] = None,
-               count: Optional[int] = None, progress=None):
-    """The problems of a batched workload; progress(i, problem) is called after each one."""
+               count: Optional[int] = None, progress=None, only=None):
+    """The problems of a batched workload; progress(i, problem) is called after each one. The ABC
+    operators come from the symmetric builder (one octant solved, preprocess.lattice_symmetry).
+    only: indices of the instances to make (default all)."""
     w = BATCH_WORKLOADS[name]
     probs = []
     for i, sp in enumerate(batch_specs(name, count)):
+        if only is not None and i not in only:
+            continue
         p = pp.make_problem(sp["R"], sp["nu"], pulses=sp["pulses"], steps=steps or w.steps,
-                            dt=sp["dt"], dk_factor=sp["dk_factor"], workers=workers)
+                            dt=sp["dt"], dk_factor=sp["dk_factor"], workers=workers,
+                            symmetric=True)
         p.info["workload"] = name
         probs.append(p)
         if progress:

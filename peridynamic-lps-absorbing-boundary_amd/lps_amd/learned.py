@@ -97,7 +97,7 @@ def truncated_problem(nu: float, workers: Optional[int] = None) -> pp.Problem:
     if nu not in _TRUNC_CACHE:
         dt, dk = configs.sweep_pair(nu)
         _TRUNC_CACHE[nu] = pp.make_problem(R_TRUNC, nu, pulses=[], steps=1, dt=dt, dk_factor=dk,
-                                           workers=workers, robust_ls=True)
+                                           workers=workers, robust_ls=True, symmetric=True)
     return _TRUNC_CACHE[nu]
 
 

@@ -287,6 +287,27 @@ def abc_node_uv(ax: float, ay: float, tau: float, cloud_xy: np.ndarray, modes: M
     Returns two (2, 4*max_cloud) arrays; columns [4k..4k+3] weight (U_x, U_y, Unm1_x, Unm1_y) of
     cloud node k and row 0/1 give the x/y component."""
     nC = cloud_xy.shape[0]
+    ATu, tu, ATv, tvv = abc_node_uv_system(ax, ay, tau, cloud_xy, modes, dt)
+    xu = _dgelsd(ATu, tu)
+    xv = _dgelsd(ATv, tvv)
+    Gu = np.zeros((2, 4 * max_cloud))
+    Gv = np.zeros((2, 4 * max_cloud))
+    for tl in range(ABCS_STEP):
+        src = tl * 2 * nC + 2 * np.arange(nC)
+        dst = 4 * np.arange(nC) + 2 * tl
+        for r in range(2):
+            Gu[r, dst] = xu[src, r]
+            Gu[r, dst + 1] = xu[src + 1, r]
+            Gv[r, dst] = xv[src, r]
+            Gv[r, dst + 1] = xv[src + 1, r]
+    return Gu, Gv
+
+
+def abc_node_uv_system(ax: float, ay: float, tau: float, cloud_xy: np.ndarray, modes: Modes,
+                       dt: float):
+    """The least-squares systems of abc_node_uv: (ATu, tu) for G_u and (ATv, tv) for G_v, rows 4
+    per mode, columns (time level, cloud node, component) = tl * 2 nC + 2 k + c."""
+    nC = cloud_xy.shape[0]
     M = len(modes)
     cosT, sinT, xl, yl = _local_frame(cloud_xy, ax, ay, tau)
     ATu = np.zeros((4 * M, 2 * nC * ABCS_STEP))
@@ -330,19 +351,7 @@ def abc_node_uv(ax: float, ay: float, tau: float, cloud_xy: np.ndarray, modes: M
         tu[b::4, 1] = sinT * lx + cosT * ly
         tvv[b::4, 0] = cosT * vx - sinT * vy
         tvv[b::4, 1] = sinT * vx + cosT * vy
-    xu = _dgelsd(ATu, tu)
-    xv = _dgelsd(ATv, tvv)
-    Gu = np.zeros((2, 4 * max_cloud))
-    Gv = np.zeros((2, 4 * max_cloud))
-    for tl in range(ABCS_STEP):
-        src = tl * 2 * nC + 2 * np.arange(nC)
-        dst = 4 * np.arange(nC) + 2 * tl
-        for r in range(2):
-            Gu[r, dst] = xu[src, r]
-            Gu[r, dst + 1] = xu[src + 1, r]
-            Gv[r, dst] = xv[src, r]
-            Gv[r, dst + 1] = xv[src + 1, r]
-    return Gu, Gv
+    return ATu, tu, ATv, tvv
 
 
 def abc_node_theta(ax: float, ay: float, tau: float, cloud_xy: np.ndarray, modes: Modes,
@@ -411,8 +420,52 @@ def abc_nodes(nodes: np.ndarray, R: float, delta: float):
     return idx.astype(np.int64), tau
 
 
+def _lattice_clouds(nodes: np.ndarray, abc_idx: np.ndarray, radius: float):
+    """abc_clouds on a lattice of spacing h = the smallest nonzero |x| step: candidate offsets
+    (i h, j h) with the same distance test as the k-d tree path, looked up by integer
+    coordinates. None when the nodes are not lattice points of one spacing."""
+    if len(abc_idx) == 0 or nodes.shape[0] < 2:
+        return None
+    xs = np.unique(np.abs(nodes[:, 0]))
+    h = float(xs[xs > 0].min()) if np.any(xs > 0) else 0.0
+    if h <= 0.0:
+        return None
+    ix = np.rint(nodes[:, 0] / h).astype(np.int64)
+    iy = np.rint(nodes[:, 1] / h).astype(np.int64)
+    if not (np.array_equal(ix * h, nodes[:, 0]) and np.array_equal(iy * h, nodes[:, 1])):
+        return None
+    off = int(max(np.abs(ix).max(), np.abs(iy).max())) + 1
+    width = 2 * off + 1
+    keys = (ix + off) * width + (iy + off)
+    order = np.argsort(keys, kind="stable")
+    skeys = keys[order]
+    m = int(radius / h) + 1
+    oi, oj = np.meshgrid(np.arange(-m, m + 1), np.arange(-m, m + 1), indexing="ij")
+    oi, oj = oi.ravel(), oj.ravel()
+    a = nodes[abc_idx]
+    cx = ix[abc_idx][:, None] + oi[None, :]
+    cy = iy[abc_idx][:, None] + oj[None, :]
+    k = (cx + off) * width + (cy + off)
+    pos = np.clip(np.searchsorted(skeys, k), 0, len(skeys) - 1)
+    hit = (skeys[pos] == k) & (np.abs(cx) < off) & (np.abs(cy) < off)
+    cand = np.where(hit, order[pos], -1)
+    # the k-d tree path's test, on the candidates' own coordinates
+    d = np.sqrt((nodes[np.maximum(cand, 0), 0] - a[:, 0:1]) ** 2 +
+                (nodes[np.maximum(cand, 0), 1] - a[:, 1:2]) ** 2)
+    keep = hit & (d <= radius + 1e-10)
+    big = np.where(keep, cand, np.iinfo(np.int64).max)
+    big.sort(axis=1)
+    counts = keep.sum(axis=1)
+    return [big[r, :counts[r]] for r in range(len(abc_idx))]
+
+
 def abc_clouds(nodes: np.ndarray, abc_idx: np.ndarray, radius: float) -> List[np.ndarray]:
-    """Nodes within radius+1e-10 of each ABC node, ascending index (preprocessor/main.cpp:884-902)."""
+    """Nodes within radius+1e-10 of each ABC node, ascending index (preprocessor/main.cpp:884-902).
+    Lattice inputs (coordinates exact multiples of one spacing) take the lattice offsets within
+    the radius, all rows at once; other inputs a k-d tree per row."""
+    fast = _lattice_clouds(nodes, abc_idx, radius)
+    if fast is not None:
+        return fast
     from scipy.spatial import cKDTree
     tree = cKDTree(nodes)
     out = []
@@ -501,35 +554,68 @@ class Problem:
 
 
 def _pool_map(fn, items, workers, shared=None):
-    """[fn(item) for item in items], on `workers` processes when workers > 1. fn reads the pool's
-    common data (modes, material, sample rows) from _W, which each worker receives once as
-    `shared`. The workers come from a 'forkserver': they start from a fresh interpreter and are
+    """[fn(item) for item in items], on `workers` processes when workers > 1. fn reads the map's
+    common data (modes, material, sample rows) from _W, installed from `shared` before each chunk
+    of items. The workers come from a 'forkserver': they start from a fresh interpreter and are
     never a copy of this process, so the map is safe after this process has loaded the HIP runtime
-    (a forked copy of a process holding GPU state must not run)."""
+    (a forked copy of a process holding GPU state must not run). The pool is kept for later maps."""
     if workers <= 1 or len(items) <= 1:
-        _worker_init(shared)
+        _install(shared)
+        _limit_threads()
         try:
             return [fn(it) for it in items]
         finally:
             _restore_threads()
-    import multiprocessing as mp
-    ctx = mp.get_context("forkserver")
-    ctx.set_forkserver_preload([__name__])
-    with ctx.Pool(workers, initializer=_worker_init, initargs=(shared,)) as pool:
-        return pool.map(fn, items, chunksize=max(1, len(items) // (workers * 8)))
+    pool = _get_pool(workers)
+    cs = max(1, len(items) // (workers * 8))
+    chunks = [(fn, shared, items[i:i + cs]) for i in range(0, len(items), cs)]
+    return [r for part in pool.map(_run_chunk, chunks, chunksize=1) for r in part]
+
+
+_POOL = {}
+
+
+def _get_pool(workers):
+    if _POOL.get("n") != workers:
+        _close_pool()
+        import multiprocessing as mp
+        ctx = mp.get_context("forkserver")
+        ctx.set_forkserver_preload([__name__])
+        _POOL.update(n=workers, pool=ctx.Pool(workers, initializer=_limit_threads))
+    return _POOL["pool"]
+
+
+def _close_pool():
+    if _POOL.get("pool") is not None:
+        _POOL["pool"].terminate()
+        _POOL["pool"].join()
+    _POOL.clear()
+
+
+import atexit  # noqa: E402
+atexit.register(_close_pool)
+
+
+def _run_chunk(args):
+    fn, shared, items = args
+    _install(shared)
+    return [fn(it) for it in items]
+
+
+def _install(shared):
+    _W.clear()
+    if shared:
+        _W.update(shared)
+    _ROBUST_LS[0] = bool(_W.get("robust_ls", False))
 
 
 _THREAD_LIMIT = []
 
 
-def _worker_init(shared):
-    """Installs the common data and limits BLAS/LAPACK to one thread. The per-row solves are small
-    (up to 196 columns); LAPACK's thread pool makes them about 200 times slower, not faster. The
-    limit reaches only libraries already loaded, so scipy's LAPACK is loaded first."""
-    _W.clear()
-    if shared:
-        _W.update(shared)
-    _ROBUST_LS[0] = bool(_W.get("robust_ls", False))
+def _limit_threads():
+    """BLAS/LAPACK on one thread: the per-row solves are small (up to 196 columns) and LAPACK's
+    thread pool makes them about 200 times slower, not faster. The limit reaches only libraries
+    already loaded, so scipy's LAPACK is loaded first."""
     try:
         import scipy.linalg.lapack  # noqa: F401 - load its BLAS before limiting
         from threadpoolctl import threadpool_limits
@@ -575,6 +661,120 @@ def _w_sweep(args):
     return max_G
 
 
+def lattice_symmetry(nodes: np.ndarray, abc_idx: np.ndarray, clouds: Sequence[np.ndarray],
+                     dx: float, dy: float):
+    """The 8 symmetries of a square lattice about the origin (x -> -x, y -> -y, x <-> y) applied to
+    the ABC rows of an origin-centred domain: every row is the image T c of a canonical row c in the
+    octant 0 <= y <= x, under a signed permutation T. Returns (canon, T, pos): canon[a] the row
+    index of row a's canonical row (canon[c] == c for canonical rows), T[a] (2 x 2, canonical ->
+    row a), pos[a, k] the position in row a's cloud of the image of cloud node k of its canonical
+    row (-1 past the cloud). None when the nodes are not lattice points (x / dx not an exact
+    integer) or a row's cloud is not the image of its canonical row's cloud.
+
+    Why it holds: a row's least-squares systems (preprocessor/main.cpp:415-785) are built in the
+    frame of its outward normal tau = atan2(y, x), over a mode grid symmetric in the incidence
+    angle (theta on -dphi .. dphi). Rotating or mirroring the row and its cloud by T therefore maps
+    its systems onto the canonical row's, with the cloud's (x, y) components transformed by T. The
+    minimum-norm solution of the mapped system is the mapped solution: each 2 x 2 block of G_u
+    and G_v becomes T G T' and each G_theta weight stays."""
+    ix = np.rint(nodes[:, 0] / dx).astype(np.int64)
+    iy = np.rint(nodes[:, 1] / dy).astype(np.int64)
+    if not (np.array_equal(ix * dx, nodes[:, 0]) and np.array_equal(iy * dy, nodes[:, 1])):
+        return None
+    C = len(abc_idx)
+    if C == 0:
+        return None
+    off = int(max(np.abs(ix).max(), np.abs(iy).max())) + 1
+    width = 2 * off + 1
+    keys = (ix + off) * width + (iy + off)
+    order = np.argsort(keys, kind="stable")
+    skeys = keys[order]
+
+    def lookup(kx, ky):
+        k = (kx + off) * width + (ky + off)
+        p = np.clip(np.searchsorted(skeys, k), 0, len(skeys) - 1)
+        ok = (skeys[p] == k) & (np.abs(kx) < off) & (np.abs(ky) < off)
+        return np.where(ok, order[p], -1)
+
+    ia, ja = ix[abc_idx], iy[abc_idx]
+    c1 = np.maximum(np.abs(ia), np.abs(ja))
+    c2 = np.minimum(np.abs(ia), np.abs(ja))
+    row_of = -np.ones(nodes.shape[0], dtype=np.int64)
+    row_of[abc_idx] = np.arange(C)
+    cnode = lookup(c1, c2)
+    if np.any(cnode < 0):
+        return None
+    canon = row_of[cnode]
+    if np.any(canon < 0):
+        return None
+    s1 = np.where(ia < 0, -1, 1)
+    s2 = np.where(ja < 0, -1, 1)
+    swap = np.abs(ia) < np.abs(ja)
+    T = np.zeros((C, 2, 2), dtype=np.int64)
+    T[:, 0, 0] = np.where(swap, 0, s1)
+    T[:, 0, 1] = np.where(swap, s1, 0)
+    T[:, 1, 0] = np.where(swap, s2, 0)
+    T[:, 1, 1] = np.where(swap, 0, s2)
+    M = max(len(c) for c in clouds)
+    lens = np.array([len(c) for c in clouds])
+    CL = -np.ones((C, M), dtype=np.int64)
+    for a, c in enumerate(clouds):
+        CL[a, :len(c)] = c
+    if not np.array_equal(lens, lens[canon]):
+        return None
+    src = CL[canon]                                   # canonical cloud nodes, per row
+    valid = src >= 0
+    ox = np.where(valid, ix[np.maximum(src, 0)] - c1[:, None], 0)
+    oy = np.where(valid, iy[np.maximum(src, 0)] - c2[:, None], 0)
+    imx = ia[:, None] + T[:, 0, 0, None] * ox + T[:, 0, 1, None] * oy
+    imy = ja[:, None] + T[:, 1, 0, None] * ox + T[:, 1, 1, None] * oy
+    img = np.where(valid, lookup(imx, imy), -1)
+    if np.any(valid & (img < 0)):
+        return None
+    # position of each image node in row a's (ascending) cloud
+    big = nodes.shape[0] + 1
+    flat = np.where(CL >= 0, np.arange(C)[:, None] * big + CL, np.iinfo(np.int64).max)
+    fo = np.argsort(flat, axis=None, kind="stable")
+    fs = flat.ravel()[fo]
+    q = np.arange(C)[:, None] * big + np.maximum(img, 0)
+    p = np.clip(np.searchsorted(fs, q), 0, fs.size - 1)
+    found = fs[p] == q
+    if np.any(valid & ~found):
+        return None
+    pos = np.where(valid, fo[p] % M, -1)
+    return canon, T, pos
+
+
+def symmetric_operators(Gu_c: np.ndarray, Gv_c: np.ndarray, Gth_c: np.ndarray,
+                        canon_rows: np.ndarray, canon: np.ndarray, T: np.ndarray, pos: np.ndarray):
+    """G_u, G_v, G_theta of every row from those of the canonical rows (lattice_symmetry):
+    Gu_c, Gv_c (2 Cc x 4M), Gth_c (Cc x 2M) in the order of canon_rows (the canonical rows'
+    indices). Block (t, k) of row a is T[a] G_c(t, k) T[a]' at cloud position pos[a, k]."""
+    C, M = pos.shape
+    slot = -np.ones(C, dtype=np.int64)
+    slot[canon_rows] = np.arange(len(canon_rows))
+    ci = slot[canon]
+    Tf = T.astype(np.float64)
+    a_idx, k_idx = np.nonzero(pos >= 0)
+    dst = pos[a_idx, k_idx]
+
+    def blocks(G):
+        B = G.reshape(-1, 2, M, 2, 2)[ci]             # (C, out, k, t, in)
+        B = np.einsum("aij,ajktl,aml->aiktm", Tf, B, Tf)
+        out = np.zeros_like(B)
+        out[a_idx, :, dst] = B[a_idx, :, k_idx]
+        return out.reshape(2 * C, 4 * M)
+
+    Gu, Gv = blocks(Gu_c), blocks(Gv_c)
+    Gth = np.zeros((C, 2 * M))
+    if Gth_c is not None and Gth_c.size:
+        th = Gth_c.reshape(-1, M, 2)[ci]
+        o = np.zeros_like(th)
+        o[a_idx, dst] = th[a_idx, k_idx]
+        Gth = o.reshape(C, 2 * M)
+    return Gu, Gv, Gth
+
+
 def default_workers() -> int:
     n = os.cpu_count() or 1
     return max(1, min(n, 16))
@@ -585,7 +785,7 @@ def make_problem(R: float, nu: float = MAT_NU, pulses: Optional[Sequence[Pulse]]
                  dx: float = DX_DEFAULT, delta: Optional[float] = None,
                  workers: Optional[int] = None, with_abc: bool = True,
                  dk_factor: Optional[float] = None, robust_ls: bool = False,
-                 ls: str = "cpu", device: int = 0) -> Problem:
+                 ls: str = "cpu", device: int = 0, symmetric: bool = False) -> Problem:
     """Generate a full solver input set for a circular lattice domain of radius R.
 
     dt=None runs the reference's (dt, delta_kappa) sweep (preprocessor/main.cpp:1047-1174);
@@ -593,13 +793,16 @@ def make_problem(R: float, nu: float = MAT_NU, pulses: Optional[Sequence[Pulse]]
     e.g. a sweep result computed once. steps=None takes ceil(210 us / dt)
     (preprocessor/main.cpp:1177-1179). robust_ls=True re-solves a least-squares problem on which
     dgelsd does not converge with dgelss (the reference keeps dgelsd's output). ls="gpu" builds
-    the operators (and the sweep's) on the GPU (lps_amd.abc_gpu) instead of with LAPACK."""
+    the operators (and the sweep's) on the GPU (lps_amd.abc_gpu) instead of with LAPACK.
+    symmetric=True (CPU builder) solves only the rows of one octant and maps them onto the others
+    (lattice_symmetry), 8 times fewer solves; the operators are least-squares solutions of every
+    row's own systems, equal to direct solves up to the systems' conditioning."""
     if ls not in ("cpu", "gpu"):
         raise ValueError("ls must be 'cpu' or 'gpu'")
     _ROBUST_LS[0] = robust_ls
     try:
         return _make_problem(R, nu, pulses, steps, dt, dx, delta, workers, with_abc, dk_factor,
-                             ls, device)
+                             ls, device, symmetric)
     finally:
         _ROBUST_LS[0] = False
 
@@ -620,7 +823,7 @@ def _sweep_gpu(nodes, abc_idx, tau, clouds, M, sample, sweep_modes, m_ref, devic
 
 
 def _make_problem(R, nu, pulses, steps, dt, dx, delta, workers, with_abc, dk_factor, ls="cpu",
-                  device=0) -> Problem:
+                  device=0, symmetric=False) -> Problem:
     delta = 4.0 * dx if delta is None else delta
     workers = default_workers() if workers is None else workers
     mc = material(nu=nu, delta=delta, dx=dx, dy=dx)
@@ -678,16 +881,22 @@ def _make_problem(R, nu, pulses, steps, dt, dx, delta, workers, with_abc, dk_fac
             info["ls"] = "gpu"
             info["ls_unconverged"] = int(np.sum(rank < 0))
         else:
-            items = _row_items(nodes, abc_idx, tau, clouds, range(C))
+            sym = lattice_symmetry(nodes, abc_idx, clouds, dx, dx) if symmetric else None
+            rows = np.arange(C) if sym is None else np.nonzero(sym[0] == np.arange(C))[0]
+            items = _row_items(nodes, abc_idx, tau, clouds, rows)
             shared = dict(modes=modes, M=M, dt=dt, mc=mc, robust_ls=robust)
             uv = _pool_map(_w_uv, items, workers, shared)
             th = _pool_map(_w_theta, items, workers, shared)
-            Gu = np.zeros((2 * C, 4 * M))
-            Gv = np.zeros((2 * C, 4 * M))
+            n = len(rows)
+            Gu = np.zeros((2 * n, 4 * M))
+            Gv = np.zeros((2 * n, 4 * M))
             for i, (gu, gv) in enumerate(uv):
                 Gu[2 * i:2 * i + 2] = gu
                 Gv[2 * i:2 * i + 2] = gv
-            Gth = np.stack(th) if C else np.zeros((0, 2 * M))
+            Gth = np.stack(th) if n else np.zeros((0, 2 * M))
+            if sym is not None:
+                Gu, Gv, Gth = symmetric_operators(Gu, Gv, Gth, rows, *sym)
+                info["ls"] = f"cpu, symmetric ({n} of {C} rows solved)"
         cl = -np.ones((C, M), dtype=np.int64)
         for i, c in enumerate(clouds):
             cl[i, :len(c)] = c

@@ -73,9 +73,10 @@ def test_forward_matches_torch_fp32():
     # rows far off
     row = np.linalg.norm(y - yr, axis=1) / np.maximum(np.linalg.norm(yr, axis=1), 1e-30)
     say("  rel L2", err, "row max", row.max(), "rows > 1e-3", int((row > 1e-3).sum()))
-    assert err <= 3e-3, err
+    # measured on the MI355X: rel L2 2.1e-4, worst row 2.6e-3, one row above 1e-3 of 200
+    assert err <= 1e-3, err
     assert (row > 1e-3).sum() <= max(2, n // 50), np.sort(row)[-10:]
-    assert row.max() <= 2e-2, row.max()
+    assert row.max() <= 1e-2, row.max()
 
 
 def test_gradients_match_torch_autograd():
@@ -102,7 +103,7 @@ def test_gradients_match_torch_autograd():
     for k in T:
         err = rel_l2(G[k], T[k].grad.numpy())
         say("  grad", k, err)
-        assert err <= 1e-2, (k, err)
+        assert err <= 1e-2, (k, err)  # measured: at most 2.8e-3 (W1)
 
 
 def test_adam_first_step():
