@@ -170,7 +170,9 @@ double lps_m_ref(const lps_ctx *ctx);
  * the step (fast path: the partial tiles' force kernel, the wait for the ABC rows' G_u / G_v, which
  * run on a second stream beside the tile kernels, and the ABC prediction), out[3] whole step.
  * lps_set_timing(ctx, 1) records four events per step, lps_set_timing(ctx, 2) only the two around
- * the force pass's main kernel (out[1]; the others read 0), 0 switches timing off. Timed steps
+ * the force pass's main kernel (out[1]; the others read 0), lps_set_timing(ctx, 3) only two around
+ * the learned closure's network kernel on its stream (out[1]; a closure must be attached), 0
+ * switches timing off. Timed steps
  * replay 30-step graphs with the events as external event-record nodes; where the runtime cannot
  * time such events, they run as separate launches. */
 int32_t lps_set_timing(lps_ctx *ctx, int32_t on);
@@ -273,6 +275,19 @@ int32_t lps_closure_evaluate(lps_closure *clo, int64_t first, int64_t count, dou
 int32_t lps_closure_collect(lps_closure *clo, lps_ctx *truth, const lps_problem *trunc,
                             int32_t nsteps, int32_t first_step, int32_t sample_every, double keep,
                             double amp_min, double y_max, uint64_t seed);
+
+/* Closed-loop samples (the next round of a DAgger-style training). trunc: a single fast-path
+ * context with a closure attached (it corrects trunc's ABC rows as it steps); truth: a context on a
+ * larger domain of the same lattice, without ABC rows, at the same step and time step. Both advance
+ * nsteps steps together; after each step n + 1 selected as in lps_closure_collect, every kept row
+ * whose s_u >= amp_min appends one sample: as inputs the history the closure read at step n (the
+ * truncated run's own, with its reflections and the closure's past corrections), as target truth's
+ * U^{n+1}, V^{n+1} at the row's node minus trunc's G_u / G_v applied to trunc's U^n, U^{n-1}, V^n,
+ * V^{n-1}: the correction that would put the truncated run back on truth. Errors are reported on
+ * trunc. */
+int32_t lps_closure_collect_closed(lps_closure *clo, lps_ctx *truth, lps_ctx *trunc,
+                                   int32_t nsteps, int32_t first_step, int32_t sample_every,
+                                   double keep, double amp_min, double y_max, uint64_t seed);
 
 /* Applies clo at every ABC row of ctx (fast path only; a batch uses each instance's nu and
  * radius) from the next step on; the history starts empty (zeros) and is cleared by lps_reset.

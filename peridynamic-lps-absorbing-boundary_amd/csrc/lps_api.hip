@@ -1067,6 +1067,7 @@ static hipError_t step_once(lps_ctx *c, bool record_events) {
     ev = &c->ev[kEv * (30 * (size_t)c->ev_set + c->ev_used)];
     if (!c->timing_force_only && (e = record_timing(c, ev[0])) != hipSuccess) return e;
   }
+  c->clo_ev = record_events && c->timing_closure ? ev : nullptr;  // recorded by the closure hook
   // fast path: the ABC rows' G_theta and G_u / G_v read only the previous steps' state and run on
   // the second stream beside the tile passes; the force pass waits for G_theta (ev_join[0]), the
   // ABC prediction for G_u / G_v (ev_join[1])
@@ -1104,9 +1105,10 @@ static hipError_t step_once(lps_ctx *c, bool record_events) {
     if ((e = launch_force_part(a, st, c->stream3)) != hipSuccess) return e;
     if ((e = hipEventRecord(c->ev_part[2], c->stream3)) != hipSuccess) return e;
   }
-  if (record_events && (e = record_timing(c, ev[1])) != hipSuccess) return e;
+  const bool ev_force = record_events && !c->timing_closure;
+  if (ev_force && (e = record_timing(c, ev[1])) != hipSuccess) return e;
   if ((e = launch_force(a, st, c->fast, c->stream)) != hipSuccess) return e;
-  if (record_events && (e = record_timing(c, ev[2])) != hipSuccess) return e;
+  if (ev_force && (e = record_timing(c, ev[2])) != hipSuccess) return e;
   // the ABC nodes' prediction folded into the partial tiles' force pass (every ABC node lies in a
   // partial tile), which then waits for G_u / G_v; not with a closure attached, whose long ABC
   // branch the partial tiles would wait for
@@ -1173,16 +1175,20 @@ static int32_t flush_events(lps_ctx *c) {
 }
 
 int32_t lps_set_timing(lps_ctx *c, int32_t on) {
+  if (on < 0 || on > 3) return c->fail(LPS_ERR_ARG, "timing mode must be 0, 1, 2 or 3");
+  if (on == 3 && !c->clo) return c->fail(LPS_ERR_ARG, "timing mode 3 needs a closure attached");
   int32_t rc = flush_events(c);
   if (rc != LPS_OK) return rc;
-  const bool fo = on == 2;
-  if (fo != c->timing_force_only) {  // the timed graphs hold the other mode's event nodes
+  const bool fo = on == 2 || on == 3, tc = on == 3;
+  if (fo != c->timing_force_only || tc != c->timing_closure) {
+    // the timed graphs hold the other mode's event nodes
     for (hipGraphExec_t &g : c->graph30t) {
       if (g) (void)hipGraphExecDestroy(g);
       g = nullptr;
     }
   }
   c->timing_force_only = fo;
+  c->timing_closure = tc;
   c->timing = on != 0;
   for (double &t : c->t_pass) t = 0.0;
   c->t_count = 0;

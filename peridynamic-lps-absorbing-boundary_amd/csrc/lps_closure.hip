@@ -110,7 +110,14 @@ hipError_t closure_step_hook(lps_ctx *c, const StepState &st, hipStream_t s) {
   if (!A->clo || A->n_rows == 0) return hipSuccess;
   const ApplyDst d{A->row_slot, A->row_inst, A->gain, A->gate, st.ux1, st.uy1, st.vx1, st.vy1,
                    A->dbg};
-  return mlp_forward_apply(A->clo->dev(), A->src(par), A->n_rows, d, A->n_rows >= A->wide_min, s);
+  // timing mode 3: the step's two timing events around the network kernel, on its stream
+  const unsigned evf = c->capturing ? hipEventRecordExternal : 0;
+  hipError_t e = hipSuccess;
+  if (c->clo_ev) e = hipEventRecordWithFlags(c->clo_ev[1], s, evf);
+  if (e == hipSuccess)
+    e = mlp_forward_apply(A->clo->dev(), A->src(par), A->n_rows, d, A->n_rows >= A->wide_min, s);
+  if (e == hipSuccess && c->clo_ev) e = hipEventRecordWithFlags(c->clo_ev[2], s, evf);
+  return e;
 }
 
 hipError_t closure_reset_hook(lps_ctx *c) {
@@ -316,10 +323,12 @@ static inline int32_t lat(double x, double d) { return (int32_t)std::lround(x / 
 struct CollectArgs {
   GatherSrc g;
   int32_t n_rows, M;
-  const int32_t *row_slot, *cloud;  // truth slots; cloud n_rows x M (-1 skip)
+  const int32_t *cloud;             // slots of the state below; cloud n_rows x M (-1 skip)
   const double *gu, *gv;            // n_rows x 8M (trunc's G_u, G_v rows)
-  const double *ux1, *uy1, *vx1, *vy1;                  // U^{n+1}, V^{n+1}
   const double *ux, *uy, *uxm, *uym, *vx, *vy, *vxm, *vym;  // U^n, U^{n-1}, V^n, V^{n-1}
+  // the target: U^{n+1}, V^{n+1} of the reflection-free run at the row's node (slot tgt_slot[r])
+  const int32_t *tgt_slot;
+  const double *ux1, *uy1, *vx1, *vy1;
   uint64_t seed, keep_thr;
   int64_t step;  // n + 1
   float amp_min;
@@ -368,7 +377,7 @@ __global__ void __launch_bounds__(256) k_collect(CollectArgs a) {
   }
   // targets in the canonical frame, units of s_u / s_v (every lane holds the sums)
   const float iu = safe_inv(su), iv = safe_inv(sv);
-  const int32_t i = a.row_slot[r];
+  const int32_t i = a.tgt_slot[r];
   const double dux = a.ux1[i] - Ux, duy = a.uy1[i] - Uy;
   const double dvx = a.vx1[i] - Vx, dvy = a.vy1[i] - Vy;
   const int f = a.g.xf[r];
@@ -496,6 +505,10 @@ int32_t lps_closure_attach(lps_ctx *c, lps_closure *clo) {
 }
 
 int32_t lps_closure_detach(lps_ctx *c) {
+  if (c->timing_closure) {  // its events would no longer be recorded
+    int32_t rc = lps_set_timing(c, 0);
+    if (rc != LPS_OK) return rc;
+  }
   CTX_TRY(c, hipStreamSynchronize(c->stream));
   CTX_TRY(c, hipStreamSynchronize(c->stream2));
   closure_free(c);
@@ -646,7 +659,7 @@ int32_t lps_closure_collect(lps_closure *clo, lps_ctx *c, const lps_problem *p, 
   CollectArgs ca{};
   ca.n_rows = C;
   ca.M = M;
-  ca.row_slot = A->row_slot;
+  ca.tgt_slot = A->row_slot;
   ca.cloud = dcloud;
   ca.gu = dgu;
   ca.gv = dgv;
@@ -684,6 +697,107 @@ int32_t lps_closure_collect(lps_closure *clo, lps_ctx *c, const lps_problem *p, 
   c->opt.use_graph = saved_graph;
   ctx_drop_graphs(c);
   delete A;
+  return rc;
+}
+
+int32_t lps_closure_collect_closed(lps_closure *clo, lps_ctx *truth, lps_ctx *c, int32_t nsteps,
+                                   int32_t first_step, int32_t sample_every, double keep,
+                                   double amp_min, double y_max, uint64_t seed) {
+  if (!clo || !truth) return c->fail(LPS_ERR_ARG, "closure and truth context are required");
+  ClosureAttach *A = c->clo;
+  if (!A || !A->clo || c->batch || !c->fast)
+    return c->fail(LPS_ERR_ARG, "the truncated run needs a single fast-path context with a closure attached");
+  if (!truth->fast || truth->batch || truth->n_abc != 0)
+    return c->fail(LPS_ERR_ARG, "the truth run needs a single fast-path context without ABC rows");
+  if (clo->device != c->opt.device || truth->opt.device != c->opt.device)
+    return c->fail(LPS_ERR_ARG, "closure and contexts on different devices");
+  if (clo->ds_cap <= 0) return c->fail(LPS_ERR_ARG, "reserve the sample store first");
+  if (nsteps < 0 || sample_every <= 0 || !(y_max > 0.0)) return c->fail(LPS_ERR_ARG, "bad sampling arguments");
+  if (truth->step != c->step) return c->fail(LPS_ERR_ARG, "truth and truncated runs are at different steps");
+  if (truth->dx != c->dx || truth->dy != c->dy || truth->dt != c->dt)
+    return c->fail(LPS_ERR_ARG, "truth and truncated runs differ in lattice or time step");
+  if (c->M > 64) return c->fail(LPS_ERR_ARG, "clouds of more than 64 nodes");
+  CTX_TRY(c, hipSetDevice(c->opt.device));
+  // the truth slot of every ABC row's node
+  std::vector<double> xs, ys, txs, tys;
+  std::vector<int8_t> tkind;
+  std::vector<int32_t> aslot;
+  int32_t rc;
+  if ((rc = download(c, xs, c->x, c->n_slots)) || (rc = download(c, ys, c->y, c->n_slots)) ||
+      (rc = download(c, aslot, c->abc_slot, c->n_abc)))
+    return rc;
+  if ((rc = download(truth, txs, truth->x, truth->n_slots)) ||
+      (rc = download(truth, tys, truth->y, truth->n_slots)) ||
+      (rc = download(truth, tkind, truth->kind, truth->n_slots)))
+    return c->fail(rc, "truth context: " + truth->err);
+  std::vector<int32_t> ix, iy;
+  for (int32_t k = 0; k < truth->n_slots; k++)
+    if (tkind[k]) { ix.push_back(lat(txs[k], c->dx)); iy.push_back(lat(tys[k], c->dy)); }
+  Grid tg;
+  tg.init(ix, iy);
+  for (int32_t k = 0; k < truth->n_slots; k++)
+    if (tkind[k]) tg.set(lat(txs[k], c->dx), lat(tys[k], c->dy), k);
+  std::vector<int32_t> tslot(c->n_abc);
+  for (int32_t r = 0; r < c->n_abc; r++) {
+    const int32_t k = aslot[r];
+    tslot[r] = tg.get(lat(xs[k], c->dx), lat(ys[k], c->dy));
+    if (tslot[r] < 0) return c->fail(LPS_ERR_ARG, "an ABC node of the truncated run is not in the truth domain");
+  }
+  int32_t *dtslot = nullptr;
+  CTX_TRY(c, hipMalloc((void **)&dtslot, sizeof(int32_t) * std::max(c->n_abc, 1)));
+  hipError_t e = hipMemcpyAsync(dtslot, tslot.data(), sizeof(int32_t) * c->n_abc, hipMemcpyHostToDevice, c->stream);
+  CollectArgs ca{};
+  ca.n_rows = c->n_abc;
+  ca.M = c->M;
+  ca.cloud = c->cloud;
+  ca.gu = c->gu;
+  ca.gv = c->gv;
+  ca.tgt_slot = dtslot;
+  ca.seed = seed;
+  ca.keep_thr = keep >= 1.0 ? ~0ull : (uint64_t)(std::max(keep, 0.0) * 18446744073709551615.0);
+  ca.amp_min = (float)amp_min;
+  ca.y_max = y_max;
+  ca.dsx = clo->ds_x;
+  ca.dsaux = clo->ds_aux;
+  ca.count = clo->ds_count;
+  ca.cap = clo->ds_cap;
+  rc = e == hipSuccess ? LPS_OK : c->hip_fail(e, "lps_closure_collect_closed");
+  for (int32_t it = 0; it < nsteps && rc == LPS_OK; it++) {
+    const int64_t n = c->step;  // the step about to run in both
+    if ((rc = lps_step(truth, 1)) != LPS_OK) {
+      rc = c->fail(rc, "truth step: " + truth->err);
+      break;
+    }
+    if ((rc = lps_step(c, 1)) != LPS_OK) break;
+    const int64_t s1 = n + 1;
+    if (s1 < first_step || s1 % sample_every != 0) continue;
+    // truth's U^{s1} (interior nodes from its prediction ring) and V^{s1}, complete before this
+    // context's stream reads them
+    if ((rc = ctx_sync_u(truth, 1)) != LPS_OK) {
+      rc = c->fail(rc, "truth: " + truth->err);
+      break;
+    }
+    if ((e = hipStreamSynchronize(truth->stream)) != hipSuccess) { rc = c->hip_fail(e, "truth stream"); break; }
+    if ((rc = ctx_sync_u(c, 3)) != LPS_OK) break;
+    // the closure's inputs of step n: the history it gathered from
+    ca.g = A->src((int)(n & 1));
+    ca.step = s1;
+    const lps_ctx::Dev2 &U0 = c->U[n % 3], &Um = c->U[(n + 2) % 3];
+    const lps_ctx::Dev2 &V0 = c->V[n % 3], &Vm = c->V[(n + 2) % 3];
+    ca.ux = U0.x; ca.uy = U0.y; ca.uxm = Um.x; ca.uym = Um.y;
+    ca.vx = V0.x; ca.vy = V0.y; ca.vxm = Vm.x; ca.vym = Vm.y;
+    const lps_ctx::Dev2 &T1 = truth->U[s1 % 3], &TV1 = truth->V[s1 % 3];
+    ca.ux1 = T1.x; ca.uy1 = T1.y; ca.vx1 = TV1.x; ca.vy1 = TV1.y;
+    if ((e = hipStreamSynchronize(c->stream2)) != hipSuccess) { rc = c->hip_fail(e, "closure stream"); break; }
+    hipLaunchKernelGGL(k_collect, dim3((c->n_abc + 3) / 4), dim3(256), 0, c->stream, ca);
+    e = hipGetLastError();
+    // done before truth's next steps overwrite what it read
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) rc = c->hip_fail(e, "k_collect");
+  }
+  e = hipStreamSynchronize(c->stream);
+  if (rc == LPS_OK && e != hipSuccess) rc = c->hip_fail(e, "lps_closure_collect_closed");
+  (void)hipFree(dtslot);
   return rc;
 }
 

@@ -88,7 +88,9 @@ struct lps_ctx {
   // timing: 2 sets of 30 steps x 4 events. ev_set receives the next steps (ev_used so far); the
   // other set may hold a replay's events not yet read (ev_prev_used steps)
   bool timing = false;
-  bool timing_force_only = false;  // lps_set_timing(ctx, 2): only the force kernel's two events
+  bool timing_force_only = false;  // lps_set_timing(ctx, 2 or 3): only two events per step
+  bool timing_closure = false;     // lps_set_timing(ctx, 3): those two around the closure's network
+  hipEvent_t *clo_ev = nullptr;    // the current step's events while it is launched (mode 3)
   std::vector<hipEvent_t> ev;
   int32_t ev_set = 0, ev_used = 0, ev_prev_used = 0;
   double t_pass[4] = {0, 0, 0, 0};  // dilatation, force (nodes), force (ABC rows), whole step

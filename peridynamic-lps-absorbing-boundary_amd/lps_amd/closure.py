@@ -113,6 +113,11 @@ class Closure:
     # samples
     def reserve(self, capacity: int):
         self._check(self._L.lps_closure_dataset_reserve(self._h, int(capacity)))
+        self._cap = max(getattr(self, "_cap", 0), int(capacity))
+
+    def capacity(self) -> int:
+        """Samples the store holds (appends beyond it are counted but dropped)."""
+        return getattr(self, "_cap", 0)
 
     def count(self) -> int:
         return self._check(self._L.lps_closure_dataset_count(self._h))
@@ -143,6 +148,17 @@ class Closure:
         out = np.zeros(3, dtype=np.float64)
         self._check(self._L.lps_closure_evaluate(self._h, first, count, _ptr(out)))
         return float(out[0]), float(out[1]), float(out[2])
+
+    def collect_closed(self, truth, trunc, nsteps: int, first_step: int = 8, sample_every: int = 1,
+                       keep: float = 1.0, amp_min: float = 0.0, y_max: float = 1e30, seed: int = 0):
+        """Closed-loop samples: `trunc` (a Solver with this closure attached) and `truth` (a Solver
+        on a larger domain without ABC) advance nsteps together; samples append to the store."""
+        rc = self._L.lps_closure_collect_closed(self._h, truth._h, trunc._h, nsteps, first_step,
+                                                sample_every, float(keep), float(amp_min),
+                                                float(y_max), seed)
+        if rc < 0:  # the message sits on the truncated context
+            raise LpsError(rc, self._L.lps_last_error(trunc._h).decode())
+        return self.count()
 
     def collect(self, truth, trunc_prob, nsteps: int, first_step: int = 8, sample_every: int = 1,
                 keep: float = 1.0, amp_min: float = 0.0, y_max: float = 1e30, seed: int = 0):

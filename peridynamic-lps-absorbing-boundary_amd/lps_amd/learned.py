@@ -146,6 +146,90 @@ def collect(clo, runs: List[Run], keep: float, sample_every: int, amp_min_rel: f
     return stats
 
 
+def collect_closed(clo, runs: List[Run], keep: float, sample_every: int, amp_min_rel: float,
+                   y_max: float = 1.0, gain: float = 1.0, chunk: int = 60, blowup: float = 1.5,
+                   workers: Optional[int] = None, device: int = 0) -> dict:
+    """Closed-loop samples (DAgger): every run on the truncated domain with the closure attached
+    (gain, no guard) beside its reflection-free truth run; inputs are the closure's own history,
+    targets the correction that would put the truncated run back on truth
+    (lps_closure_collect_closed). A run stops contributing once its energy exceeds `blowup` times
+    its energy after the first chunk (a diverged run's samples would teach nothing)."""
+    from .solver import Solver
+    stats = {"runs": 0, "steps": 0, "stopped": 0, "s": 0.0}
+    by_nu: Dict[float, List[Run]] = {}
+    for r in runs:
+        by_nu.setdefault(r.nu, []).append(r)
+    for nu in sorted(by_nu):
+        dt, dk = configs.sweep_pair(nu)
+        truth_p = pp.make_problem(R_TRUTH, nu, pulses=[], steps=1, dt=dt, with_abc=False)
+        with Solver(truth_p, abc_enabled=False, device=device) as truth:
+            for r in by_nu[nu]:
+                t1 = time.time()
+                prob = run_problem(r, r.truth_steps, workers)
+                amp = float(np.max(np.abs(prob.ic)))
+                truth.set_initial(pp.gaussian_ic(truth_p.nodes, r.pulses))
+                with Solver(prob, device=device) as trunc:
+                    trunc.attach_closure(clo)
+                    trunc.closure_guard(0.0)
+                    trunc.set_closure_gain([gain])
+                    trunc.set_closure_gate([amp_min_rel * amp])
+                    e_first, done = None, 0
+                    while done < r.truth_steps:
+                        n = min(chunk, r.truth_steps - done)
+                        clo.collect_closed(truth, trunc, n, first_step=8, sample_every=sample_every,
+                                           keep=keep, amp_min=amp_min_rel * amp, y_max=y_max,
+                                           seed=r.seed)
+                        done += n
+                        e = trunc.diagnostics()["energy"]
+                        e_first = e if e_first is None else e_first
+                        if not (e <= blowup * e_first):
+                            stats["stopped"] += 1
+                            break
+                stats["runs"] += 1
+                stats["steps"] += done
+                stats["s"] += time.time() - t1
+                log(f"closed-loop nu {nu:.3f} run {stats['runs']}/{len(runs)}: {done} of "
+                    f"{r.truth_steps} steps, samples {clo.count()} ({time.time() - t1:.1f} s)")
+    return stats
+
+
+def dagger(clo, train_runs: List[Run], val_runs: List[Run], iters: int, runs_per_iter: int,
+           gains: List[float], keep: float, sample_every: int, amp_min_rel: float, y_max: float,
+           steps: int, batch: int, lr0: float, lr1: float, eval_steps: int, seed: int = 0,
+           workers: Optional[int] = None, out: Optional[str] = None) -> dict:
+    """DAgger-style rounds on a closure fitted on teacher-forced samples: each round adds
+    closed-loop samples from runs_per_iter training runs driven by the current network, fine-tunes
+    on every sample so far (the output scales stay those of the first fit), and evaluates the
+    closed loop on val_runs against the least-squares ABC. Keeps the round whose median reflected
+    energy is lowest (weights written to `out`)."""
+    rounds, best = [], None
+    for it in range(iters):
+        part = train_runs[(it * runs_per_iter) % len(train_runs):][:runs_per_iter]
+        g = gains[min(it, len(gains) - 1)]
+        st = collect_closed(clo, part, keep, sample_every, amp_min_rel, y_max=y_max, gain=g,
+                            workers=workers)
+        n = min(clo.count(), clo.capacity())
+        t0 = time.time()
+        curve = clo.train(steps, batch, lr0, lr1, 0, n, seed=seed + 17 * (it + 1))
+        ev = evaluate(clo, val_runs, eval_steps, workers=workers, guard=1e-3,
+                      amp_min_rel=amp_min_rel, gain=1.0)
+        sm = ev["summary"]
+        rec = {"round": it, "gain_collect": g, "collect": st, "samples": n,
+               "train_s": time.time() - t0, "loss_last": float(np.mean(curve[-50:])),
+               "eval": sm}
+        rounds.append(rec)
+        log(f"DAgger round {it}: samples {n}, closed loop median LSQ {sm['median_lsq']:.3e} "
+            f"learned {sm['median_learned']:.3e}, learned <= LSQ on {sm['learned_le_lsq']}/"
+            f"{sm['runs']}, fallbacks {sm['fallbacks']}")
+        if best is None or sm["median_learned"] < best[0]:
+            best = (sm["median_learned"], it, clo.get_params())
+            if out:
+                save(clo, out, {"dagger_round": it, "eval": sm, "samples": n})
+    if best is not None:
+        clo.set_params(best[2])
+    return {"rounds": rounds, "best_round": best[1] if best else None}
+
+
 def target_scales(clo, first: int, count: int, chunk: int = 65536, pieces: int = 16):
     """Weighted RMS of the u and v targets over a spread sample of [first, first + count)."""
     su = sv = sw = 0.0
@@ -304,6 +388,17 @@ def main(argv=None):
     ap.add_argument("--report", default=None)
     ap.add_argument("--load", default=None, help="skip training: load this parameter file")
     ap.add_argument("--workers", type=int, default=None)
+    ap.add_argument("--dagger", type=int, default=0, metavar="ROUNDS",
+                    help="closed-loop (DAgger) rounds after the teacher-forced fit (or --load)")
+    ap.add_argument("--dagger-runs", type=int, default=32, help="training runs per DAgger round")
+    ap.add_argument("--dagger-gains", type=float, nargs="+", default=[0.5, 1.0],
+                    help="closure gain while collecting, per round (the last repeats)")
+    ap.add_argument("--dagger-steps", type=int, default=5000, help="Adam steps per round")
+    ap.add_argument("--dagger-eval-runs", type=int, default=32)
+    ap.add_argument("--dagger-keep", type=float, default=0.05)
+    ap.add_argument("--tf-runs", type=int, default=0,
+                    help="with --load and --dagger: teacher-forced samples of this many runs "
+                         "first, so the rounds train on both kinds")
     ap.add_argument("--export-bin", default=None, metavar="BIN",
                     help="with --load: write the weight file of PD2_LPS_ABC --abc=learned=BIN and stop")
     a = ap.parse_args(argv)
@@ -316,6 +411,18 @@ def main(argv=None):
     with Closure(seed=a.seed) as clo:
         if a.load:
             report["loaded"] = load_into(clo, a.load)
+            if a.dagger > 0:
+                clo.reserve(a.capacity)
+                if a.tf_runs > 0:
+                    report["collect_tf"] = collect(clo, draw_runs(1000 + a.seed, a.tf_runs),
+                                                   a.keep, a.sample_every, a.amp_min,
+                                                   y_max=a.y_max, workers=a.workers)
+                report["dagger"] = dagger(
+                    clo, draw_runs(4000 + a.seed, a.dagger * a.dagger_runs),
+                    draw_runs(2000 + a.seed, a.dagger_eval_runs), a.dagger, a.dagger_runs,
+                    a.dagger_gains, a.dagger_keep, a.sample_every, a.amp_min, a.y_max,
+                    a.dagger_steps, a.batch, a.lr0 * 0.3, a.lr1, a.eval_steps, a.seed,
+                    a.workers, out=a.out)
             if a.eval_runs > 0:
                 run_eval(clo, a, report, "loaded")
         else:

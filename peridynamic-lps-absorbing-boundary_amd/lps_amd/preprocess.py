@@ -579,9 +579,19 @@ def _get_pool(workers):
     if _POOL.get("n") != workers:
         _close_pool()
         import multiprocessing as mp
+        import sys
+        import types
         ctx = mp.get_context("forkserver")
         ctx.set_forkserver_preload([__name__])
-        _POOL.update(n=workers, pool=ctx.Pool(workers, initializer=_limit_threads))
+        # Workers start with the caller's main module hidden: they run only this module's
+        # functions, and a new interpreter would otherwise re-run a caller's script that has no
+        # `if __name__ == "__main__"` guard.
+        main = sys.modules["__main__"]
+        sys.modules["__main__"] = types.ModuleType("__main__")
+        try:
+            _POOL.update(n=workers, pool=ctx.Pool(workers, initializer=_limit_threads))
+        finally:
+            sys.modules["__main__"] = main
     return _POOL["pool"]
 
 

@@ -129,6 +129,9 @@ SIGNATURES = {
     "lps_closure_collect": (C.c_int32, [C.c_void_p, C.c_void_p, C.POINTER(LpsProblem), C.c_int32,
                                         C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double,
                                         C.c_uint64]),
+    "lps_closure_collect_closed": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32,
+                                               C.c_int32, C.c_int32, C.c_double, C.c_double,
+                                               C.c_double, C.c_uint64]),
     "lps_closure_attach": (C.c_int32, [C.c_void_p, C.c_void_p]),
     "lps_closure_detach": (C.c_int32, [C.c_void_p]),
     "lps_closure_set_gain": (C.c_int32, [C.c_void_p, C.c_void_p]),
@@ -314,8 +317,10 @@ class Solver:
         return self._L.lps_m_ref(self._h)
 
     def set_timing(self, on):
-        """True / 1: the four pass events per step; 2: only the force kernel's two; False / 0: off."""
-        mode = 2 if (not isinstance(on, bool) and on == 2) else (1 if on else 0)
+        """True / 1: the four pass events per step; 2: only the force kernel's two; 3: only the two
+        around the learned closure's network kernel (a closure attached); False / 0: off. In modes
+        2 and 3 pass_times()["force_ms"] holds that kernel's time."""
+        mode = int(on) if (not isinstance(on, bool) and on in (2, 3)) else (1 if on else 0)
         self._check(self._L.lps_set_timing(self._h, mode))
 
     def pass_times(self):

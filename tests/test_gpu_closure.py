@@ -402,3 +402,46 @@ def test_wide_kernel_gives_the_same_bits(monkeypatch):
     assert np.array_equal(out[0][1], out[1][1])
     for k in ("u", "v", "a"):
         assert np.array_equal(out[0][0][k], out[1][0][k]), k
+
+
+def test_closed_loop_samples_target_the_way_back_to_truth():
+    """lps_closure_collect_closed at gain 0 (the truncated run is the least-squares run): after
+    120 steps beside a reflection-free truth run (R = 30 mm, same lattice, dt and pulse: the waves
+    have reached every ABC row), one
+    sampled step gives, for every row, a target whose norm is |truth U^{n+1} - trunc U^{n+1}| / s_u
+    at the row's node (frames are signed permutations), with s_u the closure's own scale of that
+    row, and inputs equal to what the closure gathered."""
+    say("closed-loop samples")
+    prob, _ = load_golden("abc_r12_nu033")
+    pulse = pp.Pulse(3.6, -2.4, 4.8, 1.0, (1.0, 1.0))
+    truth_p = pp.make_problem(30.0, 0.33, pulses=[pulse], steps=1, dt=prob.dt, with_abc=False)
+    assert np.array_equal(pp.gaussian_ic(prob.nodes, [pulse]), prob.ic)
+    tkey = {(round(x * 2), round(y * 2)): i for i, (x, y) in enumerate(truth_p.nodes)}
+    with cl.Closure(seed=21) as c, solver.Solver(truth_p, abc_enabled=False) as truth, \
+            solver.Solver(prob) as trunc:
+        c.set_params(random_params(22))
+        c.reserve(4 * prob.C)
+        trunc.attach_closure(c)
+        trunc.closure_guard(0.0)
+        trunc.set_closure_gain([0.0])
+        truth.step(120)
+        trunc.step(120)
+        n = c.collect_closed(truth, trunc, 1, first_step=0, sample_every=1, keep=1.0,
+                             amp_min=0.0, y_max=1e30, seed=3)
+        x_in, scale, _ = trunc.closure_inspect(prob.C)
+        ft, fq = truth.fetch(), trunc.fetch()
+        xs, aux = c.read(0, n)
+    assert 0.9 * prob.C <= n <= prob.C
+    rows = aux[:, 7].astype(np.int64)
+    assert len(set(rows.tolist())) == n
+    for k, r in enumerate(rows):
+        node = prob.abc[r]
+        x, y = prob.nodes[node]
+        tu = ft["u"].reshape(-1, 2)[tkey[(round(x * 2), round(y * 2))]]
+        qu = fq["u"].reshape(-1, 2)[node]
+        su = aux[k, 5]
+        assert su == scale[r, 0]
+        want = np.linalg.norm(tu - qu) / su
+        got = np.hypot(aux[k, 0], aux[k, 1])
+        assert abs(got - want) <= 1e-5 * max(want, 1e-30) + 1e-12, (r, got, want)
+        assert np.array_equal(xs[k], x_in[r])
