@@ -18,7 +18,7 @@ closure's overhead and the gate counts.
 
 roofline: the learned leg's dominant kernel, the closure's network (k_mlp_fwd_wide, BF16 MFMA), its
 average duration from HIP events on its own stream inside the timed window (timing mode 3), against
-its algorithmic FLOPs (1,084,416 per ABC row); the LSQ leg's force kernel (k_force_rb, HBM, 105 B
+its algorithmic FLOPs (1,083,392 per ABC row); the LSQ leg's force kernel (k_force_rb, HBM, 105 B
 per node) is in config. cpu_baseline: the reference solver compiled from its own sources
 (oracle/_ref, when present) on one run of the workload for a bounded number of steps, compute-only
 (output block off) with the I/O-inclusive figure beside it, and the oracle restatement on the same
@@ -199,6 +199,19 @@ def cpu_baseline_for(prob, steps, threads, note):
     return out
 
 
+def run_logged(ctx, steps, what, chunk=1000):
+    """ctx.run(steps) in chunks of `chunk` steps with a progress line after each (a long untimed
+    pass stays visibly alive); the records of every instance, concatenated."""
+    out, done = None, 0
+    while done < steps:
+        n = min(chunk, steps - done)
+        recs = ctx.run(n)
+        out = recs if out is None else [a + b for a, b in zip(out, recs)]
+        done += n
+        log(f"[bench]   {what}: {done}/{steps} steps")
+    return out
+
+
 def summary(refl):
     import numpy as np
     r = np.asarray(refl, dtype=float)
@@ -231,13 +244,14 @@ def bench_learned(args):
     n_rows = sum(p.C for p in probs)
     log(f"[bench] {len(probs)} instances, {n_nodes} nodes, {n_rows} ABC rows, generated in "
         f"{time.time() - t:.1f}s")
+    t = time.time()
     steps = args.steps if args.steps is not None else 60
     wpath = os.path.join(ROOT, "peridynamic-lps-absorbing-boundary_amd", "lps_amd", "data",
                          "closure_r125.npz")
     with cl.Closure() as clo, solver.Batch(probs, variant=args.variant,
                                            concurrent=not args.serial) as b:
         meta = L.load_into(clo, wpath)
-        log(f"[bench] GPU setup done, path: {b.path}")
+        log(f"[bench] GPU setup {time.time() - t:.1f}s, path: {b.path}")
         gates = [args.amp_min * float(np.max(np.abs(p.ic))) for p in probs]
 
         def learned_on():
@@ -272,13 +286,13 @@ def bench_learned(args):
         # untimed full-length passes: the gate
         b.reset()
         tg = time.perf_counter()
-        rl = b.run(eval_steps)
+        rl = run_logged(b, eval_steps, "learned, full run")
         t_full_clo = time.perf_counter() - tg
         fb = b.closure_guard()
         b.detach_closure()
         b.reset()
         tg = time.perf_counter()
-        rq = b.run(eval_steps)
+        rq = run_logged(b, eval_steps, "LSQ, full run")
         t_full_lsq = time.perf_counter() - tg
         path = b.path
     refl_l = [1.0 + r[-1]["de_rel"] for r in rl]
@@ -370,7 +384,13 @@ def bench_single(args):
     steps = args.steps if args.steps is not None else 60
     workers = args.workers or min(16, os.cpu_count() or 1)
     t = time.time()
-    prob = configs.make(w.name, workers=workers)
+    if args.cache and os.path.exists(args.cache):
+        prob = configs.load(args.cache)
+        log(f"[bench] loaded {w.name} from {args.cache}")
+    else:
+        prob = configs.make(w.name, workers=workers)
+        if args.cache:
+            configs.save(prob, args.cache)
     log(f"[bench] N={prob.N} C={prob.C} dt={prob.dt:.3e} ready in {time.time() - t:.1f}s")
     with solver.Solver(prob, exact=args.exact, use_graph=True, variant=args.variant,
                        concurrent=not args.serial) as s:
@@ -434,6 +454,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=90)
     ap.add_argument("--workers", type=int, default=None)
+    ap.add_argument("--cache", default=None, metavar="NPZ",
+                    help="cfg1/cfg2: load the generated problem from NPZ, or generate it and save "
+                         "it there (profiled runs generate outside the profiler)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="run only a plain copy kernel over BYTES (HBM counter calibration)")
     args = ap.parse_args()

@@ -884,15 +884,19 @@ def _make_problem(R, nu, pulses, steps, dt, dx, delta, workers, with_abc, dk_fac
             dk = dk0 * (1.0 if dk_factor is None else dk_factor)
         modes = compute_all_modes(mc, tg, dk).sorted_by_kappa()
         C = len(abc_idx)
+        sym = lattice_symmetry(nodes, abc_idx, clouds, dx, dx) if symmetric else None
+        rows = np.arange(C) if sym is None else np.nonzero(sym[0] == np.arange(C))[0]
         if ls == "gpu":
             from . import abc_gpu
-            Gu, Gv, Gth, rank = abc_gpu.operators(nodes, abc_idx, tau, clouds, M, modes, dt,
+            Gu, Gv, Gth, rank = abc_gpu.operators(nodes, abc_idx[rows], tau[rows],
+                                                  [clouds[i] for i in rows], M, modes, dt,
                                                   mc.m_ref, device=device)
             info["ls"] = "gpu"
             info["ls_unconverged"] = int(np.sum(rank < 0))
+            if sym is not None:
+                Gu, Gv, Gth = symmetric_operators(Gu, Gv, Gth, rows, *sym)
+                info["ls"] = f"gpu, symmetric ({len(rows)} of {C} rows solved)"
         else:
-            sym = lattice_symmetry(nodes, abc_idx, clouds, dx, dx) if symmetric else None
-            rows = np.arange(C) if sym is None else np.nonzero(sym[0] == np.arange(C))[0]
             items = _row_items(nodes, abc_idx, tau, clouds, rows)
             shared = dict(modes=modes, M=M, dt=dt, mc=mc, robust_ls=robust)
             uv = _pool_map(_w_uv, items, workers, shared)
