@@ -171,14 +171,17 @@ def cpu_baseline_for(prob, steps, threads, note):
     if all(os.path.exists(b) for b in bins.values()):
         tmp = tempfile.mkdtemp(prefix="lps_cpu_")
         try:
+            log(f"[bench] CPU baseline: writing the job files ({N} nodes)")
             files.write_job(sample, os.path.join(tmp, "job"))
             for k, b in bins.items():
+                log(f"[bench] CPU baseline: reference solver ({k}), {steps} steps")
                 r = subprocess.run([b, "_out", "job"], cwd=tmp, env=env, capture_output=True,
                                    text=True, timeout=600)
                 t[k] = parse_marching(r.stdout) if r.returncode == 0 else None
                 log(f"[bench] reference ({k}) rc {r.returncode}: {t[k]} s for {steps} steps")
         finally:
             subprocess.run(["rm", "-rf", tmp])
+    log(f"[bench] CPU baseline: oracle, {steps} steps")
     o = pyoracle.Oracle(sample, threads=threads)
     tc = time.perf_counter()
     o.run(steps)

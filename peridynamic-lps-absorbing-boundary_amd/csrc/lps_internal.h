@@ -23,6 +23,14 @@ namespace lps {
 constexpr int kSlice = 64;        // SELL slice height = wavefront size
 constexpr int kBlock = 256;       // threads per block of the node passes (4 wavefronts)
 constexpr int kWavesPerBlock = kBlock / 64;
+
+// Blocks are spread over the 8 XCDs round-robin (block b runs on XCD b % 8). Work item
+// (b % 8) * per_xcd + b / 8 of n goes to block b, so each XCD takes a contiguous range and
+// neighbouring items share its L2; launch ceil(n / 8) * 8 blocks (items past n: the block exits).
+__device__ __forceinline__ int32_t xcd_block(int32_t b, int32_t n) {
+  const int32_t per_xcd = (n + 7) / 8;
+  return (b % 8) * per_xcd + b / 8;
+}
 constexpr int kMaxHorizon = 128;  // MAX_HORIZON_LENGTH, src/main.cpp:35
 constexpr int kTile = 16;         // tile edge (lattice points)
 constexpr int kMaxHalo = 4;       // largest stencil reach of the tile path (lattice cells)

@@ -450,12 +450,10 @@ __global__ void __launch_bounds__(kBlock) k_idw(StepArgs a, StepState st) {
 // ---------------------------------------------------------------------------------------------
 // Tile passes (fast path)
 // ---------------------------------------------------------------------------------------------
-// Blocks are spread over the 8 XCDs round-robin (block b runs on XCD b % 8). Tile t is given to
-// block b with t = (b % 8) * per_xcd + b / 8, so each XCD sweeps a contiguous band of tile rows and
-// a tile's halo neighbours sit in the same XCD's L2.
+// Tile t of block b (xcd_block): each XCD sweeps a contiguous band of tile rows, so a tile's halo
+// neighbours sit in the same XCD's L2.
 __device__ __forceinline__ int32_t tile_of_block(int32_t b, int32_t n_tiles) {
-  const int32_t per_xcd = (n_tiles + 7) / 8;
-  return (b % 8) * per_xcd + b / 8;
+  return xcd_block(b, n_tiles);
 }
 
 // Slot of lattice point (gx, gy), or -1 outside the tiled area / in an absent tile.
@@ -1034,10 +1032,14 @@ __global__ void __launch_bounds__(64, 3) k_force_rb(StepArgs a, StepState st) {
 
 // Fast path, ABC rows' dilatation (G_theta, or the direct value for IDW), one wavefront per row.
 // Reads only the theta history (or the prediction), so it may run beside the tile kernels.
+// The ABC rows' kernels give each XCD a contiguous range of rows (tile_of_block's mapping over
+// row blocks): rows are in node order, so neighbouring rows share cloud nodes and their state, and
+// the shared values are then read through one L2. Launched on abc_blocks_xcd(a) blocks.
 __global__ void __launch_bounds__(kBlock) k_abc_theta(StepArgs a, StepState st) {
   __shared__ AbcScratch sabc[kWavesPerBlock];
   const int32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  const int32_t row = blockIdx.x * kWavesPerBlock + wid;
+  const int32_t nrb = (a.n_abc + kWavesPerBlock - 1) / kWavesPerBlock;
+  const int32_t row = tile_of_block(blockIdx.x, nrb) * kWavesPerBlock + wid;
   if (row < a.n_abc) abc_theta_row<false>(a, st, row, lane, &sabc[wid]);
 }
 
@@ -1047,7 +1049,8 @@ __global__ void __launch_bounds__(kBlock) k_abc_theta(StepArgs a, StepState st) 
 // member comes from the prediction ring r, of an ABC member from U.
 __global__ void __launch_bounds__(kBlock) k_abc_uv(StepArgs a, StepState st, RingU r) {
   const int32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  const int32_t row = blockIdx.x * kWavesPerBlock + wid;
+  const int32_t nrb = (a.n_abc + kWavesPerBlock - 1) / kWavesPerBlock;
+  const int32_t row = tile_of_block(blockIdx.x, nrb) * kWavesPerBlock + wid;
   if (row >= a.n_abc) return;
   const double *gu0 = a.gu + (int64_t)row * 8 * a.M, *gu1 = gu0 + 4 * a.M;
   const double *gv0 = a.gv + (int64_t)row * 8 * a.M, *gv1 = gv0 + 4 * a.M;
@@ -1436,6 +1439,8 @@ static inline int32_t blocks_for(int64_t n) { return (int32_t)((n + kBlock - 1) 
 static inline int32_t abc_blocks(const StepArgs &a) {
   return (a.n_abc + kWavesPerBlock - 1) / kWavesPerBlock;
 }
+// Blocks of the XCD-contiguous row kernels: 8 equal ranges, the tail blocks exit.
+static inline int32_t abc_blocks_xcd(const StepArgs &a) { return (abc_blocks(a) + 7) / 8 * 8; }
 static inline int32_t tile_blocks(const StepArgs &a) { return (a.tile.n_tiles + 7) / 8 * 8; }
 
 hipError_t launch_predict(const StepArgs &a, const double *ux, const double *uy, const double *vx,
@@ -1484,7 +1489,7 @@ hipError_t launch_theta(const StepArgs &a, const StepState &st, bool fast, hipSt
 
 hipError_t launch_abc_theta(const StepArgs &a, const StepState &st, hipStream_t s) {
   if (a.n_abc == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_abc_theta, dim3(abc_blocks(a)), dim3(kBlock), 0, s, a, st);
+  hipLaunchKernelGGL(k_abc_theta, dim3(abc_blocks_xcd(a)), dim3(kBlock), 0, s, a, st);
   return hipGetLastError();
 }
 
@@ -1529,7 +1534,7 @@ hipError_t launch_force_part(const StepArgs &a, const StepState &st, hipStream_t
 
 hipError_t launch_abc_uv(const StepArgs &a, const StepState &st, const RingU &r, hipStream_t s) {
   if (a.n_abc == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_abc_uv, dim3(abc_blocks(a)), dim3(kBlock), 0, s, a, st, r);
+  hipLaunchKernelGGL(k_abc_uv, dim3(abc_blocks_xcd(a)), dim3(kBlock), 0, s, a, st, r);
   return hipGetLastError();
 }
 

@@ -14,6 +14,7 @@
 
 #include "../../include/lps_amd.h"
 #include "lps_closure.h"
+#include "lps_internal.h"
 
 namespace lps {
 
@@ -216,7 +217,9 @@ __global__ void __launch_bounds__(256) k_mlp_fwd_wide(MlpDev m, const __bf16 *x,
   static_assert(sizeof(int32_t) * kWideRows * kClOffsets <= sizeof(__bf16) * kWideRows * kHsPitch,
                 "history indices fit in the activation buffer");
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-  const int r0 = blockIdx.x * kWideRows;
+  // XCD-contiguous row ranges: neighbouring rows share history records through one L2
+  const int r0 = xcd_block(blockIdx.x, (n + kWideRows - 1) / kWideRows) * kWideRows;
+  if (r0 >= n) return;  // the whole block: before any barrier
   const int ar = lane & 31, ak = 8 * (lane >> 5);
   const int lrow = 32 * wv + ar;
   const int32_t row = r0 + lrow;
@@ -392,7 +395,7 @@ hipError_t mlp_forward_apply(const MlpDev &m, const GatherSrc &g, int32_t n, con
                              bool wide, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   if (wide) {
-    hipLaunchKernelGGL(k_mlp_fwd_wide<true>, dim3((n + kWideRows - 1) / kWideRows), dim3(256), 0,
+    hipLaunchKernelGGL(k_mlp_fwd_wide<true>, dim3((n + 8 * kWideRows - 1) / (8 * kWideRows) * 8), dim3(256), 0,
                        s, m, (const __bf16 *)nullptr, g, n, (float *)nullptr, d);
     return hipGetLastError();
   }
