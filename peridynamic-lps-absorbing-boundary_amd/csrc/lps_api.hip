@@ -9,6 +9,7 @@
 // replays from any step that is a multiple of 30.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -35,7 +36,8 @@ StepArgs lps_ctx::args() const {
   a.row_ptr = hb.row_ptr; a.col = hb.col;
   a.sell_off = hb.sell_off; a.sell_w = hb.sell_w; a.sell_col = hb.sell_col;
   a.m = m; a.c2m = c2m; a.is_bc = is_bc;
-  a.cloud = cloud; a.gu = gu; a.gv = gv; a.gth = gth;
+  a.abc_off = abc_off; a.cloud = cloud; a.gu = gu; a.gv = gv; a.gth = gth;
+  a.abc_gofs = abc_gofs; a.abc_xf = abc_xf; a.abc_pmap = abc_pmap;
   a.theta_method = opt.theta_method;
   a.inst = inst_d;
   a.tile = tile;
@@ -94,6 +96,7 @@ extern "C" {
 static int32_t reset_state(lps_ctx *c);
 static int32_t flush_events(lps_ctx *c);
 static int32_t choose_variant(lps_ctx *c);
+static int32_t choose_abc_predict(lps_ctx *c, bool tune);
 static void drop_graphs(lps_ctx *c);
 
 void lps_default_options(lps_options *opt) {
@@ -147,6 +150,9 @@ void lps_destroy(lps_ctx *ctx) {
   for (hipEvent_t ev : ctx->ev_part)
     if (ev) (void)hipEventDestroy(ev);
   if (ctx->stream3) (void)hipStreamDestroy(ctx->stream3);
+  if (ctx->stream4) (void)hipStreamSynchronize(ctx->stream4);
+  if (ctx->ev_clo) (void)hipEventDestroy(ctx->ev_clo);
+  if (ctx->stream4) (void)hipStreamDestroy(ctx->stream4);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -175,11 +181,109 @@ static uint64_t abc_inputs_hash(const lps_problem *p, bool abc_on, int32_t theta
   return h;
 }
 
+// Lattice images among the ABC rows. On a square lattice about the origin each row (lattice
+// position (i, j)) has a canonical row, the one at (max(|i|, |j|), min(|i|, |j|)) in the octant
+// 0 <= y <= x, and a signed permutation T with (i, j) = T (c1, c2). The row is an image when its
+// cloud is T applied to the canonical cloud (same member count) and every 2 x 2 block of its G_u
+// and G_v is T B T' of the canonical member's block B, and its G_theta weights are the canonical
+// ones: exactly, bit for bit. Then it needs no operators of its own (the kernels map the canonical
+// block, lattice_map). True when every row is an image; canon[r] the canonical row, xf[r] the
+// transform (bit 0 swap, bit 1 cross terms negated), pmap[off[r] + jj] the canonical member of
+// member jj. Operators solved row by row (the reference's preprocessor) are images only up to
+// rounding, so they keep a block per row; operators built once per canonical row and mapped by
+// the same transforms are exact images.
+static bool abc_lattice_images(const lps_problem *p, const std::vector<int32_t> &rows, int32_t M,
+                               const std::vector<int32_t> &off, bool gth_on,
+                               std::vector<int32_t> &canon, std::vector<int8_t> &xf,
+                               std::vector<uint8_t> &pmap) {
+  const int32_t R = (int32_t)rows.size();
+  if (R == 0 || M > 256) return false;
+  auto lat = [p](int32_t k, long long &i, long long &j) {
+    const double x = p->nodes[2 * (int64_t)k], y = p->nodes[2 * (int64_t)k + 1];
+    i = std::llround(x / p->dx);
+    j = std::llround(y / p->dy);
+    return (double)i * p->dx == x && (double)j * p->dy == y && std::llabs(i) < (1 << 30) &&
+           std::llabs(j) < (1 << 30);
+  };
+  auto key = [](long long i, long long j) { return (int64_t)((uint64_t)(uint32_t)i << 32 | (uint32_t)j); };
+  std::vector<std::pair<int64_t, int32_t>> at(R);  // lattice key -> row
+  std::vector<long long> ri(R), rj(R);
+  for (int32_t r = 0; r < R; r++) {
+    if (!lat(p->inf_nodes[rows[r]], ri[r], rj[r])) return false;
+    at[r] = {key(ri[r], rj[r]), r};
+  }
+  std::sort(at.begin(), at.end());
+  auto row_at = [&at](int64_t k) {
+    auto it = std::lower_bound(at.begin(), at.end(), std::make_pair(k, (int32_t)INT32_MIN));
+    return it != at.end() && it->first == k ? it->second : -1;
+  };
+  auto map_block = [](int t, const double *b0, const double *b1, double *o0, double *o1) {
+    if (t & 1) {
+      const double x0[4] = {b1[1], b1[0], b1[3], b1[2]}, x1[4] = {b0[1], b0[0], b0[3], b0[2]};
+      for (int q = 0; q < 4; q++) { o0[q] = x0[q]; o1[q] = x1[q]; }
+    } else {
+      for (int q = 0; q < 4; q++) { o0[q] = b0[q]; o1[q] = b1[q]; }
+    }
+    if (t & 2) { o0[1] = -o0[1]; o0[3] = -o0[3]; o1[0] = -o1[0]; o1[2] = -o1[2]; }
+  };
+  canon.assign(R, -1);
+  std::vector<std::pair<int64_t, int32_t>> mk;  // member keys of a row -> member
+  std::vector<char> hit;
+  for (int32_t r = 0; r < R; r++) {
+    const long long ia = ri[r], ja = rj[r];
+    const long long c1 = std::max(std::llabs(ia), std::llabs(ja)), c2 = std::min(std::llabs(ia), std::llabs(ja));
+    const int32_t cr = row_at(key(c1, c2));
+    if (cr < 0) return false;
+    const int s1 = ia < 0 ? -1 : 1, s2 = ja < 0 ? -1 : 1;
+    const bool sw = std::llabs(ia) < std::llabs(ja);
+    const int t = (sw ? 1 : 0) | (s1 * s2 < 0 ? 2 : 0);
+    const int32_t L = off[r + 1] - off[r];
+    if (off[cr + 1] - off[cr] != L) return false;
+    const int64_t a = rows[r], ac = rows[cr];
+    mk.clear();
+    for (int32_t jj = 0; jj < L; jj++) {
+      const int32_t k = p->iclouds[a * M + jj];
+      long long x, y;
+      if (k < 0 || k >= p->node_no || !lat(k, x, y)) return false;
+      mk.push_back({key(x, y), jj});
+    }
+    std::sort(mk.begin(), mk.end());
+    hit.assign(L, 0);
+    for (int32_t m = 0; m < L; m++) {
+      const int32_t kc = p->iclouds[ac * M + m];
+      long long x, y;
+      if (kc < 0 || kc >= p->node_no || !lat(kc, x, y)) return false;
+      const long long ox = x - c1, oy = y - c2;
+      const long long qx = ia + (sw ? s1 * oy : s1 * ox), qy = ja + (sw ? s2 * ox : s2 * oy);
+      auto it = std::lower_bound(mk.begin(), mk.end(), std::make_pair(key(qx, qy), (int32_t)INT32_MIN));
+      if (it == mk.end() || it->first != key(qx, qy) || hit[it->second]) return false;
+      const int32_t jj = it->second;
+      hit[jj] = 1;
+      pmap[(size_t)off[r] + jj] = (uint8_t)m;
+      // the values, exactly
+      for (const double *G : {p->ci_nodes_u, p->ci_nodes_v}) {
+        double o0[4], o1[4];
+        map_block(t, G + ac * 8 * M + 4 * m, G + ac * 8 * M + 4 * M + 4 * m, o0, o1);
+        const double *g0 = G + a * 8 * M + 4 * jj, *g1 = G + a * 8 * M + 4 * M + 4 * jj;
+        for (int q = 0; q < 4; q++)
+          if (!(o0[q] == g0[q]) || !(o1[q] == g1[q])) return false;
+      }
+      if (gth_on && (!(p->ci_nodes_theta[ac * 2 * M + 2 * m] == p->ci_nodes_theta[a * 2 * M + 2 * jj]) ||
+                     !(p->ci_nodes_theta[ac * 2 * M + 2 * m + 1] == p->ci_nodes_theta[a * 2 * M + 2 * jj + 1])))
+        return false;
+    }
+    canon[r] = cr;
+    xf[r] = (int8_t)t;
+  }
+  return true;
+}
+
 static int32_t create_impl(lps_ctx *c, const lps_problem *p) {
   const int32_t N = p->node_no, C = p->constrains_no, M = p->max_cloud_size;
   if (N <= 0 || !p->nodes || !p->bctypes || !p->icvalues || !p->icdvalues)
     return c->fail(LPS_ERR_ARG, "node_no must be positive and nodes/bctypes/icvalues/icdvalues set");
   if (C < 0 || (C > 0 && M <= 0)) return c->fail(LPS_ERR_ARG, "inconsistent constrains_no / max_cloud_size");
+  if (C > 0 && M > 256) return c->fail(LPS_ERR_ARG, "max_cloud_size above 256");
   const bool abc_on = c->opt.abc_enabled && C > 0;
   if (abc_on && (!p->inf_nodes || !p->iclouds || !p->ci_nodes_u || !p->ci_nodes_v))
     return c->fail(LPS_ERR_ARG, "ABC rows need inf_nodes, iclouds, ci_nodes_u and ci_nodes_v");
@@ -505,22 +609,86 @@ static int32_t create_impl(lps_ctx *c, const lps_problem *p) {
   }
 
   // ---- ABC rows on the device, active rows only, cloud ids as slots ----
+  // Ragged rows. On the tile path a row ends after its last member that adds anything: trailing
+  // members that are pads (-1) or have all-zero G_u, G_v (and G_theta) coefficients are dropped.
+  // Their terms are zeros at the end of the row, so the wavefront sums keep their bits. The exact
+  // path keeps every member, for the reference's order of additions.
   if (c->n_abc > 0) {
     const int32_t R = c->n_abc;
-    std::vector<int32_t> hcl((size_t)R * M);
-    std::vector<double> hgu((size_t)R * 8 * M), hgv((size_t)R * 8 * M), hgt;
-    if (c->opt.theta_method == LPS_THETA_GTHETA) hgt.resize((size_t)R * 2 * M);
+    const bool gth_on = c->opt.theta_method == LPS_THETA_GTHETA;
+    std::vector<int32_t> off(R + 1, 0);
     for (int32_t r = 0; r < R; r++) {
       const int64_t a = active_rows[r];
-      for (int32_t j = 0; j < M; j++) {
-        const int32_t k = p->iclouds[a * M + j];
-        hcl[(size_t)r * M + j] = (k < 0) ? -1 : slot_of[k];
+      int32_t L = (int32_t)M;
+      if (c->fast) {
+        auto adds = [&](int32_t j) {
+          if (p->iclouds[a * M + j] < 0) return false;
+          for (int b = 0; b < 2; b++)
+            for (int q = 0; q < 4; q++)
+              if (p->ci_nodes_u[a * 8 * M + b * 4 * M + 4 * j + q] != 0.0 ||
+                  p->ci_nodes_v[a * 8 * M + b * 4 * M + 4 * j + q] != 0.0)
+                return true;
+          return gth_on && (p->ci_nodes_theta[a * 2 * M + 2 * j] != 0.0 ||
+                            p->ci_nodes_theta[a * 2 * M + 2 * j + 1] != 0.0);
+        };
+        while (L > 0 && !adds(L - 1)) L--;
       }
-      std::memcpy(&hgu[(size_t)r * 8 * M], p->ci_nodes_u + a * 8 * M, sizeof(double) * 8 * M);
-      std::memcpy(&hgv[(size_t)r * 8 * M], p->ci_nodes_v + a * 8 * M, sizeof(double) * 8 * M);
-      if (!hgt.empty())
-        std::memcpy(&hgt[(size_t)r * 2 * M], p->ci_nodes_theta + a * 2 * M, sizeof(double) * 2 * M);
+      if ((int64_t)off[r] + L > INT32_MAX) return c->fail(LPS_ERR_ARG, "too many ABC cloud members");
+      off[r + 1] = off[r] + L;
     }
+    const size_t NM = (size_t)off[R];
+    c->abc_members = (int64_t)NM;
+    // operator blocks: shared by lattice images (tile path), else one per row
+    std::vector<int32_t> canon;
+    std::vector<int8_t> xf(R, 0);
+    std::vector<uint8_t> pmap(std::max<size_t>(NM, 1), 0);
+    const char *sym_env = std::getenv("LPS_ABC_SYMMETRY");
+    const bool try_sym = c->fast && !(sym_env && !strcmp(sym_env, "off"));
+    bool shared = try_sym && abc_lattice_images(p, active_rows, M, off, gth_on, canon, xf, pmap);
+    if (!shared) {
+      canon.resize(R);
+      for (int32_t r = 0; r < R; r++) {
+        canon[r] = r;
+        xf[r] = 0;
+        for (int32_t j = 0; j < off[r + 1] - off[r]; j++) pmap[(size_t)off[r] + j] = (uint8_t)j;
+      }
+    }
+    std::vector<int32_t> gofs(R), bofs(R, -1);
+    int64_t GM = 0;  // members of the operator blocks
+    for (int32_t r = 0; r < R; r++)
+      if (canon[r] == r) {
+        bofs[r] = (int32_t)GM;
+        GM += off[r + 1] - off[r];
+      }
+    for (int32_t r = 0; r < R; r++) gofs[r] = bofs[canon[r]];
+    c->abc_block_members = GM;
+    c->abc_shared_blocks = shared;
+    std::vector<int32_t> hcl(std::max<size_t>(NM, 1), -1);
+    std::vector<double> hgu(std::max<int64_t>(8 * GM, 1)), hgv(std::max<int64_t>(8 * GM, 1)), hgt;
+    if (gth_on) hgt.resize(std::max<int64_t>(2 * GM, 1));
+    for (int32_t r = 0; r < R; r++) {
+      const int64_t a = active_rows[r];
+      const size_t o = (size_t)off[r];
+      const int32_t L = off[r + 1] - off[r];
+      for (int32_t j = 0; j < L; j++) {
+        const int32_t k = p->iclouds[a * M + j];
+        hcl[o + j] = (k < 0) ? -1 : slot_of[k];
+      }
+      if (canon[r] != r) continue;
+      const size_t g = (size_t)bofs[r];
+      for (int b = 0; b < 2; b++) {  // the x block, then the y block, 4 L doubles each
+        std::memcpy(&hgu[8 * g + 4 * b * L], p->ci_nodes_u + a * 8 * M + 4 * b * M, sizeof(double) * 4 * L);
+        std::memcpy(&hgv[8 * g + 4 * b * L], p->ci_nodes_v + a * 8 * M + 4 * b * M, sizeof(double) * 4 * L);
+      }
+      if (gth_on) std::memcpy(&hgt[2 * g], p->ci_nodes_theta + a * 2 * M, sizeof(double) * 2 * L);
+    }
+    CTX_TRY(c, c->upload(&c->abc_gofs, gofs));
+    CTX_TRY(c, c->upload(&c->abc_xf, xf));
+    CTX_TRY(c, c->upload(&c->abc_pmap, pmap));
+    if (shared)
+      c->path += "; ABC operators of " + std::to_string(GM) + " of " + std::to_string(NM) +
+                 " members stored, the rest lattice images";
+    CTX_TRY(c, c->upload(&c->abc_off, off));
     CTX_TRY(c, c->upload(&c->cloud, hcl));
     CTX_TRY(c, c->upload(&c->gu, hgu));
     CTX_TRY(c, c->upload(&c->gv, hgv));
@@ -691,6 +859,8 @@ static int32_t new_context(const lps_options *options, lps_ctx **out) {
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream3, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream4, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_clo, hipEventDisableTiming);
   for (int q = 0; q < 3; q++)
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_part[q], hipEventDisableTiming);
   for (int q = 0; q < 2; q++) {
@@ -881,49 +1051,73 @@ static int32_t merge_batch(lps_ctx *c, const std::vector<lps_ctx *> &sub) {
       return rc;
     T.delta = d_delta; T.hx = d_hx; T.hy = d_hy; T.g1 = d_g1; T.g2 = d_g2; T.xi = d_xi;
   }
-  // ABC rows: slots shifted, clouds and operators padded to the widest cloud
+  // ABC rows: slots shifted; the instances' ragged rows and operator blocks one after the other
   int32_t R = 0;
-  for (int32_t i = 0; i < B; i++) R += sub[i]->n_abc;
+  int64_t NM = 0, GM = 0;
+  bool shared = true;
+  for (int32_t i = 0; i < B; i++) {
+    R += sub[i]->n_abc;
+    NM += sub[i]->abc_members;
+    GM += sub[i]->abc_block_members;
+    shared = shared && (sub[i]->n_abc == 0 || sub[i]->abc_shared_blocks);
+  }
   c->n_abc = R;
+  c->abc_members = NM;
+  c->abc_block_members = GM;
+  c->abc_shared_blocks = shared && R > 0;
+  if (NM > INT32_MAX) return c->fail(LPS_ERR_ARG, "too many ABC cloud members in the batch");
   if (R > 0) {
-    std::vector<int32_t> aslot, cl((size_t)R * M, -1);
-    CTX_TRY(c, c->alloc(&c->gu, (size_t)R * 8 * M));
-    CTX_TRY(c, c->alloc(&c->gv, (size_t)R * 8 * M));
-    CTX_TRY(c, hipMemsetAsync(c->gu, 0, sizeof(double) * R * 8 * M, c->stream));
-    CTX_TRY(c, hipMemsetAsync(c->gv, 0, sizeof(double) * R * 8 * M, c->stream));
+    const size_t nm = (size_t)std::max<int64_t>(NM, 1), gm = (size_t)std::max<int64_t>(GM, 1);
+    std::vector<int32_t> aslot, off, gofs, cl;
+    std::vector<int8_t> xf;
+    std::vector<uint8_t> pmap;
+    off.reserve(R + 1);
+    gofs.reserve(R);
+    xf.reserve(R);
+    cl.reserve(nm);
+    pmap.reserve(nm);
+    off.push_back(0);
+    CTX_TRY(c, c->alloc(&c->gu, 8 * gm));
+    CTX_TRY(c, c->alloc(&c->gv, 8 * gm));
     const bool gth = c->opt.theta_method == LPS_THETA_GTHETA;
-    if (gth) {
-      CTX_TRY(c, c->alloc(&c->gth, (size_t)R * 2 * M));
-      CTX_TRY(c, hipMemsetAsync(c->gth, 0, sizeof(double) * R * 2 * M, c->stream));
-    }
-    int32_t r0 = 0;
+    if (gth) CTX_TRY(c, c->alloc(&c->gth, 2 * gm));
+    int64_t m0 = 0, g0 = 0;
     for (int32_t i = 0; i < B; i++) {
       const lps_ctx &q = *sub[i];
-      const int32_t Ri = q.n_abc, Mi = q.M;
+      const int32_t Ri = q.n_abc;
+      const int64_t Mi = q.abc_members, Gi = q.abc_block_members;
       if (Ri == 0) continue;
       std::vector<int32_t> h;
       if ((rc = to_host(c, h, q.abc_slot, (size_t)Ri))) return rc;
       for (int32_t v : h) aslot.push_back((int32_t)(v + c->slot_off[i]));
-      if ((rc = to_host(c, h, q.cloud, (size_t)Ri * Mi))) return rc;
-      for (int32_t r = 0; r < Ri; r++)
-        for (int32_t j = 0; j < Mi; j++) {
-          const int32_t v = h[(size_t)r * Mi + j];
-          cl[(size_t)(r0 + r) * M + j] = v < 0 ? -1 : (int32_t)(v + c->slot_off[i]);
-        }
-      // G_u / G_v rows: two component blocks of 4 Mi doubles each, placed at 0 and 4 M
-      CTX_TRY(c, hipMemcpy2DAsync(c->gu + (size_t)r0 * 8 * M, sizeof(double) * 4 * M, q.gu,
-                                  sizeof(double) * 4 * Mi, sizeof(double) * 4 * Mi, 2 * (size_t)Ri,
-                                  hipMemcpyDeviceToDevice, c->stream));
-      CTX_TRY(c, hipMemcpy2DAsync(c->gv + (size_t)r0 * 8 * M, sizeof(double) * 4 * M, q.gv,
-                                  sizeof(double) * 4 * Mi, sizeof(double) * 4 * Mi, 2 * (size_t)Ri,
-                                  hipMemcpyDeviceToDevice, c->stream));
+      if ((rc = to_host(c, h, q.abc_off, (size_t)Ri + 1))) return rc;
+      for (int32_t r = 1; r <= Ri; r++) off.push_back((int32_t)(m0 + h[r]));
+      if ((rc = to_host(c, h, q.abc_gofs, (size_t)Ri))) return rc;
+      for (int32_t v : h) gofs.push_back((int32_t)(g0 + v));
+      if ((rc = to_host(c, h, q.cloud, (size_t)Mi))) return rc;
+      for (int32_t v : h) cl.push_back(v < 0 ? -1 : (int32_t)(v + c->slot_off[i]));
+      std::vector<int8_t> hx;
+      if ((rc = to_host(c, hx, q.abc_xf, (size_t)Ri))) return rc;
+      xf.insert(xf.end(), hx.begin(), hx.end());
+      std::vector<uint8_t> hp;
+      if ((rc = to_host(c, hp, q.abc_pmap, (size_t)Mi))) return rc;
+      pmap.insert(pmap.end(), hp.begin(), hp.end());
+      CTX_TRY(c, hipMemcpyAsync(c->gu + 8 * g0, q.gu, sizeof(double) * 8 * Gi, hipMemcpyDeviceToDevice, c->stream));
+      CTX_TRY(c, hipMemcpyAsync(c->gv + 8 * g0, q.gv, sizeof(double) * 8 * Gi, hipMemcpyDeviceToDevice, c->stream));
       if (gth)
-        CTX_TRY(c, hipMemcpy2DAsync(c->gth + (size_t)r0 * 2 * M, sizeof(double) * 2 * M, q.gth,
-                                    sizeof(double) * 2 * Mi, sizeof(double) * 2 * Mi, (size_t)Ri,
-                                    hipMemcpyDeviceToDevice, c->stream));
-      r0 += Ri;
+        CTX_TRY(c, hipMemcpyAsync(c->gth + 2 * g0, q.gth, sizeof(double) * 2 * Gi, hipMemcpyDeviceToDevice, c->stream));
+      m0 += Mi;
+      g0 += Gi;
+    }
+    if (cl.empty()) {
+      cl.push_back(-1);
+      pmap.push_back(0);
     }
     CTX_TRY(c, c->upload(&c->abc_slot, aslot));
+    CTX_TRY(c, c->upload(&c->abc_off, off));
+    CTX_TRY(c, c->upload(&c->abc_gofs, gofs));
+    CTX_TRY(c, c->upload(&c->abc_xf, xf));
+    CTX_TRY(c, c->upload(&c->abc_pmap, pmap));
     CTX_TRY(c, c->upload(&c->cloud, cl));
   }
   // instances
@@ -958,6 +1152,9 @@ static int32_t merge_batch(lps_ctx *c, const std::vector<lps_ctx *> &sub) {
             names[T.theta_kernel] + ", force " + names[T.force_kernel] + " (16x16 tiles, " +
             std::to_string(NT) + " tiles, " + std::to_string(n_full) + " full, " +
             std::to_string(c->n) + " nodes, " + std::to_string(R) + " ABC rows)";
+  if (c->abc_shared_blocks)
+    c->path += "; ABC operators of " + std::to_string(GM) + " of " + std::to_string(NM) +
+               " members stored, the rest lattice images";
   return reset_state(c);
 }
 
@@ -1027,6 +1224,7 @@ int32_t lps_create_batch(const lps_problem *problems, int32_t count, const lps_o
   }
   if (rc == LPS_OK) rc = merge_batch(c, sub);
   for (lps_ctx *q : sub) lps_destroy(q);
+  if (rc == LPS_OK) rc = choose_abc_predict(c, opt.variant == 0);
   if (rc != LPS_OK) {
     g_create_error = c->err;
     lps_destroy(c);
@@ -1037,6 +1235,42 @@ int32_t lps_create_batch(const lps_problem *problems, int32_t count, const lps_o
 }
 
 int32_t lps_instance_count(const lps_ctx *ctx) { return ctx->n_inst; }
+
+// Fast path with ABC rows on their own stream: the ABC nodes' prediction folded into the partial
+// tiles' force pass (which then waits for G_u / G_v) or a kernel of its own after G_u / G_v (the
+// partial tiles then run beside them). The same bits either way. With tune, two rounds of 3
+// warm-up and 10 timed steps each and the faster by the whole step; LPS_ABC_PREDICT=fold or
+// =apart sets the choice (tests of both schedules).
+static int32_t choose_abc_predict(lps_ctx *c, bool tune) {
+  const char *force_pred = std::getenv("LPS_ABC_PREDICT");
+  if (force_pred && (!strcmp(force_pred, "fold") || !strcmp(force_pred, "apart"))) {
+    c->fold_pred = !strcmp(force_pred, "fold");
+  } else if (tune && c->n_abc > 0 && c->opt.concurrent == 1 && c->abc_in_part &&
+             c->tile.force_kernel != kGeneric) {
+    double tf[2] = {1e300, 1e300};
+    for (int round = 0; round < 2; round++) {
+      for (int q = 0; q < 2; q++) {
+        double t[4] = {0, 0, 0, 0};
+        c->fold_pred = q == 0;
+        int32_t rc = lps_set_timing(c, 1);
+        if (rc == LPS_OK) rc = lps_step(c, 3);
+        if (rc == LPS_OK) rc = lps_set_timing(c, 1);
+        if (rc == LPS_OK) rc = lps_step(c, 10);
+        if (rc == LPS_OK) rc = lps_pass_times(c, t);
+        if (rc == LPS_OK) rc = lps_set_timing(c, 0);
+        if (rc == LPS_OK) rc = reset_state(c);
+        if (rc != LPS_OK) return rc;
+        tf[q] = std::min(tf[q], t[3]);
+      }
+    }
+    c->fold_pred = tf[0] <= tf[1];
+  } else {
+    return LPS_OK;
+  }
+  drop_graphs(c);
+  c->path += c->fold_pred ? "; ABC prediction in the partial tiles' force pass" : "; ABC prediction apart";
+  return LPS_OK;
+}
 
 // Fast path: pick the node kernels of each pass. With variant 0 every candidate (generic tile
 // kernels, unrolled 68-bond kernels, register-blocked 68-bond kernels) runs a few timed steps on
@@ -1083,7 +1317,7 @@ static int32_t choose_variant(lps_ctx *c) {
   const char *names[3] = {"generic", "unrolled", "blocked"};
   c->path = std::string("tile kernels: dilatation ") + names[th] + ", force " + names[fo] +
             (tune ? " [autotuned]" : "") + c->path;
-  return LPS_OK;
+  return choose_abc_predict(c, tune);
 }
 
 constexpr int kEv = 4;  // events per timed step
@@ -1122,10 +1356,20 @@ static hipError_t step_once(lps_ctx *c, bool record_events) {
   }
   if (fork) {
     if ((e = hipStreamWaitEvent(c->stream2, c->ev_fork[0], 0)) != hipSuccess) return e;
+    // the closure's history shift and network read only step n's state: on the fourth stream
+    // from the step's start, their corrections stored and added after G_u / G_v
+    if (c->clo) {
+      if ((e = hipStreamWaitEvent(c->stream4, c->ev_fork[0], 0)) != hipSuccess) return e;
+      if ((e = closure_step_hook(c, st, c->stream4, true)) != hipSuccess) return e;
+      if ((e = hipEventRecord(c->ev_clo, c->stream4)) != hipSuccess) return e;
+    }
     if ((e = launch_abc_theta(a, st, c->stream2)) != hipSuccess) return e;
     if ((e = hipEventRecord(c->ev_join[0], c->stream2)) != hipSuccess) return e;
     if ((e = launch_abc_uv(a, st, c->ring(c->step), c->stream2)) != hipSuccess) return e;
-    if (c->clo && (e = closure_step_hook(c, st, c->stream2)) != hipSuccess) return e;
+    if (c->clo) {
+      if ((e = hipStreamWaitEvent(c->stream2, c->ev_clo, 0)) != hipSuccess) return e;
+      if ((e = closure_apply_hook(c, st, c->stream2)) != hipSuccess) return e;
+    }
     if ((e = hipEventRecord(c->ev_join[1], c->stream2)) != hipSuccess) return e;
   } else if (abc && (e = launch_abc_theta(a, st, c->stream)) != hipSuccess) {
     return e;
@@ -1148,7 +1392,8 @@ static hipError_t step_once(lps_ctx *c, bool record_events) {
   // the ABC nodes' prediction folded into the partial tiles' force pass (every ABC node lies in a
   // partial tile), which then waits for G_u / G_v; not with a closure attached, whose long ABC
   // branch the partial tiles would wait for
-  const bool fold = abc && !split && !c->clo && c->abc_in_part && a.tile.force_kernel != kGeneric;
+  const bool fold = abc && !split && !c->clo && c->abc_in_part && c->fold_pred &&
+                    a.tile.force_kernel != kGeneric;
   if (c->fast && fold) {
     if (fork && (e = hipStreamWaitEvent(c->stream, c->ev_join[1], 0)) != hipSuccess) return e;
     if (!fork && (e = launch_abc_uv(a, st, c->ring(c->step), c->stream)) != hipSuccess) return e;
@@ -1158,10 +1403,10 @@ static hipError_t step_once(lps_ctx *c, bool record_events) {
     if (split && (e = hipStreamWaitEvent(c->stream, c->ev_part[2], 0)) != hipSuccess) return e;
     if (fork && (e = hipStreamWaitEvent(c->stream, c->ev_join[1], 0)) != hipSuccess) return e;
     if (abc && !fork && (e = launch_abc_uv(a, st, c->ring(c->step), c->stream)) != hipSuccess) return e;
-    if (abc && !fork && c->clo && (e = closure_step_hook(c, st, c->stream)) != hipSuccess) return e;
+    if (abc && !fork && c->clo && (e = closure_step_hook(c, st, c->stream, false)) != hipSuccess) return e;
     if (abc && (e = launch_abc_predict(a, st, c->stream)) != hipSuccess) return e;
   }
-  if (c->clo && !abc && (e = closure_step_hook(c, st, c->stream)) != hipSuccess) return e;
+  if (c->clo && !abc && (e = closure_step_hook(c, st, c->stream, false)) != hipSuccess) return e;
   if (record_events) {
     if (!c->timing_force_only && (e = record_timing(c, ev[3])) != hipSuccess) return e;
     c->ev_used++;

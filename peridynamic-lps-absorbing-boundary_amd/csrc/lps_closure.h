@@ -64,7 +64,6 @@ __device__ __host__ __forceinline__ int64_t pack_index(int64_t o, int64_t i) {
   const int64_t k = i >> 4, w = i & 15;
   return (((k * 8 + (o >> 5)) * 64 + (o & 31) + 32 * (w >> 3)) << 3) + (w & 7);
 }
-constexpr int32_t kWideRows = 128;  // rows per workgroup of the wide forward kernel
 
 // Inputs of the ABC rows gathered from the per-node history (see the header comment).
 struct GatherSrc {
@@ -149,11 +148,17 @@ struct ApplyDst {
   const float *gate;  // per instance: no correction where s_u < gate
   double *ux1, *uy1, *vx1, *vy1;
   float *dbg;  // n x 8 when not null: the 4 outputs, s_u, s_v, and 0, 0 (tests)
+  // n x 8 when not null: the network stores each row's 4 outputs, s_u, s_v, 0, 0 here instead of
+  // adding its correction; mlp_apply_deferred adds them later (the same bits)
+  float *defer = nullptr;
 };
-// wide: 128 rows per workgroup (every wavefront all 256 columns, weights staged in LDS), for
-// large row counts; otherwise 32 rows per workgroup. Both give the same bits.
+// wide_waves 4 or 8: 32 wide_waves rows per workgroup (every wavefront all 256 columns, weights
+// staged in LDS), for large row counts; otherwise (0) 32 rows per workgroup. All give the same bits.
 hipError_t mlp_forward_apply(const MlpDev &m, const GatherSrc &g, int32_t n, const ApplyDst &d,
-                             bool wide, hipStream_t s);
+                             int wide_waves, hipStream_t s);
+// The corrections stored by mlp_forward_apply with d.defer set, added to the state.
+hipError_t mlp_apply_deferred(const MlpDev &m, const GatherSrc &g, int32_t n, const ApplyDst &d,
+                              hipStream_t s);
 // Rows of gathered inputs written out (n x 1600 BF16) with their scales (n x 2).
 hipError_t gather_rows(const GatherSrc &g, int32_t n, __bf16 *x, float *scale, hipStream_t s);
 

@@ -29,12 +29,14 @@ struct ClosureAttach {
   float *gate = nullptr;                            // n_inst: no correction where s_u < gate
   float *hist[2] = {nullptr, nullptr}, *mx[2] = {nullptr, nullptr};  // by step parity
   float *dbg = nullptr;                             // n_rows x 8, last applied correction
+  float *defer = nullptr;  // n_rows x 8: the network's outputs and scales of the step, to be applied
   // per-instance gains: as set by the caller, and as in force (the guard zeroes them)
   std::vector<float> gain_user, gain_now;
   std::vector<double> e_prev;      // energy at the previous output record
   std::vector<int64_t> off_step;   // step of the fallback to least squares, -1 none
   double guard_tol = 1e-3;         // <= 0: no guard
   int64_t wide_min = 16384;        // row count from which the wide forward kernel runs
+  int wide_waves = 8;              // its wavefronts per workgroup, 4 or 8
   std::vector<void *> allocs;
   template <class T>
   hipError_t alloc(T **q, size_t count) {
@@ -96,7 +98,7 @@ __global__ void __launch_bounds__(256) k_hist_shift(int32_t n_hist, const int32_
   mx[2 * (int64_t)b + 1] = mv;
 }
 
-hipError_t closure_step_hook(lps_ctx *c, const StepState &st, hipStream_t s) {
+hipError_t closure_step_hook(lps_ctx *c, const StepState &st, hipStream_t s, bool defer) {
   ClosureAttach *A = c->clo;
   const int par = (int)(c->step & 1);
   if (A->n_hist > 0) {
@@ -109,15 +111,24 @@ hipError_t closure_step_hook(lps_ctx *c, const StepState &st, hipStream_t s) {
   }
   if (!A->clo || A->n_rows == 0) return hipSuccess;
   const ApplyDst d{A->row_slot, A->row_inst, A->gain, A->gate, st.ux1, st.uy1, st.vx1, st.vy1,
-                   A->dbg};
+                   A->dbg, defer ? A->defer : nullptr};
   // timing mode 3: the step's two timing events around the network kernel, on its stream
   const unsigned evf = c->capturing ? hipEventRecordExternal : 0;
   hipError_t e = hipSuccess;
   if (c->clo_ev) e = hipEventRecordWithFlags(c->clo_ev[1], s, evf);
   if (e == hipSuccess)
-    e = mlp_forward_apply(A->clo->dev(), A->src(par), A->n_rows, d, A->n_rows >= A->wide_min, s);
+    e = mlp_forward_apply(A->clo->dev(), A->src(par), A->n_rows, d,
+                          A->n_rows >= A->wide_min ? A->wide_waves : 0, s);
   if (e == hipSuccess && c->clo_ev) e = hipEventRecordWithFlags(c->clo_ev[2], s, evf);
   return e;
+}
+
+hipError_t closure_apply_hook(lps_ctx *c, const StepState &st, hipStream_t s) {
+  ClosureAttach *A = c->clo;
+  if (!A->clo || A->n_rows == 0) return hipSuccess;
+  const ApplyDst d{A->row_slot, A->row_inst, A->gain, A->gate, st.ux1, st.uy1, st.vx1, st.vy1,
+                   A->dbg, A->defer};
+  return mlp_apply_deferred(A->clo->dev(), A->src((int)(c->step & 1)), A->n_rows, d, s);
 }
 
 hipError_t closure_reset_hook(lps_ctx *c) {
@@ -287,6 +298,7 @@ static int32_t make_attach(lps_ctx *c, const Tables &t, lps_closure *clo,
   A->n_inst = n_inst;
   // LPS_CLOSURE_WIDE_MIN: row count from which the 128-row kernel runs (both give the same bits)
   if (const char *w = std::getenv("LPS_CLOSURE_WIDE_MIN")) A->wide_min = std::atoll(w);
+  if (const char *w = std::getenv("LPS_CLOSURE_WIDE_WAVES")) A->wide_waves = std::atoi(w) == 4 ? 4 : 8;
   A->gain_user.assign(std::max(n_inst, 1), 1.0f);
   A->gain_now = A->gain_user;
   A->e_prev.assign(std::max(n_inst, 1), -1.0);
@@ -305,6 +317,7 @@ static int32_t make_attach(lps_ctx *c, const Tables &t, lps_closure *clo,
     if (e == hipSuccess) e = A->alloc(&A->mx[q], (size_t)std::max(A->n_hist, 1) * 2);
   }
   if (e == hipSuccess) e = A->alloc(&A->dbg, (size_t)std::max(A->n_rows, 1) * 8);
+  if (e == hipSuccess && A->clo) e = A->alloc(&A->defer, (size_t)std::max(A->n_rows, 1) * 8);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   if (e != hipSuccess) {
     delete A;
@@ -322,9 +335,14 @@ static inline int32_t lat(double x, double d) { return (int32_t)std::lround(x / 
 // ---------------------------------------------------------------------------------------------
 struct CollectArgs {
   GatherSrc g;
-  int32_t n_rows, M;
-  const int32_t *cloud;             // slots of the state below; cloud n_rows x M (-1 skip)
-  const double *gu, *gv;            // n_rows x 8M (trunc's G_u, G_v rows)
+  int32_t n_rows;
+  // trunc's G_u, G_v rows in the solver's layout (StepArgs): row r has members off[r] .. off[r+1],
+  // its operator block at gofs[r] under the lattice transform xf[r], member map pmap
+  const int32_t *off, *gofs;
+  const int8_t *xf;
+  const uint8_t *pmap;
+  const int32_t *cloud;             // slots of the state below (-1 skip)
+  const double *gu, *gv;
   const double *ux, *uy, *uxm, *uym, *vx, *vy, *vxm, *vym;  // U^n, U^{n-1}, V^n, V^{n-1}
   // the target: U^{n+1}, V^{n+1} of the reflection-free run at the row's node (slot tgt_slot[r])
   const int32_t *tgt_slot;
@@ -355,20 +373,26 @@ __global__ void __launch_bounds__(256) k_collect(CollectArgs a) {
   row_scale(a.g, r, su, sv);
   if (!(su >= a.amp_min) || su < 1e-30f) return;
   // least-squares update on the truth state, as the solver's G_u / G_v rows
-  const double *gu0 = a.gu + (int64_t)r * 8 * a.M, *gu1 = gu0 + 4 * a.M;
-  const double *gv0 = a.gv + (int64_t)r * 8 * a.M, *gv1 = gv0 + 4 * a.M;
+  const int64_t o = a.off[r], gb = a.gofs[r];
+  const int32_t L = a.off[r + 1] - a.off[r];
+  const int xf = a.xf[r];
   double Ux = 0.0, Uy = 0.0, Vx = 0.0, Vy = 0.0;
-  for (int base = 0; base < a.M; base += 64) {
+  for (int base = 0; base < L; base += 64) {
     const int jj = base + lane;
-    const int32_t k = jj < a.M ? a.cloud[(int64_t)r * a.M + jj] : -1;
+    const int32_t k = jj < L ? a.cloud[o + jj] : -1;
     double t0 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0;
     if (k >= 0) {
+      const int m = a.pmap[o + jj];
+      const double *bu = a.gu + 8 * gb + 4 * m, *bv = a.gv + 8 * gb + 4 * m;
+      double4 g0, g1, h0, h1;
+      lattice_map(xf, *(const double4 *)bu, *(const double4 *)(bu + 4 * L), g0, g1);
+      lattice_map(xf, *(const double4 *)bv, *(const double4 *)(bv + 4 * L), h0, h1);
       const double ux = a.ux[k], uy = a.uy[k], uxm = a.uxm[k], uym = a.uym[k];
       const double vx = a.vx[k], vy = a.vy[k], vxm = a.vxm[k], vym = a.vym[k];
-      t0 = gu0[4 * jj] * ux + gu0[4 * jj + 1] * uy + gu0[4 * jj + 2] * uxm + gu0[4 * jj + 3] * uym;
-      t1 = gu1[4 * jj] * ux + gu1[4 * jj + 1] * uy + gu1[4 * jj + 2] * uxm + gu1[4 * jj + 3] * uym;
-      t2 = gv0[4 * jj] * vx + gv0[4 * jj + 1] * vy + gv0[4 * jj + 2] * vxm + gv0[4 * jj + 3] * vym;
-      t3 = gv1[4 * jj] * vx + gv1[4 * jj + 1] * vy + gv1[4 * jj + 2] * vxm + gv1[4 * jj + 3] * vym;
+      t0 = g0.x * ux + g0.y * uy + g0.z * uxm + g0.w * uym;
+      t1 = g1.x * ux + g1.y * uy + g1.z * uxm + g1.w * uym;
+      t2 = h0.x * vx + h0.y * vy + h0.z * vxm + h0.w * vym;
+      t3 = h1.x * vx + h1.y * vy + h1.z * vxm + h1.w * vym;
     }
     Ux += wsum(t0);
     Uy += wsum(t1);
@@ -636,10 +660,20 @@ int32_t lps_closure_collect(lps_closure *clo, lps_ctx *c, const lps_problem *p, 
     return c->fail(LPS_ERR_ARG, "a node of the truncated problem is not in the truth domain");
   ClosureAttach *A = nullptr;
   if ((rc = make_attach(c, t, nullptr, row_slot, std::vector<int32_t>(C, 0), 1, &A))) return rc;
-  int32_t *dcloud = nullptr;
+  int32_t *dcloud = nullptr, *doff = nullptr;
+  int8_t *dxf = nullptr;
+  uint8_t *dpmap = nullptr;
   double *dgu = nullptr, *dgv = nullptr;
   const size_t gl = (size_t)C * 8 * M;
+  // the problem's rows as they are: M members each, operators of their own (identity map)
+  std::vector<int32_t> off(C + 1);
+  for (int32_t a = 0; a <= C; a++) off[a] = a * M;
+  std::vector<uint8_t> pmap((size_t)C * M);
+  for (size_t q = 0; q < pmap.size(); q++) pmap[q] = (uint8_t)(q % M);
   hipError_t e = A->upload(&dcloud, cloud, c->stream);
+  if (e == hipSuccess) e = A->upload(&doff, off, c->stream);
+  if (e == hipSuccess) e = A->upload(&dxf, std::vector<int8_t>(std::max(C, 1), 0), c->stream);
+  if (e == hipSuccess) e = A->upload(&dpmap, pmap, c->stream);
   if (e == hipSuccess) e = A->alloc(&dgu, gl);
   if (e == hipSuccess) e = A->alloc(&dgv, gl);
   if (e == hipSuccess) e = hipMemcpyAsync(dgu, p->ci_nodes_u, sizeof(double) * gl, hipMemcpyHostToDevice, c->stream);
@@ -658,7 +692,10 @@ int32_t lps_closure_collect(lps_closure *clo, lps_ctx *c, const lps_problem *p, 
   rc = closure_reset_hook(c) == hipSuccess ? LPS_OK : c->fail(LPS_ERR_HIP, "clearing the history");
   CollectArgs ca{};
   ca.n_rows = C;
-  ca.M = M;
+  ca.off = doff;
+  ca.gofs = doff;
+  ca.xf = dxf;
+  ca.pmap = dpmap;
   ca.tgt_slot = A->row_slot;
   ca.cloud = dcloud;
   ca.gu = dgu;
@@ -748,7 +785,10 @@ int32_t lps_closure_collect_closed(lps_closure *clo, lps_ctx *truth, lps_ctx *c,
   hipError_t e = hipMemcpyAsync(dtslot, tslot.data(), sizeof(int32_t) * c->n_abc, hipMemcpyHostToDevice, c->stream);
   CollectArgs ca{};
   ca.n_rows = c->n_abc;
-  ca.M = c->M;
+  ca.off = c->abc_off;
+  ca.gofs = c->abc_gofs;
+  ca.xf = c->abc_xf;
+  ca.pmap = c->abc_pmap;
   ca.cloud = c->cloud;
   ca.gu = c->gu;
   ca.gv = c->gv;

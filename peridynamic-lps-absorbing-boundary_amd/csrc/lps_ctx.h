@@ -31,6 +31,8 @@ struct lps_ctx {
   hipStream_t stream = nullptr;
   hipStream_t stream2 = nullptr;  // the ABC rows' kernels beside the tile kernels
   hipStream_t stream3 = nullptr;  // the partial tiles' kernels beside the full tiles' kernels
+  hipStream_t stream4 = nullptr;  // the learned closure's history and network beside the ABC rows
+  hipEvent_t ev_clo = nullptr;    // the closure's network done (its corrections stored)
   hipEvent_t ev_fork[2] = {nullptr, nullptr}, ev_join[2] = {nullptr, nullptr};
   hipEvent_t ev_part[3] = {nullptr, nullptr, nullptr};  // partial theta done, theta done, partial force done
   std::string err, path;
@@ -40,6 +42,9 @@ struct lps_ctx {
   bool fast = false;
   bool stencil68_ok = false;  // the unrolled 68-bond kernels apply to this problem
   bool abc_in_part = false;   // tile path: every ABC node lies in a partial tile
+  // the ABC nodes' prediction folded into the partial tiles' force pass (which then waits for
+  // G_u / G_v); else a kernel of its own after G_u / G_v, the partial tiles beside them
+  bool fold_pred = true;
   int64_t n_full_nodes = 0;   // nodes in the full tiles
   // instances: one for lps_create, several for lps_create_batch
   std::vector<InstParams> inst_h;
@@ -64,7 +69,13 @@ struct lps_ctx {
   double *x = nullptr, *y = nullptr, *m = nullptr, *c2m = nullptr;
   int8_t *kind = nullptr, *is_bc = nullptr;
   int32_t *perm_d = nullptr, *abc_slot = nullptr;
-  int32_t *cloud = nullptr;
+  // ABC rows, ragged (StepArgs): row r's members abc_off[r] .. abc_off[r + 1], its operator
+  // block at member abc_gofs[r] of gu / gv / gth
+  int32_t *abc_off = nullptr, *cloud = nullptr, *abc_gofs = nullptr;
+  int8_t *abc_xf = nullptr;
+  uint8_t *abc_pmap = nullptr;
+  int64_t abc_members = 0, abc_block_members = 0;
+  bool abc_shared_blocks = false;  // lattice images share their canonical row's block
   double *gu = nullptr, *gv = nullptr, *gth = nullptr;
   TileArgs tile{};
   // Prediction ring: step s reads P[s % 6] (U^{s+1} before the ABC overwrite) and writes
@@ -128,10 +139,13 @@ struct lps_ctx {
 };
 
 namespace lps {
-// Learned ABC closure hooks (lps_closure.hip), active when ctx->clo is set. step: after the ABC
-// rows' G_u / G_v on their stream (at the end of the step when the context has no ABC rows):
-// records U^n, V^n into the history and adds the correction to U^{n+1}, V^{n+1} at the ABC nodes.
-hipError_t closure_step_hook(lps_ctx *c, const StepState &st, hipStream_t s);
+// Learned ABC closure hooks (lps_closure.hip), active when ctx->clo is set. step: records U^n, V^n
+// into the history and runs the network. Without defer it runs after the ABC rows' G_u / G_v on
+// their stream (at the end of the step when the context has no ABC rows) and adds each correction
+// to U^{n+1}, V^{n+1} at its ABC node. With defer it only reads step n's state, so it may run from
+// the step's start; it stores the corrections, and apply adds them after G_u / G_v.
+hipError_t closure_step_hook(lps_ctx *c, const StepState &st, hipStream_t s, bool defer);
+hipError_t closure_apply_hook(lps_ctx *c, const StepState &st, hipStream_t s);
 hipError_t closure_reset_hook(lps_ctx *c);  // clears the history and the fallbacks (lps_reset)
 // At each output record (count instances, de_rel set): an instance whose energy rose by more than
 // the guard tolerance times its E0 since the previous record falls back to the least-squares ABC.

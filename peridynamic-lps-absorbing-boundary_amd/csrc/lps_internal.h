@@ -102,15 +102,61 @@ struct StepArgs {
   const double *m;                 // weighted volume m_i
   const double *c2m;               // 2 / m_i, or 0 when m_i <= 0
   const int8_t *is_bc;             // bctypes[2i] == 1 (IDW neighbour filter)
-  // ABC rows
-  const int32_t *cloud;            // n_abc x M, slots, -1 = skip
-  const double *gu, *gv;           // n_abc x 2 x 4M
-  const double *gth;               // n_abc x 2M
+  // ABC rows, ragged: row r has the L = abc_off[r + 1] - abc_off[r] cloud members from
+  // o = abc_off[r] on (at most M; the tile path drops trailing members that add nothing)
+  const int32_t *abc_off;          // n_abc + 1
+  const int32_t *cloud;            // member slots, -1 = skip
+  // Operator blocks: a row's G_u / G_v / G_theta coefficients are those of an operator block of L
+  // members at member offset g = abc_gofs[r] (the x block, 4 L doubles at 8 g, then the y block;
+  // G_theta 2 per member at 2 g), its member jj being the block's member abc_pmap[o + jj], mapped
+  // by the lattice transform abc_xf[r] (abc_uv_coeffs). A row with operators of its own has its own
+  // block, the identity map and transform 0; rows that are images of one row under the square
+  // lattice's symmetries share that row's block.
+  const int32_t *abc_gofs;         // n_abc
+  const int8_t *abc_xf;            // n_abc: bit 0 swap x and y, bit 1 negate the cross terms
+  const uint8_t *abc_pmap;         // per member
+  const double *gu, *gv;
+  const double *gth;
   int32_t theta_method;
   const InstParams *inst;          // per instance; the fast-path kernels read it through
                                    // tile.tile_inst, the exact path uses the scalars above
   TileArgs tile;
 };
+
+// Where ABC row `row` lies in the ragged arrays: its first member and its member count.
+__device__ __forceinline__ void abc_row_span(const StepArgs &a, int32_t row, int64_t &o, int32_t &L) {
+  const int32_t o0 = a.abc_off[row];
+  o = o0;
+  L = a.abc_off[row + 1] - o0;
+}
+
+// A 2 x 2 x 2 block of G_u or G_v (b0: into x, b1: into y; each (U^n_x, U^n_y, U^{n-1}_x,
+// U^{n-1}_y)) under the lattice transform T of code xf: T B T' is B with x and y swapped (bit 0)
+// and the cross terms (x into y, y into x) negated (bit 1), since T is a signed permutation. Exact.
+__device__ __forceinline__ void lattice_map(int xf, double4 b0, double4 b1, double4 &o0, double4 &o1) {
+  if (xf & 1) {
+    o0 = make_double4(b1.y, b1.x, b1.w, b1.z);
+    o1 = make_double4(b0.y, b0.x, b0.w, b0.z);
+  } else {
+    o0 = b0;
+    o1 = b1;
+  }
+  if (xf & 2) {
+    o0.y = -o0.y; o0.w = -o0.w;
+    o1.x = -o1.x; o1.z = -o1.z;
+  }
+}
+
+// G_u (g0, g1) and G_v (h0, h1) coefficients of member jj of a row (members from o, L of them,
+// block at g, transform xf).
+__device__ __forceinline__ void abc_uv_coeffs(const StepArgs &a, int64_t o, int32_t L, int64_t g,
+                                              int xf, int jj, double4 &g0, double4 &g1, double4 &h0,
+                                              double4 &h1) {
+  const int m = a.abc_pmap[o + jj];
+  const double *bu = a.gu + 8 * g + 4 * m, *bv = a.gv + 8 * g + 4 * m;
+  lattice_map(xf, *(const double4 *)bu, *(const double4 *)(bu + 4 * L), g0, g1);
+  lattice_map(xf, *(const double4 *)bv, *(const double4 *)(bv + 4 * L), h0, h1);
+}
 
 // State buffers for one step (SoA).
 struct StepState {

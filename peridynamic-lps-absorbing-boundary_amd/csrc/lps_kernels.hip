@@ -86,10 +86,9 @@ __device__ __forceinline__ double predict_exact(double u, double v, double a, do
 }
 
 // G_u / G_v term of one cloud slot: g0 U_x + g1 U_y + g2 Unm1_x + g3 Unm1_y (src/main.cpp:903-912).
-__device__ __forceinline__ double cloud_term_exact(const double *g, double a, double b, double c,
-                                                   double d) {
+__device__ __forceinline__ double cloud_term_exact(double4 g, double a, double b, double c, double d) {
 #pragma clang fp contract(off)
-  return g[0] * a + g[1] * b + g[2] * c + g[3] * d;
+  return g.x * a + g.y * b + g.z * c + g.w * d;
 }
 
 __device__ __forceinline__ double gtheta_term_exact(double g0, double g1, double a, double b) {
@@ -194,23 +193,29 @@ struct AbcScratch {
 template <bool EXACT>
 __device__ __forceinline__ double abc_gtheta(const StepArgs &a, const StepState &st, int32_t row,
                                              int32_t lane, AbcScratch *sw) {
-  const double *g = a.gth + (int64_t)row * 2 * a.M;
-  const int32_t *cl = a.cloud + (int64_t)row * a.M;
+  int64_t o;
+  int32_t L;
+  abc_row_span(a, row, o, L);
+  const double *g = a.gth + 2 * (int64_t)a.abc_gofs[row];  // G_theta needs no lattice transform
+  const int32_t *cl = a.cloud + o;
   double acc = 0.0;
-  for (int base = 0; base < a.M; base += 64) {
+  for (int base = 0; base < L; base += 64) {
     const int jj = base + lane;
     double t = 0.0;
     int32_t k = -1;
-    if (jj < a.M) {
+    if (jj < L) {
       k = cl[jj];
-      if (k >= 0) t = gtheta_term_exact(g[2 * jj], g[2 * jj + 1], st.thn[k], st.thnm1[k]);
+      if (k >= 0) {
+        const double2 gm = *(const double2 *)(g + 2 * a.abc_pmap[o + jj]);
+        t = gtheta_term_exact(gm.x, gm.y, st.thn[k], st.thnm1[k]);
+      }
     }
     if (EXACT) {
       sw->c[0][lane] = t;
       sw->flag[lane] = (k >= 0);
       __threadfence_block();
       if (lane == 0) {
-        const int lim = min(64, a.M - base);
+        const int lim = min(64, L - base);
         for (int q = 0; q < lim; q++)
           if (sw->flag[q]) acc = add_exact(acc, sw->c[0][q]);
       }
@@ -269,23 +274,28 @@ __device__ __forceinline__ void abc_force_row(const StepArgs &a, const StepState
     }
   }
   // G_u and G_v over the cloud (src/main.cpp:883-916, 933-974)
-  const double *gu0 = a.gu + (int64_t)row * 8 * a.M, *gu1 = gu0 + 4 * a.M;
-  const double *gv0 = a.gv + (int64_t)row * 8 * a.M, *gv1 = gv0 + 4 * a.M;
-  const int32_t *cl = a.cloud + (int64_t)row * a.M;
+  int64_t o;
+  int32_t L;
+  abc_row_span(a, row, o, L);
+  const int64_t gb = a.abc_gofs[row];
+  const int xf = a.abc_xf[row];
+  const int32_t *cl = a.cloud + o;
   double Ux = 0.0, Uy = 0.0, Vx = 0.0, Vy = 0.0;
-  for (int base = 0; base < a.M; base += 64) {
+  for (int base = 0; base < L; base += 64) {
     const int jj = base + lane;
     double t0 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0;
     int32_t k = -1;
-    if (jj < a.M) {
+    if (jj < L) {
       k = cl[jj];
       if (k >= 0) {
+        double4 g0, g1, h0, h1;
+        abc_uv_coeffs(a, o, L, gb, xf, jj, g0, g1, h0, h1);
         const double ux = st.ux[k], uy = st.uy[k], uxm = st.uxm[k], uym = st.uym[k];
         const double vx = st.vx[k], vy = st.vy[k], vxm = st.vxm[k], vym = st.vym[k];
-        t0 = cloud_term_exact(gu0 + 4 * jj, ux, uy, uxm, uym);
-        t1 = cloud_term_exact(gu1 + 4 * jj, ux, uy, uxm, uym);
-        t2 = cloud_term_exact(gv0 + 4 * jj, vx, vy, vxm, vym);
-        t3 = cloud_term_exact(gv1 + 4 * jj, vx, vy, vxm, vym);
+        t0 = cloud_term_exact(g0, ux, uy, uxm, uym);
+        t1 = cloud_term_exact(g1, ux, uy, uxm, uym);
+        t2 = cloud_term_exact(h0, vx, vy, vxm, vym);
+        t3 = cloud_term_exact(h1, vx, vy, vxm, vym);
       }
     }
     if (EXACT) {
@@ -296,7 +306,7 @@ __device__ __forceinline__ void abc_force_row(const StepArgs &a, const StepState
       sw->flag[lane] = (k >= 0);
       __threadfence_block();
       if (lane == 0) {
-        const int lim = min(64, a.M - base);
+        const int lim = min(64, L - base);
         for (int q = 0; q < lim; q++) {
           if (!sw->flag[q]) continue;
           Ux = add_exact(Ux, sw->c[0][q]);
@@ -1052,17 +1062,20 @@ __global__ void __launch_bounds__(kBlock) k_abc_uv(StepArgs a, StepState st, Rin
   const int32_t nrb = (a.n_abc + kWavesPerBlock - 1) / kWavesPerBlock;
   const int32_t row = tile_of_block(blockIdx.x, nrb) * kWavesPerBlock + wid;
   if (row >= a.n_abc) return;
-  const double *gu0 = a.gu + (int64_t)row * 8 * a.M, *gu1 = gu0 + 4 * a.M;
-  const double *gv0 = a.gv + (int64_t)row * 8 * a.M, *gv1 = gv0 + 4 * a.M;
-  const int32_t *cl = a.cloud + (int64_t)row * a.M;
+  int64_t o;
+  int32_t L;
+  abc_row_span(a, row, o, L);
+  const int64_t gb = a.abc_gofs[row];
+  const int xf = a.abc_xf[row];
+  const int32_t *cl = a.cloud + o;
   double Ux = 0.0, Uy = 0.0, Vx = 0.0, Vy = 0.0;
-  for (int base = 0; base < a.M; base += 64) {
+  for (int base = 0; base < L; base += 64) {
     const int jj = base + lane;
-    const int32_t k = jj < a.M ? cl[jj] : -1;
+    const int32_t k = jj < L ? cl[jj] : -1;
     double t0 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0;
     if (k >= 0) {
-      const double4 g0 = *(const double4 *)(gu0 + 4 * jj), g1 = *(const double4 *)(gu1 + 4 * jj);
-      const double4 h0 = *(const double4 *)(gv0 + 4 * jj), h1 = *(const double4 *)(gv1 + 4 * jj);
+      double4 g0, g1, h0, h1;
+      abc_uv_coeffs(a, o, L, gb, xf, jj, g0, g1, h0, h1);
       const bool in = a.kind[k] == 1;
       const double ux = (in ? r.ux : st.ux)[k], uy = (in ? r.uy : st.uy)[k];
       const double uxm = (in ? r.uxm : st.uxm)[k], uym = (in ? r.uym : st.uym)[k];

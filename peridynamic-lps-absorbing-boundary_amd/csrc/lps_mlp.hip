@@ -98,6 +98,31 @@ __device__ __forceinline__ void apply_row(const MlpDev &m, const GatherSrc &g, c
   }
 }
 
+// A row's end in the gathered path: its correction added to the state, or with d.defer its
+// outputs and scales stored, for k_apply_deferred to add.
+__device__ __forceinline__ void finish_row(const MlpDev &m, const GatherSrc &g, const ApplyDst &d,
+                                           int32_t row, const float *y, float su, float sv) {
+  if (d.defer) {
+    float4 *o = (float4 *)(d.defer + (int64_t)row * 8);
+    o[0] = make_float4(y[0], y[1], y[2], y[3]);
+    o[1] = make_float4(su, sv, 0.0f, 0.0f);
+    return;
+  }
+  apply_row(m, g, d, row, y, su, sv);
+}
+
+// The stored corrections of n rows added to the state: the same arithmetic as the network's own
+// epilogue, so a deferred step gives the bits of an undeferred one.
+__global__ void __launch_bounds__(256) k_apply_deferred(MlpDev m, GatherSrc g, int32_t n,
+                                                        ApplyDst d) {
+  const int32_t row = blockIdx.x * 256 + threadIdx.x;
+  if (row >= n) return;
+  const float4 *o = (const float4 *)(d.defer + (int64_t)row * 8);
+  const float4 a = o[0], b = o[1];
+  const float y[4] = {a.x, a.y, a.z, a.w};
+  apply_row(m, g, d, row, y, b.x, b.y);
+}
+
 // The network over 32 rows per workgroup (4 wavefronts, 64 output columns each). kGather: inputs
 // gathered from the history (gather8) and the correction added to the state; else inputs from x.
 template <bool kGather>
@@ -191,41 +216,45 @@ __global__ void __launch_bounds__(256) k_mlp_fwd(MlpDev m, const __bf16 *x, Gath
       for (int c = 0; c < kClOut; c++) yo[(int64_t)row * kClOut + c] = y[c];
     }
   } else {
-    apply_row(m, g, d, row, y, sc[r][2], sc[r][3]);
+    finish_row(m, g, d, row, y, sc[r][2], sc[r][3]);
   }
 }
 
-// The network over 128 rows per workgroup: wavefront w owns rows 32 w .. 32 w + 31 and all 256
-// columns (8 accumulator tiles); each K step's weight fragments are staged once per workgroup in
-// LDS (double buffered). The operands of K step k + kPf are loaded while step k computes (one
-// wavefront per SIMD: the prefetch hides the memory latency). Same arithmetic order as k_mlp_fwd,
-// so the same bits.
+// The network over 32 kW rows per workgroup of kW wavefronts: wavefront w owns rows 32 w .. 32 w + 31
+// and all 256 columns (8 accumulator tiles); each K step's weight fragments are staged once per
+// workgroup in LDS (double buffered). The operands of K step k + kPf are loaded while step k
+// computes. Same arithmetic order as k_mlp_fwd, so the same bits.
+//  kW = 4: 128 rows, 86 KB of LDS, one wavefront per SIMD.
+//  kW = 8: 256 rows, 152 KB of LDS (one workgroup per CU), two wavefronts per SIMD: half the
+//    weight traffic per row, and each SIMD switches to the other wavefront while one waits.
 constexpr int kPf = 4;
 
 struct RawG {  // 8 history values (4 (x, y) pairs) of one K step, FP32
   float4 a, b;
 };
 
-template <bool kGather>
-__global__ void __launch_bounds__(256) k_mlp_fwd_wide(MlpDev m, const __bf16 *x, GatherSrc g,
-                                                      int32_t n, float *yo, ApplyDst d) {
+template <bool kGather, int kW>
+__global__ void __launch_bounds__(64 * kW) k_mlp_fwd_wide(MlpDev m, const __bf16 *x, GatherSrc g,
+                                                          int32_t n, float *yo, ApplyDst d) {
+  constexpr int kT = 64 * kW, kRows = 32 * kW;
+  constexpr int kLd = 512 / kT;  // weight fragments (of 512 per K step) each thread stages
   __shared__ bf16x8 bs[2][8][64];
-  __shared__ __bf16 hs[kWideRows][kHsPitch];
-  __shared__ float sc[kWideRows][4];
+  __shared__ __bf16 hs[kRows][kHsPitch];
+  __shared__ float sc[kRows][4];
   // layer 1 keeps the rows' history indices where the activations go later
   int32_t(*hi)[kClOffsets] = reinterpret_cast<int32_t(*)[kClOffsets]>(&hs[0][0]);
-  static_assert(sizeof(int32_t) * kWideRows * kClOffsets <= sizeof(__bf16) * kWideRows * kHsPitch,
+  static_assert(sizeof(int32_t) * kRows * kClOffsets <= sizeof(__bf16) * kRows * kHsPitch,
                 "history indices fit in the activation buffer");
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   // XCD-contiguous row ranges: neighbouring rows share history records through one L2
-  const int r0 = xcd_block(blockIdx.x, (n + kWideRows - 1) / kWideRows) * kWideRows;
+  const int r0 = xcd_block(blockIdx.x, (n + kRows - 1) / kRows) * kRows;
   if (r0 >= n) return;  // the whole block: before any barrier
   const int ar = lane & 31, ak = 8 * (lane >> 5);
   const int lrow = 32 * wv + ar;
   const int32_t row = r0 + lrow;
   const bool live = row < n;
   if constexpr (kGather) {
-    for (int q = t; q < kWideRows * kClOffsets; q += 256) {
+    for (int q = t; q < kRows * kClOffsets; q += kT) {
       const int rr = q / kClOffsets, jj = q % kClOffsets;
       hi[rr][jj] = r0 + rr < n ? g.hidx[(int64_t)(r0 + rr) * kClOffsets + jj] : -1;
     }
@@ -304,27 +333,30 @@ __global__ void __launch_bounds__(256) k_mlp_fwd_wide(MlpDev m, const __bf16 *x,
 #pragma unroll
     for (int q = 0; q < 8; q++) acc[q] = f32x16{};
     RawG ra[kPf];
-    bf16x8 xa[kPf], rb0[kPf], rb1[kPf];
+    bf16x8 xa[kPf], rb[kPf][kLd];
 #pragma unroll
     for (int dd = 0; dd < kPf; dd++) {
       const bf16x8 *src = (const bf16x8 *)(wp + (int64_t)dd * 4096);
-      rb0[dd] = src[t];
-      rb1[dd] = src[t + 256];
+#pragma unroll
+      for (int u = 0; u < kLd; u++) rb[dd][u] = src[t + kT * u];
       if (layer == 0) issue_a(dd, ra[dd], xa[dd]);
     }
     for (int k0 = 0; k0 < nk; k0 += kPf) {
 #pragma unroll
       for (int dd = 0; dd < kPf; dd++) {
         const int k = k0 + dd;
-        bs[k & 1][t >> 6][t & 63] = rb0[dd];
-        bs[k & 1][(t >> 6) + 4][t & 63] = rb1[dd];
+#pragma unroll
+        for (int u = 0; u < kLd; u++) {
+          const int q = t + kT * u;
+          bs[k & 1][q >> 6][q & 63] = rb[dd][u];
+        }
         bf16x8 a;
         if (layer == 0) a = make_a(k, ra[dd], xa[dd]);
         else a = *(const bf16x8 *)&hs[lrow][16 * k + ak];
         if (k + kPf < nk) {
           const bf16x8 *src = (const bf16x8 *)(wp + (int64_t)(k + kPf) * 4096);
-          rb0[dd] = src[t];
-          rb1[dd] = src[t + 256];
+#pragma unroll
+          for (int u = 0; u < kLd; u++) rb[dd][u] = src[t + kT * u];
           if (layer == 0) issue_a(k + kPf, ra[dd], xa[dd]);
         }
         __syncthreads();
@@ -379,7 +411,7 @@ __global__ void __launch_bounds__(256) k_mlp_fwd_wide(MlpDev m, const __bf16 *x,
       for (int c = 0; c < kClOut; c++) yo[(int64_t)orow * kClOut + c] = y[c];
     }
   } else {
-    apply_row(m, g, d, orow, y, sc[r][2], sc[r][3]);
+    finish_row(m, g, d, orow, y, sc[r][2], sc[r][3]);
   }
 }
 
@@ -392,16 +424,30 @@ hipError_t mlp_forward_dataset(const MlpDev &m, const __bf16 *x, int32_t n, __bf
 }
 
 hipError_t mlp_forward_apply(const MlpDev &m, const GatherSrc &g, int32_t n, const ApplyDst &d,
-                             bool wide, hipStream_t s) {
+                             int wide_waves, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  if (wide) {
-    hipLaunchKernelGGL(k_mlp_fwd_wide<true>, dim3((n + 8 * kWideRows - 1) / (8 * kWideRows) * 8), dim3(256), 0,
-                       s, m, (const __bf16 *)nullptr, g, n, (float *)nullptr, d);
+  if (wide_waves == 4 || wide_waves == 8) {
+    const int rows = 32 * wide_waves;  // blocks: a multiple of 8 (XCD-contiguous row ranges)
+    const dim3 grid((n + 8 * rows - 1) / (8 * rows) * 8);
+    if (wide_waves == 8)
+      hipLaunchKernelGGL((k_mlp_fwd_wide<true, 8>), grid, dim3(512), 0, s, m, (const __bf16 *)nullptr,
+                         g, n, (float *)nullptr, d);
+    else
+      hipLaunchKernelGGL((k_mlp_fwd_wide<true, 4>), grid, dim3(256), 0, s, m, (const __bf16 *)nullptr,
+                         g, n, (float *)nullptr, d);
     return hipGetLastError();
   }
   hipLaunchKernelGGL(k_mlp_fwd<true>, dim3((n + kClRows - 1) / kClRows), dim3(256), 0, s, m,
                      (const __bf16 *)nullptr, g, n, (__bf16 *)nullptr, (__bf16 *)nullptr,
                      (__bf16 *)nullptr, (float *)nullptr, d);
+  return hipGetLastError();
+}
+
+hipError_t mlp_apply_deferred(const MlpDev &m, const GatherSrc &g, int32_t n, const ApplyDst &d,
+                              hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  if (!d.defer) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_apply_deferred, dim3((n + 255) / 256), dim3(256), 0, s, m, g, n, d);
   return hipGetLastError();
 }
 

@@ -382,26 +382,53 @@ def test_teacher_forced_samples():
         assert aux[r, 4] == 1.0
 
 
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_closure_beside_the_abc_rows_gives_the_same_bits(use_graph):
+    """With the ABC rows on their own stream the closure's network runs from the step's start on
+    another stream and its corrections are added after G_u / G_v; one stream runs the network after
+    G_u / G_v and adds them in its epilogue. Both give the same bits."""
+    say("closure deferred vs in-line", use_graph)
+    prob = with_wide_pulse(load_golden("abc_r10_nu025")[0])
+    c, _ = closure_with(26, 0.0)
+    c.set_params(random_params(27, out_scale=0.05))
+    c.set_scales(1e-3, 1e-3)
+    out = []
+    with c:
+        for concurrent in (False, True):
+            with solver.Solver(prob, variant=3, use_graph=use_graph, concurrent=concurrent) as s:
+                s.attach_closure(c)
+                s.run(90)
+                out.append((s.fetch(), s.closure_inspect(prob.C)[2]))
+    assert np.any(out[0][1][:, :4] != 0)
+    assert np.array_equal(out[0][1], out[1][1])
+    for k in ("u", "v", "a"):
+        assert np.array_equal(out[0][0][k], out[1][0][k]), k
+
+
 def test_wide_kernel_gives_the_same_bits(monkeypatch):
-    """The 128-row forward kernel (large row counts) and the 32-row kernel apply the same bits."""
+    """The wide forward kernels for large row counts (256 rows per workgroup of 8 wavefronts, 128
+    rows per workgroup of 4) and the 32-row kernel apply the same bits."""
     say("closure wide vs narrow kernel")
     prob = with_wide_pulse(load_golden("abc_r10_nu025")[0])
+    assert prob.C > 256  # the 256-row kernel's blocks: one full, one partial
     c, _ = closure_with(24, 0.0)
     c.set_params(random_params(25, out_scale=0.05))
     c.set_scales(1e-3, 1e-3)
     out = []
     with c:
-        for wide_min in ("0", "1000000000"):
+        for wide_min, waves in (("0", "8"), ("0", "4"), ("1000000000", "8")):
             monkeypatch.setenv("LPS_CLOSURE_WIDE_MIN", wide_min)
+            monkeypatch.setenv("LPS_CLOSURE_WIDE_WAVES", waves)
             with solver.Solver(prob, variant=3) as s:
                 s.attach_closure(c)
                 s.run(60)
                 f = s.fetch()
                 out.append((f, s.closure_inspect(prob.C)[2]))
     assert np.any(out[0][1][:, :4] != 0)
-    assert np.array_equal(out[0][1], out[1][1])
-    for k in ("u", "v", "a"):
-        assert np.array_equal(out[0][0][k], out[1][0][k]), k
+    for q in (1, 2):
+        assert np.array_equal(out[0][1], out[q][1]), q
+        for k in ("u", "v", "a"):
+            assert np.array_equal(out[0][0][k], out[q][0][k]), (q, k)
 
 
 def test_closed_loop_samples_target_the_way_back_to_truth():

@@ -161,6 +161,56 @@ def test_tile_path_matches_oracle_on_a_larger_domain(variant, r40_problem):
                         ENERGY_KEYS, 1e-10, prob, "R40")
 
 
+@pytest.mark.parametrize("exact", [False, True])
+def test_members_that_add_nothing_change_no_bit(exact):
+    """The ABC rows are ragged: the tile path drops each cloud's trailing members that add nothing
+    (pads, -1, and nodes whose G_u, G_v and G_theta coefficients are all zero, as the reference's
+    reader turns -1 into node 0). A problem with seven such members appended to every cloud runs
+    to the same bits as the problem itself."""
+    say("padded clouds", exact)
+    import dataclasses
+    prob, _ = load_golden("abc_r12_nu033")
+    C = prob.C
+    extra = np.array([-1, 0, -1, 0, 0, -1, 0] * C).reshape(C, 7)
+    pad = lambda G, w: np.concatenate([G, np.zeros((G.shape[0], w * 7))], axis=1)
+    big = dataclasses.replace(prob, clouds=np.concatenate([prob.clouds, extra], axis=1),
+                              Gu=pad(prob.Gu, 4), Gv=pad(prob.Gv, 4), Gth=pad(prob.Gth, 2))
+    out = []
+    for p in (prob, big):
+        with solver.Solver(p, exact=exact, variant=0 if exact else 3) as s:
+            recs = s.run(150)
+            out.append((s.fetch(), recs))
+    for k in ("u", "v", "a", "theta"):
+        assert np.array_equal(out[0][0][k], out[1][0][k]), k
+    assert out[0][1] == out[1][1]
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_abc_prediction_folded_or_apart_gives_the_same_bits(graph, monkeypatch, r40_problem):
+    """The ABC nodes' prediction folded into the partial tiles' force pass, or a kernel of its own
+    after G_u / G_v with the partial tiles beside them (the autotuner times both): the same bits,
+    on one domain and on a batch."""
+    say("prediction fold/apart", graph)
+    g12, _ = load_golden("abc_r12_nu033")
+    out = []
+    for mode in ("fold", "apart"):
+        monkeypatch.setenv("LPS_ABC_PREDICT", mode)
+        with solver.Solver(r40_problem, variant=3, use_graph=graph) as s:
+            assert ("ABC prediction " + ("in" if mode == "fold" else "apart")) in s.path, s.path
+            recs = s.run(95)
+            one = (s.fetch(), recs)
+        with solver.Batch([g12, r40_problem], variant=3, use_graph=graph) as b:
+            brecs = b.run(95)
+            many = ([b.fetch(i) for i in range(2)], brecs)
+        out.append((one, many))
+    (a1, am), (b1, bm) = out
+    for k in ("u", "v", "a", "theta"):
+        assert np.array_equal(a1[0][k], b1[0][k]), k
+        for i in range(2):
+            assert np.array_equal(am[0][i][k], bm[0][i][k]), (i, k)
+    assert a1[1] == b1[1] and am[1] == bm[1]
+
+
 @pytest.mark.parametrize("graph", [True, False])
 def test_timing_mode_reports_pass_times_and_changes_no_result(graph, r40_problem):
     """lps_set_timing: per-pass events (mode 1) or the force kernel's two events only (mode 2),
