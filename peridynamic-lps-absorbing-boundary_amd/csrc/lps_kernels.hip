@@ -840,7 +840,8 @@ __device__ __forceinline__ void rb_bonds(S &s, std::integer_sequence<int, R...>)
 // has.
 __device__ __forceinline__ void pin(double &v) { asm volatile("" : "+v"(v)); }
 
-// Full-tile dilatation: theta_i = sum_k C c1_k (u_j . o_k).
+// Full-tile dilatation: theta_i = sum_k C c1_k (u_j . o_k). W: the window's row stride in LDS.
+template <int W = kW68>
 struct ThetaFull {
   const double2 *su;
   int32_t c0;
@@ -849,7 +850,7 @@ struct ThetaFull {
   double2 u;
   template <int D, int OY>
   __device__ __forceinline__ void cell() {
-    u = su[c0 + D * kW68 + OY];
+    u = su[c0 + D * W + OY];
     rb_bonds<D, OY>(*this, std::make_integer_sequence<int, kRB>{});
   }
   template <int OX, int OY, int R>
@@ -873,7 +874,9 @@ struct ThetaFull {
 
 // Full-tile force: F_i = sum_k (lambda - mu) C c1_k theta_j o_k + 8 mu C c2_k ((u_j - u_i) . o_k) o_k.
 // Weights wt (theta_j terms, theta scaled by the instance's th_scale in the window) and wb
-// (bond-extension terms); the sum is scaled by the instance's f_scale.
+// (bond-extension terms); the sum is scaled by the instance's f_scale. W: the window's row stride
+// in LDS.
+template <int W = kW68>
 struct ForceFull {
   const double2 *su;
   const double *sth;
@@ -885,8 +888,8 @@ struct ForceFull {
   double th;
   template <int D, int OY>
   __device__ __forceinline__ void cell() {
-    u = su[c0 + D * kW68 + OY];
-    th = sth[c0 + D * kW68 + OY];
+    u = su[c0 + D * W + OY];
+    th = sth[c0 + D * W + OY];
     rb_bonds<D, OY>(*this, std::make_integer_sequence<int, kRB>{});
   }
   template <int OX, int OY, int R>
@@ -930,7 +933,7 @@ __device__ __forceinline__ void theta_part_quarter(const StepArgs &a, const Step
                                                    double2 *su, int32_t t, int32_t b) {
   const TileArgs &T = a.tile;
   const int32_t *nbr = T.tile_nbr + 9 * t;
-  const int32_t lane = threadIdx.x;
+  const int32_t lane = threadIdx.x & 63;
   int8_t *ok = reinterpret_cast<int8_t *>(su + 12 * kW68);
   for (int32_t c = lane; c < 12 * kW68; c += 64) {
     const int32_t s = window_slot68(nbr, 4 * b * kW68 + c);
@@ -982,7 +985,81 @@ __global__ void __launch_bounds__(64, 4) k_theta_rb(StepArgs a, StepState st) {
   fill_window68(nb, lane, [&](WinCell w) { su[w.cell] = make_double2(st.px[w.slot], st.py[w.slot]); },
                 std::make_integer_sequence<int, 9>{});
   __syncthreads();
-  ThetaFull P{su, (lx0 + 4) * kW68 + ly + 4, T, {}, {}};
+  ThetaFull<> P{su, (lx0 + 4) * kW68 + ly + 4, T, {}, {}};
+#pragma unroll
+  for (int r = 0; r < kRB; r++) P.acc[r] = 0.0;
+  rb_sweep(P, std::make_integer_sequence<int, kRB + 8>{});
+#pragma unroll
+  for (int r = 0; r < kRB; r++)
+    if (kd[r] == 1) st.theta[t * (kTile * kTile) + (lx0 + r) * kTile + ly] = P.acc[r];
+}
+
+// LDS of theta_part_quarter: 12 x 24 cells of u, then their masks as bytes (in double2 units).
+constexpr int kQuarterCells = 12 * kW68 + (12 * kW68 + 15) / 16;
+constexpr int kGW = 2 * kTile + 8;  // 40: the window edge of a 2 x 2 group of tiles (halo 4)
+
+// Loads group g's neighbourhood table into snb and returns the wavefront's tile (w / ny, w % ny)
+// of the group, or -1 when the group has fewer tiles than the wavefront's index.
+__device__ __forceinline__ int32_t group_tile(const TileArgs &T, int32_t g, int32_t *snb,
+                                              int32_t &nx, int32_t &ny, int32_t &qx, int32_t &qy) {
+  const int32_t *G = T.groups + (int64_t)kGroupInts * g;
+  nx = G[16];
+  ny = G[17];
+  if (threadIdx.x < 16) snb[threadIdx.x] = G[threadIdx.x];
+  const int32_t w = threadIdx.x >> 6;
+  qx = ny == 2 ? (w >> 1) : w;
+  qy = ny == 2 ? (w & 1) : 0;
+  return w < nx * ny ? G[(qx + 1) * 4 + qy + 1] : -1;
+}
+
+// Stages group window cells into LDS (row stride kGW) with the 256 threads of the block: cell
+// (wx, wy) is the point (wx - 4, wy - 4) from the anchor's first node; with 16 added, (v >> 4)
+// picks the neighbourhood tile and (v & 15) the row or column inside it. f(cell, slot).
+template <class F>
+__device__ __forceinline__ void fill_group_window(const int32_t *snb, int32_t nx, int32_t ny, F &&f) {
+  const int32_t cols = kTile * ny + 8, ncell = (kTile * nx + 8) * cols;
+#pragma unroll
+  for (int k = 0; k < (kGW * kGW + kBlock - 1) / kBlock; k++) {
+    const int32_t c = threadIdx.x + kBlock * k;
+    if (c < ncell) {
+      const int32_t wx = ny == 2 ? c / kGW : c / kW68;
+      const int32_t wy = c - wx * cols;
+      const int32_t gx = wx + 12, gy = wy + 12;
+      f(wx * kGW + wy, snb[(gx >> 4) * 4 + (gy >> 4)] * (kTile * kTile) + (gx & 15) * kTile + (gy & 15));
+    }
+  }
+}
+
+// Dilatation of the full tiles by groups of up to 2 x 2 tiles (TileArgs::groups), one workgroup of
+// 4 wavefronts per group sharing the group's window (25.6 KB of LDS for 1,024 nodes, against
+// 9.2 KB for 256 with a window per tile); blocks past the groups' (rounded up to a multiple of 8)
+// take the partial tiles, one per block, a quarter per wavefront.
+__global__ void __launch_bounds__(kBlock, 4) k_theta_grp(StepArgs a, StepState st) {
+  __shared__ double2 su[kGW * kGW];
+  __shared__ int32_t snb[16];
+  static_assert(4 * kQuarterCells <= kGW * kGW, "partial-tile quarters fit the group window");
+  const TileArgs &T = a.tile;
+  const int32_t ngb = (T.n_groups + 7) / 8 * 8;
+  if ((int32_t)blockIdx.x >= ngb) {
+    const int32_t p = blockIdx.x - ngb;
+    if (p < T.n_part)
+      theta_part_quarter(a, st, su + (threadIdx.x >> 6) * kQuarterCells, T.part_list[p], threadIdx.x >> 6);
+    return;
+  }
+  const int32_t g = tile_of_block(blockIdx.x, T.n_groups);
+  if (g >= T.n_groups) return;
+  int32_t nx, ny, qx, qy;
+  const int32_t t = group_tile(T, g, snb, nx, ny, qx, qy);
+  const int32_t lane = threadIdx.x & 63;
+  const int32_t ly = lane & 15, lx0 = (lane >> 4) * kRB;
+  int8_t kd[kRB];
+#pragma unroll
+  for (int r = 0; r < kRB; r++) kd[r] = t >= 0 ? a.kind[t * (kTile * kTile) + (lx0 + r) * kTile + ly] : 0;
+  __syncthreads();
+  fill_group_window(snb, nx, ny, [&](int32_t c, int32_t s) { su[c] = make_double2(st.px[s], st.py[s]); });
+  __syncthreads();
+  if (t < 0) return;
+  ThetaFull<kGW> P{su, (kTile * qx + lx0 + 4) * kGW + kTile * qy + ly + 4, T, {}, {}};
 #pragma unroll
   for (int r = 0; r < kRB; r++) P.acc[r] = 0.0;
   rb_sweep(P, std::make_integer_sequence<int, kRB + 8>{});
@@ -1022,10 +1099,64 @@ __global__ void __launch_bounds__(64, 3) k_force_rb(StepArgs a, StepState st) {
   }, std::make_integer_sequence<int, 9>{});
   __syncthreads();
   const int32_t c0 = (lx0 + 4) * kW68 + ly + 4;
-  ForceFull P{su, sth, c0, T, {}, {}, {}, {}, 0.0};
+  ForceFull<> P{su, sth, c0, T, {}, {}, {}, {}, 0.0};
 #pragma unroll
   for (int r = 0; r < kRB; r++) {
     P.ui[r] = su[c0 + r * kW68];
+    P.fx[r] = 0.0;
+    P.fy[r] = 0.0;
+  }
+  rb_sweep(P, std::make_integer_sequence<int, kRB + 8>{});
+  const InstParams &I = a.inst[T.tile_inst[t]];
+  const double scale = I.f_scale;
+#pragma unroll
+  for (int r = 0; r < kRB; r++) {
+    const int32_t i = t * (kTile * kTile) + (lx0 + r) * kTile + ly;
+    if (kd[r] != 0)
+      node_update(I, st, i, kd[r], P.ui[r], vi[r], ai[r], scale * P.fx[r], scale * P.fy[r]);
+  }
+}
+
+// Force pass of the full tiles by groups of up to 2 x 2 tiles (TileArgs::groups), with the fused
+// Verlet update: one workgroup of 4 wavefronts per group, wavefront w on the group's tile
+// (w / ny, w % ny). The group's (16 nx + 8) x (16 ny + 8) window is staged once in LDS (row stride
+// kGW) and read by all of its wavefronts with the sweep of k_force_rb. A 2 x 2 group stages 1,600
+// cells for 1,024 nodes, where four single-tile windows stage 2,304, and its 38.4 KB of LDS lets
+// 16 wavefronts share a CU, against 11 with a 13.8 KB window per wavefront.
+__global__ void __launch_bounds__(kBlock, 4) k_force_grp(StepArgs a, StepState st) {
+  __shared__ double2 su[kGW * kGW];
+  __shared__ double sth[kGW * kGW];
+  __shared__ int32_t snb[16];
+  const TileArgs &T = a.tile;
+  const int32_t g = tile_of_block(blockIdx.x, T.n_groups);
+  if (g >= T.n_groups) return;
+  int32_t nx, ny, qx, qy;
+  const int32_t t = group_tile(T, g, snb, nx, ny, qx, qy);
+  const int32_t lane = threadIdx.x & 63;
+  const int32_t ly = lane & 15, lx0 = (lane >> 4) * kRB;
+  // the epilogue's inputs first: their loads overlap the window's
+  int8_t kd[kRB];
+  double2 vi[kRB], ai[kRB];
+#pragma unroll
+  for (int r = 0; r < kRB; r++) {
+    const int32_t i = (t >= 0 ? t : 0) * (kTile * kTile) + (lx0 + r) * kTile + ly;
+    kd[r] = t >= 0 ? a.kind[i] : 0;
+    vi[r] = t >= 0 ? make_double2(st.vx[i], st.vy[i]) : make_double2(0.0, 0.0);
+    ai[r] = t >= 0 ? make_double2(st.ax[i], st.ay[i]) : make_double2(0.0, 0.0);
+  }
+  const double ratio = a.inst[T.tile_inst[T.groups[(int64_t)kGroupInts * g + 5]]].th_scale;  // anchor
+  __syncthreads();
+  fill_group_window(snb, nx, ny, [&](int32_t c, int32_t s) {
+    su[c] = make_double2(st.px[s], st.py[s]);
+    sth[c] = ratio * st.theta[s];
+  });
+  __syncthreads();
+  if (t < 0) return;
+  const int32_t c0 = (kTile * qx + lx0 + 4) * kGW + kTile * qy + ly + 4;
+  ForceFull<kGW> P{su, sth, c0, T, {}, {}, {}, {}, 0.0};
+#pragma unroll
+  for (int r = 0; r < kRB; r++) {
+    P.ui[r] = su[c0 + r * kGW];
     P.fx[r] = 0.0;
     P.fy[r] = 0.0;
   }
@@ -1482,7 +1613,12 @@ hipError_t launch_theta(const StepArgs &a, const StepState &st, bool fast, hipSt
                         bool with_part) {
   if (fast) {
     const TileArgs &T = a.tile;
-    if (T.theta_kernel == kBlocked) {
+    if (T.theta_kernel == kBlocked && (T.grouped & 2) && T.n_groups > 0) {
+      // the partial tiles in the same launch (a block each) unless another stream takes them
+      const int32_t nb = list_blocks(T.n_groups) + (with_part ? T.n_part : 0);
+      hipLaunchKernelGGL(k_theta_grp, dim3(nb), dim3(kBlock), 0, s, a, st);
+      return hipGetLastError();
+    } else if (T.theta_kernel == kBlocked) {
       // the partial tiles in the same launch (4 blocks each) unless another stream takes them
       const int32_t nb = list_blocks(T.n_full) + (with_part ? 4 * T.n_part : 0);
       if (nb > 0) hipLaunchKernelGGL(k_theta_rb, dim3(nb), dim3(64), 0, s, a, st);
@@ -1515,7 +1651,9 @@ hipError_t launch_idw(const StepArgs &a, const StepState &st, hipStream_t s) {
 hipError_t launch_force(const StepArgs &a, const StepState &st, bool fast, hipStream_t s) {
   if (fast) {
     const int32_t nf = a.tile.n_full;
-    if (a.tile.force_kernel == kBlocked) {
+    if (a.tile.force_kernel == kBlocked && (a.tile.grouped & 1) && a.tile.n_groups > 0) {
+      hipLaunchKernelGGL(k_force_grp, dim3(list_blocks(a.tile.n_groups)), dim3(kBlock), 0, s, a, st);
+    } else if (a.tile.force_kernel == kBlocked) {
       if (nf > 0) hipLaunchKernelGGL(k_force_rb, dim3(list_blocks(nf)), dim3(64), 0, s, a, st);
     } else if (a.tile.force_kernel == kUnrolled) {
       if (nf > 0)
