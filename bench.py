@@ -87,8 +87,6 @@ def force_roofline(path, ti, n_nodes, force_ms, workload, passes=None):
     fk = path.split("force ")[1].split(" ")[0].rstrip(",") if "force " in path else "exact"
     kernel = {"blocked": "k_force_rb", "unrolled": "k_force_t68<true>",
               "generic": "k_force_tile", "exact": "k_force_exact"}.get(fk, fk)
-    if kernel == "k_force_rb" and "force of the full tiles by groups" in path:
-        kernel = "k_force_grp"  # the full tiles by groups of up to 2 x 2 tiles
     units = ti["full_tile_nodes"] if fk in ("blocked", "unrolled") else n_nodes
     per_unit = 121 if fk == "exact" else 105
     force_bytes = per_unit * units

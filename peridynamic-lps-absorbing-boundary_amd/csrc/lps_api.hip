@@ -91,76 +91,12 @@ static bool lattice_coords(const lps_problem *p, std::vector<int32_t> &ix, std::
   return true;
 }
 
-// Groups of full tiles for the grouped force kernel (TileArgs::groups), from the 3 x 3 neighbour
-// table nbr (tile t's neighbour (ax, ay) at 9 t + (ax + 1) * 3 + ay + 1), the full flags and the
-// tile coordinates (tx << 16 | ty). The tile grid is cut into aligned 2 x 2 blocks: a block whose
-// four tiles are full is one group; otherwise its full tiles pair up along y, then along x, and the
-// rest stand alone. A group is kept only if every tile its window touches exists (always true: the
-// window of a group is the union of its full tiles' windows, every cell of which is a node).
-// Groups are listed in the order of their first tile, so each XCD's contiguous share of them is a
-// spatial band.
-static std::vector<int32_t> build_groups(const std::vector<int32_t> &nbr, const std::vector<int8_t> &full,
-                                         const std::vector<int32_t> &txy) {
-  const int32_t nt = (int32_t)full.size();
-  auto step = [&](int32_t t, int32_t ax, int32_t ay) { return t < 0 ? -1 : nbr[(size_t)t * 9 + (ax + 1) * 3 + ay + 1]; };
-  auto at = [&](int32_t t, int32_t i, int32_t j) {  // the tile i along x, j along y from t
-    for (int32_t k = 0; k < (i < 0 ? -i : i); k++) t = step(t, i < 0 ? -1 : 1, 0);
-    for (int32_t k = 0; k < (j < 0 ? -j : j); k++) t = step(t, 0, j < 0 ? -1 : 1);
-    return t;
-  };
-  std::vector<int8_t> used(nt, 0);
-  std::vector<int32_t> out;
-  auto add = [&](int32_t anchor, int32_t nx, int32_t ny) {
-    int32_t e[kGroupInts];
-    for (int32_t i = -1; i <= 2; i++)
-      for (int32_t j = -1; j <= 2; j++) {
-        const bool need = i <= nx && j <= ny;
-        e[(i + 1) * 4 + j + 1] = need ? at(anchor, i, j) : -1;
-        if (need && e[(i + 1) * 4 + j + 1] < 0) return false;
-      }
-    e[16] = nx;
-    e[17] = ny;
-    out.insert(out.end(), e, e + kGroupInts);
-    for (int32_t i = 0; i < nx; i++)
-      for (int32_t j = 0; j < ny; j++) used[at(anchor, i, j)] = 1;
-    return true;
-  };
-  for (int32_t t = 0; t < nt; t++) {
-    if (!full[t] || used[t]) continue;
-    const int32_t tx = txy[t] >> 16, ty = txy[t] & 0xffff;
-    int32_t m[2][2];  // the block's tiles, direct neighbours of t (or t)
-    for (int32_t i = 0; i < 2; i++)
-      for (int32_t j = 0; j < 2; j++) {
-        const int32_t q = step(t, (tx & ~1) + i - tx, (ty & ~1) + j - ty);
-        m[i][j] = (q >= 0 && full[q] && !used[q]) ? q : -1;
-      }
-    if (m[0][0] >= 0 && m[0][1] >= 0 && m[1][0] >= 0 && m[1][1] >= 0 && add(m[0][0], 2, 2)) continue;
-    for (int32_t i = 0; i < 2; i++)  // pairs along y
-      if (m[i][0] >= 0 && m[i][1] >= 0 && add(m[i][0], 1, 2)) m[i][0] = m[i][1] = -1;
-    for (int32_t j = 0; j < 2; j++)  // pairs along x
-      if (m[0][j] >= 0 && m[1][j] >= 0 && add(m[0][j], 2, 1)) m[0][j] = m[1][j] = -1;
-    for (int32_t i = 0; i < 2; i++)
-      for (int32_t j = 0; j < 2; j++)
-        if (m[i][j] >= 0) add(m[i][j], 1, 1);
-  }
-  return out;
-}
-
-static void count_group_shapes(const std::vector<int32_t> &grp, int32_t *shapes) {
-  shapes[0] = shapes[1] = shapes[2] = 0;
-  for (size_t g = 0; g + kGroupInts <= grp.size(); g += kGroupInts) {
-    const int32_t n = grp[g + 16] * grp[g + 17];
-    shapes[n == 4 ? 0 : (n == 2 ? 1 : 2)]++;
-  }
-}
-
 extern "C" {
 
 static int32_t reset_state(lps_ctx *c);
 static int32_t flush_events(lps_ctx *c);
 static int32_t choose_variant(lps_ctx *c);
 static int32_t choose_abc_predict(lps_ctx *c, bool tune);
-static int32_t choose_force_groups(lps_ctx *c, bool tune);
 static void drop_graphs(lps_ctx *c);
 
 void lps_default_options(lps_options *opt) {
@@ -605,6 +541,22 @@ static int32_t create_impl(lps_ctx *c, const lps_problem *p) {
         for (int32_t z = 0; z < 2; z++) c->tile.wt[q][z] = C * c->tile.c1[q] * ms[z];
         for (int32_t z = 0; z < 3; z++) c->tile.wb[q][z] = 8.0 * C * c->tile.c2[q] * ps[z];
       }
+      // sums of the bond-extension weights of o_x^2 and of o_y^2 over the 68 offsets
+      c->tile.wbsum[0] = c->tile.wbsum[1] = 0.0;
+      for (int32_t ox = -4; ox <= 4; ox++)
+        for (int32_t oy = -4; oy <= 4; oy++) {
+          const int32_t r2 = ox * ox + oy * oy;
+          if (r2 == 0 || r2 > 20) continue;
+          int32_t q = 0;
+          while (kStencilR2[q] != r2) q++;
+          int32_t ra = 0, rb = 0;
+          for (int32_t aa = 0; aa <= 4; aa++)
+            for (int32_t bb = 0; bb <= aa; bb++)
+              if (aa * aa + bb * bb == r2) { ra = aa; rb = bb; }
+          auto slot = [&](int32_t p2) { return p2 == ra * ra ? 0 : (p2 == ra * rb ? 1 : 2); };
+          c->tile.wbsum[0] += c->tile.wb[q][slot(ox * ox)];
+          c->tile.wbsum[1] += c->tile.wb[q][slot(oy * oy)];
+        }
     }
     std::vector<int32_t> tinst(c->tile.n_tiles, 0);
     int32_t *d_tinst;
@@ -628,14 +580,6 @@ static int32_t create_impl(lps_ctx *c, const lps_problem *p) {
     c->tile.part_list = d_pl;
     c->tile.n_full = (int32_t)full_list.size();
     c->tile.n_part = (int32_t)part_list.size();
-    {
-      const std::vector<int32_t> grp = build_groups(nbr, full, tile_xy);
-      int32_t *d_grp = nullptr;
-      if (!grp.empty()) CTX_TRY(c, c->upload(&d_grp, grp));
-      c->tile.groups = d_grp;
-      c->tile.n_groups = (int32_t)(grp.size() / kGroupInts);
-      count_group_shapes(grp, c->group_shapes);
-    }
     c->abc_in_part = true;  // every ABC node in a partial tile (its horizon reaches past the rim)
     for (int32_t r = 0; r < c->n_abc; r++)
       if (full[abc_slot[r] / (kTile * kTile)]) c->abc_in_part = false;
@@ -1110,14 +1054,6 @@ static int32_t merge_batch(lps_ctx *c, const std::vector<lps_ctx *> &sub) {
   CTX_TRY(c, c->upload(&d_full, tfull));
   T.tile_nbr = d_nbr; T.full_list = d_fl; T.part_list = d_pl; T.tile_inst = d_ti;
   T.tile_xy = d_txy; T.tile_full = d_full;
-  {
-    const std::vector<int32_t> grp = build_groups(nbr, tfull, txy);
-    int32_t *d_grp = nullptr;
-    if (!grp.empty()) CTX_TRY(c, c->upload(&d_grp, grp));
-    T.groups = d_grp;
-    T.n_groups = (int32_t)(grp.size() / kGroupInts);
-    count_group_shapes(grp, c->group_shapes);
-  }
   {  // the stencil tables (the same for every instance)
     const size_t nt = (size_t)s0.tile.nt;
     int32_t *d_delta;
@@ -1304,7 +1240,6 @@ int32_t lps_create_batch(const lps_problem *problems, int32_t count, const lps_o
   }
   if (rc == LPS_OK) rc = merge_batch(c, sub);
   for (lps_ctx *q : sub) lps_destroy(q);
-  if (rc == LPS_OK) rc = choose_force_groups(c, opt.variant == 0);
   if (rc == LPS_OK) rc = choose_abc_predict(c, opt.variant == 0);
   if (rc != LPS_OK) {
     g_create_error = c->err;
@@ -1316,63 +1251,6 @@ int32_t lps_create_batch(const lps_problem *problems, int32_t count, const lps_o
 }
 
 int32_t lps_instance_count(const lps_ctx *ctx) { return ctx->n_inst; }
-
-// Register-blocked kernels: the full tiles one per wavefront (k_theta_rb, k_force_rb, a 24 x 24
-// window each) or by groups of up to 2 x 2 tiles, four wavefronts sharing one window (k_theta_grp,
-// k_force_grp), for either pass (TileArgs::grouped: bit 0 force, bit 1 dilatation). The same
-// arithmetic in the same order either way, so the same bits. With tune, every allowed combination
-// runs two rounds of 3 warm-up and 10 timed steps and the fastest by the whole step is kept;
-// LPS_FORCE_GROUPS=on, =off, =force or =theta sets the choice.
-static int32_t choose_force_groups(lps_ctx *c, bool tune) {
-  int32_t allowed = 0;
-  if (c->fast && c->tile.n_groups > 0) {
-    if (c->tile.force_kernel == kBlocked) allowed |= 1;
-    if (c->tile.theta_kernel == kBlocked) allowed |= 2;
-  }
-  c->tile.grouped = 0;
-  if (!allowed) return LPS_OK;
-  const char *env = std::getenv("LPS_FORCE_GROUPS");
-  const char *names[4] = {"off", "force", "theta", "on"};
-  int32_t forced = -1;
-  for (int32_t q = 0; q < 4 && env; q++)
-    if (!strcmp(env, names[q])) forced = q;
-  if (forced >= 0) {
-    c->tile.grouped = forced & allowed;
-  } else if (tune) {
-    double tf[4] = {1e300, 1e300, 1e300, 1e300};
-    for (int round = 0; round < 2; round++) {
-      for (int32_t q = 0; q < 4; q++) {
-        if (q & ~allowed) continue;
-        double t[4] = {0, 0, 0, 0};
-        c->tile.grouped = q;
-        drop_graphs(c);
-        int32_t rc = lps_set_timing(c, 1);
-        if (rc == LPS_OK) rc = lps_step(c, 3);
-        if (rc == LPS_OK) rc = lps_set_timing(c, 1);
-        if (rc == LPS_OK) rc = lps_step(c, 10);
-        if (rc == LPS_OK) rc = lps_pass_times(c, t);
-        if (rc == LPS_OK) rc = lps_set_timing(c, 0);
-        if (rc == LPS_OK) rc = reset_state(c);
-        if (rc != LPS_OK) return rc;
-        tf[q] = std::min(tf[q], t[3]);
-      }
-    }
-    int32_t best = 0;
-    for (int32_t q = 1; q < 4; q++)
-      if (tf[q] < tf[best]) best = q;
-    c->tile.grouped = best;
-  } else {
-    return LPS_OK;
-  }
-  drop_graphs(c);
-  if (c->tile.grouped) {
-    const char *which[4] = {"", "force", "dilatation", "dilatation and force"};
-    c->path += std::string("; ") + which[c->tile.grouped] + " of the full tiles by groups of tiles (" +
-               std::to_string(c->group_shapes[0]) + " of 2 x 2, " + std::to_string(c->group_shapes[1]) +
-               " of 2, " + std::to_string(c->group_shapes[2]) + " single)";
-  }
-  return LPS_OK;
-}
 
 // Fast path with ABC rows on their own stream: the ABC nodes' prediction folded into the partial
 // tiles' force pass (which then waits for G_u / G_v) or a kernel of its own after G_u / G_v (the
@@ -1455,8 +1333,6 @@ static int32_t choose_variant(lps_ctx *c) {
   const char *names[3] = {"generic", "unrolled", "blocked"};
   c->path = std::string("tile kernels: dilatation ") + names[th] + ", force " + names[fo] +
             (tune ? " [autotuned]" : "") + c->path;
-  const int32_t rc = choose_force_groups(c, tune);
-  if (rc != LPS_OK) return rc;
   return choose_abc_predict(c, tune);
 }
 

@@ -46,7 +46,6 @@ struct lps_ctx {
   // G_u / G_v); else a kernel of its own after G_u / G_v, the partial tiles beside them
   bool fold_pred = true;
   int64_t n_full_nodes = 0;   // nodes in the full tiles
-  int32_t group_shapes[3] = {0, 0, 0};  // groups of full tiles: 2 x 2, two tiles, one tile
   // instances: one for lps_create, several for lps_create_batch
   std::vector<InstParams> inst_h;
   InstParams *inst_d = nullptr;

@@ -68,17 +68,9 @@ struct TileArgs {
   // arithmetic is the same alone or in a batch. These tables are kernel arguments; the kernel
   // argument segment is kept small (1.7 KB instead of 1.4 KB made the force kernel 35% slower).
   double wt[12][2], wb[12][3];
+  double wbsum[2];                 // sum over the stencil of wb of o_x^2 and of o_y^2
   const int32_t *tile_inst;        // n_tiles: the instance (domain) a tile belongs to
-  // Groups of full tiles (1 x 1, 1 x 2, 2 x 1 or 2 x 2) for the grouped force kernel, which stages
-  // one LDS window per group: group g is kGroupInts int32 at groups + kGroupInts g, the 4 x 4
-  // tiles around its anchor tile (entry (i + 1) * 4 + j + 1 for the tile i tiles along x and j
-  // along y from the anchor, i, j in -1 .. 2; -1 where the group's window does not reach), then
-  // its extent nx, ny in tiles. grouped: bit 0, the force pass of the full tiles runs on the
-  // groups; bit 1, the dilatation pass.
-  const int32_t *groups;
-  int32_t n_groups, grouped;
 };
-constexpr int kGroupInts = 18;
 
 // Per instance (one domain of a batch; a single problem is a batch of one): time step, material,
 // and the factors of the full tiles' force with the tables of TileArgs:

@@ -804,119 +804,180 @@ constexpr bool in_stencil68(int ox, int oy) {
   return ox * ox + oy * oy > 0 && ox * ox + oy * oy <= 20;
 }
 
-// The sweep: rows D = -4 .. kRB + 3 (window row lx0 + 4 + D), columns OY = -4 .. 4 (window column
-// ly + 4 + OY). A cell is read once if a node r of the lane has the bond (D - r, OY); the pass
-// object S reads it (S::cell<D, OY>) and applies its bonds (S::bond<D - r, OY, r>). At the end of
-// each row S::end_row pins the accumulators.
-template <class S, int D, int OY, int... R>
-__device__ __forceinline__ void rb_cell(S &s, std::integer_sequence<int, R...>) {
-  if constexpr ((in_stencil68(D - R, OY) || ...)) s.template cell<D, OY>();
-}
-
-template <class S, int D, int... J>
-__device__ __forceinline__ void rb_row(S &s, std::integer_sequence<int, J...>) {
-  (rb_cell<S, D, J - 4>(s, std::make_integer_sequence<int, kRB>{}), ...);
-  s.end_row();
-}
-
-template <class S, int... E>
-__device__ __forceinline__ void rb_sweep(S &s, std::integer_sequence<int, E...>) {
-  (rb_row<S, E - 4>(s, std::make_integer_sequence<int, 9>{}), ...);
-}
-
-template <int OX, int OY, int R, class S>
-__device__ __forceinline__ void rb_bond_if(S &s) {
-  if constexpr (in_stencil68(OX, OY)) s.template bond<OX, OY, R>();
-}
-
-// Applies bond<D - r, OY, r> for every node r of the lane that has that bond.
-template <int D, int OY, class S, int... R>
-__device__ __forceinline__ void rb_bonds(S &s, std::integer_sequence<int, R...>) {
-  (rb_bond_if<D - R, OY, R>(s), ...);
-}
-
 // Keep each row's reads next to its bonds (the accumulators are complete at the end of the row):
 // hoisting every read of the sweep ahead of the arithmetic needs more registers than a wavefront
 // has.
 __device__ __forceinline__ void pin(double &v) { asm volatile("" : "+v"(v)); }
 
-// Full-tile dilatation: theta_i = sum_k C c1_k (u_j . o_k). W: the window's row stride in LDS.
-template <int W = kW68>
-struct ThetaFull {
+// ---------------------------------------------------------------------------------------------
+// Row-symmetric sweeps. On a full tile every term is linear in the window values, with a weight
+// that is even or odd in each component of the offset. The cells (D, +oy) and (D, -oy) of a sweep
+// row therefore enter every node of the lane through their sum or their difference: a row forms
+// these once for the lane's kRB nodes, and each node then takes one FMA per offset (ox, |oy|) and
+// term, instead of one per bond and term.
+//   dilatation:  x part on the sums of u_x, y part on the differences of u_y;
+//   force, x:    C c1 ox theta_j on the sums of theta, B_xx on the sums of ubar_x, B_xy on the
+//                differences of ubar_y;
+//   force, y:    C c1 oy theta_j on the differences of theta, B_yy on the sums of ubar_y, B_xy on
+//                the differences of ubar_x;
+// with B_pq = 8 C c2 o_p o_q (the bond-extension weights wb). The bond extension (u_j - u_i) . o
+// enters as ubar_j - ubar_i, ubar = u - c with c the displacement of one node of the lane:
+//   sum_k B_k (u_j - u_i) = sum_k B_k ubar_j - (sum_k B_k) ubar_i,
+// the sums of B over the stencil being TileArgs::wbsum. c lies within 7 cells of every cell the
+// lane reads, so ubar is a difference of nearby values and the sum stays the size of a first
+// difference of u, as with per-bond differences, not the field's own size (whose cancellation in
+// the second difference would cost (sigma / dx)^2 in rounding).
+// Per node: 64 FMAs for the dilatation (120 per bond), 180 for the force (466 operations per bond),
+// plus per row of the lane 8 (dilatation) or 42 (force) operations shared by its kRB nodes.
+// ---------------------------------------------------------------------------------------------
+template <class F, int... I>
+__device__ __forceinline__ void static_for(F &&f, std::integer_sequence<int, I...>) {
+  (f(std::integral_constant<int, I>{}), ...);
+}
+
+// A cell (D, OY) of a sweep row is read when some node r of the lane has the bond (D - r, OY).
+constexpr bool row_cell_needed(int D, int OY) {
+  for (int r = 0; r < kRB; r++)
+    if (in_stencil68(D - r, OY)) return true;
+  return false;
+}
+
+// Class weights of the offset (OX, OY), OY >= 0.
+template <int OX, int OY>
+__device__ __forceinline__ double w_ax(const TileArgs &T) {  // C c1 ox
+  constexpr int cls = stencil_class(OX * OX + OY * OY);
+  constexpr int z = rep_slot(cls, iabs(OX));
+  return OX > 0 ? T.wt[cls][z] : -T.wt[cls][z];
+}
+template <int OX, int OY>
+__device__ __forceinline__ double w_ay(const TileArgs &T) {  // C c1 oy, OY > 0
+  constexpr int cls = stencil_class(OX * OX + OY * OY);
+  return T.wt[cls][rep_slot(cls, OY)];
+}
+template <int OX, int OY>
+__device__ __forceinline__ double w_bxx(const TileArgs &T) {  // 8 C c2 ox^2, OX != 0
+  constexpr int cls = stencil_class(OX * OX + OY * OY);
+  return T.wb[cls][prod_slot(cls, OX * OX)];
+}
+template <int OX, int OY>
+__device__ __forceinline__ double w_byy(const TileArgs &T) {  // 8 C c2 oy^2, OY > 0
+  constexpr int cls = stencil_class(OX * OX + OY * OY);
+  return T.wb[cls][prod_slot(cls, OY * OY)];
+}
+template <int OX, int OY>
+__device__ __forceinline__ double w_bxy(const TileArgs &T) {  // 8 C c2 ox oy, OX != 0, OY > 0
+  constexpr int cls = stencil_class(OX * OX + OY * OY);
+  constexpr int z = prod_slot(cls, iabs(OX) * OY);
+  return OX > 0 ? T.wb[cls][z] : -T.wb[cls][z];
+}
+
+// Full-tile dilatation, row-symmetric: theta_i = sum_k C c1_k (u_j . o_k); c0 is the window cell
+// of the lane's first node.
+struct ThetaSym {
   const double2 *su;
   int32_t c0;
   const TileArgs &T;
   double acc[kRB];
-  double2 u;
-  template <int D, int OY>
-  __device__ __forceinline__ void cell() {
-    u = su[c0 + D * W + OY];
-    rb_bonds<D, OY>(*this, std::make_integer_sequence<int, kRB>{});
-  }
-  template <int OX, int OY, int R>
-  __device__ __forceinline__ void bond() {
-    constexpr int cls = stencil_class(OX * OX + OY * OY);
-    if constexpr (OX != 0) {
-      constexpr int z = rep_slot(cls, iabs(OX));
-      acc[R] = fma(OX > 0 ? T.wt[cls][z] : -T.wt[cls][z], u.x, acc[R]);
-    }
-    if constexpr (OY != 0) {
-      constexpr int z = rep_slot(cls, iabs(OY));
-      acc[R] = fma(OY > 0 ? T.wt[cls][z] : -T.wt[cls][z], u.y, acc[R]);
-    }
-  }
-  __device__ __forceinline__ void end_row() {
+  template <int D>
+  __device__ __forceinline__ void row() {
+    double2 u[9];
+    static_for([&](auto k) {
+      constexpr int OY = decltype(k)::value - 4;
+      if constexpr (row_cell_needed(D, OY)) u[OY + 4] = su[c0 + D * kW68 + OY];
+    }, std::make_integer_sequence<int, 9>{});
+    double sx[5], dy[5];
+    static_for([&](auto k) {
+      constexpr int OY = decltype(k)::value;
+      if constexpr (row_cell_needed(D, OY)) {
+        if constexpr (OY == 0) {
+          sx[0] = u[4].x;
+        } else {
+          sx[OY] = u[4 + OY].x + u[4 - OY].x;
+          dy[OY] = u[4 + OY].y - u[4 - OY].y;
+        }
+      }
+    }, std::make_integer_sequence<int, 5>{});
+    static_for([&](auto rr) {
+      constexpr int R = decltype(rr)::value, OX = D - R;
+      if constexpr (OX >= -4 && OX <= 4) {
+        static_for([&](auto yy) {
+          constexpr int OY = decltype(yy)::value;
+          if constexpr (in_stencil68(OX, OY)) {
+            if constexpr (OX != 0) acc[R] = fma(w_ax<OX, OY>(T), sx[OY], acc[R]);
+            if constexpr (OY != 0) acc[R] = fma(w_ay<OX, OY>(T), dy[OY], acc[R]);
+          }
+        }, std::make_integer_sequence<int, 5>{});
+      }
+    }, std::make_integer_sequence<int, kRB>{});
 #pragma unroll
     for (int r = 0; r < kRB; r++) pin(acc[r]);
     __builtin_amdgcn_sched_barrier(0);
   }
 };
 
-// Full-tile force: F_i = sum_k (lambda - mu) C c1_k theta_j o_k + 8 mu C c2_k ((u_j - u_i) . o_k) o_k.
-// Weights wt (theta_j terms, theta scaled by the instance's th_scale in the window) and wb
-// (bond-extension terms); the sum is scaled by the instance's f_scale. W: the window's row stride
-// in LDS.
-template <int W = kW68>
-struct ForceFull {
+// Full-tile force, row-symmetric (see above):
+//   F_i = sum_k (lambda - mu) C c1_k theta_j o_k + 8 mu C c2_k ((u_j - u_i) . o_k) o_k;
+// the theta_i part vanishes by the stencil's symmetry. The window holds theta scaled by the
+// instance's th_scale = (lambda - mu) / mu and the caller scales the sum by its f_scale = mu, so the
+// class tables hold no material. cu: the lane's reference displacement c; fx, fy start at
+// -(sum of B) ubar_i.
+struct ForceSym {
   const double2 *su;
   const double *sth;
   int32_t c0;
-  const TileArgs &T;    // the class tables (kernel arguments)
-  double2 ui[kRB];
+  const TileArgs &T;
+  double2 cu;
   double fx[kRB], fy[kRB];
-  double2 u;
-  double th;
-  template <int D, int OY>
-  __device__ __forceinline__ void cell() {
-    u = su[c0 + D * W + OY];
-    th = sth[c0 + D * W + OY];
-    rb_bonds<D, OY>(*this, std::make_integer_sequence<int, kRB>{});
-  }
-  template <int OX, int OY, int R>
-  __device__ __forceinline__ void bond() {
-    constexpr int cls = stencil_class(OX * OX + OY * OY);
-    const double dux = u.x - ui[R].x, duy = u.y - ui[R].y;
-    if constexpr (OX != 0) {
-      constexpr int za = rep_slot(cls, iabs(OX)), zb = prod_slot(cls, OX * OX);
-      const double wa = T.wt[cls][za];
-      fx[R] = fma(OX > 0 ? wa : -wa, th, fx[R]);
-      fx[R] = fma(T.wb[cls][zb], dux, fx[R]);
-    }
-    if constexpr (OY != 0) {
-      constexpr int za = rep_slot(cls, iabs(OY)), zb = prod_slot(cls, OY * OY);
-      const double wa = T.wt[cls][za];
-      fy[R] = fma(OY > 0 ? wa : -wa, th, fy[R]);
-      fy[R] = fma(T.wb[cls][zb], duy, fy[R]);
-    }
-    if constexpr (OX != 0 && OY != 0) {
-      constexpr int z = prod_slot(cls, iabs(OX * OY));
-      const double wb = T.wb[cls][z];
-      const double w = ((OX > 0) == (OY > 0)) ? wb : -wb;
-      fx[R] = fma(w, duy, fx[R]);
-      fy[R] = fma(w, dux, fy[R]);
-    }
-  }
-  __device__ __forceinline__ void end_row() {
+  template <int D>
+  __device__ __forceinline__ void row() {
+    double2 u[9];
+    double th[9];
+    static_for([&](auto k) {
+      constexpr int OY = decltype(k)::value - 4;
+      if constexpr (row_cell_needed(D, OY)) {
+        const double2 v = su[c0 + D * kW68 + OY];
+        u[OY + 4] = make_double2(v.x - cu.x, v.y - cu.y);
+        th[OY + 4] = sth[c0 + D * kW68 + OY];
+      }
+    }, std::make_integer_sequence<int, 9>{});
+    double st[5], dt[5], sx[5], dx[5], sy[5], dy[5];
+    static_for([&](auto k) {
+      constexpr int OY = decltype(k)::value;
+      if constexpr (row_cell_needed(D, OY)) {
+        if constexpr (OY == 0) {
+          st[0] = th[4];
+          sx[0] = u[4].x;
+          sy[0] = u[4].y;
+        } else {
+          st[OY] = th[4 + OY] + th[4 - OY];
+          dt[OY] = th[4 + OY] - th[4 - OY];
+          sx[OY] = u[4 + OY].x + u[4 - OY].x;
+          dx[OY] = u[4 + OY].x - u[4 - OY].x;
+          sy[OY] = u[4 + OY].y + u[4 - OY].y;
+          dy[OY] = u[4 + OY].y - u[4 - OY].y;
+        }
+      }
+    }, std::make_integer_sequence<int, 5>{});
+    static_for([&](auto rr) {
+      constexpr int R = decltype(rr)::value, OX = D - R;
+      if constexpr (OX >= -4 && OX <= 4) {
+        static_for([&](auto yy) {
+          constexpr int OY = decltype(yy)::value;
+          if constexpr (in_stencil68(OX, OY)) {
+            if constexpr (OX != 0) {
+              fx[R] = fma(w_ax<OX, OY>(T), st[OY], fx[R]);
+              fx[R] = fma(w_bxx<OX, OY>(T), sx[OY], fx[R]);
+              if constexpr (OY != 0) fx[R] = fma(w_bxy<OX, OY>(T), dy[OY], fx[R]);
+            }
+            if constexpr (OY != 0) {
+              fy[R] = fma(w_ay<OX, OY>(T), dt[OY], fy[R]);
+              fy[R] = fma(w_byy<OX, OY>(T), sy[OY], fy[R]);
+              if constexpr (OX != 0) fy[R] = fma(w_bxy<OX, OY>(T), dx[OY], fy[R]);
+            }
+          }
+        }, std::make_integer_sequence<int, 5>{});
+      }
+    }, std::make_integer_sequence<int, kRB>{});
 #pragma unroll
     for (int r = 0; r < kRB; r++) {
       pin(fx[r]);
@@ -926,6 +987,11 @@ struct ForceFull {
   }
 };
 
+template <class S, int... E>
+__device__ __forceinline__ void sym_sweep(S &s, std::integer_sequence<int, E...>) {
+  (s.template row<E - 4>(), ...);
+}
+
 // Dilatation of a quarter of a partial tile (tile rows 4 b .. 4 b + 3, one node per lane) with the
 // masked per-bond sum of k_theta_t68 (the same arithmetic): window rows 4 b .. 4 b + 11 (12 x 24
 // cells) in LDS, u in su[0, 288), the cell masks as bytes from su[288] on.
@@ -933,7 +999,7 @@ __device__ __forceinline__ void theta_part_quarter(const StepArgs &a, const Step
                                                    double2 *su, int32_t t, int32_t b) {
   const TileArgs &T = a.tile;
   const int32_t *nbr = T.tile_nbr + 9 * t;
-  const int32_t lane = threadIdx.x & 63;
+  const int32_t lane = threadIdx.x;
   int8_t *ok = reinterpret_cast<int8_t *>(su + 12 * kW68);
   for (int32_t c = lane; c < 12 * kW68; c += 64) {
     const int32_t s = window_slot68(nbr, 4 * b * kW68 + c);
@@ -985,84 +1051,10 @@ __global__ void __launch_bounds__(64, 4) k_theta_rb(StepArgs a, StepState st) {
   fill_window68(nb, lane, [&](WinCell w) { su[w.cell] = make_double2(st.px[w.slot], st.py[w.slot]); },
                 std::make_integer_sequence<int, 9>{});
   __syncthreads();
-  ThetaFull<> P{su, (lx0 + 4) * kW68 + ly + 4, T, {}, {}};
+  ThetaSym P{su, (lx0 + 4) * kW68 + ly + 4, T, {}};
 #pragma unroll
   for (int r = 0; r < kRB; r++) P.acc[r] = 0.0;
-  rb_sweep(P, std::make_integer_sequence<int, kRB + 8>{});
-#pragma unroll
-  for (int r = 0; r < kRB; r++)
-    if (kd[r] == 1) st.theta[t * (kTile * kTile) + (lx0 + r) * kTile + ly] = P.acc[r];
-}
-
-// LDS of theta_part_quarter: 12 x 24 cells of u, then their masks as bytes (in double2 units).
-constexpr int kQuarterCells = 12 * kW68 + (12 * kW68 + 15) / 16;
-constexpr int kGW = 2 * kTile + 8;  // 40: the window edge of a 2 x 2 group of tiles (halo 4)
-
-// Loads group g's neighbourhood table into snb and returns the wavefront's tile (w / ny, w % ny)
-// of the group, or -1 when the group has fewer tiles than the wavefront's index.
-__device__ __forceinline__ int32_t group_tile(const TileArgs &T, int32_t g, int32_t *snb,
-                                              int32_t &nx, int32_t &ny, int32_t &qx, int32_t &qy) {
-  const int32_t *G = T.groups + (int64_t)kGroupInts * g;
-  nx = G[16];
-  ny = G[17];
-  if (threadIdx.x < 16) snb[threadIdx.x] = G[threadIdx.x];
-  const int32_t w = threadIdx.x >> 6;
-  qx = ny == 2 ? (w >> 1) : w;
-  qy = ny == 2 ? (w & 1) : 0;
-  return w < nx * ny ? G[(qx + 1) * 4 + qy + 1] : -1;
-}
-
-// Stages group window cells into LDS (row stride kGW) with the 256 threads of the block: cell
-// (wx, wy) is the point (wx - 4, wy - 4) from the anchor's first node; with 16 added, (v >> 4)
-// picks the neighbourhood tile and (v & 15) the row or column inside it. f(cell, slot).
-template <class F>
-__device__ __forceinline__ void fill_group_window(const int32_t *snb, int32_t nx, int32_t ny, F &&f) {
-  const int32_t cols = kTile * ny + 8, ncell = (kTile * nx + 8) * cols;
-#pragma unroll
-  for (int k = 0; k < (kGW * kGW + kBlock - 1) / kBlock; k++) {
-    const int32_t c = threadIdx.x + kBlock * k;
-    if (c < ncell) {
-      const int32_t wx = ny == 2 ? c / kGW : c / kW68;
-      const int32_t wy = c - wx * cols;
-      const int32_t gx = wx + 12, gy = wy + 12;
-      f(wx * kGW + wy, snb[(gx >> 4) * 4 + (gy >> 4)] * (kTile * kTile) + (gx & 15) * kTile + (gy & 15));
-    }
-  }
-}
-
-// Dilatation of the full tiles by groups of up to 2 x 2 tiles (TileArgs::groups), one workgroup of
-// 4 wavefronts per group sharing the group's window (25.6 KB of LDS for 1,024 nodes, against
-// 9.2 KB for 256 with a window per tile); blocks past the groups' (rounded up to a multiple of 8)
-// take the partial tiles, one per block, a quarter per wavefront.
-__global__ void __launch_bounds__(kBlock, 4) k_theta_grp(StepArgs a, StepState st) {
-  __shared__ double2 su[kGW * kGW];
-  __shared__ int32_t snb[16];
-  static_assert(4 * kQuarterCells <= kGW * kGW, "partial-tile quarters fit the group window");
-  const TileArgs &T = a.tile;
-  const int32_t ngb = (T.n_groups + 7) / 8 * 8;
-  if ((int32_t)blockIdx.x >= ngb) {
-    const int32_t p = blockIdx.x - ngb;
-    if (p < T.n_part)
-      theta_part_quarter(a, st, su + (threadIdx.x >> 6) * kQuarterCells, T.part_list[p], threadIdx.x >> 6);
-    return;
-  }
-  const int32_t g = tile_of_block(blockIdx.x, T.n_groups);
-  if (g >= T.n_groups) return;
-  int32_t nx, ny, qx, qy;
-  const int32_t t = group_tile(T, g, snb, nx, ny, qx, qy);
-  const int32_t lane = threadIdx.x & 63;
-  const int32_t ly = lane & 15, lx0 = (lane >> 4) * kRB;
-  int8_t kd[kRB];
-#pragma unroll
-  for (int r = 0; r < kRB; r++) kd[r] = t >= 0 ? a.kind[t * (kTile * kTile) + (lx0 + r) * kTile + ly] : 0;
-  __syncthreads();
-  fill_group_window(snb, nx, ny, [&](int32_t c, int32_t s) { su[c] = make_double2(st.px[s], st.py[s]); });
-  __syncthreads();
-  if (t < 0) return;
-  ThetaFull<kGW> P{su, (kTile * qx + lx0 + 4) * kGW + kTile * qy + ly + 4, T, {}, {}};
-#pragma unroll
-  for (int r = 0; r < kRB; r++) P.acc[r] = 0.0;
-  rb_sweep(P, std::make_integer_sequence<int, kRB + 8>{});
+  sym_sweep(P, std::make_integer_sequence<int, kRB + 8>{});
 #pragma unroll
   for (int r = 0; r < kRB; r++)
     if (kd[r] == 1) st.theta[t * (kTile * kTile) + (lx0 + r) * kTile + ly] = P.acc[r];
@@ -1099,75 +1091,23 @@ __global__ void __launch_bounds__(64, 3) k_force_rb(StepArgs a, StepState st) {
   }, std::make_integer_sequence<int, 9>{});
   __syncthreads();
   const int32_t c0 = (lx0 + 4) * kW68 + ly + 4;
-  ForceFull<> P{su, sth, c0, T, {}, {}, {}, {}, 0.0};
+  double2 ui[kRB];
+#pragma unroll
+  for (int r = 0; r < kRB; r++) ui[r] = su[c0 + r * kW68];
+  ForceSym P{su, sth, c0, T, ui[1], {}, {}};
 #pragma unroll
   for (int r = 0; r < kRB; r++) {
-    P.ui[r] = su[c0 + r * kW68];
-    P.fx[r] = 0.0;
-    P.fy[r] = 0.0;
+    P.fx[r] = -T.wbsum[0] * (ui[r].x - P.cu.x);
+    P.fy[r] = -T.wbsum[1] * (ui[r].y - P.cu.y);
   }
-  rb_sweep(P, std::make_integer_sequence<int, kRB + 8>{});
+  sym_sweep(P, std::make_integer_sequence<int, kRB + 8>{});
   const InstParams &I = a.inst[T.tile_inst[t]];
   const double scale = I.f_scale;
 #pragma unroll
   for (int r = 0; r < kRB; r++) {
     const int32_t i = t * (kTile * kTile) + (lx0 + r) * kTile + ly;
     if (kd[r] != 0)
-      node_update(I, st, i, kd[r], P.ui[r], vi[r], ai[r], scale * P.fx[r], scale * P.fy[r]);
-  }
-}
-
-// Force pass of the full tiles by groups of up to 2 x 2 tiles (TileArgs::groups), with the fused
-// Verlet update: one workgroup of 4 wavefronts per group, wavefront w on the group's tile
-// (w / ny, w % ny). The group's (16 nx + 8) x (16 ny + 8) window is staged once in LDS (row stride
-// kGW) and read by all of its wavefronts with the sweep of k_force_rb. A 2 x 2 group stages 1,600
-// cells for 1,024 nodes, where four single-tile windows stage 2,304, and its 38.4 KB of LDS lets
-// 16 wavefronts share a CU, against 11 with a 13.8 KB window per wavefront.
-__global__ void __launch_bounds__(kBlock, 4) k_force_grp(StepArgs a, StepState st) {
-  __shared__ double2 su[kGW * kGW];
-  __shared__ double sth[kGW * kGW];
-  __shared__ int32_t snb[16];
-  const TileArgs &T = a.tile;
-  const int32_t g = tile_of_block(blockIdx.x, T.n_groups);
-  if (g >= T.n_groups) return;
-  int32_t nx, ny, qx, qy;
-  const int32_t t = group_tile(T, g, snb, nx, ny, qx, qy);
-  const int32_t lane = threadIdx.x & 63;
-  const int32_t ly = lane & 15, lx0 = (lane >> 4) * kRB;
-  // the epilogue's inputs first: their loads overlap the window's
-  int8_t kd[kRB];
-  double2 vi[kRB], ai[kRB];
-#pragma unroll
-  for (int r = 0; r < kRB; r++) {
-    const int32_t i = (t >= 0 ? t : 0) * (kTile * kTile) + (lx0 + r) * kTile + ly;
-    kd[r] = t >= 0 ? a.kind[i] : 0;
-    vi[r] = t >= 0 ? make_double2(st.vx[i], st.vy[i]) : make_double2(0.0, 0.0);
-    ai[r] = t >= 0 ? make_double2(st.ax[i], st.ay[i]) : make_double2(0.0, 0.0);
-  }
-  const double ratio = a.inst[T.tile_inst[T.groups[(int64_t)kGroupInts * g + 5]]].th_scale;  // anchor
-  __syncthreads();
-  fill_group_window(snb, nx, ny, [&](int32_t c, int32_t s) {
-    su[c] = make_double2(st.px[s], st.py[s]);
-    sth[c] = ratio * st.theta[s];
-  });
-  __syncthreads();
-  if (t < 0) return;
-  const int32_t c0 = (kTile * qx + lx0 + 4) * kGW + kTile * qy + ly + 4;
-  ForceFull<kGW> P{su, sth, c0, T, {}, {}, {}, {}, 0.0};
-#pragma unroll
-  for (int r = 0; r < kRB; r++) {
-    P.ui[r] = su[c0 + r * kGW];
-    P.fx[r] = 0.0;
-    P.fy[r] = 0.0;
-  }
-  rb_sweep(P, std::make_integer_sequence<int, kRB + 8>{});
-  const InstParams &I = a.inst[T.tile_inst[t]];
-  const double scale = I.f_scale;
-#pragma unroll
-  for (int r = 0; r < kRB; r++) {
-    const int32_t i = t * (kTile * kTile) + (lx0 + r) * kTile + ly;
-    if (kd[r] != 0)
-      node_update(I, st, i, kd[r], P.ui[r], vi[r], ai[r], scale * P.fx[r], scale * P.fy[r]);
+      node_update(I, st, i, kd[r], ui[r], vi[r], ai[r], scale * P.fx[r], scale * P.fy[r]);
   }
 }
 
@@ -1613,12 +1553,7 @@ hipError_t launch_theta(const StepArgs &a, const StepState &st, bool fast, hipSt
                         bool with_part) {
   if (fast) {
     const TileArgs &T = a.tile;
-    if (T.theta_kernel == kBlocked && (T.grouped & 2) && T.n_groups > 0) {
-      // the partial tiles in the same launch (a block each) unless another stream takes them
-      const int32_t nb = list_blocks(T.n_groups) + (with_part ? T.n_part : 0);
-      hipLaunchKernelGGL(k_theta_grp, dim3(nb), dim3(kBlock), 0, s, a, st);
-      return hipGetLastError();
-    } else if (T.theta_kernel == kBlocked) {
+    if (T.theta_kernel == kBlocked) {
       // the partial tiles in the same launch (4 blocks each) unless another stream takes them
       const int32_t nb = list_blocks(T.n_full) + (with_part ? 4 * T.n_part : 0);
       if (nb > 0) hipLaunchKernelGGL(k_theta_rb, dim3(nb), dim3(64), 0, s, a, st);
@@ -1651,9 +1586,7 @@ hipError_t launch_idw(const StepArgs &a, const StepState &st, hipStream_t s) {
 hipError_t launch_force(const StepArgs &a, const StepState &st, bool fast, hipStream_t s) {
   if (fast) {
     const int32_t nf = a.tile.n_full;
-    if (a.tile.force_kernel == kBlocked && (a.tile.grouped & 1) && a.tile.n_groups > 0) {
-      hipLaunchKernelGGL(k_force_grp, dim3(list_blocks(a.tile.n_groups)), dim3(kBlock), 0, s, a, st);
-    } else if (a.tile.force_kernel == kBlocked) {
+    if (a.tile.force_kernel == kBlocked) {
       if (nf > 0) hipLaunchKernelGGL(k_force_rb, dim3(list_blocks(nf)), dim3(64), 0, s, a, st);
     } else if (a.tile.force_kernel == kUnrolled) {
       if (nf > 0)

@@ -23,6 +23,9 @@ class Workload:
     dt: float            # sweep result
     dk_factor: float     # sweep result
     description: str
+    # ABC operators from the symmetric builder (one octant of rows solved, the others their exact
+    # lattice images, as for the batched workloads) instead of one solve per row
+    symmetric: bool = False
 
 
 WORKLOADS = {
@@ -36,7 +39,8 @@ WORKLOADS = {
     "cfg2": Workload("cfg2", 500.0, 0.33, 6000, pp.Pulse(150.0, -100.0, 40.0, 1.0, (1.0, 1.0)),
                      17.0e-9, 0.7,
                      "R=500 mm, dx=0.5 mm, delta=2 mm, nu=0.33, Gaussian pulse r=40 mm at "
-                     "(150,-100) mm, LSQ ABC (G_theta), 6000 steps"),
+                     "(150,-100) mm, LSQ ABC (G_theta, symmetric builder), 6000 steps",
+                     symmetric=True),
 }
 
 
@@ -58,7 +62,7 @@ def load(path: str) -> pp.Problem:
 def make(name: str, workers: Optional[int] = None, steps: Optional[int] = None) -> pp.Problem:
     w = WORKLOADS[name]
     p = pp.make_problem(w.R, w.nu, pulses=[w.pulse], steps=steps or w.steps, dt=w.dt,
-                        dk_factor=w.dk_factor, workers=workers)
+                        dk_factor=w.dk_factor, workers=workers, symmetric=w.symmetric)
     p.info["workload"] = w.name
     return p
 

@@ -212,37 +212,6 @@ def test_abc_prediction_folded_or_apart_gives_the_same_bits(graph, monkeypatch, 
 
 
 @pytest.mark.parametrize("graph", [True, False])
-def test_force_by_tile_groups_gives_the_same_bits(graph, monkeypatch, r40_problem):
-    """The full tiles' force pass one tile per wavefront (k_force_rb) or by groups of up to 2 x 2
-    tiles sharing one LDS window (k_force_grp; the autotuner times both): the same bits, on one
-    domain and on a batch, and the groups of every shape appear."""
-    import re
-    say("force groups on/off", graph)
-    g12, _ = load_golden("abc_r12_nu033")
-    out = []
-    for mode in ("on", "off"):
-        monkeypatch.setenv("LPS_FORCE_GROUPS", mode)
-        with solver.Solver(r40_problem, variant=3, use_graph=graph) as s:
-            m = re.search(r"groups of tiles \((\d+) of 2 x 2, (\d+) of 2, (\d+) single\)", s.path)
-            assert (m is not None) == (mode == "on"), s.path
-            if m:
-                assert int(m.group(1)) > 0 and int(m.group(2)) + int(m.group(3)) > 0, s.path
-            recs = s.run(95)
-            one = (s.fetch(), recs)
-        with solver.Batch([g12, r40_problem], variant=3, use_graph=graph) as b:
-            assert ("groups of tiles" in b.path) == (mode == "on"), b.path
-            brecs = b.run(95)
-            many = ([b.fetch(i) for i in range(2)], brecs)
-        out.append((one, many))
-    (a1, am), (b1, bm) = out
-    for k in ("u", "v", "a", "theta"):
-        assert np.array_equal(a1[0][k], b1[0][k]), k
-        for i in range(2):
-            assert np.array_equal(am[0][i][k], bm[0][i][k]), (i, k)
-    assert a1[1] == b1[1] and am[1] == bm[1]
-
-
-@pytest.mark.parametrize("graph", [True, False])
 def test_timing_mode_reports_pass_times_and_changes_no_result(graph, r40_problem):
     """lps_set_timing: per-pass events (mode 1) or the force kernel's two events only (mode 2),
     inside the 30-step graphs when graphs are on; the state afterwards is the same bits as an
