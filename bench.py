@@ -52,6 +52,12 @@ MLP_FLOPS_PER_ROW = 2.0 * (1600 * 256 + 2 * 256 * 256 + 256 * 4)
 GATE = 1e-4
 
 
+def schedule(args):
+    """The step schedule (lps_options.concurrent): --serial is 0, else --schedule, else the
+    library's default."""
+    return 0 if args.serial else args.schedule
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -278,7 +284,7 @@ def bench_learned(args):
     wpath = os.path.join(ROOT, "peridynamic-lps-absorbing-boundary_amd", "lps_amd", "data",
                          "closure_r125.npz")
     with cl.Closure() as clo, solver.Batch(probs, variant=args.variant,
-                                           concurrent=not args.serial) as b:
+                                           concurrent=schedule(args)) as b:
         meta = L.load_into(clo, wpath)
         log(f"[bench] GPU setup {time.time() - t:.1f}s, path: {b.path}")
         gates = [args.amp_min * float(np.max(np.abs(p.ic))) for p in probs]
@@ -373,7 +379,7 @@ def bench_batch(args):
     probs = configs.make_batch(w.name, workers=workers, steps=w.steps)
     n_nodes = sum(p.N for p in probs)
     log(f"[bench] {len(probs)} instances, {n_nodes} nodes, ready in {time.time() - t:.1f}s")
-    with solver.Batch(probs, variant=args.variant, concurrent=not args.serial) as b:
+    with solver.Batch(probs, variant=args.variant, concurrent=schedule(args)) as b:
         if args.warmup > 0:
             b.run(args.warmup)
             b.set_timing(2)
@@ -385,7 +391,8 @@ def bench_batch(args):
         passes = pass_breakdown(b, steps)
         b.reset()
         tg = time.perf_counter()
-        recs = b.run(w.steps)
+        full_steps = args.eval_steps or w.steps
+        recs = b.run(full_steps)
         full_s = time.perf_counter() - tg
         path = b.path
     refl = [1.0 + r[-1]["de_rel"] for r in recs]
@@ -398,7 +405,7 @@ def bench_batch(args):
         "data": "synthetic",
         "config": {"workload": f"{w.name}: {w.description}", "instances": len(probs),
                    "nodes": n_nodes, "kernels": path, "abc": "least-squares only",
-                   "full_run": {"steps": w.steps, "node_steps_per_s": n_nodes * w.steps / full_s},
+                   "full_run": {"steps": full_steps, "node_steps_per_s": n_nodes * full_steps / full_s},
                    "reflected": summary(refl)},
         "roofline": force_roofline(path, ti, n_nodes, pt["force_ms"], w.name, passes),
         "cpu_baseline": cpu,
@@ -422,7 +429,7 @@ def bench_single(args):
             configs.save(prob, args.cache)
     log(f"[bench] N={prob.N} C={prob.C} dt={prob.dt:.3e} ready in {time.time() - t:.1f}s")
     with solver.Solver(prob, exact=args.exact, use_graph=True, variant=args.variant,
-                       concurrent=not args.serial) as s:
+                       concurrent=schedule(args)) as s:
         rp, _ = s.horizons()
         nnz = int(rp[-1]) - prob.N
         if args.warmup > 0:
@@ -435,7 +442,8 @@ def bench_single(args):
         ti = s.tile_info()
         passes = pass_breakdown(s, steps)
         tg = time.perf_counter()
-        recs = s.run(w.steps)
+        full_steps = args.eval_steps or w.steps
+        recs = s.run(full_steps)
         full_s = time.perf_counter() - tg
         path = s.path
     refl = 1.0 + recs[-1]["de_rel"]
@@ -449,7 +457,7 @@ def bench_single(args):
         "config": {"workload": f"{w.name}: {w.description}", "nodes": prob.N, "abc_nodes": prob.C,
                    "bonds": nnz, "dt": prob.dt, "exact": bool(args.exact), "kernels": path,
                    "abc": "least-squares only", "window_start_step": window_start(steps),
-                   "full_run": {"steps": w.steps, "node_steps_per_s": prob.N * w.steps / full_s,
+                   "full_run": {"steps": full_steps, "node_steps_per_s": prob.N * full_steps / full_s,
                                 "reflected_energy_final": refl,
                                 "gate_reflected_le_1e-4": bool(refl <= GATE)}},
         "roofline": force_roofline(path, ti, prob.N, pt["force_ms"], w.name, passes),
@@ -468,8 +476,8 @@ def main():
                          "learned vs LSQ), cfg2 (R=500 mm), cfg1 (Example 1), cfg3 (64 domains)")
     ap.add_argument("--runs", type=int, default=256, help="cfg4: held-out runs (one batch)")
     ap.add_argument("--eval-steps", type=int, default=None,
-                    help="learned workloads: length of the untimed gate pass (default: the "
-                         "workload's, 6000 for cfg4)")
+                    help="length of the untimed full pass (the gate of the learned workloads; "
+                         "default: the workload's, 6000 for cfg4)")
     ap.add_argument("--amp-min", type=float, default=1e-4,
                     help="closure amplitude gate, relative to the run's largest initial |u|")
     ap.add_argument("--guard", type=float, default=1e-3, help="closure energy guard (units of E0)")
@@ -480,6 +488,10 @@ def main():
     ap.add_argument("--serial", action="store_true",
                     help="run the ABC rows' kernels on the main stream instead of beside the "
                          "tile kernels on a second stream")
+    ap.add_argument("--schedule", type=int, default=None, choices=(0, 1, 2, 3),
+                    help="step schedule: 0 ABC rows after the tile kernels, 1 on a second stream "
+                         "beside them, 2 also the partial tiles on a third, 3 the ABC rows inside "
+                         "the dilatation launch (default: the library's)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=90)
     ap.add_argument("--workers", type=int, default=None)

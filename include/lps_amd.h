@@ -69,8 +69,9 @@ typedef struct lps_options {
                             3 when the 68-bond stencil applies, else generic) */
   int32_t concurrent;    /* fast path: 1 runs the ABC rows' kernels (G_theta, G_u, G_v) on a
                             second stream beside the tile kernels; 2 also the partial tiles'
-                            kernels on a third stream beside the full tiles'; 0 on the one
-                            stream */
+                            kernels on a third stream beside the full tiles'; 3 runs the ABC rows
+                            as extra workgroups of the dilatation launch, one stream; 0 the row
+                            kernels after the tile kernels, one stream */
 } lps_options;
 
 /* One output record: the quantities src/main.cpp:986-1062 computes every 30 steps. */

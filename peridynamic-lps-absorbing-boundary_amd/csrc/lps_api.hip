@@ -858,6 +858,11 @@ static int32_t new_context(const lps_options *options, lps_ctx **out) {
   lps_ctx *c = new lps_ctx();
   if (options) c->opt = *options;
   else lps_default_options(&c->opt);
+  if (c->opt.concurrent < 0 || c->opt.concurrent > 3) {
+    g_create_error = "concurrent must be 0, 1, 2 or 3";
+    delete c;
+    return LPS_ERR_ARG;
+  }
   int ndev = 0;
   hipError_t e = hipGetDeviceCount(&ndev);
   if (e != hipSuccess || ndev <= 0) {
@@ -1344,6 +1349,36 @@ static hipError_t record_timing(lps_ctx *c, hipEvent_t e) {
   return hipEventRecordWithFlags(e, c->stream, c->capturing ? hipEventRecordExternal : 0);
 }
 
+// Fast path, concurrent == 3: one stream, three launches. The ABC rows' G_theta and G_u / G_v run as
+// extra workgroups of the dilatation launch (they read only the previous steps' state), then the
+// full tiles' force, then the partial tiles' force with the ABC nodes' prediction. A learned
+// closure runs on the fourth stream from the step's start and its corrections are added before
+// the ABC nodes' prediction.
+static hipError_t step_fused(lps_ctx *c, const StepArgs &a, const StepState &st, hipEvent_t *ev,
+                             bool record_events) {
+  hipError_t e;
+  if (c->clo) {
+    if ((e = hipEventRecord(c->ev_fork[0], c->stream)) != hipSuccess) return e;
+    if ((e = hipStreamWaitEvent(c->stream4, c->ev_fork[0], 0)) != hipSuccess) return e;
+    if ((e = closure_step_hook(c, st, c->stream4, true)) != hipSuccess) return e;
+    if ((e = hipEventRecord(c->ev_clo, c->stream4)) != hipSuccess) return e;
+  }
+  if ((e = launch_theta_rows(a, st, c->ring(c->step), c->stream)) != hipSuccess) return e;
+  if (a.theta_method == LPS_THETA_IDW && (e = launch_idw(a, st, c->stream)) != hipSuccess) return e;
+  const bool ev_force = record_events && !c->timing_closure;
+  if (ev_force && (e = record_timing(c, ev[1])) != hipSuccess) return e;
+  if ((e = launch_force(a, st, true, c->stream)) != hipSuccess) return e;
+  if (ev_force && (e = record_timing(c, ev[2])) != hipSuccess) return e;
+  if (!c->clo && c->abc_in_part && a.tile.force_kernel != kGeneric)
+    return launch_force_part(a, st, c->stream, true);
+  if ((e = launch_force_part(a, st, c->stream)) != hipSuccess) return e;
+  if (c->clo) {
+    if ((e = hipStreamWaitEvent(c->stream, c->ev_clo, 0)) != hipSuccess) return e;
+    if ((e = closure_apply_hook(c, st, c->stream)) != hipSuccess) return e;
+  }
+  return launch_abc_predict(a, st, c->stream);
+}
+
 static hipError_t step_once(lps_ctx *c, bool record_events) {
   const StepArgs a = c->args();
   const StepState st = c->state(c->step);
@@ -1354,6 +1389,16 @@ static hipError_t step_once(lps_ctx *c, bool record_events) {
     if (!c->timing_force_only && (e = record_timing(c, ev[0])) != hipSuccess) return e;
   }
   c->clo_ev = record_events && c->timing_closure ? ev : nullptr;  // recorded by the closure hook
+  if (c->fast && a.n_abc > 0 && c->opt.concurrent == 3) {
+    if ((e = step_fused(c, a, st, ev, record_events)) != hipSuccess) return e;
+    if (record_events) {
+      if (!c->timing_force_only && (e = record_timing(c, ev[3])) != hipSuccess) return e;
+      c->ev_used++;
+    }
+    c->step++;
+    c->u_synced = 0;
+    return hipSuccess;
+  }
   // fast path: the ABC rows' G_theta and G_u / G_v read only the previous steps' state and run on
   // the second stream beside the tile passes; the force pass waits for G_theta (ev_join[0]), the
   // ABC prediction for G_u / G_v (ev_join[1])

@@ -208,6 +208,9 @@ hipError_t launch_force_part(const StepArgs &a, const StepState &st, hipStream_t
 // and the prediction, launch_abc_uv (G_u, G_v) only the previous steps' state: both may run on a
 // second stream beside the tile passes; the force pass needs abc_theta, launch_abc_predict (the
 // next prediction at the ABC nodes) needs abc_uv and the force pass.
+// The dilatation with the ABC rows' G_theta and G_u / G_v in the same launch (extra workgroups of
+// the register-blocked kernel; other families: the row kernels then the dilatation, on s).
+hipError_t launch_theta_rows(const StepArgs &a, const StepState &st, const RingU &r, hipStream_t s);
 hipError_t launch_abc_theta(const StepArgs &a, const StepState &st, hipStream_t s);
 hipError_t launch_abc_uv(const StepArgs &a, const StepState &st, const RingU &r, hipStream_t s);
 hipError_t launch_abc_predict(const StepArgs &a, const StepState &st, hipStream_t s);

@@ -1025,19 +1025,131 @@ __device__ __forceinline__ void theta_part_quarter(const StepArgs &a, const Step
   st.theta[i] = a.c2m[i] * ((acc[0] + acc[1]) + (acc[2] + acc[3]));
 }
 
+// Fast-path ABC rows: 16 lanes per row, four rows per wavefront, lane q of a row on its cloud
+// members q, q + 16, ... (a cloud has at most 49), then a sum over the row's 16 lanes. One wavefront
+// per row left 15 to 25 of its lanes idle on the (ragged) clouds and spent 12 cross-lane moves per
+// value on the 64-lane sum.
+// The row kernels give each XCD a contiguous range of rows (tile_of_block's mapping over row
+// blocks): rows are in node order, so neighbouring rows share cloud nodes and their state, and the
+// shared values are then read through one L2.
+constexpr int kRowLanes = 16;
+constexpr int kRowsPerBlock = kBlock / kRowLanes;
+
+// Sum over the 16 lanes of a row group; every lane of the group gets it.
+__device__ __forceinline__ double group16_sum(double v) {
+  for (int o = kRowLanes / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kRowLanes);
+  return v;
+}
+
+// ABC row `row`'s dilatation on its 16 lanes (q = lane in the row): G_theta (src/main.cpp:792-812),
+// or for IDW the direct value (overridden by k_idw where a non-ABC neighbour exists,
+// src/main.cpp:765-790). Reads only the theta history (or the prediction).
+__device__ __forceinline__ void abc_theta_rows16(const StepArgs &a, const StepState &st,
+                                                 int32_t row, int32_t q) {
+  const int32_t node = a.abc_slot[row];
+  double th;
+  if (a.theta_method == 1) {
+    int64_t o;
+    int32_t L;
+    abc_row_span(a, row, o, L);
+    const double *g = a.gth + 2 * (int64_t)a.abc_gofs[row];  // G_theta needs no lattice transform
+    const int32_t *cl = a.cloud + o;
+    double acc = 0.0;
+    for (int jj = q; jj < L; jj += kRowLanes) {
+      const int32_t k = cl[jj];
+      if (k >= 0) {
+        const double2 gm = *(const double2 *)(g + 2 * a.abc_pmap[o + jj]);
+        acc += gtheta_term_exact(gm.x, gm.y, st.thn[k], st.thnm1[k]);
+      }
+    }
+    th = group16_sum(acc);
+  } else {
+    th = q == 0 ? theta_node_exact(a, st.px, st.py, node) : 0.0;
+  }
+  if (q == 0) st.theta[node] = th;
+}
+
+// ABC row `row` on its 16 lanes: U^{n+1} = G_u [U^n, U^{n-1}] and V^{n+1} = G_v [V^n, V^{n-1}]
+// over the row's cloud (src/main.cpp:883-916, 933-974). The tile kernels compute the force at the
+// ABC nodes like at any node. U of an interior cloud member comes from the prediction ring r, of an
+// ABC member from U.
+__device__ __forceinline__ void abc_uv_rows16(const StepArgs &a, const StepState &st, const RingU &r,
+                                              int32_t row, int32_t q) {
+  int64_t o;
+  int32_t L;
+  abc_row_span(a, row, o, L);
+  const int64_t gb = a.abc_gofs[row];
+  const int xf = a.abc_xf[row];
+  const int32_t *cl = a.cloud + o;
+  double Ux = 0.0, Uy = 0.0, Vx = 0.0, Vy = 0.0;
+  for (int jj = q; jj < L; jj += kRowLanes) {
+    const int32_t k = cl[jj];
+    if (k < 0) continue;
+    double4 g0, g1, h0, h1;
+    abc_uv_coeffs(a, o, L, gb, xf, jj, g0, g1, h0, h1);
+    const bool in = a.kind[k] == 1;
+    const double ux = (in ? r.ux : st.ux)[k], uy = (in ? r.uy : st.uy)[k];
+    const double uxm = (in ? r.uxm : st.uxm)[k], uym = (in ? r.uym : st.uym)[k];
+    const double vx = st.vx[k], vy = st.vy[k], vxm = st.vxm[k], vym = st.vym[k];
+    Ux += g0.x * ux + g0.y * uy + g0.z * uxm + g0.w * uym;
+    Uy += g1.x * ux + g1.y * uy + g1.z * uxm + g1.w * uym;
+    Vx += h0.x * vx + h0.y * vy + h0.z * vxm + h0.w * vym;
+    Vy += h1.x * vx + h1.y * vy + h1.z * vxm + h1.w * vym;
+  }
+  Ux = group16_sum(Ux);
+  Uy = group16_sum(Uy);
+  Vx = group16_sum(Vx);
+  Vy = group16_sum(Vy);
+  if (q == 0) {
+    const int32_t node = a.abc_slot[row];
+    st.ux1[node] = Ux; st.uy1[node] = Uy;
+    st.vx1[node] = Vx; st.vy1[node] = Vy;
+  }
+}
+
+// Fast path, ABC rows' dilatation, kRowsPerBlock rows per block; may run beside the tile kernels.
+// Launched on abc_blocks_xcd(a) blocks.
+__global__ void __launch_bounds__(kBlock) k_abc_theta(StepArgs a, StepState st) {
+  const int32_t nrb = (a.n_abc + kRowsPerBlock - 1) / kRowsPerBlock;
+  const int32_t row = tile_of_block(blockIdx.x, nrb) * kRowsPerBlock + (int32_t)threadIdx.x / kRowLanes;
+  if (row < a.n_abc) abc_theta_rows16(a, st, row, threadIdx.x & (kRowLanes - 1));
+}
+
+// Fast path, ABC rows' G_u / G_v, kRowsPerBlock rows per block. Launched on abc_blocks_xcd(a)
+// blocks.
+__global__ void __launch_bounds__(kBlock) k_abc_uv(StepArgs a, StepState st, RingU r) {
+  const int32_t nrb = (a.n_abc + kRowsPerBlock - 1) / kRowsPerBlock;
+  const int32_t row = tile_of_block(blockIdx.x, nrb) * kRowsPerBlock + (int32_t)threadIdx.x / kRowLanes;
+  if (row < a.n_abc) abc_uv_rows16(a, st, r, row, threadIdx.x & (kRowLanes - 1));
+}
+
 // Dilatation of the full tiles, one wavefront per tile (full_list order); blocks past the full
 // tiles' (rounded up to a multiple of 8) take the partial tiles, four wavefronts per tile, so they
-// fill the tail of the full tiles' launch instead of following it.
-__global__ void __launch_bounds__(64, 4) k_theta_rb(StepArgs a, StepState st) {
+// fill the tail of the full tiles' launch instead of following it. With nrow_blocks > 0 (a
+// multiple of 8) the first nrow_blocks blocks take the ABC rows instead, four per block: G_theta
+// and G_u / G_v read only the previous steps' state, so the step's ABC work runs in the same launch
+// as the dilatation.
+__global__ void __launch_bounds__(64, 4) k_theta_rb(StepArgs a, StepState st, RingU ring,
+                                                     int32_t nrow_blocks) {
   __shared__ double2 su[kCells68];
   const TileArgs &T = a.tile;
+  if ((int32_t)blockIdx.x < nrow_blocks) {
+    const int32_t nrb = (a.n_abc + 3) / 4;
+    const int32_t row = tile_of_block(blockIdx.x, nrb) * 4 + (int32_t)threadIdx.x / kRowLanes;
+    if (row < a.n_abc) {
+      abc_theta_rows16(a, st, row, threadIdx.x & (kRowLanes - 1));
+      abc_uv_rows16(a, st, ring, row, threadIdx.x & (kRowLanes - 1));
+    }
+    return;
+  }
+  const int32_t b = blockIdx.x - nrow_blocks;
   const int32_t nfb = (T.n_full + 7) / 8 * 8;
-  if ((int32_t)blockIdx.x >= nfb) {
-    const int32_t pb = blockIdx.x - nfb;
+  if (b >= nfb) {
+    const int32_t pb = b - nfb;
     if ((pb >> 2) < T.n_part) theta_part_quarter(a, st, su, T.part_list[pb >> 2], pb & 3);
     return;
   }
-  const int32_t q = tile_of_block(blockIdx.x, T.n_full);
+  const int32_t q = tile_of_block(b, T.n_full);
   if (q >= T.n_full) return;
   const int32_t t = T.full_list[q];
   int32_t nb[9];
@@ -1108,63 +1220,6 @@ __global__ void __launch_bounds__(64, 3) k_force_rb(StepArgs a, StepState st) {
     const int32_t i = t * (kTile * kTile) + (lx0 + r) * kTile + ly;
     if (kd[r] != 0)
       node_update(I, st, i, kd[r], ui[r], vi[r], ai[r], scale * P.fx[r], scale * P.fy[r]);
-  }
-}
-
-// Fast path, ABC rows' dilatation (G_theta, or the direct value for IDW), one wavefront per row.
-// Reads only the theta history (or the prediction), so it may run beside the tile kernels.
-// The ABC rows' kernels give each XCD a contiguous range of rows (tile_of_block's mapping over
-// row blocks): rows are in node order, so neighbouring rows share cloud nodes and their state, and
-// the shared values are then read through one L2. Launched on abc_blocks_xcd(a) blocks.
-__global__ void __launch_bounds__(kBlock) k_abc_theta(StepArgs a, StepState st) {
-  __shared__ AbcScratch sabc[kWavesPerBlock];
-  const int32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  const int32_t nrb = (a.n_abc + kWavesPerBlock - 1) / kWavesPerBlock;
-  const int32_t row = tile_of_block(blockIdx.x, nrb) * kWavesPerBlock + wid;
-  if (row < a.n_abc) abc_theta_row<false>(a, st, row, lane, &sabc[wid]);
-}
-
-// Fast path, ABC rows: U^{n+1} = G_u [U^n, U^{n-1}] and V^{n+1} = G_v [V^n, V^{n-1}] over the
-// row's cloud (src/main.cpp:883-916, 933-974), one wavefront per row, lane k on cloud member k.
-// The tile kernels compute the force at the ABC nodes like at any node. U of an interior cloud
-// member comes from the prediction ring r, of an ABC member from U.
-__global__ void __launch_bounds__(kBlock) k_abc_uv(StepArgs a, StepState st, RingU r) {
-  const int32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  const int32_t nrb = (a.n_abc + kWavesPerBlock - 1) / kWavesPerBlock;
-  const int32_t row = tile_of_block(blockIdx.x, nrb) * kWavesPerBlock + wid;
-  if (row >= a.n_abc) return;
-  int64_t o;
-  int32_t L;
-  abc_row_span(a, row, o, L);
-  const int64_t gb = a.abc_gofs[row];
-  const int xf = a.abc_xf[row];
-  const int32_t *cl = a.cloud + o;
-  double Ux = 0.0, Uy = 0.0, Vx = 0.0, Vy = 0.0;
-  for (int base = 0; base < L; base += 64) {
-    const int jj = base + lane;
-    const int32_t k = jj < L ? cl[jj] : -1;
-    double t0 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0;
-    if (k >= 0) {
-      double4 g0, g1, h0, h1;
-      abc_uv_coeffs(a, o, L, gb, xf, jj, g0, g1, h0, h1);
-      const bool in = a.kind[k] == 1;
-      const double ux = (in ? r.ux : st.ux)[k], uy = (in ? r.uy : st.uy)[k];
-      const double uxm = (in ? r.uxm : st.uxm)[k], uym = (in ? r.uym : st.uym)[k];
-      const double vx = st.vx[k], vy = st.vy[k], vxm = st.vxm[k], vym = st.vym[k];
-      t0 = g0.x * ux + g0.y * uy + g0.z * uxm + g0.w * uym;
-      t1 = g1.x * ux + g1.y * uy + g1.z * uxm + g1.w * uym;
-      t2 = h0.x * vx + h0.y * vy + h0.z * vxm + h0.w * vym;
-      t3 = h1.x * vx + h1.y * vy + h1.z * vxm + h1.w * vym;
-    }
-    Ux += wave_sum(t0);
-    Uy += wave_sum(t1);
-    Vx += wave_sum(t2);
-    Vy += wave_sum(t3);
-  }
-  if (lane == 0) {
-    const int32_t node = a.abc_slot[row];
-    st.ux1[node] = Ux; st.uy1[node] = Uy;
-    st.vx1[node] = Vx; st.vy1[node] = Vy;
   }
 }
 
@@ -1523,8 +1578,11 @@ static inline int32_t blocks_for(int64_t n) { return (int32_t)((n + kBlock - 1) 
 static inline int32_t abc_blocks(const StepArgs &a) {
   return (a.n_abc + kWavesPerBlock - 1) / kWavesPerBlock;
 }
-// Blocks of the XCD-contiguous row kernels: 8 equal ranges, the tail blocks exit.
-static inline int32_t abc_blocks_xcd(const StepArgs &a) { return (abc_blocks(a) + 7) / 8 * 8; }
+// Blocks of the fast-path row kernels (kRowsPerBlock rows each) in 8 equal XCD ranges; the tail
+// blocks exit.
+static inline int32_t abc_blocks_xcd(const StepArgs &a) {
+  return ((a.n_abc + kRowsPerBlock - 1) / kRowsPerBlock + 7) / 8 * 8;
+}
 static inline int32_t tile_blocks(const StepArgs &a) { return (a.tile.n_tiles + 7) / 8 * 8; }
 
 hipError_t launch_predict(const StepArgs &a, const double *ux, const double *uy, const double *vx,
@@ -1556,7 +1614,7 @@ hipError_t launch_theta(const StepArgs &a, const StepState &st, bool fast, hipSt
     if (T.theta_kernel == kBlocked) {
       // the partial tiles in the same launch (4 blocks each) unless another stream takes them
       const int32_t nb = list_blocks(T.n_full) + (with_part ? 4 * T.n_part : 0);
-      if (nb > 0) hipLaunchKernelGGL(k_theta_rb, dim3(nb), dim3(64), 0, s, a, st);
+      if (nb > 0) hipLaunchKernelGGL(k_theta_rb, dim3(nb), dim3(64), 0, s, a, st, RingU{}, 0);
       return hipGetLastError();
     } else if (T.theta_kernel == kUnrolled) {
       hipLaunchKernelGGL(k_theta_t68, dim3(tile_blocks(a)), dim3(kBlock), 0, s, a, st,
@@ -1568,6 +1626,20 @@ hipError_t launch_theta(const StepArgs &a, const StepState &st, bool fast, hipSt
     const int32_t nb = blocks_for(a.n);
     hipLaunchKernelGGL(k_theta_exact, dim3(nb + abc_blocks(a)), dim3(kBlock), 0, s, a, st, nb);
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_theta_rows(const StepArgs &a, const StepState &st, const RingU &r, hipStream_t s) {
+  const TileArgs &T = a.tile;
+  if (T.theta_kernel != kBlocked) {  // the row kernels first, then the dilatation, on one stream
+    hipError_t e;
+    if ((e = launch_abc_theta(a, st, s)) != hipSuccess) return e;
+    if ((e = launch_abc_uv(a, st, r, s)) != hipSuccess) return e;
+    return launch_theta(a, st, true, s, true);
+  }
+  const int32_t nrow = ((a.n_abc + 3) / 4 + 7) / 8 * 8;
+  const int32_t nb = nrow + list_blocks(T.n_full) + 4 * T.n_part;
+  if (nb > 0) hipLaunchKernelGGL(k_theta_rb, dim3(nb), dim3(64), 0, s, a, st, r, nrow);
   return hipGetLastError();
 }
 

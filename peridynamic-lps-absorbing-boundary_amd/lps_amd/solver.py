@@ -213,7 +213,10 @@ def _lps_options(L, theta_method, abc_enabled, exact, use_graph, device, horizon
     o.device = device
     o.horizon_radius = horizon_radius
     o.variant = variant
-    o.concurrent = int(concurrent)  # 0 serial, 1 ABC rows beside the tiles, 2 also partial tiles
+    if concurrent is not None:
+        # 0 the ABC rows after the tiles, 1 beside them (second stream), 2 also the partial tiles
+        # beside the full ones (third stream), 3 the ABC rows inside the dilatation launch
+        o.concurrent = int(concurrent)
     return o
 
 
@@ -222,7 +225,7 @@ class Solver:
 
     def __init__(self, prob, theta_method: int = LPS_THETA_GTHETA, abc_enabled: bool = True,
                  exact: bool = False, use_graph: bool = True, device: int = 0,
-                 horizon_radius: float = 0.0, variant: int = 0, concurrent: bool = True):
+                 horizon_radius: float = 0.0, variant: int = 0, concurrent: Optional[int] = None):
         L = load_library()
         self._keep: List[np.ndarray] = []
         N = prob.N
@@ -378,7 +381,7 @@ class Batch(Solver):
     delta, each with its own radius, nu, dt, ABC operators and initial conditions."""
 
     def __init__(self, probs, theta_method: int = LPS_THETA_GTHETA, use_graph: bool = True,
-                 device: int = 0, variant: int = 0, concurrent: bool = True):
+                 device: int = 0, variant: int = 0, concurrent: Optional[int] = None):
         L = load_library()
         self._keep: List[np.ndarray] = []
         structs = (LpsProblem * len(probs))()

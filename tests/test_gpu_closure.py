@@ -386,7 +386,9 @@ def test_teacher_forced_samples():
 def test_closure_beside_the_abc_rows_gives_the_same_bits(use_graph):
     """With the ABC rows on their own stream the closure's network runs from the step's start on
     another stream and its corrections are added after G_u / G_v; one stream runs the network after
-    G_u / G_v and adds them in its epilogue. Both give the same bits."""
+    G_u / G_v and adds them in its epilogue; with the ABC rows inside the dilatation launch the
+    network runs beside the tile kernels and its corrections are added before the ABC nodes'
+    prediction. All give the same bits."""
     say("closure deferred vs in-line", use_graph)
     prob = with_wide_pulse(load_golden("abc_r10_nu025")[0])
     c, _ = closure_with(26, 0.0)
@@ -394,15 +396,16 @@ def test_closure_beside_the_abc_rows_gives_the_same_bits(use_graph):
     c.set_scales(1e-3, 1e-3)
     out = []
     with c:
-        for concurrent in (False, True):
+        for concurrent in (0, 1, 3):
             with solver.Solver(prob, variant=3, use_graph=use_graph, concurrent=concurrent) as s:
                 s.attach_closure(c)
                 s.run(90)
                 out.append((s.fetch(), s.closure_inspect(prob.C)[2]))
     assert np.any(out[0][1][:, :4] != 0)
-    assert np.array_equal(out[0][1], out[1][1])
-    for k in ("u", "v", "a"):
-        assert np.array_equal(out[0][0][k], out[1][0][k]), k
+    for other in out[1:]:
+        assert np.array_equal(out[0][1], other[1])
+        for k in ("u", "v", "a"):
+            assert np.array_equal(out[0][0][k], other[0][k]), k
 
 
 def test_wide_kernel_gives_the_same_bits(monkeypatch):

@@ -63,8 +63,8 @@ def test_exact_mode_matches_oracle_bitwise(name):
 
 
 @pytest.mark.parametrize("name", CASES)
-@pytest.mark.parametrize("graph,variant,concurrent", [(True, 0, True), (False, 0, True),
-                                                      (True, 3, True), (True, 2, False)])
+@pytest.mark.parametrize("graph,variant,concurrent", [(True, 0, 1), (False, 0, 1), (True, 3, 1),
+                                                      (True, 2, 0), (True, 0, 3), (False, 3, 3)])
 def test_fast_mode_matches_reference_golden(name, graph, variant, concurrent):
     """Fast path (tile kernels, FMA) against the reference's outputs: 1e-10 relative L2; autotuned,
     register-blocked and unrolled node kernels, ABC rows beside or after the tile kernels."""
@@ -209,6 +209,32 @@ def test_abc_prediction_folded_or_apart_gives_the_same_bits(graph, monkeypatch, 
         for i in range(2):
             assert np.array_equal(am[0][i][k], bm[0][i][k]), (i, k)
     assert a1[1] == b1[1] and am[1] == bm[1]
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_step_schedules_give_the_same_bits(graph, r40_problem):
+    """The ABC rows after the tile kernels (0), on a second stream (1) or as extra workgroups of the
+    dilatation launch (3): the same kernels' arithmetic in every schedule, so the same bits, on one
+    domain and on a batch. (Schedule 2 runs the partial tiles' dilatation in a kernel of its own, a
+    different association of the same sums: it is held to the oracle in the other tests.)"""
+    say("schedules", graph)
+    g12, _ = load_golden("abc_r12_nu033")
+    out = []
+    for sched in (0, 1, 3):
+        with solver.Solver(r40_problem, variant=3, use_graph=graph, concurrent=sched) as s:
+            recs = s.run(95)
+            one = (s.fetch(), recs)
+        with solver.Batch([g12, r40_problem], variant=3, use_graph=graph, concurrent=sched) as b:
+            brecs = b.run(95)
+            many = ([b.fetch(i) for i in range(2)], brecs)
+        out.append((one, many))
+    (a1, am) = out[0]
+    for b1, bm in out[1:]:
+        for k in ("u", "v", "a", "theta"):
+            assert np.array_equal(a1[0][k], b1[0][k]), k
+            for i in range(2):
+                assert np.array_equal(am[0][i][k], bm[0][i][k]), (i, k)
+        assert a1[1] == b1[1] and am[1] == bm[1]
 
 
 @pytest.mark.parametrize("graph", [True, False])
