@@ -261,20 +261,32 @@ def bench_learned(args):
     from lps_amd import closure as cl, configs, learned as L, solver
     workers = args.workers or min(16, os.cpu_count() or 1)
     t = time.time()
+    cached = bool(args.cache) and os.path.exists(args.cache)
     if args.workload == "cfg4":
-        runs = L.draw_runs(3000, args.runs)
         eval_steps = args.eval_steps or 6000
-        probs = [L.run_problem(r, eval_steps, workers) for r in runs]
+        if cached:
+            probs = configs.load_batch(args.cache)
+        else:
+            runs = L.draw_runs(3000, args.runs)
+            probs = [L.run_problem(r, eval_steps, workers) for r in runs]
         desc = (f"cfg4: {len(probs)} held-out runs of the learned-closure workload, R=125 mm, "
                 f"dx=0.5 mm, delta=2 mm, 1-4 P/S Gaussian pulses, nu on the 0.025 grid, one batch; "
                 f"gate over {eval_steps} steps")
     else:
         w = configs.BATCH_WORKLOADS["cfg5"]
         eval_steps = args.eval_steps or w.steps
-        probs = configs.make_batch("cfg5", workers=workers, steps=eval_steps,
-                                   progress=lambda i, p: log(f"[bench]   instance {i}: R={p.info['R']} "
-                                                             f"nu={p.info['nu']} N={p.N} C={p.C}"))
+        if cached:
+            probs = configs.load_batch(args.cache)
+        else:
+            probs = configs.make_batch("cfg5", workers=workers, steps=eval_steps,
+                                       progress=lambda i, p: log(f"[bench]   instance {i}: "
+                                                                 f"R={p.info['R']} nu={p.info['nu']} "
+                                                                 f"N={p.N} C={p.C}"))
         desc = f"cfg5: {w.description}; gate over {eval_steps} steps"
+    if args.cache and not cached:
+        configs.save_batch(probs, args.cache)
+    elif cached:
+        log(f"[bench] loaded {len(probs)} instances from {args.cache}")
     n_nodes = sum(p.N for p in probs)
     n_rows = sum(p.C for p in probs)
     log(f"[bench] {len(probs)} instances, {n_nodes} nodes, {n_rows} ABC rows, generated in "
@@ -496,8 +508,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=90)
     ap.add_argument("--workers", type=int, default=None)
     ap.add_argument("--cache", default=None, metavar="NPZ",
-                    help="cfg1/cfg2: load the generated problem from NPZ, or generate it and save "
-                         "it there (profiled runs generate outside the profiler)")
+                    help="cfg1/cfg2/cfg4/cfg5: load the generated problems from NPZ, or generate "
+                         "them and save them there (profiled runs generate outside the profiler)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="run only a plain copy kernel over BYTES (HBM counter calibration)")
     args = ap.parse_args()
