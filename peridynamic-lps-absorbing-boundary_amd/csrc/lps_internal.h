@@ -69,6 +69,8 @@ struct TileArgs {
   // argument segment is kept small (1.7 KB instead of 1.4 KB made the force kernel 35% slower).
   double wt[12][2], wb[12][3];
   double wbsum[2];                 // sum over the stencil of wb of o_x^2 and of o_y^2
+  // register-blocked kernels: nodes per lane, 4 (a wavefront per tile) or 2 (two per tile)
+  int32_t theta_nr, force_nr;
   const int32_t *tile_inst;        // n_tiles: the instance (domain) a tile belongs to
 };
 

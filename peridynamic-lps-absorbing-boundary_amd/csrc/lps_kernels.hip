@@ -726,10 +726,10 @@ __global__ void __launch_bounds__(kBlock, FULL ? 5 : 2) k_force_t68(StepArgs a, 
 
 // ---------------------------------------------------------------------------------------------
 // Register-blocked passes over the full tiles (every window cell a node with m == m_ref).
-// One wavefront per tile; lane l owns the kRB nodes (4 (l / 16) + r, l % 16), r < kRB, of the
-// tile. The wavefront sweeps the window rows that these nodes reach: each row's cells are read
-// from LDS once and feed the bonds of every node of the lane that reaches the row, so a node
-// costs (9 kRB + 60) / kRB cell reads instead of 68.
+// A lane owns NR nodes in consecutive rows of one column of the tile (NR = 4: a wavefront per
+// tile; NR = 2: two wavefronts per tile). The lane sweeps the window rows that these nodes reach:
+// each row's cells are read from LDS once and feed the bonds of every node of the lane that
+// reaches the row, so a node costs (9 NR + 60) / NR cell reads instead of 68.
 // On a full tile m_j = m_i = m_ref for every bond, so the bond term collapses to class weights
 // folded with the material (TileArgs::wt, wa, wb) and the theta_i part of the force,
 // theta_i sum_k c1_k xi_hat_k, vanishes by the stencil's symmetry:
@@ -739,7 +739,6 @@ __global__ void __launch_bounds__(kBlock, FULL ? 5 : 2) k_force_t68(StepArgs a, 
 // same way (sum_k c1_k o_k = 0); the force keeps u_j - u_i, whose difference of nearby values is
 // exact, since there the sum is a second difference.
 // ---------------------------------------------------------------------------------------------
-constexpr int kRB = 4;
 
 // The 24 x 24 window (halo 4) of a full tile in 9 wavefront-wide pieces: piece k of lane l is the
 // pair (slot, window cell). Pieces 0-3 are the tile itself (64 consecutive slots each), 4 / 5 the
@@ -813,7 +812,7 @@ __device__ __forceinline__ void pin(double &v) { asm volatile("" : "+v"(v)); }
 // Row-symmetric sweeps. On a full tile every term is linear in the window values, with a weight
 // that is even or odd in each component of the offset. The cells (D, +oy) and (D, -oy) of a sweep
 // row therefore enter every node of the lane through their sum or their difference: a row forms
-// these once for the lane's kRB nodes, and each node then takes one FMA per offset (ox, |oy|) and
+// these once for the lane's NR nodes, and each node then takes one FMA per offset (ox, |oy|) and
 // term, instead of one per bond and term.
 //   dilatation:  x part on the sums of u_x, y part on the differences of u_y;
 //   force, x:    C c1 ox theta_j on the sums of theta, B_xx on the sums of ubar_x, B_xy on the
@@ -828,16 +827,16 @@ __device__ __forceinline__ void pin(double &v) { asm volatile("" : "+v"(v)); }
 // difference of u, as with per-bond differences, not the field's own size (whose cancellation in
 // the second difference would cost (sigma / dx)^2 in rounding).
 // Per node: 64 FMAs for the dilatation (120 per bond), 180 for the force (466 operations per bond),
-// plus per row of the lane 8 (dilatation) or 42 (force) operations shared by its kRB nodes.
+// plus per row of the lane 8 (dilatation) or 42 (force) operations shared by its NR nodes.
 // ---------------------------------------------------------------------------------------------
 template <class F, int... I>
 __device__ __forceinline__ void static_for(F &&f, std::integer_sequence<int, I...>) {
   (f(std::integral_constant<int, I>{}), ...);
 }
 
-// A cell (D, OY) of a sweep row is read when some node r of the lane has the bond (D - r, OY).
-constexpr bool row_cell_needed(int D, int OY) {
-  for (int r = 0; r < kRB; r++)
+// A cell (D, OY) of a sweep row is read when some node r < NR of the lane has the bond (D - r, OY).
+constexpr bool row_cell_needed(int D, int OY, int NR) {
+  for (int r = 0; r < NR; r++)
     if (in_stencil68(D - r, OY)) return true;
   return false;
 }
@@ -872,23 +871,24 @@ __device__ __forceinline__ double w_bxy(const TileArgs &T) {  // 8 C c2 ox oy, O
 }
 
 // Full-tile dilatation, row-symmetric: theta_i = sum_k C c1_k (u_j . o_k); c0 is the window cell
-// of the lane's first node.
+// of the lane's first node, NR its nodes (consecutive rows).
+template <int NR>
 struct ThetaSym {
   const double2 *su;
   int32_t c0;
   const TileArgs &T;
-  double acc[kRB];
+  double acc[NR];
   template <int D>
   __device__ __forceinline__ void row() {
     double2 u[9];
     static_for([&](auto k) {
       constexpr int OY = decltype(k)::value - 4;
-      if constexpr (row_cell_needed(D, OY)) u[OY + 4] = su[c0 + D * kW68 + OY];
+      if constexpr (row_cell_needed(D, OY, NR)) u[OY + 4] = su[c0 + D * kW68 + OY];
     }, std::make_integer_sequence<int, 9>{});
     double sx[5], dy[5];
     static_for([&](auto k) {
       constexpr int OY = decltype(k)::value;
-      if constexpr (row_cell_needed(D, OY)) {
+      if constexpr (row_cell_needed(D, OY, NR)) {
         if constexpr (OY == 0) {
           sx[0] = u[4].x;
         } else {
@@ -908,9 +908,9 @@ struct ThetaSym {
           }
         }, std::make_integer_sequence<int, 5>{});
       }
-    }, std::make_integer_sequence<int, kRB>{});
+    }, std::make_integer_sequence<int, NR>{});
 #pragma unroll
-    for (int r = 0; r < kRB; r++) pin(acc[r]);
+    for (int r = 0; r < NR; r++) pin(acc[r]);
     __builtin_amdgcn_sched_barrier(0);
   }
 };
@@ -920,21 +920,22 @@ struct ThetaSym {
 // the theta_i part vanishes by the stencil's symmetry. The window holds theta scaled by the
 // instance's th_scale = (lambda - mu) / mu and the caller scales the sum by its f_scale = mu, so the
 // class tables hold no material. cu: the lane's reference displacement c; fx, fy start at
-// -(sum of B) ubar_i.
+// -(sum of B) ubar_i. NR: the lane's nodes (consecutive rows).
+template <int NR>
 struct ForceSym {
   const double2 *su;
   const double *sth;
   int32_t c0;
   const TileArgs &T;
   double2 cu;
-  double fx[kRB], fy[kRB];
+  double fx[NR], fy[NR];
   template <int D>
   __device__ __forceinline__ void row() {
     double2 u[9];
     double th[9];
     static_for([&](auto k) {
       constexpr int OY = decltype(k)::value - 4;
-      if constexpr (row_cell_needed(D, OY)) {
+      if constexpr (row_cell_needed(D, OY, NR)) {
         const double2 v = su[c0 + D * kW68 + OY];
         u[OY + 4] = make_double2(v.x - cu.x, v.y - cu.y);
         th[OY + 4] = sth[c0 + D * kW68 + OY];
@@ -943,7 +944,7 @@ struct ForceSym {
     double st[5], dt[5], sx[5], dx[5], sy[5], dy[5];
     static_for([&](auto k) {
       constexpr int OY = decltype(k)::value;
-      if constexpr (row_cell_needed(D, OY)) {
+      if constexpr (row_cell_needed(D, OY, NR)) {
         if constexpr (OY == 0) {
           st[0] = th[4];
           sx[0] = u[4].x;
@@ -977,9 +978,9 @@ struct ForceSym {
           }
         }, std::make_integer_sequence<int, 5>{});
       }
-    }, std::make_integer_sequence<int, kRB>{});
+    }, std::make_integer_sequence<int, NR>{});
 #pragma unroll
-    for (int r = 0; r < kRB; r++) {
+    for (int r = 0; r < NR; r++) {
       pin(fx[r]);
       pin(fy[r]);
     }
@@ -999,7 +1000,7 @@ __device__ __forceinline__ void theta_part_quarter(const StepArgs &a, const Step
                                                    double2 *su, int32_t t, int32_t b) {
   const TileArgs &T = a.tile;
   const int32_t *nbr = T.tile_nbr + 9 * t;
-  const int32_t lane = threadIdx.x;
+  const int32_t lane = threadIdx.x & 63;
   int8_t *ok = reinterpret_cast<int8_t *>(su + 12 * kW68);
   for (int32_t c = lane; c < 12 * kW68; c += 64) {
     const int32_t s = window_slot68(nbr, 4 * b * kW68 + c);
@@ -1123,101 +1124,126 @@ __global__ void __launch_bounds__(kBlock) k_abc_uv(StepArgs a, StepState st, Rin
   if (row < a.n_abc) abc_uv_rows16(a, st, r, row, threadIdx.x & (kRowLanes - 1));
 }
 
-// Dilatation of the full tiles, one wavefront per tile (full_list order); blocks past the full
-// tiles' (rounded up to a multiple of 8) take the partial tiles, four wavefronts per tile, so they
-// fill the tail of the full tiles' launch instead of following it. With nrow_blocks > 0 (a
-// multiple of 8) the first nrow_blocks blocks take the ABC rows instead, four per block: G_theta
-// and G_u / G_v read only the previous steps' state, so the step's ABC work runs in the same launch
-// as the dilatation.
-__global__ void __launch_bounds__(64, 4) k_theta_rb(StepArgs a, StepState st, RingU ring,
-                                                     int32_t nrow_blocks) {
-  __shared__ double2 su[kCells68];
+// The register-blocked tile kernels take NR nodes per lane: NR = 4, one wavefront per tile; NR = 2,
+// two wavefronts per tile sharing its window (half the LDS per wavefront, so twice the wavefronts per
+// CU, for more rows swept per node). Lane t of the tile's 256 / NR threads owns the nodes
+// (NR (t / 16) + r, t % 16), r < NR. Both fill the window with the same 9 pieces (the wavefronts of
+// an NR = 2 block take alternate pieces), and their node arithmetic is the same, so they give the
+// same bits.
+template <int NR, class F>
+__device__ __forceinline__ void fill_tile_window(const int32_t *nb, int32_t t, F &&f) {
+  if constexpr (NR == 4) {
+    fill_window68(nb, t, f, std::make_integer_sequence<int, 9>{});
+  } else {
+    if ((t >> 6) == 0) fill_window68(nb, t & 63, f, std::integer_sequence<int, 0, 2, 4, 6, 8>{});
+    else fill_window68(nb, t & 63, f, std::integer_sequence<int, 1, 3, 5, 7>{});
+  }
+}
+
+// LDS of theta_part_quarter: 12 x 24 cells of u, then their masks as bytes (in double2 units).
+constexpr int kQuarterCells = 12 * kW68 + (12 * kW68 + 15) / 16;
+
+// Dilatation of the full tiles, one tile per block (full_list order); blocks past the full tiles'
+// (rounded up to a multiple of 8) take the partial tiles, a quarter of a tile per wavefront, so they
+// fill the tail of the full tiles' launch instead of following it. With nrow_blocks > 0 (a multiple
+// of 8) the first nrow_blocks blocks take the ABC rows instead, four per wavefront: G_theta and
+// G_u / G_v read only the previous steps' state, so the step's ABC work runs in the same launch as
+// the dilatation.
+template <int NR>
+__global__ void __launch_bounds__(256 / NR, NR == 4 ? 4 : 6) k_theta_rb(StepArgs a, StepState st,
+                                                                        RingU ring,
+                                                                        int32_t nrow_blocks) {
+  constexpr int kWaves = 4 / NR;
+  __shared__ double2 su[kCells68 > kWaves * kQuarterCells ? kCells68 : kWaves * kQuarterCells];
   const TileArgs &T = a.tile;
+  const int32_t t = threadIdx.x, wv = t >> 6;
   if ((int32_t)blockIdx.x < nrow_blocks) {
-    const int32_t nrb = (a.n_abc + 3) / 4;
-    const int32_t row = tile_of_block(blockIdx.x, nrb) * 4 + (int32_t)threadIdx.x / kRowLanes;
+    const int32_t nrb = (a.n_abc + 4 * kWaves - 1) / (4 * kWaves);
+    const int32_t row = tile_of_block(blockIdx.x, nrb) * 4 * kWaves + t / kRowLanes;
     if (row < a.n_abc) {
-      abc_theta_rows16(a, st, row, threadIdx.x & (kRowLanes - 1));
-      abc_uv_rows16(a, st, ring, row, threadIdx.x & (kRowLanes - 1));
+      abc_theta_rows16(a, st, row, t & (kRowLanes - 1));
+      abc_uv_rows16(a, st, ring, row, t & (kRowLanes - 1));
     }
     return;
   }
   const int32_t b = blockIdx.x - nrow_blocks;
   const int32_t nfb = (T.n_full + 7) / 8 * 8;
   if (b >= nfb) {
-    const int32_t pb = b - nfb;
-    if ((pb >> 2) < T.n_part) theta_part_quarter(a, st, su, T.part_list[pb >> 2], pb & 3);
+    const int32_t pq = (b - nfb) * kWaves + wv;  // quarter pq & 3 of partial tile pq >> 2
+    if ((pq >> 2) < T.n_part)
+      theta_part_quarter(a, st, su + wv * kQuarterCells, T.part_list[pq >> 2], pq & 3);
     return;
   }
   const int32_t q = tile_of_block(b, T.n_full);
   if (q >= T.n_full) return;
-  const int32_t t = T.full_list[q];
+  const int32_t tl = T.full_list[q];
   int32_t nb[9];
 #pragma unroll
-  for (int k = 0; k < 9; k++) nb[k] = T.tile_nbr[9 * t + k];
-  const int32_t lane = threadIdx.x;
-  const int32_t ly = lane & 15, lx0 = (lane >> 4) * kRB;
-  int8_t kd[kRB];
+  for (int k = 0; k < 9; k++) nb[k] = T.tile_nbr[9 * tl + k];
+  const int32_t ly = t & 15, lx0 = (t >> 4) * NR;
+  int8_t kd[NR];
 #pragma unroll
-  for (int r = 0; r < kRB; r++) kd[r] = a.kind[t * (kTile * kTile) + (lx0 + r) * kTile + ly];
-  fill_window68(nb, lane, [&](WinCell w) { su[w.cell] = make_double2(st.px[w.slot], st.py[w.slot]); },
-                std::make_integer_sequence<int, 9>{});
+  for (int r = 0; r < NR; r++) kd[r] = a.kind[tl * (kTile * kTile) + (lx0 + r) * kTile + ly];
+  fill_tile_window<NR>(nb, t, [&](WinCell w) { su[w.cell] = make_double2(st.px[w.slot], st.py[w.slot]); });
   __syncthreads();
-  ThetaSym P{su, (lx0 + 4) * kW68 + ly + 4, T, {}};
+  ThetaSym<NR> P{su, (lx0 + 4) * kW68 + ly + 4, T, {}};
 #pragma unroll
-  for (int r = 0; r < kRB; r++) P.acc[r] = 0.0;
-  sym_sweep(P, std::make_integer_sequence<int, kRB + 8>{});
+  for (int r = 0; r < NR; r++) P.acc[r] = 0.0;
+  sym_sweep(P, std::make_integer_sequence<int, NR + 8>{});
 #pragma unroll
-  for (int r = 0; r < kRB; r++)
-    if (kd[r] == 1) st.theta[t * (kTile * kTile) + (lx0 + r) * kTile + ly] = P.acc[r];
+  for (int r = 0; r < NR; r++)
+    if (kd[r] == 1) st.theta[tl * (kTile * kTile) + (lx0 + r) * kTile + ly] = P.acc[r];
 }
 
-// Force pass of the full tiles with the fused Verlet corrector and predictor; one wavefront per
-// tile (full_list order).
-__global__ void __launch_bounds__(64, 3) k_force_rb(StepArgs a, StepState st) {
+// Force pass of the full tiles with the fused Verlet corrector and predictor; one tile per block
+// (full_list order), NR nodes per lane.
+template <int NR>
+__global__ void __launch_bounds__(256 / NR, NR == 4 ? 3 : 4) k_force_rb(StepArgs a, StepState st) {
   __shared__ double2 su[kCells68];
   __shared__ double sth[kCells68];
   const TileArgs &T = a.tile;
   const int32_t q = tile_of_block(blockIdx.x, T.n_full);
   if (q >= T.n_full) return;
-  const int32_t t = T.full_list[q];
+  const int32_t tl = T.full_list[q];
   int32_t nb[9];
 #pragma unroll
-  for (int k = 0; k < 9; k++) nb[k] = T.tile_nbr[9 * t + k];
-  const int32_t lane = threadIdx.x;
-  const int32_t ly = lane & 15, lx0 = (lane >> 4) * kRB;
+  for (int k = 0; k < 9; k++) nb[k] = T.tile_nbr[9 * tl + k];
+  const int32_t t = threadIdx.x;
+  const int32_t ly = t & 15, lx0 = (t >> 4) * NR;
   // the epilogue's inputs first: their loads overlap the window's
-  int8_t kd[kRB];
-  double2 vi[kRB], ai[kRB];
+  int8_t kd[NR];
+  double2 vi[NR], ai[NR];
 #pragma unroll
-  for (int r = 0; r < kRB; r++) {
-    const int32_t i = t * (kTile * kTile) + (lx0 + r) * kTile + ly;
+  for (int r = 0; r < NR; r++) {
+    const int32_t i = tl * (kTile * kTile) + (lx0 + r) * kTile + ly;
     kd[r] = a.kind[i];
     vi[r] = make_double2(st.vx[i], st.vy[i]);
     ai[r] = make_double2(st.ax[i], st.ay[i]);
   }
-  const double ratio = a.inst[T.tile_inst[t]].th_scale;
-  fill_window68(nb, lane, [&](WinCell w) {
+  const double ratio = a.inst[T.tile_inst[tl]].th_scale;
+  fill_tile_window<NR>(nb, t, [&](WinCell w) {
     su[w.cell] = make_double2(st.px[w.slot], st.py[w.slot]);
     sth[w.cell] = ratio * st.theta[w.slot];
-  }, std::make_integer_sequence<int, 9>{});
+  });
   __syncthreads();
   const int32_t c0 = (lx0 + 4) * kW68 + ly + 4;
-  double2 ui[kRB];
+  double2 ui[NR];
 #pragma unroll
-  for (int r = 0; r < kRB; r++) ui[r] = su[c0 + r * kW68];
-  ForceSym P{su, sth, c0, T, ui[1], {}, {}};
+  for (int r = 0; r < NR; r++) ui[r] = su[c0 + r * kW68];
+  // c: the displacement of row 4 floor(lx0 / 4) + 1 of the lane's column, the same node for NR = 4
+  // and NR = 2 (the same bits)
+  ForceSym<NR> P{su, sth, c0, T, su[((lx0 & ~3) + 5) * kW68 + ly + 4], {}, {}};
 #pragma unroll
-  for (int r = 0; r < kRB; r++) {
+  for (int r = 0; r < NR; r++) {
     P.fx[r] = -T.wbsum[0] * (ui[r].x - P.cu.x);
     P.fy[r] = -T.wbsum[1] * (ui[r].y - P.cu.y);
   }
-  sym_sweep(P, std::make_integer_sequence<int, kRB + 8>{});
-  const InstParams &I = a.inst[T.tile_inst[t]];
+  sym_sweep(P, std::make_integer_sequence<int, NR + 8>{});
+  const InstParams &I = a.inst[T.tile_inst[tl]];
   const double scale = I.f_scale;
 #pragma unroll
-  for (int r = 0; r < kRB; r++) {
-    const int32_t i = t * (kTile * kTile) + (lx0 + r) * kTile + ly;
+  for (int r = 0; r < NR; r++) {
+    const int32_t i = tl * (kTile * kTile) + (lx0 + r) * kTile + ly;
     if (kd[r] != 0)
       node_update(I, st, i, kd[r], ui[r], vi[r], ai[r], scale * P.fx[r], scale * P.fy[r]);
   }
@@ -1612,9 +1638,16 @@ hipError_t launch_theta(const StepArgs &a, const StepState &st, bool fast, hipSt
   if (fast) {
     const TileArgs &T = a.tile;
     if (T.theta_kernel == kBlocked) {
-      // the partial tiles in the same launch (4 blocks each) unless another stream takes them
-      const int32_t nb = list_blocks(T.n_full) + (with_part ? 4 * T.n_part : 0);
-      if (nb > 0) hipLaunchKernelGGL(k_theta_rb, dim3(nb), dim3(64), 0, s, a, st, RingU{}, 0);
+      // the partial tiles in the same launch (a quarter per wavefront) unless another stream takes
+      // them
+      const int32_t np = with_part ? T.n_part : 0;
+      if (T.theta_nr == 2) {
+        const int32_t nb = list_blocks(T.n_full) + 2 * np;
+        if (nb > 0) hipLaunchKernelGGL(k_theta_rb<2>, dim3(nb), dim3(128), 0, s, a, st, RingU{}, 0);
+      } else {
+        const int32_t nb = list_blocks(T.n_full) + 4 * np;
+        if (nb > 0) hipLaunchKernelGGL(k_theta_rb<4>, dim3(nb), dim3(64), 0, s, a, st, RingU{}, 0);
+      }
       return hipGetLastError();
     } else if (T.theta_kernel == kUnrolled) {
       hipLaunchKernelGGL(k_theta_t68, dim3(tile_blocks(a)), dim3(kBlock), 0, s, a, st,
@@ -1637,9 +1670,15 @@ hipError_t launch_theta_rows(const StepArgs &a, const StepState &st, const RingU
     if ((e = launch_abc_uv(a, st, r, s)) != hipSuccess) return e;
     return launch_theta(a, st, true, s, true);
   }
-  const int32_t nrow = ((a.n_abc + 3) / 4 + 7) / 8 * 8;
-  const int32_t nb = nrow + list_blocks(T.n_full) + 4 * T.n_part;
-  if (nb > 0) hipLaunchKernelGGL(k_theta_rb, dim3(nb), dim3(64), 0, s, a, st, r, nrow);
+  if (T.theta_nr == 2) {
+    const int32_t nrow = ((a.n_abc + 7) / 8 + 7) / 8 * 8;
+    const int32_t nb = nrow + list_blocks(T.n_full) + 2 * T.n_part;
+    if (nb > 0) hipLaunchKernelGGL(k_theta_rb<2>, dim3(nb), dim3(128), 0, s, a, st, r, nrow);
+  } else {
+    const int32_t nrow = ((a.n_abc + 3) / 4 + 7) / 8 * 8;
+    const int32_t nb = nrow + list_blocks(T.n_full) + 4 * T.n_part;
+    if (nb > 0) hipLaunchKernelGGL(k_theta_rb<4>, dim3(nb), dim3(64), 0, s, a, st, r, nrow);
+  }
   return hipGetLastError();
 }
 
@@ -1659,7 +1698,10 @@ hipError_t launch_force(const StepArgs &a, const StepState &st, bool fast, hipSt
   if (fast) {
     const int32_t nf = a.tile.n_full;
     if (a.tile.force_kernel == kBlocked) {
-      if (nf > 0) hipLaunchKernelGGL(k_force_rb, dim3(list_blocks(nf)), dim3(64), 0, s, a, st);
+      if (nf > 0 && a.tile.force_nr == 2)
+        hipLaunchKernelGGL(k_force_rb<2>, dim3(list_blocks(nf)), dim3(128), 0, s, a, st);
+      else if (nf > 0)
+        hipLaunchKernelGGL(k_force_rb<4>, dim3(list_blocks(nf)), dim3(64), 0, s, a, st);
     } else if (a.tile.force_kernel == kUnrolled) {
       if (nf > 0)
         hipLaunchKernelGGL(k_force_t68<true>, dim3(list_blocks(nf)), dim3(kBlock), 0, s, a, st,

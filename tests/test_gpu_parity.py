@@ -238,6 +238,33 @@ def test_step_schedules_give_the_same_bits(graph, r40_problem):
 
 
 @pytest.mark.parametrize("graph", [True, False])
+def test_two_wavefronts_per_tile_give_the_same_bits(graph, monkeypatch, r40_problem):
+    """The register-blocked dilatation and force kernels with 4 nodes per lane (a wavefront per
+    tile) or 2 (two wavefronts per tile sharing its window), in every combination: the same node
+    arithmetic, so the same bits, on one domain and on a batch, in both step schedules."""
+    say("nodes per lane", graph)
+    g12, _ = load_golden("abc_r12_nu033")
+    out = []
+    for nodes in ("4,4", "2,2", "2,4", "4,2"):
+        monkeypatch.setenv("LPS_RB_NODES", nodes)
+        for sched in (1, 3):
+            with solver.Solver(r40_problem, variant=3, use_graph=graph, concurrent=sched) as s:
+                recs = s.run(95)
+                one = (s.fetch(), recs)
+            with solver.Batch([g12, r40_problem], variant=3, use_graph=graph, concurrent=sched) as b:
+                brecs = b.run(95)
+                many = ([b.fetch(i) for i in range(2)], brecs)
+            out.append((one, many))
+    (a1, am) = out[0]
+    for b1, bm in out[1:]:
+        for k in ("u", "v", "a", "theta"):
+            assert np.array_equal(a1[0][k], b1[0][k]), k
+            for i in range(2):
+                assert np.array_equal(am[0][i][k], bm[0][i][k]), (i, k)
+        assert a1[1] == b1[1] and am[1] == bm[1]
+
+
+@pytest.mark.parametrize("graph", [True, False])
 def test_timing_mode_reports_pass_times_and_changes_no_result(graph, r40_problem):
     """lps_set_timing: per-pass events (mode 1) or the force kernel's two events only (mode 2),
     inside the 30-step graphs when graphs are on; the state afterwards is the same bits as an
