@@ -37,7 +37,7 @@ StepArgs lps_ctx::args() const {
   a.sell_off = hb.sell_off; a.sell_w = hb.sell_w; a.sell_col = hb.sell_col;
   a.m = m; a.c2m = c2m; a.is_bc = is_bc;
   a.abc_off = abc_off; a.cloud = cloud; a.gu = gu; a.gv = gv; a.gth = gth;
-  a.abc_gofs = abc_gofs; a.abc_xf = abc_xf; a.abc_pmap = abc_pmap;
+  a.abc_gofs = abc_gofs; a.abc_xf = abc_xf; a.abc_pmap = abc_pmap; a.abc_in = abc_in;
   a.theta_method = opt.theta_method;
   a.inst = inst_d;
   a.tile = tile;
@@ -718,6 +718,8 @@ static int32_t create_impl(lps_ctx *c, const lps_problem *p) {
                  " members stored, the rest lattice images";
     CTX_TRY(c, c->upload(&c->abc_off, off));
     CTX_TRY(c, c->upload(&c->cloud, hcl));
+    CTX_TRY(c, c->alloc(&c->abc_in, hcl.size()));
+    CTX_TRY(c, launch_member_in((int64_t)hcl.size(), c->cloud, c->kind, c->abc_in, s));
     CTX_TRY(c, c->upload(&c->gu, hgu));
     CTX_TRY(c, c->upload(&c->gv, hgv));
     if (!hgt.empty()) CTX_TRY(c, c->upload(&c->gth, hgt));
@@ -1152,6 +1154,8 @@ static int32_t merge_batch(lps_ctx *c, const std::vector<lps_ctx *> &sub) {
     CTX_TRY(c, c->upload(&c->abc_xf, xf));
     CTX_TRY(c, c->upload(&c->abc_pmap, pmap));
     CTX_TRY(c, c->upload(&c->cloud, cl));
+    CTX_TRY(c, c->alloc(&c->abc_in, cl.size()));
+    CTX_TRY(c, launch_member_in((int64_t)cl.size(), c->cloud, c->kind, c->abc_in, c->stream));
   }
   // instances
   c->inst_h.clear();

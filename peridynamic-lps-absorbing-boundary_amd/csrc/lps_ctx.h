@@ -74,6 +74,7 @@ struct lps_ctx {
   int32_t *abc_off = nullptr, *cloud = nullptr, *abc_gofs = nullptr;
   int8_t *abc_xf = nullptr;
   uint8_t *abc_pmap = nullptr;
+  uint8_t *abc_in = nullptr;  // per member: an interior node (the row kernels read its U from the ring)
   int64_t abc_members = 0, abc_block_members = 0;
   bool abc_shared_blocks = false;  // lattice images share their canonical row's block
   double *gu = nullptr, *gv = nullptr, *gth = nullptr;

@@ -118,6 +118,8 @@ struct StepArgs {
   const int32_t *abc_gofs;         // n_abc
   const int8_t *abc_xf;            // n_abc: bit 0 swap x and y, bit 1 negate the cross terms
   const uint8_t *abc_pmap;         // per member
+  const uint8_t *abc_in;           // per member: 1 for an interior node (kind 1), whose U^n lives
+                                   // in the prediction ring (fast path)
   const double *gu, *gv;
   const double *gth;
   int32_t theta_method;
@@ -213,6 +215,9 @@ hipError_t launch_force_part(const StepArgs &a, const StepState &st, hipStream_t
 // The dilatation with the ABC rows' G_theta and G_u / G_v in the same launch (extra workgroups of
 // the register-blocked kernel; other families: the row kernels then the dilatation, on s).
 hipError_t launch_theta_rows(const StepArgs &a, const StepState &st, const RingU &r, hipStream_t s);
+// abc_in of every ABC cloud member (n members of cloud): 1 where the member's slot is of kind 1.
+hipError_t launch_member_in(int64_t n, const int32_t *cloud, const int8_t *kind, uint8_t *out,
+                            hipStream_t s);
 hipError_t launch_abc_theta(const StepArgs &a, const StepState &st, hipStream_t s);
 hipError_t launch_abc_uv(const StepArgs &a, const StepState &st, const RingU &r, hipStream_t s);
 hipError_t launch_abc_predict(const StepArgs &a, const StepState &st, hipStream_t s);

@@ -1056,11 +1056,23 @@ __device__ __forceinline__ void abc_theta_rows16(const StepArgs &a, const StepSt
     const double *g = a.gth + 2 * (int64_t)a.abc_gofs[row];  // G_theta needs no lattice transform
     const int32_t *cl = a.cloud + o;
     double acc = 0.0;
-    for (int jj = q; jj < L; jj += kRowLanes) {
-      const int32_t k = cl[jj];
-      if (k >= 0) {
+    // four members per lane at a time (q, q + 16, q + 32, q + 48): their loads are unconditional
+    // (clamped indices), so the four members' gathers are in flight together; a member past the
+    // cloud or a pad adds nothing
+    for (int base = q; base < L; base += 4 * kRowLanes) {
+      int32_t k[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int jj = base + u * kRowLanes;
+        k[u] = jj < L ? cl[jj] : -1;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int jj = min(base + u * kRowLanes, L - 1);
+        const int32_t ks = k[u] >= 0 ? k[u] : 0;
         const double2 gm = *(const double2 *)(g + 2 * a.abc_pmap[o + jj]);
-        acc += gtheta_term_exact(gm.x, gm.y, st.thn[k], st.thnm1[k]);
+        const double t = gtheta_term_exact(gm.x, gm.y, st.thn[ks], st.thnm1[ks]);
+        acc = k[u] >= 0 ? acc + t : acc;
       }
     }
     th = group16_sum(acc);
@@ -1083,19 +1095,36 @@ __device__ __forceinline__ void abc_uv_rows16(const StepArgs &a, const StepState
   const int xf = a.abc_xf[row];
   const int32_t *cl = a.cloud + o;
   double Ux = 0.0, Uy = 0.0, Vx = 0.0, Vy = 0.0;
-  for (int jj = q; jj < L; jj += kRowLanes) {
-    const int32_t k = cl[jj];
-    if (k < 0) continue;
-    double4 g0, g1, h0, h1;
-    abc_uv_coeffs(a, o, L, gb, xf, jj, g0, g1, h0, h1);
-    const bool in = a.kind[k] == 1;
-    const double ux = (in ? r.ux : st.ux)[k], uy = (in ? r.uy : st.uy)[k];
-    const double uxm = (in ? r.uxm : st.uxm)[k], uym = (in ? r.uym : st.uym)[k];
-    const double vx = st.vx[k], vy = st.vy[k], vxm = st.vxm[k], vym = st.vym[k];
-    Ux += g0.x * ux + g0.y * uy + g0.z * uxm + g0.w * uym;
-    Uy += g1.x * ux + g1.y * uy + g1.z * uxm + g1.w * uym;
-    Vx += h0.x * vx + h0.y * vy + h0.z * vxm + h0.w * vym;
-    Vy += h1.x * vx + h1.y * vy + h1.z * vxm + h1.w * vym;
+  // four members per lane at a time, loads unconditional (see abc_theta_rows16)
+  for (int base = q; base < L; base += 4 * kRowLanes) {
+    int32_t k[4];
+    bool in[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int jj = base + u * kRowLanes;
+      k[u] = jj < L ? cl[jj] : -1;
+      in[u] = jj < L && a.abc_in[o + jj];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int jj = min(base + u * kRowLanes, L - 1);
+      const int32_t ks = k[u] >= 0 ? k[u] : 0;
+      double4 g0, g1, h0, h1;
+      abc_uv_coeffs(a, o, L, gb, xf, jj, g0, g1, h0, h1);
+      const double ux = (in[u] ? r.ux : st.ux)[ks], uy = (in[u] ? r.uy : st.uy)[ks];
+      const double uxm = (in[u] ? r.uxm : st.uxm)[ks], uym = (in[u] ? r.uym : st.uym)[ks];
+      const double vx = st.vx[ks], vy = st.vy[ks], vxm = st.vxm[ks], vym = st.vym[ks];
+      const double tux = g0.x * ux + g0.y * uy + g0.z * uxm + g0.w * uym;
+      const double tuy = g1.x * ux + g1.y * uy + g1.z * uxm + g1.w * uym;
+      const double tvx = h0.x * vx + h0.y * vy + h0.z * vxm + h0.w * vym;
+      const double tvy = h1.x * vx + h1.y * vy + h1.z * vxm + h1.w * vym;
+      if (k[u] >= 0) {
+        Ux += tux;
+        Uy += tuy;
+        Vx += tvx;
+        Vy += tvy;
+      }
+    }
   }
   Ux = group16_sum(Ux);
   Uy = group16_sum(Uy);
@@ -1679,6 +1708,22 @@ hipError_t launch_theta_rows(const StepArgs &a, const StepState &st, const RingU
     const int32_t nb = nrow + list_blocks(T.n_full) + 4 * T.n_part;
     if (nb > 0) hipLaunchKernelGGL(k_theta_rb<4>, dim3(nb), dim3(64), 0, s, a, st, r, nrow);
   }
+  return hipGetLastError();
+}
+
+__global__ void __launch_bounds__(kBlock) k_member_in(int64_t n, const int32_t *cloud,
+                                                       const int8_t *kind, uint8_t *out) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const int32_t k = cloud[i];
+  out[i] = k >= 0 && kind[k] == 1;
+}
+
+hipError_t launch_member_in(int64_t n, const int32_t *cloud, const int8_t *kind, uint8_t *out,
+                            hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_member_in, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
+                     n, cloud, kind, out);
   return hipGetLastError();
 }
 
