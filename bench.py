@@ -500,10 +500,11 @@ def main():
     ap.add_argument("--serial", action="store_true",
                     help="run the ABC rows' kernels on the main stream instead of beside the "
                          "tile kernels on a second stream")
-    ap.add_argument("--schedule", type=int, default=None, choices=(0, 1, 2, 3),
+    ap.add_argument("--schedule", type=int, default=None, choices=(0, 1, 2, 3, 4),
                     help="step schedule: 0 ABC rows after the tile kernels, 1 on a second stream "
                          "beside them, 2 also the partial tiles on a third, 3 the ABC rows inside "
-                         "the dilatation launch (default: the library's)")
+                         "the dilatation launch, 4 as 1 with the deep tiles' dilatation and force "
+                         "in one launch (default: the library's)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=90)
     ap.add_argument("--workers", type=int, default=None)

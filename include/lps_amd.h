@@ -70,8 +70,13 @@ typedef struct lps_options {
   int32_t concurrent;    /* fast path: 1 runs the ABC rows' kernels (G_theta, G_u, G_v) on a
                             second stream beside the tile kernels; 2 also the partial tiles'
                             kernels on a third stream beside the full tiles'; 3 runs the ABC rows
-                            as extra workgroups of the dilatation launch, one stream; 0 the row
-                            kernels after the tile kernels, one stream */
+                            as extra workgroups of the dilatation launch, one stream; 4 as 1,
+                            with the "deep" full tiles (full 3 x 3 neighbourhood, interior nodes
+                            only in the 24 x 24 window) computing dilatation and force in one
+                            launch while the other tiles' dilatation runs on a third stream; 0
+                            the row kernels after the tile kernels, one stream. Every schedule
+                            gives the same bits except 2 (another partial-tile dilatation kernel,
+                            within rounding) */
 } lps_options;
 
 /* One output record: the quantities src/main.cpp:986-1062 computes every 30 steps. */

@@ -592,6 +592,30 @@ static int32_t create_impl(lps_ctx *c, const lps_problem *p) {
     c->tile.part_list = d_pl;
     c->tile.n_full = (int32_t)full_list.size();
     c->tile.n_part = (int32_t)part_list.size();
+    // deep full tiles: the 3 x 3 tiles around them full and every cell of the 24 x 24 window an
+    // interior node (kind 1); only with the 68-offset stencil (halo 4). The lists stay alive until
+    // the stream synchronisation below (asynchronous uploads).
+    std::vector<int32_t> deep_list, shallow_list;
+    {
+      for (int32_t t : full_list) {
+        bool deep = K == 4;
+        for (int32_t k = 0; k < 9 && deep; k++) {
+          const int32_t tt = nbr[(size_t)t * 9 + k];
+          deep = tt >= 0 && full[tt];
+        }
+        for (int32_t cc = 0; cc < (kTile + 8) * (kTile + 8) && deep; cc++) {
+          const int32_t rx = cc / (kTile + 8) - 4, ry = cc % (kTile + 8) - 4;
+          const int32_t ax = rx < 0 ? 0 : (rx < kTile ? 1 : 2), ay = ry < 0 ? 0 : (ry < kTile ? 1 : 2);
+          const int32_t tt = nbr[(size_t)t * 9 + ax * 3 + ay];
+          deep = kind[(size_t)tt * kTile * kTile + (rx - (ax - 1) * kTile) * kTile + (ry - (ay - 1) * kTile)] == 1;
+        }
+        (deep ? deep_list : shallow_list).push_back(t);
+      }
+      CTX_TRY(c, c->upload(&c->deep_list, deep_list));
+      CTX_TRY(c, c->upload(&c->shallow_list, shallow_list));
+      c->n_deep = (int32_t)deep_list.size();
+      c->n_shallow = (int32_t)shallow_list.size();
+    }
     c->abc_in_part = true;  // every ABC node in a partial tile (its horizon reaches past the rim)
     for (int32_t r = 0; r < c->n_abc; r++)
       if (full[abc_slot[r] / (kTile * kTile)]) c->abc_in_part = false;
@@ -630,7 +654,8 @@ static int32_t create_impl(lps_ctx *c, const lps_problem *p) {
         if (full[t])
           for (int32_t k = 0; k < kTile * kTile; k++) c->n_full_nodes += kind[(size_t)t * kTile * kTile + k] != 0;
       c->path = " (16x16 tiles, " + std::to_string(ntl) + " tiles, " + std::to_string(nfull) +
-                " full, " + std::to_string(nt) + " bonds, halo " + std::to_string(K) + ")";
+                " full, " + std::to_string(c->n_deep) + " deep, " + std::to_string(nt) +
+                " bonds, halo " + std::to_string(K) + ")";
     } else {
       c->path += " (" + std::to_string(mis) + " horizons differ from the stencil)";
     }
@@ -872,8 +897,8 @@ static int32_t new_context(const lps_options *options, lps_ctx **out) {
   lps_ctx *c = new lps_ctx();
   if (options) c->opt = *options;
   else lps_default_options(&c->opt);
-  if (c->opt.concurrent < 0 || c->opt.concurrent > 3) {
-    g_create_error = "concurrent must be 0, 1, 2 or 3";
+  if (c->opt.concurrent < 0 || c->opt.concurrent > 4) {
+    g_create_error = "concurrent must be 0, 1, 2, 3 or 4";
     delete c;
     return LPS_ERR_ARG;
   }
@@ -1036,11 +1061,15 @@ static int32_t merge_batch(lps_ctx *c, const std::vector<lps_ctx *> &sub) {
       c->perm_h[c->slot_off[i] + k] = o < 0 ? -1 : (int32_t)(o + c->node_off[i]);
     }
   // tiles: neighbour tables and lists shifted by the instances' tile offsets
-  std::vector<int32_t> nbr, full_list, part_list, tinst(NT), txy;
+  std::vector<int32_t> nbr, full_list, part_list, tinst(NT), txy, deep_list, shallow_list;
   std::vector<int8_t> tfull;
   for (int32_t i = 0; i < B; i++) {
     const TileArgs &T = sub[i]->tile;
     std::vector<int32_t> h;
+    if ((rc = to_host(c, h, sub[i]->deep_list, (size_t)sub[i]->n_deep))) return rc;
+    for (int32_t v : h) deep_list.push_back(v + c->tile_off[i]);
+    if ((rc = to_host(c, h, sub[i]->shallow_list, (size_t)sub[i]->n_shallow))) return rc;
+    for (int32_t v : h) shallow_list.push_back(v + c->tile_off[i]);
     if ((rc = to_host(c, h, T.tile_nbr, (size_t)9 * T.n_tiles))) return rc;
     for (int32_t v : h) nbr.push_back(v < 0 ? -1 : v + c->tile_off[i]);
     if ((rc = to_host(c, h, T.full_list, (size_t)T.n_full))) return rc;
@@ -1068,6 +1097,10 @@ static int32_t merge_batch(lps_ctx *c, const std::vector<lps_ctx *> &sub) {
   CTX_TRY(c, c->upload(&d_nbr, nbr));
   CTX_TRY(c, c->upload(&d_fl, full_list));
   CTX_TRY(c, c->upload(&d_pl, part_list));
+  CTX_TRY(c, c->upload(&c->deep_list, deep_list));
+  CTX_TRY(c, c->upload(&c->shallow_list, shallow_list));
+  c->n_deep = (int32_t)deep_list.size();
+  c->n_shallow = (int32_t)shallow_list.size();
   CTX_TRY(c, c->upload(&d_ti, tinst));
   CTX_TRY(c, c->upload(&d_txy, txy));
   CTX_TRY(c, c->upload(&d_full, tfull));
@@ -1188,7 +1221,7 @@ static int32_t merge_batch(lps_ctx *c, const std::vector<lps_ctx *> &sub) {
   c->path = "batch of " + std::to_string(B) + " instances: tile kernels: dilatation " +
             names[T.theta_kernel] + ", force " + names[T.force_kernel] + " (16x16 tiles, " +
             std::to_string(NT) + " tiles, " + std::to_string(n_full) + " full, " +
-            std::to_string(c->n) + " nodes, " + std::to_string(R) + " ABC rows)";
+            std::to_string(c->n_deep) + " deep, " + std::to_string(c->n) + " nodes, " + std::to_string(R) + " ABC rows)";
   if (c->abc_shared_blocks)
     c->path += "; ABC operators of " + std::to_string(GM) + " of " + std::to_string(NM) +
                " members stored, the rest lattice images";
@@ -1325,8 +1358,9 @@ static int32_t choose_variant(lps_ctx *c) {
     // two rounds of 3 warm-up and 10 timed steps per family, the minimum of the rounds; the
     // dilatation family by its pass, the force family by the rest of the step (with the partial
     // tiles on a third stream the force's share moves between the timed intervals)
-    double best_th = 1e300, best_fo = 1e300;
+    double best_th = 1e300, best_fo = 1e300, best_step = 1e300;
     double tth[3] = {1e300, 1e300, 1e300}, tfo[3] = {1e300, 1e300, 1e300};
+    double tst[3] = {1e300, 1e300, 1e300};
     for (int round = 0; round < 2; round++) {
       for (int32_t m : {kGeneric, kUnrolled, kBlocked}) {
         double t[4] = {0, 0, 0, 0};
@@ -1341,12 +1375,18 @@ static int32_t choose_variant(lps_ctx *c) {
         if (rc != LPS_OK) return rc;
         tth[m] = std::min(tth[m], t[0]);
         tfo[m] = std::min(tfo[m], t[3] - t[0]);
+        tst[m] = std::min(tst[m], t[3]);
       }
     }
     for (int32_t m : {kGeneric, kUnrolled, kBlocked}) {
       if (tth[m] < best_th) { best_th = tth[m]; th = m; }
       if (tfo[m] < best_fo) { best_fo = tfo[m]; fo = m; }
     }
+    // concurrent == 4 fuses the deep tiles' two passes when both are register-blocked, so its
+    // first interval holds force work: one family for both passes, by the whole step
+    if (c->opt.concurrent == 4)
+      for (int32_t m : {kGeneric, kUnrolled, kBlocked})
+        if (tst[m] < best_step) { best_step = tst[m]; th = fo = m; }
   }
   c->tile.theta_kernel = th;
   c->tile.force_kernel = fo;
@@ -1423,8 +1463,19 @@ static hipError_t step_once(lps_ctx *c, bool record_events) {
   // concurrent == 2: the partial tiles' kernels on a third stream, beside the full tiles' kernels
   const bool split = c->fast && c->opt.concurrent == 2 && a.tile.n_part > 0 &&
                      a.tile.theta_kernel == kBlocked && a.tile.force_kernel != kGeneric;
-  if (fork || split) {
+  // concurrent == 4: the deep full tiles' dilatation + force in one launch on the main stream, the
+  // other tiles' dilatation on the third stream beside it; the shallow full tiles' force then
+  // waits for that dilatation and for G_theta
+  const bool deep = c->fast && c->opt.concurrent == 4 && c->n_deep > 0 &&
+                    a.tile.theta_kernel == kBlocked && a.tile.force_kernel == kBlocked;
+  if (fork || split || deep) {
     if ((e = hipEventRecord(c->ev_fork[0], c->stream)) != hipSuccess) return e;
+  }
+  if (deep) {
+    if ((e = hipStreamWaitEvent(c->stream3, c->ev_fork[0], 0)) != hipSuccess) return e;
+    if ((e = launch_theta_list(a, st, c->shallow_list, c->n_shallow, c->stream3)) != hipSuccess)
+      return e;
+    if ((e = hipEventRecord(c->ev_part[0], c->stream3)) != hipSuccess) return e;
   }
   if (split) {
     if ((e = hipStreamWaitEvent(c->stream3, c->ev_fork[0], 0)) != hipSuccess) return e;
@@ -1451,8 +1502,13 @@ static hipError_t step_once(lps_ctx *c, bool record_events) {
   } else if (abc && (e = launch_abc_theta(a, st, c->stream)) != hipSuccess) {
     return e;
   }
-  if ((e = launch_theta(a, st, c->fast, c->stream, !split)) != hipSuccess) return e;
-  if (split && (e = hipStreamWaitEvent(c->stream, c->ev_part[0], 0)) != hipSuccess) return e;
+  if (deep) {
+    if ((e = launch_force_deep(a, st, c->deep_list, c->n_deep, c->stream)) != hipSuccess) return e;
+  } else if ((e = launch_theta(a, st, c->fast, c->stream, !split)) != hipSuccess) {
+    return e;
+  }
+  if ((split || deep) && (e = hipStreamWaitEvent(c->stream, c->ev_part[0], 0)) != hipSuccess)
+    return e;
   if (fork && (e = hipStreamWaitEvent(c->stream, c->ev_join[0], 0)) != hipSuccess) return e;
   if (a.n_abc > 0 && a.theta_method == LPS_THETA_IDW && (e = launch_idw(a, st, c->stream)) != hipSuccess)
     return e;
@@ -1464,7 +1520,12 @@ static hipError_t step_once(lps_ctx *c, bool record_events) {
   }
   const bool ev_force = record_events && !c->timing_closure;
   if (ev_force && (e = record_timing(c, ev[1])) != hipSuccess) return e;
-  if ((e = launch_force(a, st, c->fast, c->stream)) != hipSuccess) return e;
+  if (deep) {
+    if ((e = launch_force_list(a, st, c->shallow_list, c->n_shallow, c->stream)) != hipSuccess)
+      return e;
+  } else if ((e = launch_force(a, st, c->fast, c->stream)) != hipSuccess) {
+    return e;
+  }
   if (ev_force && (e = record_timing(c, ev[2])) != hipSuccess) return e;
   // the ABC nodes' prediction folded into the partial tiles' force pass (every ABC node lies in a
   // partial tile), which then waits for G_u / G_v; not with a closure attached, whose long ABC

@@ -46,6 +46,10 @@ struct lps_ctx {
   // G_u / G_v); else a kernel of its own after G_u / G_v, the partial tiles beside them
   bool fold_pred = true;
   int64_t n_full_nodes = 0;   // nodes in the full tiles
+  // the full tiles split for concurrent == 4: deep (the 3 x 3 neighbourhood full, interior nodes
+  // only in the 24 x 24 window: dilatation and force in one launch) and shallow (the others)
+  int32_t *deep_list = nullptr, *shallow_list = nullptr;
+  int32_t n_deep = 0, n_shallow = 0;
   // instances: one for lps_create, several for lps_create_batch
   std::vector<InstParams> inst_h;
   InstParams *inst_d = nullptr;

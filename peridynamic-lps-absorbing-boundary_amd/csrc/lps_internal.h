@@ -215,6 +215,17 @@ hipError_t launch_force_part(const StepArgs &a, const StepState &st, hipStream_t
 // The dilatation with the ABC rows' G_theta and G_u / G_v in the same launch (extra workgroups of
 // the register-blocked kernel; other families: the row kernels then the dilatation, on s).
 hipError_t launch_theta_rows(const StepArgs &a, const StepState &st, const RingU &r, hipStream_t s);
+// Register-blocked family on a list of full tiles: the dilatation of the list and of every partial
+// tile (launch_theta_list), the force of the list (launch_force_list), and the dilatation + force
+// of deep tiles in one launch (launch_force_deep: a full tile whose 3 x 3 neighbourhood is full
+// and whose 24 x 24 window holds interior nodes only; it reads u alone, so it needs no dilatation
+// pass in front of it, and writes its tile's theta for the other tiles' force).
+hipError_t launch_theta_list(const StepArgs &a, const StepState &st, const int32_t *list,
+                             int32_t count, hipStream_t s);
+hipError_t launch_force_list(const StepArgs &a, const StepState &st, const int32_t *list,
+                             int32_t count, hipStream_t s);
+hipError_t launch_force_deep(const StepArgs &a, const StepState &st, const int32_t *list,
+                             int32_t count, hipStream_t s);
 // abc_in of every ABC cloud member (n members of cloud): 1 where the member's slot is of kind 1.
 hipError_t launch_member_in(int64_t n, const int32_t *cloud, const int8_t *kind, uint8_t *out,
                             hipStream_t s);

@@ -872,7 +872,7 @@ __device__ __forceinline__ double w_bxy(const TileArgs &T) {  // 8 C c2 ox oy, O
 
 // Full-tile dilatation, row-symmetric: theta_i = sum_k C c1_k (u_j . o_k); c0 is the window cell
 // of the lane's first node, NR its nodes (consecutive rows).
-template <int NR>
+template <int NR, int W = kW68>
 struct ThetaSym {
   const double2 *su;
   int32_t c0;
@@ -883,7 +883,7 @@ struct ThetaSym {
     double2 u[9];
     static_for([&](auto k) {
       constexpr int OY = decltype(k)::value - 4;
-      if constexpr (row_cell_needed(D, OY, NR)) u[OY + 4] = su[c0 + D * kW68 + OY];
+      if constexpr (row_cell_needed(D, OY, NR)) u[OY + 4] = su[c0 + D * W + OY];
     }, std::make_integer_sequence<int, 9>{});
     double sx[5], dy[5];
     static_for([&](auto k) {
@@ -921,11 +921,11 @@ struct ThetaSym {
 // instance's th_scale = (lambda - mu) / mu and the caller scales the sum by its f_scale = mu, so the
 // class tables hold no material. cu: the lane's reference displacement c; fx, fy start at
 // -(sum of B) ubar_i. NR: the lane's nodes (consecutive rows).
-template <int NR>
+template <int NR, int WU = kW68, int WT = kW68>
 struct ForceSym {
   const double2 *su;
   const double *sth;
-  int32_t c0;
+  int32_t cu0, ct0;  // the lane's first node in the u window (row stride WU) and theta window (WT)
   const TileArgs &T;
   double2 cu;
   double fx[NR], fy[NR];
@@ -936,9 +936,9 @@ struct ForceSym {
     static_for([&](auto k) {
       constexpr int OY = decltype(k)::value - 4;
       if constexpr (row_cell_needed(D, OY, NR)) {
-        const double2 v = su[c0 + D * kW68 + OY];
+        const double2 v = su[cu0 + D * WU + OY];
         u[OY + 4] = make_double2(v.x - cu.x, v.y - cu.y);
-        th[OY + 4] = sth[c0 + D * kW68 + OY];
+        th[OY + 4] = sth[ct0 + D * WT + OY];
       }
     }, std::make_integer_sequence<int, 9>{});
     double st[5], dt[5], sx[5], dx[5], sy[5], dy[5];
@@ -1181,7 +1181,9 @@ constexpr int kQuarterCells = 12 * kW68 + (12 * kW68 + 15) / 16;
 template <int NR>
 __global__ void __launch_bounds__(256 / NR, NR == 4 ? 4 : 6) k_theta_rb(StepArgs a, StepState st,
                                                                         RingU ring,
-                                                                        int32_t nrow_blocks) {
+                                                                        int32_t nrow_blocks,
+                                                                        const int32_t *flist,
+                                                                        int32_t nfull) {
   constexpr int kWaves = 4 / NR;
   __shared__ double2 su[kCells68 > kWaves * kQuarterCells ? kCells68 : kWaves * kQuarterCells];
   const TileArgs &T = a.tile;
@@ -1196,16 +1198,16 @@ __global__ void __launch_bounds__(256 / NR, NR == 4 ? 4 : 6) k_theta_rb(StepArgs
     return;
   }
   const int32_t b = blockIdx.x - nrow_blocks;
-  const int32_t nfb = (T.n_full + 7) / 8 * 8;
+  const int32_t nfb = (nfull + 7) / 8 * 8;
   if (b >= nfb) {
     const int32_t pq = (b - nfb) * kWaves + wv;  // quarter pq & 3 of partial tile pq >> 2
     if ((pq >> 2) < T.n_part)
       theta_part_quarter(a, st, su + wv * kQuarterCells, T.part_list[pq >> 2], pq & 3);
     return;
   }
-  const int32_t q = tile_of_block(b, T.n_full);
-  if (q >= T.n_full) return;
-  const int32_t tl = T.full_list[q];
+  const int32_t q = tile_of_block(b, nfull);
+  if (q >= nfull) return;
+  const int32_t tl = flist[q];
   int32_t nb[9];
 #pragma unroll
   for (int k = 0; k < 9; k++) nb[k] = T.tile_nbr[9 * tl + k];
@@ -1227,13 +1229,15 @@ __global__ void __launch_bounds__(256 / NR, NR == 4 ? 4 : 6) k_theta_rb(StepArgs
 // Force pass of the full tiles with the fused Verlet corrector and predictor; one tile per block
 // (full_list order), NR nodes per lane.
 template <int NR>
-__global__ void __launch_bounds__(256 / NR, NR == 4 ? 3 : 4) k_force_rb(StepArgs a, StepState st) {
+__global__ void __launch_bounds__(256 / NR, NR == 4 ? 3 : 4) k_force_rb(StepArgs a, StepState st,
+                                                                        const int32_t *flist,
+                                                                        int32_t nfull) {
   __shared__ double2 su[kCells68];
   __shared__ double sth[kCells68];
   const TileArgs &T = a.tile;
-  const int32_t q = tile_of_block(blockIdx.x, T.n_full);
-  if (q >= T.n_full) return;
-  const int32_t tl = T.full_list[q];
+  const int32_t q = tile_of_block(blockIdx.x, nfull);
+  if (q >= nfull) return;
+  const int32_t tl = flist[q];
   int32_t nb[9];
 #pragma unroll
   for (int k = 0; k < 9; k++) nb[k] = T.tile_nbr[9 * tl + k];
@@ -1261,7 +1265,7 @@ __global__ void __launch_bounds__(256 / NR, NR == 4 ? 3 : 4) k_force_rb(StepArgs
   for (int r = 0; r < NR; r++) ui[r] = su[c0 + r * kW68];
   // c: the displacement of row 4 floor(lx0 / 4) + 1 of the lane's column, the same node for NR = 4
   // and NR = 2 (the same bits)
-  ForceSym<NR> P{su, sth, c0, T, su[((lx0 & ~3) + 5) * kW68 + ly + 4], {}, {}};
+  ForceSym<NR> P{su, sth, c0, c0, T, su[((lx0 & ~3) + 5) * kW68 + ly + 4], {}, {}};
 #pragma unroll
   for (int r = 0; r < NR; r++) {
     P.fx[r] = -T.wbsum[0] * (ui[r].x - P.cu.x);
@@ -1272,6 +1276,99 @@ __global__ void __launch_bounds__(256 / NR, NR == 4 ? 3 : 4) k_force_rb(StepArgs
   const double scale = I.f_scale;
 #pragma unroll
   for (int r = 0; r < NR; r++) {
+    const int32_t i = tl * (kTile * kTile) + (lx0 + r) * kTile + ly;
+    if (kd[r] != 0)
+      node_update(I, st, i, kd[r], ui[r], vi[r], ai[r], scale * P.fx[r], scale * P.fy[r]);
+  }
+}
+
+constexpr int kW32 = kTile + 16;  // u window of k_force_deep: 32 x 32 cells (halo 8)
+
+// Dilatation and force of a "deep" full tile in one launch. A deep tile's 3 x 3 neighbourhood is
+// full and its 24 x 24 window holds interior nodes only (kind 1), so the theta of every window node
+// is k_theta_rb's full-tile sum over u alone: the block computes the 576 values itself from a
+// 32 x 32 window of u (lanes 0..95, 6 window rows each), stores the tile's own 256 for the other
+// tiles' force pass, and runs k_force_rb<2>'s sweep and epilogue on them. The per-node arithmetic
+// of both sums is that of k_theta_rb / k_force_rb<2> (the same FMA sequence per node), so the
+// results are the same bits as the two-launch pass; the dilatation of the halo ring is computed
+// twice (by each tile whose window holds it), 2.25 times the dilatation arithmetic for the deep
+// tiles, in exchange for no dilatation launch in front of their force.
+__global__ void __launch_bounds__(128, 3) k_force_deep(StepArgs a, StepState st,
+                                                       const int32_t *list, int32_t count) {
+  __shared__ double2 su[kW32 * kW32];
+  __shared__ double sth[kCells68];
+  const TileArgs &T = a.tile;
+  const int32_t q = tile_of_block(blockIdx.x, count);
+  if (q >= count) return;
+  const int32_t tl = list[q];
+  int32_t nb[9];
+#pragma unroll
+  for (int k = 0; k < 9; k++) nb[k] = T.tile_nbr[9 * tl + k];
+  const int32_t t = threadIdx.x;
+  const int32_t ly = t & 15, lx0 = (t >> 4) * 2;
+  const double ratio = a.inst[T.tile_inst[tl]].th_scale;
+  {
+    // u window: column wc = t & 31 (tile column wc - 8), rows (t >> 5) + 4 k, k = 0..7 (tile rows
+    // -8..-5 | 0..15 | 16..23: the x-neighbour -1, 0, +1 known per k at compile time)
+    const int32_t wc = t & 31, y = wc - 8;
+    const int32_t ay = y < 0 ? -1 : (y < kTile ? 0 : 1);
+    const int32_t yy = y - ay * kTile;
+    int32_t tcol[3];  // the tile of this column in x-neighbour row -1, 0, +1
+#pragma unroll
+    for (int ax = 0; ax < 3; ax++)
+      tcol[ax] = ay < 0 ? nb[3 * ax] : (ay == 0 ? nb[3 * ax + 1] : nb[3 * ax + 2]);
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const int32_t wr = (t >> 5) + 4 * k, x = wr - 8;
+      constexpr int kAx[8] = {-1, -1, 0, 0, 0, 0, 1, 1};
+      const int32_t s = tcol[kAx[k] + 1] * (kTile * kTile) + (x - kAx[k] * kTile) * kTile + yy;
+      su[wr * kW32 + wc] = make_double2(st.px[s], st.py[s]);
+    }
+  }
+  __syncthreads();
+  if (t < 96) {
+    const int32_t tc = t % 24, tr0 = (t / 24) * 6;  // theta-window column and first row
+    ThetaSym<6, kW32> P{su, (tr0 + 4) * kW32 + tc + 4, T, {}};
+#pragma unroll
+    for (int r = 0; r < 6; r++) P.acc[r] = 0.0;
+    sym_sweep(P, std::make_integer_sequence<int, 14>{});
+#pragma unroll
+    for (int r = 0; r < 6; r++) {
+      const int32_t cr = tr0 + r;
+      sth[cr * kW68 + tc] = ratio * P.acc[r];
+      const int32_t x = cr - 4, y = tc - 4;
+      if (x >= 0 && x < kTile && y >= 0 && y < kTile)
+        st.theta[tl * (kTile * kTile) + x * kTile + y] = P.acc[r];
+    }
+  }
+  __syncthreads();
+  // the epilogue's inputs (here, not in front of the dilatation sweep, which needs the registers):
+  // their loads overlap the force sweep
+  int8_t kd[2];
+  double2 vi[2], ai[2];
+#pragma unroll
+  for (int r = 0; r < 2; r++) {
+    const int32_t i = tl * (kTile * kTile) + (lx0 + r) * kTile + ly;
+    kd[r] = a.kind[i];
+    vi[r] = make_double2(st.vx[i], st.vy[i]);
+    ai[r] = make_double2(st.ax[i], st.ay[i]);
+  }
+  const int32_t cu0 = (lx0 + 8) * kW32 + ly + 8, ct0 = (lx0 + 4) * kW68 + ly + 4;
+  double2 ui[2];
+#pragma unroll
+  for (int r = 0; r < 2; r++) ui[r] = su[cu0 + r * kW32];
+  // c: the displacement of tile row 4 floor(lx0 / 4) + 1 of the lane's column, as k_force_rb
+  ForceSym<2, kW32, kW68> P{su, sth, cu0, ct0, T, su[((lx0 & ~3) + 9) * kW32 + ly + 8], {}, {}};
+#pragma unroll
+  for (int r = 0; r < 2; r++) {
+    P.fx[r] = -T.wbsum[0] * (ui[r].x - P.cu.x);
+    P.fy[r] = -T.wbsum[1] * (ui[r].y - P.cu.y);
+  }
+  sym_sweep(P, std::make_integer_sequence<int, 10>{});
+  const InstParams &I = a.inst[T.tile_inst[tl]];
+  const double scale = I.f_scale;
+#pragma unroll
+  for (int r = 0; r < 2; r++) {
     const int32_t i = tl * (kTile * kTile) + (lx0 + r) * kTile + ly;
     if (kd[r] != 0)
       node_update(I, st, i, kd[r], ui[r], vi[r], ai[r], scale * P.fx[r], scale * P.fy[r]);
@@ -1672,10 +1769,14 @@ hipError_t launch_theta(const StepArgs &a, const StepState &st, bool fast, hipSt
       const int32_t np = with_part ? T.n_part : 0;
       if (T.theta_nr == 2) {
         const int32_t nb = list_blocks(T.n_full) + 2 * np;
-        if (nb > 0) hipLaunchKernelGGL(k_theta_rb<2>, dim3(nb), dim3(128), 0, s, a, st, RingU{}, 0);
+        if (nb > 0)
+          hipLaunchKernelGGL(k_theta_rb<2>, dim3(nb), dim3(128), 0, s, a, st, RingU{}, 0, T.full_list,
+                             T.n_full);
       } else {
         const int32_t nb = list_blocks(T.n_full) + 4 * np;
-        if (nb > 0) hipLaunchKernelGGL(k_theta_rb<4>, dim3(nb), dim3(64), 0, s, a, st, RingU{}, 0);
+        if (nb > 0)
+          hipLaunchKernelGGL(k_theta_rb<4>, dim3(nb), dim3(64), 0, s, a, st, RingU{}, 0, T.full_list,
+                             T.n_full);
       }
       return hipGetLastError();
     } else if (T.theta_kernel == kUnrolled) {
@@ -1702,11 +1803,13 @@ hipError_t launch_theta_rows(const StepArgs &a, const StepState &st, const RingU
   if (T.theta_nr == 2) {
     const int32_t nrow = ((a.n_abc + 7) / 8 + 7) / 8 * 8;
     const int32_t nb = nrow + list_blocks(T.n_full) + 2 * T.n_part;
-    if (nb > 0) hipLaunchKernelGGL(k_theta_rb<2>, dim3(nb), dim3(128), 0, s, a, st, r, nrow);
+    if (nb > 0)
+      hipLaunchKernelGGL(k_theta_rb<2>, dim3(nb), dim3(128), 0, s, a, st, r, nrow, T.full_list, T.n_full);
   } else {
     const int32_t nrow = ((a.n_abc + 3) / 4 + 7) / 8 * 8;
     const int32_t nb = nrow + list_blocks(T.n_full) + 4 * T.n_part;
-    if (nb > 0) hipLaunchKernelGGL(k_theta_rb<4>, dim3(nb), dim3(64), 0, s, a, st, r, nrow);
+    if (nb > 0)
+      hipLaunchKernelGGL(k_theta_rb<4>, dim3(nb), dim3(64), 0, s, a, st, r, nrow, T.full_list, T.n_full);
   }
   return hipGetLastError();
 }
@@ -1744,9 +1847,11 @@ hipError_t launch_force(const StepArgs &a, const StepState &st, bool fast, hipSt
     const int32_t nf = a.tile.n_full;
     if (a.tile.force_kernel == kBlocked) {
       if (nf > 0 && a.tile.force_nr == 2)
-        hipLaunchKernelGGL(k_force_rb<2>, dim3(list_blocks(nf)), dim3(128), 0, s, a, st);
+        hipLaunchKernelGGL(k_force_rb<2>, dim3(list_blocks(nf)), dim3(128), 0, s, a, st,
+                           a.tile.full_list, nf);
       else if (nf > 0)
-        hipLaunchKernelGGL(k_force_rb<4>, dim3(list_blocks(nf)), dim3(64), 0, s, a, st);
+        hipLaunchKernelGGL(k_force_rb<4>, dim3(list_blocks(nf)), dim3(64), 0, s, a, st,
+                           a.tile.full_list, nf);
     } else if (a.tile.force_kernel == kUnrolled) {
       if (nf > 0)
         hipLaunchKernelGGL(k_force_t68<true>, dim3(list_blocks(nf)), dim3(kBlock), 0, s, a, st,
@@ -1758,6 +1863,38 @@ hipError_t launch_force(const StepArgs &a, const StepState &st, bool fast, hipSt
   } else {
     const int32_t nb = blocks_for(a.n);
     hipLaunchKernelGGL(k_force_exact, dim3(nb + abc_blocks(a)), dim3(kBlock), 0, s, a, st, nb);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_force_deep(const StepArgs &a, const StepState &st, const int32_t *list,
+                             int32_t count, hipStream_t s) {
+  if (count == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_force_deep, dim3(list_blocks(count)), dim3(128), 0, s, a, st, list, count);
+  return hipGetLastError();
+}
+
+hipError_t launch_force_list(const StepArgs &a, const StepState &st, const int32_t *list,
+                             int32_t count, hipStream_t s) {
+  if (count == 0) return hipSuccess;
+  if (a.tile.force_nr == 2)
+    hipLaunchKernelGGL(k_force_rb<2>, dim3(list_blocks(count)), dim3(128), 0, s, a, st, list, count);
+  else
+    hipLaunchKernelGGL(k_force_rb<4>, dim3(list_blocks(count)), dim3(64), 0, s, a, st, list, count);
+  return hipGetLastError();
+}
+
+hipError_t launch_theta_list(const StepArgs &a, const StepState &st, const int32_t *list,
+                             int32_t count, hipStream_t s) {
+  const int32_t np = a.tile.n_part;
+  if (a.tile.theta_nr == 2) {
+    const int32_t nb = list_blocks(count) + 2 * np;
+    if (nb > 0)
+      hipLaunchKernelGGL(k_theta_rb<2>, dim3(nb), dim3(128), 0, s, a, st, RingU{}, 0, list, count);
+  } else {
+    const int32_t nb = list_blocks(count) + 4 * np;
+    if (nb > 0)
+      hipLaunchKernelGGL(k_theta_rb<4>, dim3(nb), dim3(64), 0, s, a, st, RingU{}, 0, list, count);
   }
   return hipGetLastError();
 }

@@ -396,7 +396,7 @@ def test_closure_beside_the_abc_rows_gives_the_same_bits(use_graph):
     c.set_scales(1e-3, 1e-3)
     out = []
     with c:
-        for concurrent in (0, 1, 3):
+        for concurrent in (0, 1, 3, 4):
             with solver.Solver(prob, variant=3, use_graph=use_graph, concurrent=concurrent) as s:
                 s.attach_closure(c)
                 s.run(90)

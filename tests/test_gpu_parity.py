@@ -1,4 +1,5 @@
 """The HIP path against the oracle and the reference's golden outputs, through the C ABI."""
+import re
 import sys
 
 import numpy as np
@@ -64,7 +65,8 @@ def test_exact_mode_matches_oracle_bitwise(name):
 
 @pytest.mark.parametrize("name", CASES)
 @pytest.mark.parametrize("graph,variant,concurrent", [(True, 0, 1), (False, 0, 1), (True, 3, 1),
-                                                      (True, 2, 0), (True, 0, 3), (False, 3, 3)])
+                                                      (True, 2, 0), (True, 0, 3), (False, 3, 3),
+                                                      (True, 3, 4), (False, 0, 4)])
 def test_fast_mode_matches_reference_golden(name, graph, variant, concurrent):
     """Fast path (tile kernels, FMA) against the reference's outputs: 1e-10 relative L2; autotuned,
     register-blocked and unrolled node kernels, ABC rows beside or after the tile kernels."""
@@ -213,15 +215,20 @@ def test_abc_prediction_folded_or_apart_gives_the_same_bits(graph, monkeypatch, 
 
 @pytest.mark.parametrize("graph", [True, False])
 def test_step_schedules_give_the_same_bits(graph, r40_problem):
-    """The ABC rows after the tile kernels (0), on a second stream (1) or as extra workgroups of the
-    dilatation launch (3): the same kernels' arithmetic in every schedule, so the same bits, on one
-    domain and on a batch. (Schedule 2 runs the partial tiles' dilatation in a kernel of its own, a
-    different association of the same sums: it is held to the oracle in the other tests.)"""
+    """The ABC rows after the tile kernels (0), on a second stream (1), as extra workgroups of the
+    dilatation launch (3), or on a second stream with the deep tiles' dilatation and force in one
+    launch (4): the same node arithmetic in every schedule, so the same bits, on one domain and on a
+    batch. (Schedule 2 runs the partial tiles' dilatation in a kernel of its own, a different
+    association of the same sums: it is held to the oracle in the other tests.)"""
     say("schedules", graph)
     g12, _ = load_golden("abc_r12_nu033")
     out = []
-    for sched in (0, 1, 3):
+    for sched in (0, 1, 3, 4):
         with solver.Solver(r40_problem, variant=3, use_graph=graph, concurrent=sched) as s:
+            # the R = 40 mm disc has deep tiles (the fused kernel runs) and shallow ones
+            deep = int(re.search(r"(\d+) deep", s.path).group(1))
+            full = int(re.search(r"(\d+) full", s.path).group(1))
+            assert 0 < deep < full, s.path
             recs = s.run(95)
             one = (s.fetch(), recs)
         with solver.Batch([g12, r40_problem], variant=3, use_graph=graph, concurrent=sched) as b:
@@ -241,13 +248,13 @@ def test_step_schedules_give_the_same_bits(graph, r40_problem):
 def test_two_wavefronts_per_tile_give_the_same_bits(graph, monkeypatch, r40_problem):
     """The register-blocked dilatation and force kernels with 4 nodes per lane (a wavefront per
     tile) or 2 (two wavefronts per tile sharing its window), in every combination: the same node
-    arithmetic, so the same bits, on one domain and on a batch, in both step schedules."""
+    arithmetic, so the same bits, on one domain and on a batch, in schedules 1, 3 and 4."""
     say("nodes per lane", graph)
     g12, _ = load_golden("abc_r12_nu033")
     out = []
     for nodes in ("4,4", "2,2", "2,4", "4,2"):
         monkeypatch.setenv("LPS_RB_NODES", nodes)
-        for sched in (1, 3):
+        for sched in (1, 3, 4):
             with solver.Solver(r40_problem, variant=3, use_graph=graph, concurrent=sched) as s:
                 recs = s.run(95)
                 one = (s.fetch(), recs)
