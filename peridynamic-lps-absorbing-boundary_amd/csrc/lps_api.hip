@@ -37,7 +37,7 @@ StepArgs lps_ctx::args() const {
   a.sell_off = hb.sell_off; a.sell_w = hb.sell_w; a.sell_col = hb.sell_col;
   a.m = m; a.c2m = c2m; a.is_bc = is_bc;
   a.abc_off = abc_off; a.cloud = cloud; a.gu = gu; a.gv = gv; a.gth = gth;
-  a.abc_gofs = abc_gofs; a.abc_xf = abc_xf; a.abc_pmap = abc_pmap; a.abc_in = abc_in;
+  a.abc_gofs = abc_gofs; a.abc_xf = abc_xf; a.abc_pmap = abc_pmap;
   a.theta_method = opt.theta_method;
   a.inst = inst_d;
   a.tile = tile;
@@ -179,6 +179,28 @@ static uint64_t abc_inputs_hash(const lps_problem *p, bool abc_on, int32_t theta
   mix(p->ci_nodes_v, sizeof(double) * 2 * C * 4 * M);
   if (theta_method == LPS_THETA_GTHETA) mix(p->ci_nodes_theta, sizeof(double) * C * 2 * M);
   return h;
+}
+
+// The fast path's packed member state (AbcPack) for the clouds cl (slots, -1 pads): the distinct
+// member slots in ascending order, each entry's index among them, and the record buffers.
+static int32_t build_member_pack(lps_ctx *c, const std::vector<int32_t> &cl, int64_t n_slots,
+                                 bool with_theta) {
+  std::vector<int32_t> of((size_t)std::max<int64_t>(n_slots, 1), -1), slots, idx(cl.size(), -1);
+  for (int32_t v : cl)
+    if (v >= 0) of[v] = 0;
+  for (int64_t i = 0; i < n_slots; i++)
+    if (of[i] == 0) {
+      of[i] = (int32_t)slots.size();
+      slots.push_back((int32_t)i);
+    }
+  for (size_t j = 0; j < cl.size(); j++) idx[j] = cl[j] < 0 ? -1 : of[cl[j]];
+  c->n_pk = (int32_t)slots.size();
+  CTX_TRY(c, c->upload(&c->pk_slot, slots));
+  CTX_TRY(c, c->upload(&c->pk_idx, idx));
+  CTX_TRY(c, c->alloc(&c->pk_u, 2 * (size_t)c->n_pk));
+  if (with_theta) CTX_TRY(c, c->alloc(&c->pk_th, (size_t)c->n_pk));
+  CTX_TRY(c, hipStreamSynchronize(c->stream));  // the host vectors end here
+  return LPS_OK;
 }
 
 // Lattice images among the ABC rows. On a square lattice about the origin each row (lattice
@@ -743,8 +765,10 @@ static int32_t create_impl(lps_ctx *c, const lps_problem *p) {
                  " members stored, the rest lattice images";
     CTX_TRY(c, c->upload(&c->abc_off, off));
     CTX_TRY(c, c->upload(&c->cloud, hcl));
-    CTX_TRY(c, c->alloc(&c->abc_in, hcl.size()));
-    CTX_TRY(c, launch_member_in((int64_t)hcl.size(), c->cloud, c->kind, c->abc_in, s));
+    if (c->fast) {
+      const int32_t rc = build_member_pack(c, hcl, c->n_slots, gth_on);
+      if (rc != LPS_OK) return rc;
+    }
     CTX_TRY(c, c->upload(&c->gu, hgu));
     CTX_TRY(c, c->upload(&c->gv, hgv));
     if (!hgt.empty()) CTX_TRY(c, c->upload(&c->gth, hgt));
@@ -1123,11 +1147,43 @@ static int32_t merge_batch(lps_ctx *c, const std::vector<lps_ctx *> &sub) {
   int32_t R = 0;
   int64_t NM = 0, GM = 0;
   bool shared = true;
+  const bool gth = c->opt.theta_method == LPS_THETA_GTHETA;
+  // Instances with the same ABC inputs (equal hashes of node list, clouds and operators, and
+  // byte-equal operator blocks) share one copy of the blocks: a batch of runs of one geometry
+  // and material then streams one set of operators, which stays in the caches, instead of one
+  // per run. gbase: the first block member of each instance's blocks.
+  std::vector<int64_t> gbase(B, -1);
+  int32_t n_dedup = 0;
   for (int32_t i = 0; i < B; i++) {
     R += sub[i]->n_abc;
     NM += sub[i]->abc_members;
-    GM += sub[i]->abc_block_members;
     shared = shared && (sub[i]->n_abc == 0 || sub[i]->abc_shared_blocks);
+    const lps_ctx &q = *sub[i];
+    const int64_t Gi = q.abc_block_members;
+    if (q.n_abc == 0) continue;
+    for (int32_t k = 0; k < i && gbase[i] < 0; k++) {
+      const lps_ctx &p0 = *sub[k];
+      if (gbase[k] < 0 || p0.abc_hash[0] != q.abc_hash[0] || p0.abc_block_members != Gi) continue;
+      bool same = true;
+      std::vector<double> ha, hb;
+      const std::pair<const double *, const double *> blk[3] = {
+          {p0.gu, q.gu}, {p0.gv, q.gv}, {gth ? p0.gth : nullptr, gth ? q.gth : nullptr}};
+      for (int b = 0; b < 3 && same; b++) {
+        if (!blk[b].first) continue;
+        const size_t len = (size_t)(b < 2 ? 8 : 2) * Gi;
+        if ((rc = to_host(c, ha, blk[b].first, len)) || (rc = to_host(c, hb, blk[b].second, len)))
+          return rc;
+        same = std::memcmp(ha.data(), hb.data(), sizeof(double) * len) == 0;
+      }
+      if (same) {
+        gbase[i] = gbase[k];
+        n_dedup++;
+      }
+    }
+    if (gbase[i] < 0) {
+      gbase[i] = GM;
+      GM += Gi;
+    }
   }
   c->n_abc = R;
   c->abc_members = NM;
@@ -1147,14 +1203,16 @@ static int32_t merge_batch(lps_ctx *c, const std::vector<lps_ctx *> &sub) {
     off.push_back(0);
     CTX_TRY(c, c->alloc(&c->gu, 8 * gm));
     CTX_TRY(c, c->alloc(&c->gv, 8 * gm));
-    const bool gth = c->opt.theta_method == LPS_THETA_GTHETA;
     if (gth) CTX_TRY(c, c->alloc(&c->gth, 2 * gm));
-    int64_t m0 = 0, g0 = 0;
+    int64_t m0 = 0;
     for (int32_t i = 0; i < B; i++) {
       const lps_ctx &q = *sub[i];
       const int32_t Ri = q.n_abc;
-      const int64_t Mi = q.abc_members, Gi = q.abc_block_members;
+      const int64_t Mi = q.abc_members, Gi = q.abc_block_members, g0 = gbase[i];
       if (Ri == 0) continue;
+      // the blocks are copied by the first instance that holds them
+      bool first = true;
+      for (int32_t k = 0; k < i && first; k++) first = !(sub[k]->n_abc > 0 && gbase[k] == g0);
       std::vector<int32_t> h;
       if ((rc = to_host(c, h, q.abc_slot, (size_t)Ri))) return rc;
       for (int32_t v : h) aslot.push_back((int32_t)(v + c->slot_off[i]));
@@ -1170,12 +1228,13 @@ static int32_t merge_batch(lps_ctx *c, const std::vector<lps_ctx *> &sub) {
       std::vector<uint8_t> hp;
       if ((rc = to_host(c, hp, q.abc_pmap, (size_t)Mi))) return rc;
       pmap.insert(pmap.end(), hp.begin(), hp.end());
-      CTX_TRY(c, hipMemcpyAsync(c->gu + 8 * g0, q.gu, sizeof(double) * 8 * Gi, hipMemcpyDeviceToDevice, c->stream));
-      CTX_TRY(c, hipMemcpyAsync(c->gv + 8 * g0, q.gv, sizeof(double) * 8 * Gi, hipMemcpyDeviceToDevice, c->stream));
-      if (gth)
-        CTX_TRY(c, hipMemcpyAsync(c->gth + 2 * g0, q.gth, sizeof(double) * 2 * Gi, hipMemcpyDeviceToDevice, c->stream));
+      if (first) {
+        CTX_TRY(c, hipMemcpyAsync(c->gu + 8 * g0, q.gu, sizeof(double) * 8 * Gi, hipMemcpyDeviceToDevice, c->stream));
+        CTX_TRY(c, hipMemcpyAsync(c->gv + 8 * g0, q.gv, sizeof(double) * 8 * Gi, hipMemcpyDeviceToDevice, c->stream));
+        if (gth)
+          CTX_TRY(c, hipMemcpyAsync(c->gth + 2 * g0, q.gth, sizeof(double) * 2 * Gi, hipMemcpyDeviceToDevice, c->stream));
+      }
       m0 += Mi;
-      g0 += Gi;
     }
     if (cl.empty()) {
       cl.push_back(-1);
@@ -1187,8 +1246,7 @@ static int32_t merge_batch(lps_ctx *c, const std::vector<lps_ctx *> &sub) {
     CTX_TRY(c, c->upload(&c->abc_xf, xf));
     CTX_TRY(c, c->upload(&c->abc_pmap, pmap));
     CTX_TRY(c, c->upload(&c->cloud, cl));
-    CTX_TRY(c, c->alloc(&c->abc_in, cl.size()));
-    CTX_TRY(c, launch_member_in((int64_t)cl.size(), c->cloud, c->kind, c->abc_in, c->stream));
+    if ((rc = build_member_pack(c, cl, NS, gth)) != LPS_OK) return rc;
   }
   // instances
   c->inst_h.clear();
@@ -1225,6 +1283,8 @@ static int32_t merge_batch(lps_ctx *c, const std::vector<lps_ctx *> &sub) {
   if (c->abc_shared_blocks)
     c->path += "; ABC operators of " + std::to_string(GM) + " of " + std::to_string(NM) +
                " members stored, the rest lattice images";
+  if (n_dedup > 0)
+    c->path += "; " + std::to_string(n_dedup) + " instances share an earlier one's ABC operators";
   return reset_state(c);
 }
 
@@ -1419,7 +1479,10 @@ static hipError_t step_fused(lps_ctx *c, const StepArgs &a, const StepState &st,
     if ((e = closure_step_hook(c, st, c->stream4, true)) != hipSuccess) return e;
     if ((e = hipEventRecord(c->ev_clo, c->stream4)) != hipSuccess) return e;
   }
-  if ((e = launch_theta_rows(a, st, c->ring(c->step), c->stream)) != hipSuccess) return e;
+  if ((e = launch_abc_pack(c->n_pk, c->pk_slot, c->kind, st, c->ring(c->step), c->pk_u, c->pk_th,
+                           c->stream)) != hipSuccess)
+    return e;
+  if ((e = launch_theta_rows(a, st, c->pack(), c->stream)) != hipSuccess) return e;
   if (a.theta_method == LPS_THETA_IDW && (e = launch_idw(a, st, c->stream)) != hipSuccess) return e;
   const bool ev_force = record_events && !c->timing_closure;
   if (ev_force && (e = record_timing(c, ev[1])) != hipSuccess) return e;
@@ -1491,16 +1554,22 @@ static hipError_t step_once(lps_ctx *c, bool record_events) {
       if ((e = closure_step_hook(c, st, c->stream4, true)) != hipSuccess) return e;
       if ((e = hipEventRecord(c->ev_clo, c->stream4)) != hipSuccess) return e;
     }
-    if ((e = launch_abc_theta(a, st, c->stream2)) != hipSuccess) return e;
+    if ((e = launch_abc_pack(c->n_pk, c->pk_slot, c->kind, st, c->ring(c->step), c->pk_u,
+                             c->pk_th, c->stream2)) != hipSuccess)
+      return e;
+    if ((e = launch_abc_theta(a, st, c->pack(), c->stream2)) != hipSuccess) return e;
     if ((e = hipEventRecord(c->ev_join[0], c->stream2)) != hipSuccess) return e;
-    if ((e = launch_abc_uv(a, st, c->ring(c->step), c->stream2)) != hipSuccess) return e;
+    if ((e = launch_abc_uv(a, st, c->pack(), c->stream2)) != hipSuccess) return e;
     if (c->clo) {
       if ((e = hipStreamWaitEvent(c->stream2, c->ev_clo, 0)) != hipSuccess) return e;
       if ((e = closure_apply_hook(c, st, c->stream2)) != hipSuccess) return e;
     }
     if ((e = hipEventRecord(c->ev_join[1], c->stream2)) != hipSuccess) return e;
-  } else if (abc && (e = launch_abc_theta(a, st, c->stream)) != hipSuccess) {
-    return e;
+  } else if (abc) {
+    if ((e = launch_abc_pack(c->n_pk, c->pk_slot, c->kind, st, c->ring(c->step), c->pk_u, c->pk_th,
+                             c->stream)) != hipSuccess)
+      return e;
+    if ((e = launch_abc_theta(a, st, c->pack(), c->stream)) != hipSuccess) return e;
   }
   if (deep) {
     if ((e = launch_force_deep(a, st, c->deep_list, c->n_deep, c->stream)) != hipSuccess) return e;
@@ -1534,13 +1603,13 @@ static hipError_t step_once(lps_ctx *c, bool record_events) {
                     a.tile.force_kernel != kGeneric;
   if (c->fast && fold) {
     if (fork && (e = hipStreamWaitEvent(c->stream, c->ev_join[1], 0)) != hipSuccess) return e;
-    if (!fork && (e = launch_abc_uv(a, st, c->ring(c->step), c->stream)) != hipSuccess) return e;
+    if (!fork && (e = launch_abc_uv(a, st, c->pack(), c->stream)) != hipSuccess) return e;
     if ((e = launch_force_part(a, st, c->stream, true)) != hipSuccess) return e;
   } else if (c->fast) {
     if (!split && (e = launch_force_part(a, st, c->stream)) != hipSuccess) return e;
     if (split && (e = hipStreamWaitEvent(c->stream, c->ev_part[2], 0)) != hipSuccess) return e;
     if (fork && (e = hipStreamWaitEvent(c->stream, c->ev_join[1], 0)) != hipSuccess) return e;
-    if (abc && !fork && (e = launch_abc_uv(a, st, c->ring(c->step), c->stream)) != hipSuccess) return e;
+    if (abc && !fork && (e = launch_abc_uv(a, st, c->pack(), c->stream)) != hipSuccess) return e;
     if (abc && !fork && c->clo && (e = closure_step_hook(c, st, c->stream, false)) != hipSuccess) return e;
     if (abc && (e = launch_abc_predict(a, st, c->stream)) != hipSuccess) return e;
   }

@@ -78,10 +78,15 @@ struct lps_ctx {
   int32_t *abc_off = nullptr, *cloud = nullptr, *abc_gofs = nullptr;
   int8_t *abc_xf = nullptr;
   uint8_t *abc_pmap = nullptr;
-  uint8_t *abc_in = nullptr;  // per member: an interior node (the row kernels read its U from the ring)
   int64_t abc_members = 0, abc_block_members = 0;
   bool abc_shared_blocks = false;  // lattice images share their canonical row's block
   double *gu = nullptr, *gv = nullptr, *gth = nullptr;
+  // the clouds' members packed per member for the fast path's row kernels (AbcPack): their slots,
+  // each cloud entry's member index (-1 pad) and the records, rewritten at each step's start
+  int32_t *pk_slot = nullptr, *pk_idx = nullptr;
+  int32_t n_pk = 0;
+  double4 *pk_u = nullptr;
+  double2 *pk_th = nullptr;
   TileArgs tile{};
   // Prediction ring: step s reads P[s % 6] (U^{s+1} before the ABC overwrite) and writes
   // P[(s + 1) % 6]. At the interior nodes (kind 1) U^{s+1} equals that prediction and the fast
@@ -141,6 +146,7 @@ struct lps_ctx {
     const Dev2 &a = ring_u(s, 0), &b = ring_u(s, 1);
     return RingU{a.x, a.y, b.x, b.y};
   }
+  lps::AbcPack pack() const { return lps::AbcPack{pk_idx, pk_u, pk_th}; }
 };
 
 namespace lps {

@@ -118,8 +118,6 @@ struct StepArgs {
   const int32_t *abc_gofs;         // n_abc
   const int8_t *abc_xf;            // n_abc: bit 0 swap x and y, bit 1 negate the cross terms
   const uint8_t *abc_pmap;         // per member
-  const uint8_t *abc_in;           // per member: 1 for an interior node (kind 1), whose U^n lives
-                                   // in the prediction ring (fast path)
   const double *gu, *gv;
   const double *gth;
   int32_t theta_method;
@@ -182,6 +180,15 @@ struct RingU {
   const double *ux, *uy, *uxm, *uym;
 };
 
+// Fast path, ABC rows: the state of the clouds' members packed per member at the step's start
+// (launch_abc_pack), so a member's G_u / G_v terms gather one 64-byte record (and G_theta one
+// 16-byte record) instead of eight (two) values from separate arrays.
+struct AbcPack {
+  const int32_t *idx;  // per cloud entry: its member's record, -1 for a pad
+  const double4 *u;    // per member: (U^n_x, U^n_y, U^{n-1}_x, U^{n-1}_y), (V^n_x, .., V^{n-1}_y)
+  const double2 *th;   // per member: (theta^n, theta^{n-1}) (G_theta only)
+};
+
 // Per-block partial sums of the output block.
 struct DiagPartial {
   double energy, ek, ep, px, py, max_u, max_v, max_a, th_max, th_min, ub, vb;
@@ -214,7 +221,7 @@ hipError_t launch_force_part(const StepArgs &a, const StepState &st, hipStream_t
 // next prediction at the ABC nodes) needs abc_uv and the force pass.
 // The dilatation with the ABC rows' G_theta and G_u / G_v in the same launch (extra workgroups of
 // the register-blocked kernel; other families: the row kernels then the dilatation, on s).
-hipError_t launch_theta_rows(const StepArgs &a, const StepState &st, const RingU &r, hipStream_t s);
+hipError_t launch_theta_rows(const StepArgs &a, const StepState &st, const AbcPack &p, hipStream_t s);
 // Register-blocked family on a list of full tiles: the dilatation of the list and of every partial
 // tile (launch_theta_list), the force of the list (launch_force_list), and the dilatation + force
 // of deep tiles in one launch (launch_force_deep: a full tile whose 3 x 3 neighbourhood is full
@@ -226,11 +233,12 @@ hipError_t launch_force_list(const StepArgs &a, const StepState &st, const int32
                              int32_t count, hipStream_t s);
 hipError_t launch_force_deep(const StepArgs &a, const StepState &st, const int32_t *list,
                              int32_t count, hipStream_t s);
-// abc_in of every ABC cloud member (n members of cloud): 1 where the member's slot is of kind 1.
-hipError_t launch_member_in(int64_t n, const int32_t *cloud, const int8_t *kind, uint8_t *out,
-                            hipStream_t s);
-hipError_t launch_abc_theta(const StepArgs &a, const StepState &st, hipStream_t s);
-hipError_t launch_abc_uv(const StepArgs &a, const StepState &st, const RingU &r, hipStream_t s);
+// The n members' packed state (slot: their slots; th null without G_theta); U of an interior
+// member (kind 1) from the prediction ring r, of an ABC member from U. Before the step's row kernels.
+hipError_t launch_abc_pack(int32_t n, const int32_t *slot, const int8_t *kind, const StepState &st,
+                           const RingU &r, double4 *u, double2 *th, hipStream_t s);
+hipError_t launch_abc_theta(const StepArgs &a, const StepState &st, const AbcPack &p, hipStream_t s);
+hipError_t launch_abc_uv(const StepArgs &a, const StepState &st, const AbcPack &p, hipStream_t s);
 hipError_t launch_abc_predict(const StepArgs &a, const StepState &st, hipStream_t s);
 // U at the interior nodes (kind 1) copied from the prediction ring: ux[i] = px[i] there.
 hipError_t launch_u_from_ring(const StepArgs &a, const double *px, const double *py, double *ux,

@@ -1046,7 +1046,7 @@ __device__ __forceinline__ double group16_sum(double v) {
 // or for IDW the direct value (overridden by k_idw where a non-ABC neighbour exists,
 // src/main.cpp:765-790). Reads only the theta history (or the prediction).
 __device__ __forceinline__ void abc_theta_rows16(const StepArgs &a, const StepState &st,
-                                                 int32_t row, int32_t q) {
+                                                 const AbcPack &p, int32_t row, int32_t q) {
   const int32_t node = a.abc_slot[row];
   double th;
   if (a.theta_method == 1) {
@@ -1054,7 +1054,7 @@ __device__ __forceinline__ void abc_theta_rows16(const StepArgs &a, const StepSt
     int32_t L;
     abc_row_span(a, row, o, L);
     const double *g = a.gth + 2 * (int64_t)a.abc_gofs[row];  // G_theta needs no lattice transform
-    const int32_t *cl = a.cloud + o;
+    const int32_t *cl = p.idx + o;
     double acc = 0.0;
     // four members per lane at a time (q, q + 16, q + 32, q + 48): their loads are unconditional
     // (clamped indices), so the four members' gathers are in flight together; a member past the
@@ -1071,7 +1071,8 @@ __device__ __forceinline__ void abc_theta_rows16(const StepArgs &a, const StepSt
         const int jj = min(base + u * kRowLanes, L - 1);
         const int32_t ks = k[u] >= 0 ? k[u] : 0;
         const double2 gm = *(const double2 *)(g + 2 * a.abc_pmap[o + jj]);
-        const double t = gtheta_term_exact(gm.x, gm.y, st.thn[ks], st.thnm1[ks]);
+        const double2 h = p.th[ks];  // (theta^n, theta^{n-1}) of the member
+        const double t = gtheta_term_exact(gm.x, gm.y, h.x, h.y);
         acc = k[u] >= 0 ? acc + t : acc;
       }
     }
@@ -1084,26 +1085,24 @@ __device__ __forceinline__ void abc_theta_rows16(const StepArgs &a, const StepSt
 
 // ABC row `row` on its 16 lanes: U^{n+1} = G_u [U^n, U^{n-1}] and V^{n+1} = G_v [V^n, V^{n-1}]
 // over the row's cloud (src/main.cpp:883-916, 933-974). The tile kernels compute the force at the
-// ABC nodes like at any node. U of an interior cloud member comes from the prediction ring r, of an
-// ABC member from U.
-__device__ __forceinline__ void abc_uv_rows16(const StepArgs &a, const StepState &st, const RingU &r,
+// ABC nodes like at any node. The members' state comes packed (p: U of an interior member from the
+// prediction ring, of an ABC member from U), one 64-byte record per member.
+__device__ __forceinline__ void abc_uv_rows16(const StepArgs &a, const StepState &st, const AbcPack &p,
                                               int32_t row, int32_t q) {
   int64_t o;
   int32_t L;
   abc_row_span(a, row, o, L);
   const int64_t gb = a.abc_gofs[row];
   const int xf = a.abc_xf[row];
-  const int32_t *cl = a.cloud + o;
+  const int32_t *cl = p.idx + o;
   double Ux = 0.0, Uy = 0.0, Vx = 0.0, Vy = 0.0;
   // four members per lane at a time, loads unconditional (see abc_theta_rows16)
   for (int base = q; base < L; base += 4 * kRowLanes) {
     int32_t k[4];
-    bool in[4];
 #pragma unroll
     for (int u = 0; u < 4; u++) {
       const int jj = base + u * kRowLanes;
       k[u] = jj < L ? cl[jj] : -1;
-      in[u] = jj < L && a.abc_in[o + jj];
     }
 #pragma unroll
     for (int u = 0; u < 4; u++) {
@@ -1111,9 +1110,9 @@ __device__ __forceinline__ void abc_uv_rows16(const StepArgs &a, const StepState
       const int32_t ks = k[u] >= 0 ? k[u] : 0;
       double4 g0, g1, h0, h1;
       abc_uv_coeffs(a, o, L, gb, xf, jj, g0, g1, h0, h1);
-      const double ux = (in[u] ? r.ux : st.ux)[ks], uy = (in[u] ? r.uy : st.uy)[ks];
-      const double uxm = (in[u] ? r.uxm : st.uxm)[ks], uym = (in[u] ? r.uym : st.uym)[ks];
-      const double vx = st.vx[ks], vy = st.vy[ks], vxm = st.vxm[ks], vym = st.vym[ks];
+      const double4 su = p.u[2 * ks], sv = p.u[2 * ks + 1];
+      const double ux = su.x, uy = su.y, uxm = su.z, uym = su.w;
+      const double vx = sv.x, vy = sv.y, vxm = sv.z, vym = sv.w;
       const double tux = g0.x * ux + g0.y * uy + g0.z * uxm + g0.w * uym;
       const double tuy = g1.x * ux + g1.y * uy + g1.z * uxm + g1.w * uym;
       const double tvx = h0.x * vx + h0.y * vy + h0.z * vxm + h0.w * vym;
@@ -1139,18 +1138,32 @@ __device__ __forceinline__ void abc_uv_rows16(const StepArgs &a, const StepState
 
 // Fast path, ABC rows' dilatation, kRowsPerBlock rows per block; may run beside the tile kernels.
 // Launched on abc_blocks_xcd(a) blocks.
-__global__ void __launch_bounds__(kBlock) k_abc_theta(StepArgs a, StepState st) {
+__global__ void __launch_bounds__(kBlock) k_abc_theta(StepArgs a, StepState st, AbcPack p) {
   const int32_t nrb = (a.n_abc + kRowsPerBlock - 1) / kRowsPerBlock;
   const int32_t row = tile_of_block(blockIdx.x, nrb) * kRowsPerBlock + (int32_t)threadIdx.x / kRowLanes;
-  if (row < a.n_abc) abc_theta_rows16(a, st, row, threadIdx.x & (kRowLanes - 1));
+  if (row < a.n_abc) abc_theta_rows16(a, st, p, row, threadIdx.x & (kRowLanes - 1));
 }
 
 // Fast path, ABC rows' G_u / G_v, kRowsPerBlock rows per block. Launched on abc_blocks_xcd(a)
 // blocks.
-__global__ void __launch_bounds__(kBlock) k_abc_uv(StepArgs a, StepState st, RingU r) {
+__global__ void __launch_bounds__(kBlock) k_abc_uv(StepArgs a, StepState st, AbcPack p) {
   const int32_t nrb = (a.n_abc + kRowsPerBlock - 1) / kRowsPerBlock;
   const int32_t row = tile_of_block(blockIdx.x, nrb) * kRowsPerBlock + (int32_t)threadIdx.x / kRowLanes;
-  if (row < a.n_abc) abc_uv_rows16(a, st, r, row, threadIdx.x & (kRowLanes - 1));
+  if (row < a.n_abc) abc_uv_rows16(a, st, p, row, threadIdx.x & (kRowLanes - 1));
+}
+
+// The packed state of the ABC clouds' members (AbcPack), one thread per member.
+__global__ void __launch_bounds__(kBlock) k_abc_pack(int32_t n, const int32_t *slot,
+                                                     const int8_t *kind, StepState st, RingU r,
+                                                     double4 *u, double2 *th) {
+  const int32_t m = blockIdx.x * kBlock + threadIdx.x;
+  if (m >= n) return;
+  const int32_t i = slot[m];
+  const bool in = kind[i] == 1;
+  u[2 * m] = make_double4((in ? r.ux : st.ux)[i], (in ? r.uy : st.uy)[i], (in ? r.uxm : st.uxm)[i],
+                          (in ? r.uym : st.uym)[i]);
+  u[2 * m + 1] = make_double4(st.vx[i], st.vy[i], st.vxm[i], st.vym[i]);
+  if (th) th[m] = make_double2(st.thn[i], st.thnm1[i]);
 }
 
 // The register-blocked tile kernels take NR nodes per lane: NR = 4, one wavefront per tile; NR = 2,
@@ -1180,7 +1193,7 @@ constexpr int kQuarterCells = 12 * kW68 + (12 * kW68 + 15) / 16;
 // the dilatation.
 template <int NR>
 __global__ void __launch_bounds__(256 / NR, NR == 4 ? 4 : 6) k_theta_rb(StepArgs a, StepState st,
-                                                                        RingU ring,
+                                                                        AbcPack pack,
                                                                         int32_t nrow_blocks,
                                                                         const int32_t *flist,
                                                                         int32_t nfull) {
@@ -1192,8 +1205,8 @@ __global__ void __launch_bounds__(256 / NR, NR == 4 ? 4 : 6) k_theta_rb(StepArgs
     const int32_t nrb = (a.n_abc + 4 * kWaves - 1) / (4 * kWaves);
     const int32_t row = tile_of_block(blockIdx.x, nrb) * 4 * kWaves + t / kRowLanes;
     if (row < a.n_abc) {
-      abc_theta_rows16(a, st, row, t & (kRowLanes - 1));
-      abc_uv_rows16(a, st, ring, row, t & (kRowLanes - 1));
+      abc_theta_rows16(a, st, pack, row, t & (kRowLanes - 1));
+      abc_uv_rows16(a, st, pack, row, t & (kRowLanes - 1));
     }
     return;
   }
@@ -1770,12 +1783,12 @@ hipError_t launch_theta(const StepArgs &a, const StepState &st, bool fast, hipSt
       if (T.theta_nr == 2) {
         const int32_t nb = list_blocks(T.n_full) + 2 * np;
         if (nb > 0)
-          hipLaunchKernelGGL(k_theta_rb<2>, dim3(nb), dim3(128), 0, s, a, st, RingU{}, 0, T.full_list,
+          hipLaunchKernelGGL(k_theta_rb<2>, dim3(nb), dim3(128), 0, s, a, st, AbcPack{}, 0, T.full_list,
                              T.n_full);
       } else {
         const int32_t nb = list_blocks(T.n_full) + 4 * np;
         if (nb > 0)
-          hipLaunchKernelGGL(k_theta_rb<4>, dim3(nb), dim3(64), 0, s, a, st, RingU{}, 0, T.full_list,
+          hipLaunchKernelGGL(k_theta_rb<4>, dim3(nb), dim3(64), 0, s, a, st, AbcPack{}, 0, T.full_list,
                              T.n_full);
       }
       return hipGetLastError();
@@ -1792,11 +1805,11 @@ hipError_t launch_theta(const StepArgs &a, const StepState &st, bool fast, hipSt
   return hipGetLastError();
 }
 
-hipError_t launch_theta_rows(const StepArgs &a, const StepState &st, const RingU &r, hipStream_t s) {
+hipError_t launch_theta_rows(const StepArgs &a, const StepState &st, const AbcPack &r, hipStream_t s) {
   const TileArgs &T = a.tile;
   if (T.theta_kernel != kBlocked) {  // the row kernels first, then the dilatation, on one stream
     hipError_t e;
-    if ((e = launch_abc_theta(a, st, s)) != hipSuccess) return e;
+    if ((e = launch_abc_theta(a, st, r, s)) != hipSuccess) return e;
     if ((e = launch_abc_uv(a, st, r, s)) != hipSuccess) return e;
     return launch_theta(a, st, true, s, true);
   }
@@ -1814,25 +1827,17 @@ hipError_t launch_theta_rows(const StepArgs &a, const StepState &st, const RingU
   return hipGetLastError();
 }
 
-__global__ void __launch_bounds__(kBlock) k_member_in(int64_t n, const int32_t *cloud,
-                                                       const int8_t *kind, uint8_t *out) {
-  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
-  const int32_t k = cloud[i];
-  out[i] = k >= 0 && kind[k] == 1;
-}
-
-hipError_t launch_member_in(int64_t n, const int32_t *cloud, const int8_t *kind, uint8_t *out,
-                            hipStream_t s) {
-  if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_member_in, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
-                     n, cloud, kind, out);
+hipError_t launch_abc_theta(const StepArgs &a, const StepState &st, const AbcPack &p, hipStream_t s) {
+  if (a.n_abc == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_abc_theta, dim3(abc_blocks_xcd(a)), dim3(kBlock), 0, s, a, st, p);
   return hipGetLastError();
 }
 
-hipError_t launch_abc_theta(const StepArgs &a, const StepState &st, hipStream_t s) {
-  if (a.n_abc == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_abc_theta, dim3(abc_blocks_xcd(a)), dim3(kBlock), 0, s, a, st);
+hipError_t launch_abc_pack(int32_t n, const int32_t *slot, const int8_t *kind, const StepState &st,
+                           const RingU &r, double4 *u, double2 *th, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_abc_pack, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, n, slot, kind,
+                     st, r, u, th);
   return hipGetLastError();
 }
 
@@ -1890,11 +1895,11 @@ hipError_t launch_theta_list(const StepArgs &a, const StepState &st, const int32
   if (a.tile.theta_nr == 2) {
     const int32_t nb = list_blocks(count) + 2 * np;
     if (nb > 0)
-      hipLaunchKernelGGL(k_theta_rb<2>, dim3(nb), dim3(128), 0, s, a, st, RingU{}, 0, list, count);
+      hipLaunchKernelGGL(k_theta_rb<2>, dim3(nb), dim3(128), 0, s, a, st, AbcPack{}, 0, list, count);
   } else {
     const int32_t nb = list_blocks(count) + 4 * np;
     if (nb > 0)
-      hipLaunchKernelGGL(k_theta_rb<4>, dim3(nb), dim3(64), 0, s, a, st, RingU{}, 0, list, count);
+      hipLaunchKernelGGL(k_theta_rb<4>, dim3(nb), dim3(64), 0, s, a, st, AbcPack{}, 0, list, count);
   }
   return hipGetLastError();
 }
@@ -1912,9 +1917,9 @@ hipError_t launch_force_part(const StepArgs &a, const StepState &st, hipStream_t
   return hipGetLastError();
 }
 
-hipError_t launch_abc_uv(const StepArgs &a, const StepState &st, const RingU &r, hipStream_t s) {
+hipError_t launch_abc_uv(const StepArgs &a, const StepState &st, const AbcPack &p, hipStream_t s) {
   if (a.n_abc == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_abc_uv, dim3(abc_blocks_xcd(a)), dim3(kBlock), 0, s, a, st, r);
+  hipLaunchKernelGGL(k_abc_uv, dim3(abc_blocks_xcd(a)), dim3(kBlock), 0, s, a, st, p);
   return hipGetLastError();
 }
 

@@ -40,6 +40,31 @@ def test_batch_equals_instances_alone(variant, instances):
             assert r == q, (i, r, q)
 
 
+def test_instances_with_the_same_operators_share_them(instances):
+    """Instances whose ABC inputs are the same (one geometry and material, other initial
+    conditions) keep one copy of the operator blocks in the batch; every instance still gives the
+    same bits as alone."""
+    print("batch shared operators", flush=True)
+    probs, _ = instances
+    g10 = probs[0]
+    other = pp.Problem(**{**g10.__dict__})
+    other.ic = -0.7 * g10.ic
+    batch = [g10, probs[1], other, g10]
+    steps = 150
+    with solver.Batch(batch, variant=3) as b:
+        assert "2 instances share an earlier one's ABC operators" in b.path, b.path
+        recs = b.run(steps)
+        fields = [b.fetch(i) for i in range(len(batch))]
+    for i, p in enumerate(batch):
+        with solver.Solver(p, variant=3) as s:
+            srecs = s.run(steps)
+            sf = s.fetch()
+        for k in ("u", "v", "a", "theta"):
+            assert np.array_equal(fields[i][k], sf[k]), (i, k)
+        assert recs[i] == srecs, i
+    assert not np.array_equal(fields[0]["u"], fields[2]["u"])
+
+
 def test_batch_matches_the_reference_golden_outputs(instances):
     print("batch vs golden", flush=True)
     probs, refs = instances
