@@ -945,6 +945,7 @@ static int32_t new_context(const lps_options *options, lps_ctx **out) {
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream3, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream4, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_clo, hipEventDisableTiming);
+  if (const char *cf = std::getenv("LPS_CLOSURE_FIRST")) c->clo_first = std::atoi(cf) != 0;
   for (int q = 0; q < 3; q++)
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_part[q], hipEventDisableTiming);
   for (int q = 0; q < 2; q++) {
@@ -1531,6 +1532,11 @@ static hipError_t step_once(lps_ctx *c, bool record_events) {
   // waits for that dilatation and for G_theta
   const bool deep = c->fast && c->opt.concurrent == 4 && c->n_deep > 0 &&
                     a.tile.theta_kernel == kBlocked && a.tile.force_kernel == kBlocked;
+  // the closure's history shift and network first, alone on the GPU, on the main stream (its
+  // corrections stored, added after G_u / G_v): beside the tile kernels its 152 KB workgroups wait
+  // for CUs with that much LDS free, so the step gains nothing from the overlap
+  const bool clo_first = fork && c->clo && c->clo_first;
+  if (clo_first && (e = closure_step_hook(c, st, c->stream, true)) != hipSuccess) return e;
   if (fork || split || deep) {
     if ((e = hipEventRecord(c->ev_fork[0], c->stream)) != hipSuccess) return e;
   }
@@ -1549,7 +1555,7 @@ static hipError_t step_once(lps_ctx *c, bool record_events) {
     if ((e = hipStreamWaitEvent(c->stream2, c->ev_fork[0], 0)) != hipSuccess) return e;
     // the closure's history shift and network read only step n's state: on the fourth stream
     // from the step's start, their corrections stored and added after G_u / G_v
-    if (c->clo) {
+    if (c->clo && !clo_first) {
       if ((e = hipStreamWaitEvent(c->stream4, c->ev_fork[0], 0)) != hipSuccess) return e;
       if ((e = closure_step_hook(c, st, c->stream4, true)) != hipSuccess) return e;
       if ((e = hipEventRecord(c->ev_clo, c->stream4)) != hipSuccess) return e;
@@ -1561,7 +1567,7 @@ static hipError_t step_once(lps_ctx *c, bool record_events) {
     if ((e = hipEventRecord(c->ev_join[0], c->stream2)) != hipSuccess) return e;
     if ((e = launch_abc_uv(a, st, c->pack(), c->stream2)) != hipSuccess) return e;
     if (c->clo) {
-      if ((e = hipStreamWaitEvent(c->stream2, c->ev_clo, 0)) != hipSuccess) return e;
+      if (!clo_first && (e = hipStreamWaitEvent(c->stream2, c->ev_clo, 0)) != hipSuccess) return e;
       if ((e = closure_apply_hook(c, st, c->stream2)) != hipSuccess) return e;
     }
     if ((e = hipEventRecord(c->ev_join[1], c->stream2)) != hipSuccess) return e;

@@ -31,7 +31,8 @@ struct lps_ctx {
   hipStream_t stream = nullptr;
   hipStream_t stream2 = nullptr;  // the ABC rows' kernels beside the tile kernels
   hipStream_t stream3 = nullptr;  // the partial tiles' kernels beside the full tiles' kernels
-  hipStream_t stream4 = nullptr;  // the learned closure's history and network beside the ABC rows
+  hipStream_t stream4 = nullptr;  // the learned closure's history and network beside the tile
+                                  // kernels (LPS_CLOSURE_FIRST=0; by default they run first on stream)
   hipEvent_t ev_clo = nullptr;    // the closure's network done (its corrections stored)
   hipEvent_t ev_fork[2] = {nullptr, nullptr}, ev_join[2] = {nullptr, nullptr};
   hipEvent_t ev_part[3] = {nullptr, nullptr, nullptr};  // partial theta done, theta done, partial force done
@@ -45,6 +46,9 @@ struct lps_ctx {
   // the ABC nodes' prediction folded into the partial tiles' force pass (which then waits for
   // G_u / G_v); else a kernel of its own after G_u / G_v, the partial tiles beside them
   bool fold_pred = true;
+  // with the ABC rows on their own stream: the closure's network alone at the step's start on the
+  // main stream (else on the fourth stream beside the tile kernels); LPS_CLOSURE_FIRST=0/1
+  bool clo_first = true;
   int64_t n_full_nodes = 0;   // nodes in the full tiles
   // the full tiles split for concurrent == 4: deep (the 3 x 3 neighbourhood full, interior nodes
   // only in the 24 x 24 window: dilatation and force in one launch) and shallow (the others)
