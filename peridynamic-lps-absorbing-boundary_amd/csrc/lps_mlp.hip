@@ -278,12 +278,20 @@ __global__ void __launch_bounds__(64 * kW) k_mlp_fwd_wide(MlpDev m, const __bf16
     }
     __syncthreads();
   }
-  float iu = 0.0f, iv = 0.0f;
-  int f = 0;
+  // the row's frame transform folded into its scales: to_frame swaps a pair's components (bit 0)
+  // and negates the first (bit 1) and second (bit 2); (-c) * inv and c * (-inv) are the same
+  // floats, so the inputs keep their bits. su_a / su_b: the first / second component's scale of
+  // the u part, sv_a / sv_b of the v part.
+  float su_a = 0.0f, su_b = 0.0f, sv_a = 0.0f, sv_b = 0.0f;
+  bool swap = false;
   if constexpr (kGather) {
-    iu = sc[lrow][0];
-    iv = sc[lrow][1];
-    f = live ? g.xf[row] : 0;
+    const float iu = sc[lrow][0], iv = sc[lrow][1];
+    const int f = live ? g.xf[row] : 0;
+    swap = (f & 1) != 0;
+    su_a = (f & 2) ? -iu : iu;
+    su_b = (f & 4) ? -iu : iu;
+    sv_a = (f & 2) ? -iv : iv;
+    sv_b = (f & 4) ? -iv : iv;
   }
   // loads of layer-1 inputs: history record (gathered) or the input row
   auto issue_a = [&](int k, RawG &raw, bf16x8 &xr) {
@@ -315,12 +323,15 @@ __global__ void __launch_bounds__(64 * kW) k_mlp_fwd_wide(MlpDev m, const __bf16
         }
         return out;
       }
-      const float inv = (k & 1) ? iv : iu;
-      float cx, cy;
-      to_frame(f, raw.a.x, raw.a.y, cx, cy); out[0] = (__bf16)(cx * inv); out[1] = (__bf16)(cy * inv);
-      to_frame(f, raw.a.z, raw.a.w, cx, cy); out[2] = (__bf16)(cx * inv); out[3] = (__bf16)(cy * inv);
-      to_frame(f, raw.b.x, raw.b.y, cx, cy); out[4] = (__bf16)(cx * inv); out[5] = (__bf16)(cy * inv);
-      to_frame(f, raw.b.z, raw.b.w, cx, cy); out[6] = (__bf16)(cx * inv); out[7] = (__bf16)(cy * inv);
+      const float sa = (k & 1) ? sv_a : su_a, sb = (k & 1) ? sv_b : su_b;
+      out[0] = (__bf16)((swap ? raw.a.y : raw.a.x) * sa);
+      out[1] = (__bf16)((swap ? raw.a.x : raw.a.y) * sb);
+      out[2] = (__bf16)((swap ? raw.a.w : raw.a.z) * sa);
+      out[3] = (__bf16)((swap ? raw.a.z : raw.a.w) * sb);
+      out[4] = (__bf16)((swap ? raw.b.y : raw.b.x) * sa);
+      out[5] = (__bf16)((swap ? raw.b.x : raw.b.y) * sb);
+      out[6] = (__bf16)((swap ? raw.b.w : raw.b.z) * sa);
+      out[7] = (__bf16)((swap ? raw.b.z : raw.b.w) * sb);
       return out;
     } else {
       return xr;
