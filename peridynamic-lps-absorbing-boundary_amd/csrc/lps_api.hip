@@ -181,10 +181,14 @@ static uint64_t abc_inputs_hash(const lps_problem *p, bool abc_on, int32_t theta
   return h;
 }
 
-// The fast path's packed member state (AbcPack) for the clouds cl (slots, -1 pads): the distinct
-// member slots in ascending order, each entry's index among them, and the record buffers.
-static int32_t build_member_pack(lps_ctx *c, const std::vector<int32_t> &cl, int64_t n_slots,
-                                 bool with_theta) {
+// The fast path's packed member state (AbcPack) for the clouds cl (slots, -1 pads; row r's entries
+// from off[r] on): the distinct member slots in ascending order, each entry's index among them in
+// the order of its row's operator block (entry jj of row r at off[r] + pmap[off[r] + jj], so the
+// row kernels read a member's coefficients at its own position, without the map), and the record
+// buffers.
+static int32_t build_member_pack(lps_ctx *c, const std::vector<int32_t> &cl,
+                                 const std::vector<int32_t> &off, const std::vector<uint8_t> &pmap,
+                                 int64_t n_slots, bool with_theta) {
   std::vector<int32_t> of((size_t)std::max<int64_t>(n_slots, 1), -1), slots, idx(cl.size(), -1);
   for (int32_t v : cl)
     if (v >= 0) of[v] = 0;
@@ -193,7 +197,9 @@ static int32_t build_member_pack(lps_ctx *c, const std::vector<int32_t> &cl, int
       of[i] = (int32_t)slots.size();
       slots.push_back((int32_t)i);
     }
-  for (size_t j = 0; j < cl.size(); j++) idx[j] = cl[j] < 0 ? -1 : of[cl[j]];
+  for (size_t r = 0; r + 1 < off.size(); r++)
+    for (int32_t e = off[r]; e < off[r + 1]; e++)
+      idx[(size_t)off[r] + pmap[e]] = cl[e] < 0 ? -1 : of[cl[e]];
   c->n_pk = (int32_t)slots.size();
   CTX_TRY(c, c->upload(&c->pk_slot, slots));
   CTX_TRY(c, c->upload(&c->pk_idx, idx));
@@ -766,7 +772,7 @@ static int32_t create_impl(lps_ctx *c, const lps_problem *p) {
     CTX_TRY(c, c->upload(&c->abc_off, off));
     CTX_TRY(c, c->upload(&c->cloud, hcl));
     if (c->fast) {
-      const int32_t rc = build_member_pack(c, hcl, c->n_slots, gth_on);
+      const int32_t rc = build_member_pack(c, hcl, off, pmap, c->n_slots, gth_on);
       if (rc != LPS_OK) return rc;
     }
     CTX_TRY(c, c->upload(&c->gu, hgu));
@@ -1247,7 +1253,7 @@ static int32_t merge_batch(lps_ctx *c, const std::vector<lps_ctx *> &sub) {
     CTX_TRY(c, c->upload(&c->abc_xf, xf));
     CTX_TRY(c, c->upload(&c->abc_pmap, pmap));
     CTX_TRY(c, c->upload(&c->cloud, cl));
-    if ((rc = build_member_pack(c, cl, NS, gth)) != LPS_OK) return rc;
+    if ((rc = build_member_pack(c, cl, off, pmap, NS, gth)) != LPS_OK) return rc;
   }
   // instances
   c->inst_h.clear();

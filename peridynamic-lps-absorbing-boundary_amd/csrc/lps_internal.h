@@ -184,7 +184,8 @@ struct RingU {
 // (launch_abc_pack), so a member's G_u / G_v terms gather one 64-byte record (and G_theta one
 // 16-byte record) instead of eight (two) values from separate arrays.
 struct AbcPack {
-  const int32_t *idx;  // per cloud entry: its member's record, -1 for a pad
+  const int32_t *idx;  // per cloud entry, in the order of its row's operator block (the row's entry
+                       // jj at block position abc_pmap[o + jj]): its member's record, -1 for a pad
   const double4 *u;    // per member: (U^n_x, U^n_y, U^{n-1}_x, U^{n-1}_y), (V^n_x, .., V^{n-1}_y)
   const double2 *th;   // per member: (theta^n, theta^{n-1}) (G_theta only)
 };

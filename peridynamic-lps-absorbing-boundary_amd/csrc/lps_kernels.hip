@@ -1070,7 +1070,7 @@ __device__ __forceinline__ void abc_theta_rows16(const StepArgs &a, const StepSt
       for (int u = 0; u < 4; u++) {
         const int jj = min(base + u * kRowLanes, L - 1);
         const int32_t ks = k[u] >= 0 ? k[u] : 0;
-        const double2 gm = *(const double2 *)(g + 2 * a.abc_pmap[o + jj]);
+        const double2 gm = *(const double2 *)(g + 2 * jj);  // p.idx is in block order
         const double2 h = p.th[ks];  // (theta^n, theta^{n-1}) of the member
         const double t = gtheta_term_exact(gm.x, gm.y, h.x, h.y);
         acc = k[u] >= 0 ? acc + t : acc;
@@ -1108,8 +1108,11 @@ __device__ __forceinline__ void abc_uv_rows16(const StepArgs &a, const StepState
     for (int u = 0; u < 4; u++) {
       const int jj = min(base + u * kRowLanes, L - 1);
       const int32_t ks = k[u] >= 0 ? k[u] : 0;
+      // p.idx is in block order: member jj's coefficients are the block's jj-th
+      const double *bu = a.gu + 8 * gb + 4 * jj, *bv = a.gv + 8 * gb + 4 * jj;
       double4 g0, g1, h0, h1;
-      abc_uv_coeffs(a, o, L, gb, xf, jj, g0, g1, h0, h1);
+      lattice_map(xf, *(const double4 *)bu, *(const double4 *)(bu + 4 * L), g0, g1);
+      lattice_map(xf, *(const double4 *)bv, *(const double4 *)(bv + 4 * L), h0, h1);
       const double4 su = p.u[2 * ks], sv = p.u[2 * ks + 1];
       const double ux = su.x, uy = su.y, uxm = su.z, uym = su.w;
       const double vx = sv.x, vy = sv.y, vxm = sv.z, vym = sv.w;

@@ -383,12 +383,12 @@ def test_teacher_forced_samples():
 
 
 @pytest.mark.parametrize("use_graph", [False, True])
-def test_closure_beside_the_abc_rows_gives_the_same_bits(use_graph):
-    """With the ABC rows on their own stream the closure's network runs from the step's start on
-    another stream and its corrections are added after G_u / G_v; one stream runs the network after
-    G_u / G_v and adds them in its epilogue; with the ABC rows inside the dilatation launch the
-    network runs beside the tile kernels and its corrections are added before the ABC nodes'
-    prediction. All give the same bits."""
+def test_closure_beside_the_abc_rows_gives_the_same_bits(use_graph, monkeypatch):
+    """With the ABC rows on their own stream the closure's network runs first in the step (or from
+    the step's start on another stream) and its corrections are added after G_u / G_v; one stream
+    runs the network after G_u / G_v and adds them in its epilogue; with the ABC rows inside the
+    dilatation launch the network runs beside the tile kernels and its corrections are added before
+    the ABC nodes' prediction. All give the same bits."""
     say("closure deferred vs in-line", use_graph)
     prob = with_wide_pulse(load_golden("abc_r10_nu025")[0])
     c, _ = closure_with(26, 0.0)
@@ -396,7 +396,10 @@ def test_closure_beside_the_abc_rows_gives_the_same_bits(use_graph):
     c.set_scales(1e-3, 1e-3)
     out = []
     with c:
-        for concurrent in (0, 1, 3, 4):
+        # with the rows on their own stream the network runs first on the main stream, or
+        # (LPS_CLOSURE_FIRST=0) on a fourth stream beside the tile kernels
+        for concurrent, first in ((0, "1"), (1, "1"), (1, "0"), (3, "1"), (4, "1")):
+            monkeypatch.setenv("LPS_CLOSURE_FIRST", first)
             with solver.Solver(prob, variant=3, use_graph=use_graph, concurrent=concurrent) as s:
                 s.attach_closure(c)
                 s.run(90)
