@@ -183,13 +183,13 @@ static uint64_t abc_inputs_hash(const lps_problem *p, bool abc_on, int32_t theta
 
 // The fast path's packed member state (AbcPack) for the clouds cl (slots, -1 pads; row r's entries
 // from off[r] on): the distinct member slots in ascending order, each entry's index among them in
-// the order of its row's operator block (entry jj of row r at off[r] + pmap[off[r] + jj], so the
-// row kernels read a member's coefficients at its own position, without the map), and the record
-// buffers.
+// the order of its row's operator block at a fixed stride per row (entry jj of row r at
+// r * stride + pmap[off[r] + jj], so the row kernels read a member's coefficients at its own
+// position, without the map), and the record buffers.
 static int32_t build_member_pack(lps_ctx *c, const std::vector<int32_t> &cl,
                                  const std::vector<int32_t> &off, const std::vector<uint8_t> &pmap,
                                  int64_t n_slots, bool with_theta) {
-  std::vector<int32_t> of((size_t)std::max<int64_t>(n_slots, 1), -1), slots, idx(cl.size(), -1);
+  std::vector<int32_t> of((size_t)std::max<int64_t>(n_slots, 1), -1), slots;
   for (int32_t v : cl)
     if (v >= 0) of[v] = 0;
   for (int64_t i = 0; i < n_slots; i++)
@@ -197,14 +197,20 @@ static int32_t build_member_pack(lps_ctx *c, const std::vector<int32_t> &cl,
       of[i] = (int32_t)slots.size();
       slots.push_back((int32_t)i);
     }
-  for (size_t r = 0; r + 1 < off.size(); r++)
+  const size_t R = off.size() - 1;
+  int32_t W = 1;
+  for (size_t r = 0; r < R; r++) W = std::max(W, off[r + 1] - off[r]);
+  std::vector<int32_t> idx(std::max<size_t>(R * W, 1), -1);
+  for (size_t r = 0; r < R; r++)
     for (int32_t e = off[r]; e < off[r + 1]; e++)
-      idx[(size_t)off[r] + pmap[e]] = cl[e] < 0 ? -1 : of[cl[e]];
+      idx[r * W + pmap[e]] = cl[e] < 0 ? -1 : of[cl[e]];
   c->n_pk = (int32_t)slots.size();
+  c->pk_stride = W;
   CTX_TRY(c, c->upload(&c->pk_slot, slots));
   CTX_TRY(c, c->upload(&c->pk_idx, idx));
-  CTX_TRY(c, c->alloc(&c->pk_u, 2 * (size_t)c->n_pk));
-  if (with_theta) CTX_TRY(c, c->alloc(&c->pk_th, (size_t)c->n_pk));
+  // at least one record: the row kernels read record 0 for pads
+  CTX_TRY(c, c->alloc(&c->pk_u, 2 * (size_t)std::max(c->n_pk, 1)));
+  if (with_theta) CTX_TRY(c, c->alloc(&c->pk_th, (size_t)std::max(c->n_pk, 1)));
   CTX_TRY(c, hipStreamSynchronize(c->stream));  // the host vectors end here
   return LPS_OK;
 }
@@ -745,8 +751,10 @@ static int32_t create_impl(lps_ctx *c, const lps_problem *p) {
     c->abc_block_members = GM;
     c->abc_shared_blocks = shared;
     std::vector<int32_t> hcl(std::max<size_t>(NM, 1), -1);
-    std::vector<double> hgu(std::max<int64_t>(8 * GM, 1)), hgv(std::max<int64_t>(8 * GM, 1)), hgt;
-    if (gth_on) hgt.resize(std::max<int64_t>(2 * GM, 1));
+    // one member of zeros past the last block: a row without members (L = 0) still reads its
+    // first member's coefficients (discarded) without a branch on L in the row kernels
+    std::vector<double> hgu(8 * GM + 8), hgv(8 * GM + 8), hgt;
+    if (gth_on) hgt.resize(2 * GM + 2);
     for (int32_t r = 0; r < R; r++) {
       const int64_t a = active_rows[r];
       const size_t o = (size_t)off[r];
@@ -1208,9 +1216,10 @@ static int32_t merge_batch(lps_ctx *c, const std::vector<lps_ctx *> &sub) {
     cl.reserve(nm);
     pmap.reserve(nm);
     off.push_back(0);
-    CTX_TRY(c, c->alloc(&c->gu, 8 * gm));
-    CTX_TRY(c, c->alloc(&c->gv, 8 * gm));
-    if (gth) CTX_TRY(c, c->alloc(&c->gth, 2 * gm));
+    // one member past the last block, as in the single setup (read, never used, by rows with L = 0)
+    CTX_TRY(c, c->alloc(&c->gu, 8 * gm + 8));
+    CTX_TRY(c, c->alloc(&c->gv, 8 * gm + 8));
+    if (gth) CTX_TRY(c, c->alloc(&c->gth, 2 * gm + 2));
     int64_t m0 = 0;
     for (int32_t i = 0; i < B; i++) {
       const lps_ctx &q = *sub[i];

@@ -88,7 +88,7 @@ struct lps_ctx {
   // the clouds' members packed per member for the fast path's row kernels (AbcPack): their slots,
   // each cloud entry's member index (-1 pad) and the records, rewritten at each step's start
   int32_t *pk_slot = nullptr, *pk_idx = nullptr;
-  int32_t n_pk = 0;
+  int32_t n_pk = 0, pk_stride = 0;
   double4 *pk_u = nullptr;
   double2 *pk_th = nullptr;
   TileArgs tile{};
@@ -150,7 +150,7 @@ struct lps_ctx {
     const Dev2 &a = ring_u(s, 0), &b = ring_u(s, 1);
     return RingU{a.x, a.y, b.x, b.y};
   }
-  lps::AbcPack pack() const { return lps::AbcPack{pk_idx, pk_u, pk_th}; }
+  lps::AbcPack pack() const { return lps::AbcPack{pk_idx, pk_u, pk_th, pk_stride}; }
 };
 
 namespace lps {

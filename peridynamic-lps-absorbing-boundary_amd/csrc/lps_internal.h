@@ -184,10 +184,14 @@ struct RingU {
 // (launch_abc_pack), so a member's G_u / G_v terms gather one 64-byte record (and G_theta one
 // 16-byte record) instead of eight (two) values from separate arrays.
 struct AbcPack {
-  const int32_t *idx;  // per cloud entry, in the order of its row's operator block (the row's entry
-                       // jj at block position abc_pmap[o + jj]): its member's record, -1 for a pad
+  // per row, stride entries (the widest row's length): the row's cloud entries in the order of its
+  // operator block (entry jj of row r at r * stride + abc_pmap[abc_off[r] + jj]), each its member's
+  // record, -1 for a pad and past the row's end. The fixed stride lets a row kernel load the
+  // entries before it knows where the row's ragged data start.
+  const int32_t *idx;
   const double4 *u;    // per member: (U^n_x, U^n_y, U^{n-1}_x, U^{n-1}_y), (V^n_x, .., V^{n-1}_y)
   const double2 *th;   // per member: (theta^n, theta^{n-1}) (G_theta only)
+  int32_t stride;
 };
 
 // Per-block partial sums of the output block.

@@ -1054,21 +1054,22 @@ __device__ __forceinline__ void abc_theta_rows16(const StepArgs &a, const StepSt
     int32_t L;
     abc_row_span(a, row, o, L);
     const double *g = a.gth + 2 * (int64_t)a.abc_gofs[row];  // G_theta needs no lattice transform
-    const int32_t *cl = p.idx + o;
+    const int32_t *cl = p.idx + (int64_t)row * p.stride;
     double acc = 0.0;
     // four members per lane at a time (q, q + 16, q + 32, q + 48): their loads are unconditional
     // (clamped indices), so the four members' gathers are in flight together; a member past the
-    // cloud or a pad adds nothing
-    for (int base = q; base < L; base += 4 * kRowLanes) {
+    // cloud or a pad adds nothing. The entries' loads need only the row (fixed stride), not its
+    // span, so they go out together with the span's.
+    for (int base = q; base < p.stride; base += 4 * kRowLanes) {
       int32_t k[4];
 #pragma unroll
       for (int u = 0; u < 4; u++) {
         const int jj = base + u * kRowLanes;
-        k[u] = jj < L ? cl[jj] : -1;
+        k[u] = jj < p.stride ? cl[jj] : -1;
       }
 #pragma unroll
       for (int u = 0; u < 4; u++) {
-        const int jj = min(base + u * kRowLanes, L - 1);
+        const int jj = max(min(base + u * kRowLanes, L - 1), 0);
         const int32_t ks = k[u] >= 0 ? k[u] : 0;
         const double2 gm = *(const double2 *)(g + 2 * jj);  // p.idx is in block order
         const double2 h = p.th[ks];  // (theta^n, theta^{n-1}) of the member
@@ -1094,19 +1095,19 @@ __device__ __forceinline__ void abc_uv_rows16(const StepArgs &a, const StepState
   abc_row_span(a, row, o, L);
   const int64_t gb = a.abc_gofs[row];
   const int xf = a.abc_xf[row];
-  const int32_t *cl = p.idx + o;
+  const int32_t *cl = p.idx + (int64_t)row * p.stride;
   double Ux = 0.0, Uy = 0.0, Vx = 0.0, Vy = 0.0;
   // four members per lane at a time, loads unconditional (see abc_theta_rows16)
-  for (int base = q; base < L; base += 4 * kRowLanes) {
+  for (int base = q; base < p.stride; base += 4 * kRowLanes) {
     int32_t k[4];
 #pragma unroll
     for (int u = 0; u < 4; u++) {
       const int jj = base + u * kRowLanes;
-      k[u] = jj < L ? cl[jj] : -1;
+      k[u] = jj < p.stride ? cl[jj] : -1;
     }
 #pragma unroll
     for (int u = 0; u < 4; u++) {
-      const int jj = min(base + u * kRowLanes, L - 1);
+      const int jj = max(min(base + u * kRowLanes, L - 1), 0);
       const int32_t ks = k[u] >= 0 ? k[u] : 0;
       // p.idx is in block order: member jj's coefficients are the block's jj-th
       const double *bu = a.gu + 8 * gb + 4 * jj, *bv = a.gv + 8 * gb + 4 * jj;
