@@ -1618,9 +1618,10 @@ static hipError_t step_once(lps_ctx *c, bool record_events) {
   }
   if (ev_force && (e = record_timing(c, ev[2])) != hipSuccess) return e;
   // the ABC nodes' prediction folded into the partial tiles' force pass (every ABC node lies in a
-  // partial tile), which then waits for G_u / G_v; not with a closure attached, whose long ABC
-  // branch the partial tiles would wait for
-  const bool fold = abc && !split && !c->clo && c->abc_in_part && c->fold_pred &&
+  // partial tile), which then waits for G_u / G_v (and a closure's stored corrections, when its
+  // network ran first in the step; not when the network runs beside the tile kernels, a long ABC
+  // branch the partial tiles would wait for)
+  const bool fold = abc && !split && (!c->clo || clo_first) && c->abc_in_part && c->fold_pred &&
                     a.tile.force_kernel != kGeneric;
   if (c->fast && fold) {
     if (fork && (e = hipStreamWaitEvent(c->stream, c->ev_join[1], 0)) != hipSuccess) return e;
