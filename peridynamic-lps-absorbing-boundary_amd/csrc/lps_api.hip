@@ -209,7 +209,8 @@ static int32_t build_member_pack(lps_ctx *c, const std::vector<int32_t> &cl,
   CTX_TRY(c, c->upload(&c->pk_slot, slots));
   CTX_TRY(c, c->upload(&c->pk_idx, idx));
   // at least one record: the row kernels read record 0 for pads
-  CTX_TRY(c, c->alloc(&c->pk_u, 2 * (size_t)std::max(c->n_pk, 1)));
+  for (int b = 0; b < 2; b++) CTX_TRY(c, c->alloc(&c->pk_lvl[b], (size_t)std::max(c->n_pk, 1)));
+  c->pk_last_step = -1;
   if (with_theta) CTX_TRY(c, c->alloc(&c->pk_th, (size_t)std::max(c->n_pk, 1)));
   CTX_TRY(c, hipStreamSynchronize(c->stream));  // the host vectors end here
   return LPS_OK;
@@ -866,6 +867,7 @@ static int32_t reset_state(lps_ctx *c) {
     CTX_TRY(c, reset_ring(c));
     CTX_TRY(c, hipStreamSynchronize(s));
     c->step = 0;
+    c->pk_last_step = -1;
     c->E0 = -1.0;
     c->E0_inst.assign(c->n_inst, -1.0);
     return LPS_OK;
@@ -895,6 +897,7 @@ static int32_t reset_state(lps_ctx *c) {
   CTX_TRY(c, reset_ring(c));
   CTX_TRY(c, hipStreamSynchronize(s));
   c->step = 0;
+  c->pk_last_step = -1;
   c->E0 = -1.0;
   return LPS_OK;
 }
@@ -1481,6 +1484,18 @@ static hipError_t record_timing(lps_ctx *c, hipEvent_t e) {
   return hipEventRecordWithFlags(e, c->stream, c->capturing ? hipEventRecordExternal : 0);
 }
 
+// The ABC members' records of step c->step: its U, V level, and the previous step's unless the
+// records already hold it (written as the previous step's level by the step just before). The
+// first step of a captured 30-step graph always writes both, so a replay is right after a reset
+// or a restore too.
+static hipError_t pack_members(lps_ctx *c, const StepState &st, hipStream_t s) {
+  const int64_t n = c->step;
+  const bool prev = n % 30 == 0 || c->pk_last_step != n - 1;
+  c->pk_last_step = n;
+  return launch_abc_pack(c->n_pk, c->pk_slot, c->kind, st, c->ring(n), c->pk_lvl[n & 1],
+                         prev ? c->pk_lvl[(n + 1) & 1] : nullptr, c->pk_th, s);
+}
+
 // Fast path, concurrent == 3: one stream, three launches. The ABC rows' G_theta and G_u / G_v run as
 // extra workgroups of the dilatation launch (they read only the previous steps' state), then the
 // full tiles' force, then the partial tiles' force with the ABC nodes' prediction. A learned
@@ -1495,10 +1510,8 @@ static hipError_t step_fused(lps_ctx *c, const StepArgs &a, const StepState &st,
     if ((e = closure_step_hook(c, st, c->stream4, true)) != hipSuccess) return e;
     if ((e = hipEventRecord(c->ev_clo, c->stream4)) != hipSuccess) return e;
   }
-  if ((e = launch_abc_pack(c->n_pk, c->pk_slot, c->kind, st, c->ring(c->step), c->pk_u, c->pk_th,
-                           c->stream)) != hipSuccess)
-    return e;
-  if ((e = launch_theta_rows(a, st, c->pack(), c->stream)) != hipSuccess) return e;
+  if ((e = pack_members(c, st, c->stream)) != hipSuccess) return e;
+  if ((e = launch_theta_rows(a, st, c->pack(c->step), c->stream)) != hipSuccess) return e;
   if (a.theta_method == LPS_THETA_IDW && (e = launch_idw(a, st, c->stream)) != hipSuccess) return e;
   const bool ev_force = record_events && !c->timing_closure;
   if (ev_force && (e = record_timing(c, ev[1])) != hipSuccess) return e;
@@ -1575,22 +1588,18 @@ static hipError_t step_once(lps_ctx *c, bool record_events) {
       if ((e = closure_step_hook(c, st, c->stream4, true)) != hipSuccess) return e;
       if ((e = hipEventRecord(c->ev_clo, c->stream4)) != hipSuccess) return e;
     }
-    if ((e = launch_abc_pack(c->n_pk, c->pk_slot, c->kind, st, c->ring(c->step), c->pk_u,
-                             c->pk_th, c->stream2)) != hipSuccess)
-      return e;
-    if ((e = launch_abc_theta(a, st, c->pack(), c->stream2)) != hipSuccess) return e;
+    if ((e = pack_members(c, st, c->stream2)) != hipSuccess) return e;
+    if ((e = launch_abc_theta(a, st, c->pack(c->step), c->stream2)) != hipSuccess) return e;
     if ((e = hipEventRecord(c->ev_join[0], c->stream2)) != hipSuccess) return e;
-    if ((e = launch_abc_uv(a, st, c->pack(), c->stream2)) != hipSuccess) return e;
+    if ((e = launch_abc_uv(a, st, c->pack(c->step), c->stream2)) != hipSuccess) return e;
     if (c->clo) {
       if (!clo_first && (e = hipStreamWaitEvent(c->stream2, c->ev_clo, 0)) != hipSuccess) return e;
       if ((e = closure_apply_hook(c, st, c->stream2)) != hipSuccess) return e;
     }
     if ((e = hipEventRecord(c->ev_join[1], c->stream2)) != hipSuccess) return e;
   } else if (abc) {
-    if ((e = launch_abc_pack(c->n_pk, c->pk_slot, c->kind, st, c->ring(c->step), c->pk_u, c->pk_th,
-                             c->stream)) != hipSuccess)
-      return e;
-    if ((e = launch_abc_theta(a, st, c->pack(), c->stream)) != hipSuccess) return e;
+    if ((e = pack_members(c, st, c->stream)) != hipSuccess) return e;
+    if ((e = launch_abc_theta(a, st, c->pack(c->step), c->stream)) != hipSuccess) return e;
   }
   if (deep) {
     if ((e = launch_force_deep(a, st, c->deep_list, c->n_deep, c->stream)) != hipSuccess) return e;
@@ -1625,13 +1634,13 @@ static hipError_t step_once(lps_ctx *c, bool record_events) {
                     a.tile.force_kernel != kGeneric;
   if (c->fast && fold) {
     if (fork && (e = hipStreamWaitEvent(c->stream, c->ev_join[1], 0)) != hipSuccess) return e;
-    if (!fork && (e = launch_abc_uv(a, st, c->pack(), c->stream)) != hipSuccess) return e;
+    if (!fork && (e = launch_abc_uv(a, st, c->pack(c->step), c->stream)) != hipSuccess) return e;
     if ((e = launch_force_part(a, st, c->stream, true)) != hipSuccess) return e;
   } else if (c->fast) {
     if (!split && (e = launch_force_part(a, st, c->stream)) != hipSuccess) return e;
     if (split && (e = hipStreamWaitEvent(c->stream, c->ev_part[2], 0)) != hipSuccess) return e;
     if (fork && (e = hipStreamWaitEvent(c->stream, c->ev_join[1], 0)) != hipSuccess) return e;
-    if (abc && !fork && (e = launch_abc_uv(a, st, c->pack(), c->stream)) != hipSuccess) return e;
+    if (abc && !fork && (e = launch_abc_uv(a, st, c->pack(c->step), c->stream)) != hipSuccess) return e;
     if (abc && !fork && c->clo && (e = closure_step_hook(c, st, c->stream, false)) != hipSuccess) return e;
     if (abc && (e = launch_abc_predict(a, st, c->stream)) != hipSuccess) return e;
   }

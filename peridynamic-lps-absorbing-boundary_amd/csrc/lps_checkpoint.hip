@@ -165,6 +165,7 @@ int32_t lps_checkpoint_load(lps_ctx *c, const char *path) {
   }
   c->u_synced = 2;
   c->step = h.step;
+  c->pk_last_step = -1;  // the ABC members' records hold another state
   if (c->batch) {
     c->E0_inst = e0;
     c->E0 = -1.0;

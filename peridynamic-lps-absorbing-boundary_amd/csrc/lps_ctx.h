@@ -89,7 +89,8 @@ struct lps_ctx {
   // each cloud entry's member index (-1 pad) and the records, rewritten at each step's start
   int32_t *pk_slot = nullptr, *pk_idx = nullptr;
   int32_t n_pk = 0, pk_stride = 0;
-  double4 *pk_u = nullptr;
+  double4 *pk_lvl[2] = {nullptr, nullptr};  // U, V of one level per member, by step parity
+  int64_t pk_last_step = -1;  // the step whose level the records hold last (-1: none valid)
   double2 *pk_th = nullptr;
   TileArgs tile{};
   // Prediction ring: step s reads P[s % 6] (U^{s+1} before the ABC overwrite) and writes
@@ -150,7 +151,9 @@ struct lps_ctx {
     const Dev2 &a = ring_u(s, 0), &b = ring_u(s, 1);
     return RingU{a.x, a.y, b.x, b.y};
   }
-  lps::AbcPack pack() const { return lps::AbcPack{pk_idx, pk_u, pk_th, pk_stride}; }
+  lps::AbcPack pack(int64_t s) const {
+    return lps::AbcPack{pk_idx, pk_lvl[s & 1], pk_lvl[(s + 1) & 1], pk_th, pk_stride};
+  }
 };
 
 namespace lps {

@@ -181,15 +181,19 @@ struct RingU {
 };
 
 // Fast path, ABC rows: the state of the clouds' members packed per member at the step's start
-// (launch_abc_pack), so a member's G_u / G_v terms gather one 64-byte record (and G_theta one
-// 16-byte record) instead of eight (two) values from separate arrays.
+// (launch_abc_pack), so a member's G_u / G_v terms gather two 32-byte records (and G_theta one
+// 16-byte record) instead of eight (two) values from separate arrays. The records of step n's
+// U^n, V^n are those of step n + 1's U^{n}, V^{n} (two buffers by step parity), so a step packs
+// one level.
 struct AbcPack {
   // per row, stride entries (the widest row's length): the row's cloud entries in the order of its
   // operator block (entry jj of row r at r * stride + abc_pmap[abc_off[r] + jj]), each its member's
   // record, -1 for a pad and past the row's end. The fixed stride lets a row kernel load the
   // entries before it knows where the row's ragged data start.
   const int32_t *idx;
-  const double4 *u;    // per member: (U^n_x, U^n_y, U^{n-1}_x, U^{n-1}_y), (V^n_x, .., V^{n-1}_y)
+  const double4 *cur;  // per member: (U^n_x, U^n_y, V^n_x, V^n_y)
+  const double4 *prev; // per member: (U^{n-1}_x, U^{n-1}_y, V^{n-1}_x, V^{n-1}_y), the previous
+                       // step's cur
   const double2 *th;   // per member: (theta^n, theta^{n-1}) (G_theta only)
   int32_t stride;
 };
@@ -238,10 +242,12 @@ hipError_t launch_force_list(const StepArgs &a, const StepState &st, const int32
                              int32_t count, hipStream_t s);
 hipError_t launch_force_deep(const StepArgs &a, const StepState &st, const int32_t *list,
                              int32_t count, hipStream_t s);
-// The n members' packed state (slot: their slots; th null without G_theta); U of an interior
-// member (kind 1) from the prediction ring r, of an ABC member from U. Before the step's row kernels.
+// The n members' packed state (slot: their slots; th null without G_theta): the level of U^n, V^n
+// into cur, and with prev not null also U^{n-1}, V^{n-1} into prev (after a reset, a restore, or
+// at a step that starts a captured graph); U of an interior member (kind 1) from the prediction
+// ring r, of an ABC member from U. Before the step's row kernels.
 hipError_t launch_abc_pack(int32_t n, const int32_t *slot, const int8_t *kind, const StepState &st,
-                           const RingU &r, double4 *u, double2 *th, hipStream_t s);
+                           const RingU &r, double4 *cur, double4 *prev, double2 *th, hipStream_t s);
 hipError_t launch_abc_theta(const StepArgs &a, const StepState &st, const AbcPack &p, hipStream_t s);
 hipError_t launch_abc_uv(const StepArgs &a, const StepState &st, const AbcPack &p, hipStream_t s);
 hipError_t launch_abc_predict(const StepArgs &a, const StepState &st, hipStream_t s);

@@ -1114,9 +1114,9 @@ __device__ __forceinline__ void abc_uv_rows16(const StepArgs &a, const StepState
       double4 g0, g1, h0, h1;
       lattice_map(xf, *(const double4 *)bu, *(const double4 *)(bu + 4 * L), g0, g1);
       lattice_map(xf, *(const double4 *)bv, *(const double4 *)(bv + 4 * L), h0, h1);
-      const double4 su = p.u[2 * ks], sv = p.u[2 * ks + 1];
-      const double ux = su.x, uy = su.y, uxm = su.z, uym = su.w;
-      const double vx = sv.x, vy = sv.y, vxm = sv.z, vym = sv.w;
+      const double4 sn = p.cur[ks], sm = p.prev[ks];
+      const double ux = sn.x, uy = sn.y, uxm = sm.x, uym = sm.y;
+      const double vx = sn.z, vy = sn.w, vxm = sm.z, vym = sm.w;
       const double tux = g0.x * ux + g0.y * uy + g0.z * uxm + g0.w * uym;
       const double tuy = g1.x * ux + g1.y * uy + g1.z * uxm + g1.w * uym;
       const double tvx = h0.x * vx + h0.y * vy + h0.z * vxm + h0.w * vym;
@@ -1159,14 +1159,14 @@ __global__ void __launch_bounds__(kBlock) k_abc_uv(StepArgs a, StepState st, Abc
 // The packed state of the ABC clouds' members (AbcPack), one thread per member.
 __global__ void __launch_bounds__(kBlock) k_abc_pack(int32_t n, const int32_t *slot,
                                                      const int8_t *kind, StepState st, RingU r,
-                                                     double4 *u, double2 *th) {
+                                                     double4 *cur, double4 *prev, double2 *th) {
   const int32_t m = blockIdx.x * kBlock + threadIdx.x;
   if (m >= n) return;
   const int32_t i = slot[m];
   const bool in = kind[i] == 1;
-  u[2 * m] = make_double4((in ? r.ux : st.ux)[i], (in ? r.uy : st.uy)[i], (in ? r.uxm : st.uxm)[i],
-                          (in ? r.uym : st.uym)[i]);
-  u[2 * m + 1] = make_double4(st.vx[i], st.vy[i], st.vxm[i], st.vym[i]);
+  cur[m] = make_double4((in ? r.ux : st.ux)[i], (in ? r.uy : st.uy)[i], st.vx[i], st.vy[i]);
+  if (prev)
+    prev[m] = make_double4((in ? r.uxm : st.uxm)[i], (in ? r.uym : st.uym)[i], st.vxm[i], st.vym[i]);
   if (th) th[m] = make_double2(st.thn[i], st.thnm1[i]);
 }
 
@@ -1838,10 +1838,10 @@ hipError_t launch_abc_theta(const StepArgs &a, const StepState &st, const AbcPac
 }
 
 hipError_t launch_abc_pack(int32_t n, const int32_t *slot, const int8_t *kind, const StepState &st,
-                           const RingU &r, double4 *u, double2 *th, hipStream_t s) {
+                           const RingU &r, double4 *cur, double4 *prev, double2 *th, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_abc_pack, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, n, slot, kind,
-                     st, r, u, th);
+                     st, r, cur, prev, th);
   return hipGetLastError();
 }
 
