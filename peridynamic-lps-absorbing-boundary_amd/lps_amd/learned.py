@@ -4,8 +4,8 @@ against the least-squares ABC.
 The reference has no learned closure; its ABC is the least-squares G_u / G_v update
 (src/main.cpp:883-974, operators from preprocessor/main.cpp:415-637). Here a network (lps_amd.closure)
 adds a correction to that update at every ABC row, and the least-squares ABC stays both the
-fallback (gain 0) and the judge: a closure is kept for an instance only where its reflected energy
-is at or below the least-squares ABC's.
+fallback (gain 0) and the standard it is measured against: a closure is kept for an instance only
+where its reflected energy is at or below the least-squares ABC's.
 
 Workload (truncated domain R = 125 mm, truth R = 375 mm):
   * initial conditions: 1-4 Gaussian pulses, radius U(5, 40) mm, centres uniform in the disk of
