@@ -504,7 +504,8 @@ def main():
                     help="step schedule: 0 ABC rows after the tile kernels, 1 on a second stream "
                          "beside them, 2 also the partial tiles on a third, 3 the ABC rows inside "
                          "the dilatation launch, 4 as 1 with the deep tiles' dilatation and force "
-                         "in one launch (default: the library's)")
+                         "in one launch (default: the library's, 2 for batches of small domains, "
+                         "else 1)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=90)
     ap.add_argument("--workers", type=int, default=None)

@@ -74,9 +74,9 @@ typedef struct lps_options {
                             with the "deep" full tiles (full 3 x 3 neighbourhood, interior nodes
                             only in the 24 x 24 window) computing dilatation and force in one
                             launch while the other tiles' dilatation runs on a third stream; 0
-                            the row kernels after the tile kernels, one stream. Every schedule
-                            gives the same bits except 2 (another partial-tile dilatation kernel,
-                            within rounding) */
+                            the row kernels after the tile kernels, one stream; -1 (the default)
+                            2 when the partial tiles are at least a tenth of the tiles (batches
+                            of small domains), else 1. Every schedule gives the same bits */
 } lps_options;
 
 /* One output record: the quantities src/main.cpp:986-1062 computes every 30 steps. */

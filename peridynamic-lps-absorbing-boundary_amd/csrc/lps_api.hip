@@ -107,7 +107,18 @@ void lps_default_options(lps_options *opt) {
   opt->device = 0;
   opt->horizon_radius = 0.0;
   opt->variant = 0;
-  opt->concurrent = 1;
+  opt->concurrent = -1;
+}
+
+// concurrent = -1 (the default): schedule 2 (the partial tiles on a third stream, in whole-tile
+// blocks) when the partial tiles are at least a tenth of the tiles, as in batches of small
+// domains (default workload: LSQ step 3.70 -> 3.58 ms; 64 domains 1.55 -> 1.50 ms), else 1 (one
+// large domain: even). Every schedule gives the same bits.
+static void resolve_schedule(lps_ctx *c) {
+  if (c->opt.concurrent >= 0) return;
+  const bool many_part = c->fast && (int64_t)c->tile.n_part * 10 >= (int64_t)c->tile.n_tiles;
+  c->opt.concurrent = many_part ? 2 : 1;
+  c->path += many_part ? "; schedule 2 (partial tiles on a third stream)" : "; schedule 1";
 }
 
 const char *lps_global_error(void) { return g_create_error.c_str(); }
@@ -938,8 +949,8 @@ static int32_t new_context(const lps_options *options, lps_ctx **out) {
   lps_ctx *c = new lps_ctx();
   if (options) c->opt = *options;
   else lps_default_options(&c->opt);
-  if (c->opt.concurrent < 0 || c->opt.concurrent > 4) {
-    g_create_error = "concurrent must be 0, 1, 2, 3 or 4";
+  if (c->opt.concurrent < -1 || c->opt.concurrent > 4) {
+    g_create_error = "concurrent must be -1 (automatic), 0, 1, 2, 3 or 4";
     delete c;
     return LPS_ERR_ARG;
   }
@@ -988,6 +999,7 @@ int32_t lps_create(const lps_problem *problem, const lps_options *options, lps_c
   int32_t rc = new_context(options, &c);
   if (rc != LPS_OK) return rc;
   rc = create_impl(c, problem);
+  if (rc == LPS_OK) resolve_schedule(c);
   if (rc == LPS_OK && c->fast) rc = choose_variant(c);
   if (rc != LPS_OK) {
     g_create_error = c->err;
@@ -1372,6 +1384,7 @@ int32_t lps_create_batch(const lps_problem *problems, int32_t count, const lps_o
     }
   }
   if (rc == LPS_OK) rc = merge_batch(c, sub);
+  if (rc == LPS_OK) resolve_schedule(c);
   for (lps_ctx *q : sub) lps_destroy(q);
   if (rc == LPS_OK) rc = choose_abc_predict(c, opt.variant == 0);
   if (rc != LPS_OK) {

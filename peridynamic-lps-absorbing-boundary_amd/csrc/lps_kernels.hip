@@ -580,6 +580,29 @@ union Tile68Shared {
   AbcScratch abc[kWavesPerBlock];
 };
 
+// The masked 68-bond dilatation sum of the node at window cell c (row stride kW68), before its
+// 2 / m_i: the per-bond terms of src/main.cpp:736-761 over the cells with ok set, four chains (bond
+// k in chain k % 4). The roundings are fixed here (explicit fma, no contraction left to the
+// compiler), so the partial tiles' whole-tile blocks (k_theta_t68) and quarter-tile blocks
+// (theta_part_quarter) give the same bits.
+__device__ __forceinline__ double theta_masked68(const double2 *su, const int8_t *ok, int32_t c,
+                                                 double2 ui, const TileArgs &T) {
+#pragma clang fp contract(off)
+  constexpr Stencil68 S = make_stencil68();
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int k = 0; k < 68; k++) {
+    const Off68 o = S.o[k];
+    const int32_t cj = c + o.ox * kW68 + o.oy;
+    const double2 uj = su[cj];
+    const double dx = uj.x - ui.x, dy = uj.y - ui.y;
+    const double d = o.ox == 0 ? dy * o.oy : (o.oy == 0 ? dx * o.ox : fma(dx, (double)o.ox, dy * o.oy));
+    const double term = T.c1[o.cls] * d;
+    acc[k % 4] = ok[cj] ? acc[k % 4] + term : acc[k % 4];
+  }
+  return (acc[0] + acc[1]) + (acc[2] + acc[3]);
+}
+
 // Dilatation over the tiles of list (all tiles when list is null), one node per thread.
 __global__ void __launch_bounds__(kBlock) k_theta_t68(StepArgs a, StepState st,
                                                       const int32_t *list, int32_t count) {
@@ -615,14 +638,8 @@ __global__ void __launch_bounds__(kBlock) k_theta_t68(StepArgs a, StepState st,
       acc[k % 4] += T.c1[o.cls] * dot_int(uj.x - ui.x, uj.y - ui.y, o.ox, o.oy);
     }
   } else {
-#pragma unroll
-    for (int k = 0; k < 68; k++) {
-      const Off68 o = S.o[k];
-      const int32_t cj = c + o.ox * kW68 + o.oy;
-      const double2 uj = sh.th.u[cj];
-      const double term = T.c1[o.cls] * dot_int(uj.x - ui.x, uj.y - ui.y, o.ox, o.oy);
-      acc[k % 4] += sh.th.ok[cj] ? term : 0.0;
-    }
+    st.theta[i] = a.c2m[i] * theta_masked68(sh.th.u, sh.th.ok, c, ui, T);
+    return;
   }
   st.theta[i] = a.c2m[i] * ((acc[0] + acc[1]) + (acc[2] + acc[3]));
 }
@@ -1012,18 +1029,7 @@ __device__ __forceinline__ void theta_part_quarter(const StepArgs &a, const Step
   const int32_t i = t * (kTile * kTile) + (4 * b + lane / kTile) * kTile + lane % kTile;
   if (a.kind[i] != 1) return;
   const int32_t c = (lane / kTile + 4) * kW68 + (lane % kTile + 4);
-  const double2 ui = su[c];
-  constexpr Stencil68 S = make_stencil68();
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};  // bond k feeds chain k % 4
-#pragma unroll
-  for (int k = 0; k < 68; k++) {
-    const Off68 o = S.o[k];
-    const int32_t cj = c + o.ox * kW68 + o.oy;
-    const double2 uj = su[cj];
-    const double term = T.c1[o.cls] * dot_int(uj.x - ui.x, uj.y - ui.y, o.ox, o.oy);
-    acc[k % 4] += ok[cj] ? term : 0.0;
-  }
-  st.theta[i] = a.c2m[i] * ((acc[0] + acc[1]) + (acc[2] + acc[3]));
+  st.theta[i] = a.c2m[i] * theta_masked68(su, ok, c, su[c], T);
 }
 
 // Fast-path ABC rows: 16 lanes per row, four rows per wavefront, lane q of a row on its cloud

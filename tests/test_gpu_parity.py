@@ -66,7 +66,8 @@ def test_exact_mode_matches_oracle_bitwise(name):
 @pytest.mark.parametrize("name", CASES)
 @pytest.mark.parametrize("graph,variant,concurrent", [(True, 0, 1), (False, 0, 1), (True, 3, 1),
                                                       (True, 2, 0), (True, 0, 3), (False, 3, 3),
-                                                      (True, 3, 4), (False, 0, 4)])
+                                                      (True, 3, 4), (False, 0, 4), (True, 3, 2),
+                                                      (False, 0, 2)])
 def test_fast_mode_matches_reference_golden(name, graph, variant, concurrent):
     """Fast path (tile kernels, FMA) against the reference's outputs: 1e-10 relative L2; autotuned,
     register-blocked and unrolled node kernels, ABC rows beside or after the tile kernels."""
@@ -197,11 +198,12 @@ def test_abc_prediction_folded_or_apart_gives_the_same_bits(graph, monkeypatch, 
     out = []
     for mode in ("fold", "apart"):
         monkeypatch.setenv("LPS_ABC_PREDICT", mode)
-        with solver.Solver(r40_problem, variant=3, use_graph=graph) as s:
+        # schedule 1: the fold applies to the partial tiles' force pass on the main stream
+        with solver.Solver(r40_problem, variant=3, use_graph=graph, concurrent=1) as s:
             assert ("ABC prediction " + ("in" if mode == "fold" else "apart")) in s.path, s.path
             recs = s.run(95)
             one = (s.fetch(), recs)
-        with solver.Batch([g12, r40_problem], variant=3, use_graph=graph) as b:
+        with solver.Batch([g12, r40_problem], variant=3, use_graph=graph, concurrent=1) as b:
             brecs = b.run(95)
             many = ([b.fetch(i) for i in range(2)], brecs)
         out.append((one, many))
@@ -215,15 +217,14 @@ def test_abc_prediction_folded_or_apart_gives_the_same_bits(graph, monkeypatch, 
 
 @pytest.mark.parametrize("graph", [True, False])
 def test_step_schedules_give_the_same_bits(graph, r40_problem):
-    """The ABC rows after the tile kernels (0), on a second stream (1), as extra workgroups of the
-    dilatation launch (3), or on a second stream with the deep tiles' dilatation and force in one
-    launch (4): the same node arithmetic in every schedule, so the same bits, on one domain and on a
-    batch. (Schedule 2 runs the partial tiles' dilatation in a kernel of its own, a different
-    association of the same sums: it is held to the oracle in the other tests.)"""
+    """The ABC rows after the tile kernels (0), on a second stream (1), with the partial tiles on a
+    third stream in whole-tile blocks (2), as extra workgroups of the dilatation launch (3), or on a
+    second stream with the deep tiles' dilatation and force in one launch (4): the same node
+    arithmetic in every schedule, so the same bits, on one domain and on a batch."""
     say("schedules", graph)
     g12, _ = load_golden("abc_r12_nu033")
     out = []
-    for sched in (0, 1, 3, 4):
+    for sched in (0, 1, 2, 3, 4):
         with solver.Solver(r40_problem, variant=3, use_graph=graph, concurrent=sched) as s:
             # the R = 40 mm disc has deep tiles (the fused kernel runs) and shallow ones
             deep = int(re.search(r"(\d+) deep", s.path).group(1))
